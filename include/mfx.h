@@ -1,0 +1,198 @@
+/*
+ * libmfx — C ABI of the MI355X-native per-step grid path of mantaflow:
+ * semi-Lagrangian / MacCormack advection, wall boundary conditions and
+ * buoyancy, and the pressure projection (RHS, flag-derived Laplacian,
+ * conjugate gradient with an optional multigrid preconditioner, velocity
+ * correction).
+ *
+ * Every entry point returns 0 on success and -1 on failure; the message of
+ * the last failure on the calling thread is returned by mfx_last_error().
+ * Messages reuse the reference's wording where it has one.  Calls are
+ * asynchronous on the scene's HIP stream; only downloads, reductions and
+ * solve statistics synchronise.  A scene must not be used from two threads at
+ * once.  Grids belong to the scene that created them.
+ *
+ * Host data layout (uploads/downloads) is the reference's: Real grids
+ * float[n], Int and Flag grids int32[n], Vec3 and MAC grids float[n][3]
+ * (AoS), x fastest (idx = i + nx*j + nx*ny*k).  On the device Vec3/MAC grids
+ * are stored SoA and flag grids additionally keep a packed uint8 copy.
+ *
+ * Reference citations are relative to the mantaflow source tree.
+ */
+#ifndef MFX_H
+#define MFX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mfx_scene mfx_scene;
+typedef struct mfx_grid mfx_grid;
+
+/* grid types: the reference's GridBase::GridType bits (source/grid.h:29) */
+enum {
+	MFX_GRID_REAL = 1,
+	MFX_GRID_INT = 2,
+	MFX_GRID_VEC3 = 4,
+	MFX_GRID_MAC = 8 | 4,
+	MFX_GRID_LEVELSET = 16 | 1,
+	MFX_GRID_FLAGS = 32 | 2
+};
+
+/* preconditioners (source/plugin/pressure.cpp:27) */
+enum { MFX_PC_NONE = 0, MFX_PC_MIC = 1, MFX_PC_MGDYNAMIC = 2, MFX_PC_MGSTATIC = 3 };
+
+/* reductions for mfx_grid_reduce (source/grid.h:184-195, grid.cpp:319-372) */
+enum { MFX_RED_MAX = 0, MFX_RED_MIN = 1, MFX_RED_MAXABS = 2, MFX_RED_L1 = 3, MFX_RED_L2 = 4, MFX_RED_SUM = 5 };
+
+/* shapes (source/shapes.h:37-108) */
+enum { MFX_SHAPE_BOX = 1, MFX_SHAPE_SPHERE = 2, MFX_SHAPE_CYLINDER = 3 };
+
+/* ---------------------------------------------------------------- errors */
+/* message of the last failed call on this thread (replaces the C++ exception
+ * the reference's wrappers turn into RuntimeError, source/pwrapper/pclass.cpp:57-61) */
+const char* mfx_last_error(void);
+int mfx_device_count(int* out);
+
+/* ---------------------------------------------------------------- scenes */
+/* FluidSolver(gridSize, dim) (source/fluidsolver.cpp:107-115) */
+int mfx_scene_create(int nx, int ny, int nz, int dim, mfx_scene** out);
+int mfx_scene_destroy(mfx_scene* s);
+/* FluidSolver::mDt, Python property `timestep` (source/fluidsolver.h:59) */
+int mfx_scene_set_dt(mfx_scene* s, float dt);
+int mfx_scene_get_dt(const mfx_scene* s, float* dt);
+/* wait for all work queued on the scene's stream */
+int mfx_scene_sync(mfx_scene* s);
+/* bytes of device memory held by the scene (grids, temporaries, MG cache) */
+int mfx_scene_memory(const mfx_scene* s, size_t* bytes);
+
+/* ----------------------------------------------------------------- grids */
+/* Grid<T>::Grid / MACGrid / FlagGrid constructors, zero-initialised
+ * (source/grid.cpp:47-59) */
+int mfx_grid_create(mfx_scene* s, int type, mfx_grid** out);
+int mfx_grid_destroy(mfx_grid* g);
+int mfx_grid_type(const mfx_grid* g, int* type);
+int mfx_grid_upload(mfx_grid* g, const void* host, size_t bytes);
+int mfx_grid_download(const mfx_grid* g, void* host, size_t bytes);
+/* Grid<T>::clear, copyFrom, swap (source/grid.cpp:93-110,205-210) */
+int mfx_grid_clear(mfx_grid* g);
+int mfx_grid_copy(mfx_grid* dst, const mfx_grid* src);
+int mfx_grid_swap(mfx_grid* a, mfx_grid* b);
+/* element-wise ops (source/grid.h:472-480, grid.cpp:212-280); `v` holds one
+ * value for Real/Int grids and three for Vec3/MAC grids */
+int mfx_grid_set_const(mfx_grid* g, const float* v);
+int mfx_grid_add_const(mfx_grid* g, const float* v);
+int mfx_grid_mult_const(mfx_grid* g, const float* v);
+int mfx_grid_add(mfx_grid* g, const mfx_grid* a);
+int mfx_grid_sub(mfx_grid* g, const mfx_grid* a);
+int mfx_grid_mult(mfx_grid* g, const mfx_grid* a);
+int mfx_grid_add_scaled(mfx_grid* g, const mfx_grid* a, const float* factor);
+int mfx_grid_clamp(mfx_grid* g, float vmin, float vmax);
+/* getMax/getMin/getMaxAbs/getL1/getL2 (source/grid.cpp:315-389); bnd used by L1/L2 */
+int mfx_grid_reduce(const mfx_grid* g, int op, int bnd, double* out);
+/* setBound / setBoundNeumann / setBoundMAC (source/grid.cpp:585-648) */
+int mfx_grid_set_bound(mfx_grid* g, const float* value, int boundaryWidth);
+int mfx_grid_set_bound_neumann(mfx_grid* g, int boundaryWidth);
+int mfx_grid_set_bound_mac(mfx_grid* g, const float* value, int boundaryWidth, int normalOnly);
+/* gridMaxDiff / gridMaxDiffInt / gridMaxDiffVec3 (source/grid.cpp:400-430) */
+int mfx_grid_max_diff(const mfx_grid* a, const mfx_grid* b, double* out);
+/* getComponent / setComponent / copyMacToVec3 (source/grid.cpp:438,680,685) */
+int mfx_get_component(const mfx_grid* vec, mfx_grid* real, int c);
+int mfx_set_component(const mfx_grid* real, mfx_grid* vec, int c);
+int mfx_copy_mac_to_vec3(const mfx_grid* mac, mfx_grid* vec);
+
+/* ----------------------------------------------------------------- flags */
+/* FlagGrid::initDomain (source/grid.cpp:732-842) */
+int mfx_flags_init_domain(mfx_grid* flags, int boundaryWidth, const char* wall, const char* open,
+                          const char* inflow, const char* outflow);
+/* FlagGrid::fillGrid (source/grid.cpp:856-861) */
+int mfx_flags_fill_grid(mfx_grid* flags, int type);
+/* FlagGrid::countCells (source/grid.cpp:384-398); mask may be NULL */
+int mfx_flags_count_cells(const mfx_grid* flags, int flag, int bnd, mfx_grid* mask, long long* out);
+/* setOpenBound (source/plugin/extforces.cpp:105-130) */
+int mfx_set_open_bound(mfx_grid* flags, int bWidth, const char* openBound, int type);
+/* resetOutflow without particles (source/plugin/extforces.cpp:133-160) */
+int mfx_reset_outflow(mfx_grid* flags, mfx_grid* phi, mfx_grid* real);
+
+/* ---------------------------------------------------------------- shapes */
+/* Shape::applyToGrid for Box/Sphere/Cylinder (source/shapes.cpp:71-84,151-154,240,324-329).
+ * BOX: a=p0, b=p1.  SPHERE: a=center, b=scale, r=radius.  CYLINDER: a=center, b=z, r=radius. */
+int mfx_apply_shape(mfx_grid* grid, int shape, const float* a, const float* b, float r,
+                    const float* value, const mfx_grid* respectFlags);
+
+/* ------------------------------------------------------------- advection */
+/* advectSemiLagrange (source/plugin/advection.cpp:442-460); dt is the scene's.
+ * orderSpace and orderTrace other than 1 are rejected with an error. */
+int mfx_advect_semi_lagrange(const mfx_grid* flags, const mfx_grid* vel, mfx_grid* grid, int order,
+                             float strength, int orderSpace, int clampMode, int orderTrace);
+
+/* ------------------------------------------------- forces / boundaries */
+/* setWallBcs without fractions/phiObs (source/plugin/extforces.cpp:307-315); obvel may be NULL */
+int mfx_set_wall_bcs(const mfx_grid* flags, mfx_grid* vel, const mfx_grid* obvel);
+/* addBuoyancy (source/plugin/extforces.cpp:83-87) */
+int mfx_add_buoyancy(const mfx_grid* flags, const mfx_grid* density, mfx_grid* vel,
+                     const float gravity[3], float coefficient, int scale);
+
+/* --------------------------------------------------------------- pressure */
+typedef struct {
+	/* the reference's solvePressure kwargs (source/plugin/pressure.cpp:480-495) */
+	float cgAccuracy;          /* 1e-3 */
+	float cgMaxIterFac;        /* 1.5 */
+	int preconditioner;        /* MFX_PC_MIC */
+	int enforceCompatibility;  /* 0 */
+	int useL2Norm;             /* 0 */
+	int zeroPressureFixing;    /* 0 */
+	/* extensions (not in the reference) */
+	double relTol;             /* > 0: stop on ||r||_2 <= relTol * ||b||_2 (FP64 norms) */
+	int maxIterOverride;       /* > 0: replaces the reference's iteration cap */
+	int precision;             /* 0 FP32 path; 1 mixed (BF16 MG smoothing, FP64 scalars) */
+	int trueResidualEvery;     /* > 0: recompute r = b - A x with FP64 accumulation every k its */
+} mfx_pressure_opts;
+
+typedef struct {
+	int iterations;            /* GridCg::mIterations */
+	float resNorm;             /* GridCg::mResNorm at exit */
+	double relResidual;        /* ||b - A x||_2 / ||b||_2, FP64, at exit */
+	int fixPidx;               /* pinned cell (zeroPressureFixing) or -1 */
+	int maxIter;               /* iteration cap used */
+	int preconditionerUsed;    /* preconditioner actually run on the GPU */
+	int converged;             /* 1 if the stopping criterion was met */
+	double seconds;            /* wall time of the call */
+} mfx_solve_stats;
+
+void mfx_pressure_opts_default(mfx_pressure_opts* o);
+
+/* solvePressure (source/plugin/pressure.cpp:480-521).  retRhs may be NULL.
+ * PcMIC (a sequential triangular solve in the reference,
+ * conjugategrad.cpp:135-159) runs as multigrid-preconditioned CG in 3D and as
+ * plain CG in 2D, where the reference also disables it (conjugategrad.cpp:315-321). */
+int mfx_solve_pressure(mfx_grid* vel, mfx_grid* pressure, const mfx_grid* flags,
+                       const mfx_pressure_opts* opts, mfx_solve_stats* stats, mfx_grid* retRhs);
+/* computePressureRhs (source/plugin/pressure.cpp:277-299) */
+int mfx_compute_pressure_rhs(mfx_grid* rhs, const mfx_grid* vel, const mfx_grid* flags,
+                             const mfx_pressure_opts* opts);
+/* solvePressureSystem (source/plugin/pressure.cpp:312-452); rhs may be modified
+ * by zero-pressure fixing exactly as in the reference */
+int mfx_solve_pressure_system(mfx_grid* rhs, mfx_grid* pressure, const mfx_grid* flags,
+                              const mfx_pressure_opts* opts, mfx_solve_stats* stats);
+/* correctVelocity (source/plugin/pressure.cpp:455-476) */
+int mfx_correct_velocity(mfx_grid* vel, const mfx_grid* pressure, const mfx_grid* flags);
+/* releaseMG (source/plugin/pressure.cpp:252-266); s == NULL releases all */
+int mfx_release_mg(mfx_scene* s);
+
+/* ------------------------------------------------------------- profiling */
+/* optional HIP-event timing of the main kernels on their launching stream
+ * (names: "cg_matvec", "cg_update", "mg_l0_smooth", "mg_l0_residual",
+ * "advect_sl", "advect_maccormack"); off by default, no counterpart in the
+ * reference (its TimingData times whole plugin calls, timing.cpp:23-89) */
+int mfx_kernel_timing(int enable);
+int mfx_kernel_timing_reset(void);
+int mfx_kernel_timing_get(const char* name, double* totalMs, long long* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,119 @@
+"""ctypes binding of libmfx.so (declared in include/mfx.h).
+
+The library is loaded from this package directory; there is no fallback: if
+it is missing or fails to load, importing raises.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmfx.so")
+
+GRID_REAL, GRID_INT, GRID_VEC3, GRID_MAC, GRID_LEVELSET, GRID_FLAGS = 1, 2, 4, 12, 17, 34
+RED_MAX, RED_MIN, RED_MAXABS, RED_L1, RED_L2, RED_SUM = 0, 1, 2, 3, 4, 5
+SHAPE_BOX, SHAPE_SPHERE, SHAPE_CYLINDER = 1, 2, 3
+
+
+class PressureOpts(C.Structure):
+    _fields_ = [("cgAccuracy", C.c_float), ("cgMaxIterFac", C.c_float), ("preconditioner", C.c_int),
+                ("enforceCompatibility", C.c_int), ("useL2Norm", C.c_int),
+                ("zeroPressureFixing", C.c_int), ("relTol", C.c_double),
+                ("maxIterOverride", C.c_int), ("precision", C.c_int),
+                ("trueResidualEvery", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("resNorm", C.c_float), ("relResidual", C.c_double),
+                ("fixPidx", C.c_int), ("maxIter", C.c_int), ("preconditionerUsed", C.c_int),
+                ("converged", C.c_int), ("seconds", C.c_double)]
+
+
+# every exported symbol of include/mfx.h with its ctypes signature
+_P = C.c_void_p
+_I = C.c_int
+_F = C.c_float
+_FP = C.POINTER(C.c_float)
+SIGNATURES = {
+    "mfx_last_error": (C.c_char_p, []),
+    "mfx_device_count": (_I, [C.POINTER(_I)]),
+    "mfx_scene_create": (_I, [_I, _I, _I, _I, C.POINTER(_P)]),
+    "mfx_scene_destroy": (_I, [_P]),
+    "mfx_scene_set_dt": (_I, [_P, _F]),
+    "mfx_scene_get_dt": (_I, [_P, _FP]),
+    "mfx_scene_sync": (_I, [_P]),
+    "mfx_scene_memory": (_I, [_P, C.POINTER(C.c_size_t)]),
+    "mfx_grid_create": (_I, [_P, _I, C.POINTER(_P)]),
+    "mfx_grid_destroy": (_I, [_P]),
+    "mfx_grid_type": (_I, [_P, C.POINTER(_I)]),
+    "mfx_grid_upload": (_I, [_P, _P, C.c_size_t]),
+    "mfx_grid_download": (_I, [_P, _P, C.c_size_t]),
+    "mfx_grid_clear": (_I, [_P]),
+    "mfx_grid_copy": (_I, [_P, _P]),
+    "mfx_grid_swap": (_I, [_P, _P]),
+    "mfx_grid_set_const": (_I, [_P, _FP]),
+    "mfx_grid_add_const": (_I, [_P, _FP]),
+    "mfx_grid_mult_const": (_I, [_P, _FP]),
+    "mfx_grid_add": (_I, [_P, _P]),
+    "mfx_grid_sub": (_I, [_P, _P]),
+    "mfx_grid_mult": (_I, [_P, _P]),
+    "mfx_grid_add_scaled": (_I, [_P, _P, _FP]),
+    "mfx_grid_clamp": (_I, [_P, _F, _F]),
+    "mfx_grid_reduce": (_I, [_P, _I, _I, C.POINTER(C.c_double)]),
+    "mfx_grid_set_bound": (_I, [_P, _FP, _I]),
+    "mfx_grid_set_bound_neumann": (_I, [_P, _I]),
+    "mfx_grid_set_bound_mac": (_I, [_P, _FP, _I, _I]),
+    "mfx_grid_max_diff": (_I, [_P, _P, C.POINTER(C.c_double)]),
+    "mfx_get_component": (_I, [_P, _P, _I]),
+    "mfx_set_component": (_I, [_P, _P, _I]),
+    "mfx_copy_mac_to_vec3": (_I, [_P, _P]),
+    "mfx_flags_init_domain": (_I, [_P, _I, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p]),
+    "mfx_flags_fill_grid": (_I, [_P, _I]),
+    "mfx_flags_count_cells": (_I, [_P, _I, _I, _P, C.POINTER(C.c_longlong)]),
+    "mfx_set_open_bound": (_I, [_P, _I, C.c_char_p, _I]),
+    "mfx_reset_outflow": (_I, [_P, _P, _P]),
+    "mfx_apply_shape": (_I, [_P, _I, _FP, _FP, _F, _FP, _P]),
+    "mfx_advect_semi_lagrange": (_I, [_P, _P, _P, _I, _F, _I, _I, _I]),
+    "mfx_set_wall_bcs": (_I, [_P, _P, _P]),
+    "mfx_add_buoyancy": (_I, [_P, _P, _P, _FP, _F, _I]),
+    "mfx_pressure_opts_default": (None, [C.POINTER(PressureOpts)]),
+    "mfx_solve_pressure": (_I, [_P, _P, _P, C.POINTER(PressureOpts), C.POINTER(SolveStats), _P]),
+    "mfx_compute_pressure_rhs": (_I, [_P, _P, _P, C.POINTER(PressureOpts)]),
+    "mfx_solve_pressure_system": (_I, [_P, _P, _P, C.POINTER(PressureOpts), C.POINTER(SolveStats)]),
+    "mfx_correct_velocity": (_I, [_P, _P, _P]),
+    "mfx_release_mg": (_I, [_P]),
+    "mfx_kernel_timing": (_I, [_I]),
+    "mfx_kernel_timing_reset": (_I, []),
+    "mfx_kernel_timing_get": (_I, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libmfx.so (once).  Raises if it is not built or cannot be loaded."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libmfx.so is not built (%s); run python -m mantaflow_amd.build" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+class MfxError(RuntimeError):
+    pass
+
+
+def check(rc):
+    if rc != 0:
+        raise MfxError(lib().mfx_last_error().decode(errors="replace"))
+    return rc
+
+
+def floats(*v):
+    arr = (C.c_float * max(3, len(v)))(*v)
+    return arr

@@ -1,0 +1,676 @@
+// Semi-Lagrangian / MacCormack advection, wall boundary conditions and
+// buoyancy on device-resident grids.
+//
+// Arithmetic mirrors the reference's C++ expressions operation by operation
+// (compiled with -ffp-contract=off), so results are bit-identical to the
+// reference's serial FP32 plugins:
+//   advection.cpp:25-460, util/interpol.h:50-90, grid.h:424-470,
+//   extforces.cpp:72-87,186-217.
+// Kernels are laid out for gfx950: one 64-wide wavefront per x-row segment
+// (coalesced SoA loads), 64x4 tiles, XCD-contiguous tile order; trilinear
+// gathers are served from L1/L2.  The MacCormack backward step, correction
+// and clamp are fused into one pass (the backward value of a cell is only
+// needed at that cell), so order-2 advection is two full passes.
+#include "mfx_internal.h"
+#include "mfx_prof.h"
+
+namespace mfx {
+
+struct Vel {
+	const float* __restrict__ u;
+	const float* __restrict__ v;
+	const float* __restrict__ w;
+};
+
+__device__ inline bool interior(const Dims& d, int i, int j, int k) {
+	return i >= 1 && i < d.nx - 1 && j >= 1 && j < d.ny - 1 && (d.is3d ? (k >= 1 && k < d.nz - 1) : true);
+}
+
+// MACGrid::getCentered (grid.h:424-437)
+__device__ inline void getCentered(const Vel& V, const Dims& d, long long ix, float* o) {
+	o[0] = 0.5f * (V.u[ix] + V.u[ix + 1]);
+	o[1] = 0.5f * (V.v[ix] + V.v[ix + d.nx]);
+	o[2] = 0.f;
+	if (d.is3d) o[2] = 0.5f * (V.w[ix] + V.w[ix + d.sz]);
+}
+// MACGrid::getAtMACX/Y/Z (grid.h:439-470); Z is not guarded by is3D
+__device__ inline void getAtMACX(const Vel& V, const Dims& d, long long ix, float* o) {
+	const long long Y = d.nx, Z = d.sz;
+	o[0] = V.u[ix];
+	o[1] = 0.25f * (V.v[ix] + V.v[ix - 1] + V.v[ix + Y] + V.v[ix + Y - 1]);
+	o[2] = 0.f;
+	if (d.is3d) o[2] = 0.25f * (V.w[ix] + V.w[ix - 1] + V.w[ix + Z] + V.w[ix + Z - 1]);
+}
+__device__ inline void getAtMACY(const Vel& V, const Dims& d, long long ix, float* o) {
+	const long long Y = d.nx, Z = d.sz;
+	o[0] = 0.25f * (V.u[ix] + V.u[ix - Y] + V.u[ix + 1] + V.u[ix + 1 - Y]);
+	o[1] = V.v[ix];
+	o[2] = 0.f;
+	if (d.is3d) o[2] = 0.25f * (V.w[ix] + V.w[ix - Y] + V.w[ix + Z] + V.w[ix + Z - Y]);
+}
+__device__ inline void getAtMACZ(const Vel& V, const Dims& d, long long ix, float* o) {
+	const long long Y = d.nx, Z = d.sz;
+	o[0] = 0.25f * (V.u[ix] + V.u[ix - Z] + V.u[ix + 1] + V.u[ix + 1 - Z]);
+	o[1] = 0.25f * (V.v[ix] + V.v[ix - Z] + V.v[ix + Y] + V.v[ix + Y - Z]);
+	o[2] = V.w[ix];
+}
+
+// interpol<Real> / interpolComponent (interpol.h:50-90)
+__device__ inline float interp(const float* __restrict__ data, const Dims& d, float posx, float posy,
+                               float posz) {
+	float px = posx - 0.5f, py = posy - 0.5f, pz = posz - 0.5f;
+	int xi = (int)px, yi = (int)py, zi = (int)pz;
+	float s1 = px - (float)xi, s0 = 1.0f - s1;
+	float t1 = py - (float)yi, t0 = 1.0f - t1;
+	float f1 = pz - (float)zi, f0 = 1.0f - f1;
+	if (px < 0.f) { xi = 0; s0 = 1.0f; s1 = 0.0f; }
+	if (py < 0.f) { yi = 0; t0 = 1.0f; t1 = 0.0f; }
+	if (pz < 0.f) { zi = 0; f0 = 1.0f; f1 = 0.0f; }
+	if (xi >= d.nx - 1) { xi = d.nx - 2; s0 = 0.0f; s1 = 1.0f; }
+	if (yi >= d.ny - 1) { yi = d.ny - 2; t0 = 0.0f; t1 = 1.0f; }
+	if (d.nz > 1) {
+		if (zi >= d.nz - 1) { zi = d.nz - 2; f0 = 0.0f; f1 = 1.0f; }
+	}
+	const long long X = 1, Y = d.nx, Z = d.sz;
+	const long long idx = (long long)xi + Y * yi + Z * zi;
+	return ((data[idx] * t0 + data[idx + Y] * t1) * s0 + (data[idx + X] * t0 + data[idx + X + Y] * t1) * s1) * f0 +
+	       ((data[idx + Z] * t0 + data[idx + Y + Z] * t1) * s0 +
+	        (data[idx + X + Z] * t0 + data[idx + X + Y + Z] * t1) * s1) * f1;
+}
+
+__device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+__device__ inline float clampf_ref(float v, float lo, float hi) {
+	if (v < lo) return lo;
+	if (v > hi) return hi;
+	return v;
+}
+
+#define CHECKFLAG(ix) (f8[ix] & (kFluid | kEmpty))
+
+// ---------------------------------------------------------------------------
+// forward SL step for centered grids (1 or 3 components): dst = 0 on the
+// outer layer (the reference writes a zeroed temporary with bnd=1)
+// ---------------------------------------------------------------------------
+template <int NC>
+__global__ void __launch_bounds__(kBlock) k_sl_centered(Vel V, Dims d, Tiles t, float dt, const float* __restrict__ s0,
+                                                        const float* __restrict__ s1, const float* __restrict__ s2,
+                                                        float* __restrict__ o0, float* __restrict__ o1,
+                                                        float* __restrict__ o2) {
+	CellPos p = tile_cell(t);
+	if (p.i >= d.nx || p.j >= d.ny) return;
+	const long long ix = d.idx(p.i, p.j, p.k);
+	if (!interior(d, p.i, p.j, p.k)) {
+		o0[ix] = 0.f;
+		if (NC == 3) {
+			o1[ix] = 0.f;
+			o2[ix] = 0.f;
+		}
+		return;
+	}
+	float c[3];
+	getCentered(V, d, ix, c);
+	const float px = ((float)p.i + 0.5f) - c[0] * dt;
+	const float py = ((float)p.j + 0.5f) - c[1] * dt;
+	const float pz = ((float)p.k + 0.5f) - c[2] * dt;
+	o0[ix] = interp(s0, d, px, py, pz);
+	if (NC == 3) {
+		o1[ix] = interp(s1, d, px, py, pz);
+		o2[ix] = interp(s2, d, px, py, pz);
+	}
+}
+
+// SemiLagrangeMAC value of one cell (advection.cpp:44-57)
+__device__ inline void sl_mac_cell(const Vel& V, const Dims& d, long long ix, int i, int j, int k, float dt,
+                                   const float* __restrict__ su, const float* __restrict__ sv,
+                                   const float* __restrict__ sw, float* out) {
+	const float p0x = (float)i + 0.5f, p0y = (float)j + 0.5f, p0z = (float)k + 0.5f;
+	float v[3];
+	getAtMACX(V, d, ix, v);
+	out[0] = interp(su, d, p0x - v[0] * dt, p0y - v[1] * dt, p0z - v[2] * dt);
+	getAtMACY(V, d, ix, v);
+	out[1] = interp(sv, d, p0x - v[0] * dt, p0y - v[1] * dt, p0z - v[2] * dt);
+	getAtMACZ(V, d, ix, v);
+	out[2] = interp(sw, d, p0x - v[0] * dt, p0y - v[1] * dt, p0z - v[2] * dt);
+}
+
+__global__ void __launch_bounds__(kBlock) k_sl_mac(Vel V, Dims d, Tiles t, float dt, const float* __restrict__ su,
+                                                   const float* __restrict__ sv, const float* __restrict__ sw,
+                                                   float* __restrict__ ou, float* __restrict__ ov,
+                                                   float* __restrict__ ow) {
+	CellPos p = tile_cell(t);
+	if (p.i >= d.nx || p.j >= d.ny) return;
+	const long long ix = d.idx(p.i, p.j, p.k);
+	if (!interior(d, p.i, p.j, p.k)) {
+		ou[ix] = 0.f;
+		ov[ix] = 0.f;
+		ow[ix] = 0.f;
+		return;
+	}
+	float out[3];
+	sl_mac_cell(V, d, ix, p.i, p.j, p.k, dt, su, sv, sw, out);
+	ou[ix] = out[0];
+	ov[ix] = out[1];
+	ow[ix] = out[2];
+}
+
+// ---------------------------------------------------------------------------
+// MacCormack for centered grids: backward SL (fwd, -dt), correction
+// (MacCormackCorrect, all idx) and clamp (MacCormackClamp, bnd=1) fused.
+// ---------------------------------------------------------------------------
+template <int NC>
+__global__ void __launch_bounds__(kBlock) k_maccormack_centered(Vel V, Dims d, Tiles t, float dt, float strength,
+                                                                int clampMode, const uint8_t* __restrict__ f8,
+                                                                const float* o0, const float* o1, const float* o2,
+                                                                const float* __restrict__ fw0,
+                                                                const float* __restrict__ fw1,
+                                                                const float* __restrict__ fw2, float* __restrict__ n0,
+                                                                float* __restrict__ n1, float* __restrict__ n2) {
+	CellPos p = tile_cell(t);
+	if (p.i >= d.nx || p.j >= d.ny) return;
+	const long long ix = d.idx(p.i, p.j, p.k);
+	const bool inner = interior(d, p.i, p.j, p.k);
+	const float* orig[3] = {o0, o1, o2};
+	const float* fwd[3] = {fw0, fw1, fw2};
+	float* dst[3] = {n0, n1, n2};
+	float vc[3] = {0.f, 0.f, 0.f};
+	float px = 0.f, py = 0.f, pz = 0.f;
+	if (inner) {
+		getCentered(V, d, ix, vc);
+		// bwd position uses -dt: pos - v*(-dt)
+		const float mdt = -dt;
+		px = ((float)p.i + 0.5f) - vc[0] * mdt;
+		py = ((float)p.j + 0.5f) - vc[1] * mdt;
+		pz = ((float)p.k + 0.5f) - vc[2] * mdt;
+	}
+	const bool fl = f8[ix] & kFluid;
+	float val[3], fv[3];
+	for (int c = 0; c < NC; c++) {
+		const float bwd = inner ? interp(fwd[c], d, px, py, pz) : 0.f;
+		fv[c] = fwd[c][ix];
+		float x = fv[c];
+		if (fl) {
+			if (NC == 1) {
+				// float "+=" double expression (advection.cpp:89)
+				x = (float)((double)x + (double)strength * 0.5 * (double)(orig[c][ix] - bwd));
+			} else {
+				// Vec3: double product narrowed per component, float add
+				x = x + (float)((double)strength * 0.5 * (double)(orig[c][ix] - bwd));
+			}
+		}
+		val[c] = x;
+	}
+	if (inner) {
+		// doClampComponent (advection.cpp:144-186) with pos = (i,j,k)
+		const float vdt[3] = {vc[0] * dt, vc[1] * dt, vc[2] * dt};
+		const float pos[3] = {(float)p.i, (float)p.j, (float)p.k};
+		const int gux = d.nx - 1, guy = d.ny - 1, guz = d.nz - 1;
+		float mn[3] = {3.402823466e+38f, 3.402823466e+38f, 3.402823466e+38f};
+		float mx[3] = {-3.402823466e+38f, -3.402823466e+38f, -3.402823466e+38f};
+		bool haveFl = false;
+		const int numPos = clampMode == 1 ? 2 : 1;
+		for (int l = 0; l < numPos; ++l) {
+			const float sg = l == 0 ? -1.f : 1.f;
+			int P[3];
+			for (int a = 0; a < 3; a++) P[a] = (int)(l == 0 ? pos[a] - vdt[a] : pos[a] + vdt[a]);
+			(void)sg;
+			const int i0 = clampi(P[0], 0, gux - 1), j0 = clampi(P[1], 0, guy - 1);
+			const int k0 = clampi(P[2], 0, d.is3d ? guz - 1 : 1);
+			const int i1 = i0 + 1, j1 = j0 + 1, k1 = d.is3d ? k0 + 1 : k0;
+			const int nc = d.is3d ? 8 : 4;
+			for (int q = 0; q < nc; q++) {
+				const int ci = (q & 1) ? i1 : i0, cj = (q & 2) ? j1 : j0, ck = (q & 4) ? k1 : k0;
+				const long long cx = d.idx(ci, cj, ck);
+				if (CHECKFLAG(cx)) {
+					for (int c = 0; c < NC; c++) {
+						const float ov = orig[c][cx];
+						if (ov < mn[c]) mn[c] = ov;
+						if (ov > mx[c]) mx[c] = ov;
+					}
+					haveFl = true;
+				}
+			}
+		}
+		if (!haveFl) {
+			for (int c = 0; c < NC; c++) val[c] = fv[c];
+		} else if (clampMode == 1) {
+			for (int c = 0; c < NC; c++) val[c] = clampf_ref(val[c], mn[c], mx[c]);
+		} else {
+			bool out = false;
+			for (int c = 0; c < NC; c++) out = out || (val[c] < mn[c]) || (val[c] > mx[c]);
+			if (out)
+				for (int c = 0; c < NC; c++) val[c] = fv[c];
+		}
+		if (clampMode == 1) {
+			int pf[3], pb[3];
+			for (int a = 0; a < 3; a++) {
+				const float base = pos[a] + 0.5f;
+				pf[a] = (int)(base - vdt[a]);
+				pb[a] = (int)(base + vdt[a]);
+			}
+			if (pf[0] < 0 || pf[1] < 0 || pf[2] < 0 || pb[0] < 0 || pb[1] < 0 || pb[2] < 0 || pf[0] > gux ||
+			    pf[1] > guy || ((pf[2] > guz) && d.is3d) || pb[0] > gux || pb[1] > guy ||
+			    ((pb[2] > guz) && d.is3d) || (f8[d.idx(pf[0], pf[1], pf[2])] & kObstacle) ||
+			    (f8[d.idx(pb[0], pb[1], pb[2])] & kObstacle)) {
+				for (int c = 0; c < NC; c++) val[c] = fv[c];
+			}
+		}
+	}
+	for (int c = 0; c < NC; c++) dst[c][ix] = val[c];
+}
+
+// doClampComponentMAC<c> (advection.cpp:191-235)
+__device__ inline float clamp_comp_mac(const Dims& d, const uint8_t* __restrict__ f8, int c, float dst,
+                                       const float* __restrict__ origc, float fwd, int i, int j, int k,
+                                       const float* vel, int clampMode) {
+	const int gux = d.nx - 1, guy = d.ny - 1, guz = d.nz - 1;
+	const float pos[3] = {(float)i, (float)j, (float)k};
+	if (clampMode == 2) {
+		const int o[3] = {i, j, k};
+		int nb[3] = {i, j, k};
+		nb[c] -= 1;
+		if (!(CHECKFLAG(d.idx(o[0], o[1], o[2])) && CHECKFLAG(d.idx(nb[0], nb[1], nb[2])))) return fwd;
+	}
+	float mn = 3.402823466e+38f, mx = -3.402823466e+38f;
+	const int numPos = clampMode == 1 ? 2 : 1;
+	for (int l = 0; l < numPos; ++l) {
+		int P[3];
+		for (int a = 0; a < 3; a++) P[a] = (int)(l == 0 ? pos[a] - vel[a] : pos[a] + vel[a]);
+		const int i0 = clampi(P[0], 0, gux - 1), j0 = clampi(P[1], 0, guy - 1);
+		const int k0 = clampi(P[2], 0, d.is3d ? guz - 1 : 0);
+		const int i1 = i0 + 1, j1 = j0 + 1, k1 = d.is3d ? k0 + 1 : k0;
+		const int nc = d.is3d ? 8 : 4;
+		for (int q = 0; q < nc; q++) {
+			const int ci = (q & 1) ? i1 : i0, cj = (q & 2) ? j1 : j0, ck = (q & 4) ? k1 : k0;
+			const float ov = origc[d.idx(ci, cj, ck)];
+			if (ov < mn) mn = ov;
+			if (ov > mx) mx = ov;
+		}
+	}
+	if (clampMode == 1) return clampf_ref(dst, mn, mx);
+	if (dst < mn || dst > mx) return fwd;
+	return dst;
+}
+
+// MacCormack for MAC grids: backward SemiLagrangeMAC (fwd, -dt), correction
+// (MacCormackCorrectMAC, all ijk) and clamp (MacCormackClampMAC, bnd=1) fused.
+__global__ void __launch_bounds__(kBlock) k_maccormack_mac(Vel V, Dims d, Tiles t, float dt, float strength,
+                                                           int clampMode, const uint8_t* __restrict__ f8,
+                                                           const float* ou, const float* ov, const float* ow,
+                                                           const float* __restrict__ fu, const float* __restrict__ fv,
+                                                           const float* __restrict__ fw, float* __restrict__ nu,
+                                                           float* __restrict__ nv, float* __restrict__ nw) {
+	CellPos p = tile_cell(t);
+	if (p.i >= d.nx || p.j >= d.ny) return;
+	const int i = p.i, j = p.j, k = p.k;
+	const long long ix = d.idx(i, j, k);
+	const bool inner = interior(d, i, j, k);
+	float bwd[3] = {0.f, 0.f, 0.f};
+	if (inner) sl_mac_cell(V, d, ix, i, j, k, -dt, fu, fv, fw, bwd);
+	// MacCormackCorrectMAC skip logic (isMAC = true)
+	bool skip[3] = {false, false, false};
+	if (!(f8[ix] & kFluid)) skip[0] = skip[1] = skip[2] = true;
+	if ((i > 0) && !(f8[ix - 1] & kFluid)) skip[0] = true;
+	if ((j > 0) && !(f8[ix - d.nx] & kFluid)) skip[1] = true;
+	if ((k > 0) && !(f8[ix - d.sz] & kFluid)) skip[2] = true;
+	const float* orig[3] = {ou, ov, ow};
+	const float* fwd[3] = {fu, fv, fw};
+	float val[3], fvv[3];
+	for (int c = 0; c < 3; c++) {
+		fvv[c] = fwd[c][ix];
+		if (skip[c]) val[c] = fvv[c];
+		else val[c] = (float)((double)fvv[c] + (double)strength * 0.5 * (double)(orig[c][ix] - bwd[c]));
+	}
+	if (inner) {
+		float v[3], vdt[3];
+		getAtMACX(V, d, ix, v);
+		for (int a = 0; a < 3; a++) vdt[a] = v[a] * dt;
+		val[0] = clamp_comp_mac(d, f8, 0, val[0], ou, fvv[0], i, j, k, vdt, clampMode);
+		getAtMACY(V, d, ix, v);
+		for (int a = 0; a < 3; a++) vdt[a] = v[a] * dt;
+		val[1] = clamp_comp_mac(d, f8, 1, val[1], ov, fvv[1], i, j, k, vdt, clampMode);
+		if (d.is3d) {
+			getAtMACZ(V, d, ix, v);
+			for (int a = 0; a < 3; a++) vdt[a] = v[a] * dt;
+			val[2] = clamp_comp_mac(d, f8, 2, val[2], ow, fvv[2], i, j, k, vdt, clampMode);
+		}
+	}
+	nu[ix] = val[0];
+	nv[ix] = val[1];
+	nw[ix] = val[2];
+}
+
+// ---------------------------------------------------------------------------
+// outflow extrapolation (advection.cpp:326-391): only outflow cells do work;
+// velDst is kept in a scratch Vec grid and copied back in a second pass.
+// ---------------------------------------------------------------------------
+__device__ inline bool inb(const Dims& d, int i, int j, int k) {
+	return i >= 0 && j >= 0 && k >= 0 && i < d.nx && j < d.ny && k < d.nz;
+}
+
+__global__ void __launch_bounds__(kBlock) k_outflow_extrap(Dims d, Tiles t, const uint8_t* __restrict__ f8,
+                                                           const float* __restrict__ vu, const float* __restrict__ vv,
+                                                           const float* __restrict__ vw, const float* __restrict__ pu,
+                                                           const float* __restrict__ pv, const float* __restrict__ pw,
+                                                           float timeStep, float* __restrict__ du,
+                                                           float* __restrict__ dv, float* __restrict__ dw) {
+	CellPos p = tile_cell(t);
+	if (p.i >= d.nx || p.j >= d.ny) return;
+	const int i = p.i, j = p.j, k = p.k;
+	const long long ix = d.idx(i, j, k);
+	if (!(f8[ix] & kOutflow)) return;
+	const float* vel[3] = {vu, vv, vw};
+	const float* prev[3] = {pu, pv, pw};
+	// getBulkVel
+	float avg[3] = {0.f, 0.f, 0.f};
+	int count = 0;
+	const int nmax = d.is3d ? 1 : 0;
+	for (int nn = -nmax; nn <= nmax; nn++)
+		for (int m = -1; m <= 1; m++)
+			for (int l = -1; l <= 1; l++) {
+				if (inb(d, i + l, j + m, k + nn)) {
+					const long long q = d.idx(i + l, j + m, k + nn);
+					if ((f8[q] & kFluid) || (f8[q] & kOutflow)) {
+						for (int c = 0; c < 3; c++) avg[c] += vel[c][q];
+						count++;
+					}
+				}
+			}
+	float bulk[3];
+	for (int c = 0; c < 3; c++) bulk[c] = count > 0 ? avg[c] / (float)count : avg[c];
+	const int dim = d.is3d ? 3 : 2;
+	const int cur[3] = {i, j, k};
+	int cnt = 0;
+	float vd[3] = {0.f, 0.f, 0.f};
+	for (int c = 0; c < dim; c++) {
+		int low[3], up[3], flLow[3], flUp[3];
+		for (int a = 0; a < 3; a++) low[a] = up[a] = flLow[a] = flUp[a] = cur[a];
+		const float mb = (1.0f < bulk[c]) ? bulk[c] : 1.0f;
+		const float factor = timeStep * mb;
+		low[c] = flLow[c] = cur[c] - 1;
+		up[c] = flUp[c] = cur[c] + 1;
+		for (int dd = 0; dd < 2; dd++) {
+			const bool fromLower = inb(d, flLow[0], flLow[1], flLow[2]) && (f8[d.idx(flLow[0], flLow[1], flLow[2])] & kFluid);
+			const bool fromUpper = inb(d, flUp[0], flUp[1], flUp[2]) && (f8[d.idx(flUp[0], flUp[1], flUp[2])] & kFluid);
+			if (fromLower || fromUpper) {
+				if (fromLower) {
+					const long long q = d.idx(low[0], low[1], low[2]);
+					for (int a = 0; a < 3; a++) vd[a] += ((vel[a][ix] - prev[a][ix]) / factor) + vel[a][q];
+					cnt++;
+				}
+				if (fromUpper) {
+					const long long q = d.idx(up[0], up[1], up[2]);
+					for (int a = 0; a < 3; a++) vd[a] += ((vel[a][ix] - prev[a][ix]) / factor) + vel[a][q];
+					cnt++;
+				}
+				break;
+			}
+			flLow[c]--;
+			flUp[c]++;
+		}
+	}
+	if (cnt > 0)
+		for (int a = 0; a < 3; a++) vd[a] /= (float)cnt;
+	du[ix] = vd[0];
+	dv[ix] = vd[1];
+	dw[ix] = vd[2];
+}
+
+__global__ void __launch_bounds__(kBlock) k_outflow_copy(Dims d, Tiles t, const uint8_t* __restrict__ f8,
+                                                         const float* __restrict__ du, const float* __restrict__ dv,
+                                                         const float* __restrict__ dw, float* __restrict__ vu,
+                                                         float* __restrict__ vv, float* __restrict__ vw) {
+	CellPos p = tile_cell(t);
+	if (p.i >= d.nx || p.j >= d.ny) return;
+	const long long ix = d.idx(p.i, p.j, p.k);
+	if (!(f8[ix] & kOutflow)) return;
+	vu[ix] = du[ix];
+	vv[ix] = dv[ix];
+	vw[ix] = dw[ix];
+}
+
+// ---------------------------------------------------------------------------
+// KnSetWallBcs (extforces.cpp:186-217), all ijk
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_set_wall_bcs(Dims d, Tiles t, const uint8_t* __restrict__ f8,
+                                                         float* __restrict__ u, float* __restrict__ v,
+                                                         float* __restrict__ w, const float* __restrict__ ou,
+                                                         const float* __restrict__ ov, const float* __restrict__ ow) {
+	CellPos p = tile_cell(t);
+	if (p.i >= d.nx || p.j >= d.ny) return;
+	const int i = p.i, j = p.j, k = p.k;
+	const long long ix = d.idx(i, j, k);
+	const uint8_t f = f8[ix];
+	const bool curFluid = f & kFluid, curObs = f & kObstacle;
+	if (!curFluid && !curObs) return;
+	float bx = 0.f, by = 0.f, bz = 0.f;
+	if (ou) {
+		bx = ou[ix];
+		by = ov[ix];
+		if (d.is3d) bz = ow[ix];
+	}
+	if (i > 0) {
+		const uint8_t g = f8[ix - 1];
+		if (g & kObstacle) u[ix] = bx;
+		if (curObs && (g & kFluid)) u[ix] = bx;
+	}
+	if (j > 0) {
+		const uint8_t g = f8[ix - d.nx];
+		if (g & kObstacle) v[ix] = by;
+		if (curObs && (g & kFluid)) v[ix] = by;
+	}
+	if (!d.is3d) {
+		w[ix] = 0.f;
+	} else if (k > 0) {
+		const uint8_t g = f8[ix - d.sz];
+		if (g & kObstacle) w[ix] = bz;
+		if (curObs && (g & kFluid)) w[ix] = bz;
+	}
+	if (curFluid) {
+		if ((i > 0 && (f8[ix - 1] & kStick)) || (i < d.nx - 1 && (f8[ix + 1] & kStick))) {
+			v[ix] = 0.f;
+			w[ix] = 0.f;
+		}
+		if ((j > 0 && (f8[ix - d.nx] & kStick)) || (j < d.ny - 1 && (f8[ix + d.nx] & kStick))) {
+			u[ix] = 0.f;
+			w[ix] = 0.f;
+		}
+		if (d.is3d && ((k > 0 && (f8[ix - d.sz] & kStick)) || (k < d.nz - 1 && (f8[ix + d.sz] & kStick)))) {
+			u[ix] = 0.f;
+			v[ix] = 0.f;
+		}
+	}
+}
+
+// KnAddBuoyancy (extforces.cpp:72-80), bnd=1; face update evaluated in double
+__global__ void __launch_bounds__(kBlock) k_add_buoyancy(Dims d, Tiles t, const uint8_t* __restrict__ f8,
+                                                         const float* __restrict__ den, float* __restrict__ u,
+                                                         float* __restrict__ v, float* __restrict__ w, float fx,
+                                                         float fy, float fz) {
+	CellPos p = tile_cell(t);
+	if (p.i >= d.nx || p.j >= d.ny) return;
+	if (!interior(d, p.i, p.j, p.k)) return;
+	const long long ix = d.idx(p.i, p.j, p.k);
+	if (!(f8[ix] & kFluid)) return;
+	if (f8[ix - 1] & kFluid) u[ix] = (float)((double)u[ix] + (0.5 * (double)fx) * (double)(den[ix] + den[ix - 1]));
+	if (f8[ix - d.nx] & kFluid)
+		v[ix] = (float)((double)v[ix] + (0.5 * (double)fy) * (double)(den[ix] + den[ix - d.nx]));
+	if (d.is3d && (f8[ix - d.sz] & kFluid))
+		w[ix] = (float)((double)w[ix] + (0.5 * (double)fz) * (double)(den[ix] + den[ix - d.sz]));
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static inline Vel vel_of(Grid* g) { return Vel{g->comp(0), g->comp(1), g->comp(2)}; }
+
+void op_set_wall_bcs(Scene* s, Grid* flags, Grid* vel, Grid* obvel) {
+	Tiles t = tiles_for(s->d);
+	hipLaunchKernelGGL(k_set_wall_bcs, dim3((unsigned)t.count()), dim3(kBlock), 0, s->stream, s->d, t, flags->f8,
+	                   vel->comp(0), vel->comp(1), vel->comp(2), obvel ? obvel->comp(0) : nullptr,
+	                   obvel ? obvel->comp(1) : nullptr, obvel ? obvel->comp(2) : nullptr);
+}
+
+void op_add_buoyancy(Scene* s, Grid* flags, Grid* density, Grid* vel, const float f[3]) {
+	Tiles t = tiles_for(s->d);
+	hipLaunchKernelGGL(k_add_buoyancy, dim3((unsigned)t.count()), dim3(kBlock), 0, s->stream, s->d, t, flags->f8,
+	                   (const float*)density->real(), vel->comp(0), vel->comp(1), vel->comp(2), f[0], f[1], f[2]);
+}
+
+void op_advect(Scene* s, Grid* flags, Grid* vel, Grid* grid, int order, float strength, int clampMode) {
+	const Dims d = s->d;
+	const Tiles t = tiles_for(d);
+	const dim3 gb((unsigned)t.count()), tb(kBlock);
+	const float dt = s->dt;
+	const Vel V = vel_of(vel);
+	if (grid->isMAC()) {
+		float* fwd = s->getVec();
+		float* fw[3] = {fwd, fwd + s->cstride, fwd + 2 * s->cstride};
+		{
+			KtScope kt("advect_sl_mac", s->stream);
+			hipLaunchKernelGGL(k_sl_mac, gb, tb, 0, s->stream, V, d, t, dt, (const float*)grid->comp(0),
+			                   (const float*)grid->comp(1), (const float*)grid->comp(2), fw[0], fw[1], fw[2]);
+		}
+		float* result = fwd;
+		if (order == 2) {
+			float* ng = s->getVec();
+			float* nw[3] = {ng, ng + s->cstride, ng + 2 * s->cstride};
+			KtScope kt("advect_maccormack_mac", s->stream);
+			hipLaunchKernelGGL(k_maccormack_mac, gb, tb, 0, s->stream, V, d, t, dt, strength, clampMode, flags->f8,
+			                   (const float*)grid->comp(0), (const float*)grid->comp(1), (const float*)grid->comp(2),
+			                   (const float*)fw[0], (const float*)fw[1], (const float*)fw[2], nw[0], nw[1], nw[2]);
+			s->putVec(fwd);
+			result = ng;
+		}
+		// applyOutflowBC(flags, result, orig=grid, dt): timeStep = max(1.0, dt*4)
+		{
+			const double tsd = (double)dt * 4;
+			const float ts = (float)((1.0 < tsd) ? tsd : 1.0);
+			float* vdst = s->getVec();
+			float* r3[3] = {result, result + s->cstride, result + 2 * s->cstride};
+			float* d3[3] = {vdst, vdst + s->cstride, vdst + 2 * s->cstride};
+			hipLaunchKernelGGL(k_outflow_extrap, gb, tb, 0, s->stream, d, t, flags->f8, (const float*)r3[0],
+			                   (const float*)r3[1], (const float*)r3[2], (const float*)grid->comp(0),
+			                   (const float*)grid->comp(1), (const float*)grid->comp(2), ts, d3[0], d3[1], d3[2]);
+			hipLaunchKernelGGL(k_outflow_copy, gb, tb, 0, s->stream, d, t, flags->f8, (const float*)d3[0],
+			                   (const float*)d3[1], (const float*)d3[2], r3[0], r3[1], r3[2]);
+			s->putVec(vdst);
+		}
+		// orig.swap(result)
+		s->putVec(grid->real());
+		grid->data = result;
+		return;
+	}
+	const int nc = grid->isVec() ? 3 : 1;
+	float* fwd = nc == 3 ? s->getVec() : s->getReal();
+	float* fw[3] = {fwd, nc == 3 ? fwd + s->cstride : nullptr, nc == 3 ? fwd + 2 * s->cstride : nullptr};
+	const float* src[3] = {grid->comp(0), nc == 3 ? grid->comp(1) : nullptr, nc == 3 ? grid->comp(2) : nullptr};
+	if (nc == 1)
+		hipLaunchKernelGGL(k_sl_centered<1>, gb, tb, 0, s->stream, V, d, t, dt, src[0], src[1], src[2], fw[0], fw[1],
+		                   fw[2]);
+	else
+		hipLaunchKernelGGL(k_sl_centered<3>, gb, tb, 0, s->stream, V, d, t, dt, src[0], src[1], src[2], fw[0], fw[1],
+		                   fw[2]);
+	float* result = fwd;
+	if (order == 2) {
+		float* ng = nc == 3 ? s->getVec() : s->getReal();
+		float* nw[3] = {ng, nc == 3 ? ng + s->cstride : nullptr, nc == 3 ? ng + 2 * s->cstride : nullptr};
+		if (nc == 1)
+			hipLaunchKernelGGL(k_maccormack_centered<1>, gb, tb, 0, s->stream, V, d, t, dt, strength, clampMode,
+			                   flags->f8, src[0], src[1], src[2], (const float*)fw[0], (const float*)fw[1],
+			                   (const float*)fw[2], nw[0], nw[1], nw[2]);
+		else
+			hipLaunchKernelGGL(k_maccormack_centered<3>, gb, tb, 0, s->stream, V, d, t, dt, strength, clampMode,
+			                   flags->f8, src[0], src[1], src[2], (const float*)fw[0], (const float*)fw[1],
+			                   (const float*)fw[2], nw[0], nw[1], nw[2]);
+		if (nc == 3) s->putVec(fwd);
+		else s->putReal(fwd);
+		result = ng;
+	}
+	if (nc == 3) s->putVec(grid->real());
+	else s->putReal(grid->real());
+	grid->data = result;
+}
+
+}  // namespace mfx
+
+using namespace mfx;
+struct mfx_grid : public Grid {};
+static inline Grid* G(const mfx_grid* g) { return const_cast<Grid*>(static_cast<const Grid*>(g)); }
+
+#define API_BEGIN try {
+#define API_END                                 \
+	}                                           \
+	catch (const std::exception& e) {           \
+		::mfx::set_last_error(e.what());        \
+		return -1;                              \
+	}                                           \
+	catch (...) {                               \
+		::mfx::set_last_error("unknown error"); \
+		return -1;                              \
+	}                                           \
+	return 0;
+
+static void same_scene(const Grid* a, const Grid* b) {
+	MFX_CHECK(a->s == b->s, "grids belong to different solvers");
+}
+
+extern "C" {
+
+int mfx_advect_semi_lagrange(const mfx_grid* flags, const mfx_grid* vel, mfx_grid* grid, int order, float strength,
+                             int orderSpace, int clampMode, int orderTrace) {
+	API_BEGIN
+	MFX_CHECK(flags && vel && grid, "advectSemiLagrange: flags, vel and grid are required");
+	MFX_CHECK(order == 1 || order == 2, "AdvectSemiLagrange: Only order 1 (regular SL) and 2 (MacCormack) supported");
+	MFX_CHECK(orderSpace == 1, "advectSemiLagrange: orderSpace != 1 (cubic interpolation) is not supported on the GPU path");
+	MFX_CHECK(orderTrace == 1, "advectSemiLagrange: orderTrace != 1 (midpoint backtrace) is not supported on the GPU path");
+	Grid *f = G(flags), *v = G(vel), *g = G(grid);
+	MFX_CHECK(f->isFlags(), "advectSemiLagrange: flags must be a FlagGrid");
+	MFX_CHECK(v->isMAC(), "advectSemiLagrange: vel must be a MACGrid");
+	same_scene(f, v);
+	same_scene(f, g);
+	MFX_CHECK(g->isReal() || g->isVec(), "AdvectSemiLagrange: Grid Type is not supported (only Real, Vec3, MAC, Levelset)");
+	op_advect(f->s, f, v, g, order, strength, clampMode);
+	API_END
+}
+
+int mfx_set_wall_bcs(const mfx_grid* flags, mfx_grid* vel, const mfx_grid* obvel) {
+	API_BEGIN
+	MFX_CHECK(flags && vel, "setWallBcs: flags and vel are required");
+	Grid *f = G(flags), *v = G(vel);
+	MFX_CHECK(f->isFlags() && v->isMAC(), "setWallBcs: needs a FlagGrid and a MACGrid");
+	same_scene(f, v);
+	if (obvel) {
+		MFX_CHECK(G(obvel)->isVec(), "setWallBcs: obvel must be a MACGrid");
+		same_scene(f, G(obvel));
+	}
+	op_set_wall_bcs(f->s, f, v, obvel ? G(obvel) : nullptr);
+	API_END
+}
+
+int mfx_add_buoyancy(const mfx_grid* flags, const mfx_grid* density, mfx_grid* vel, const float gravity[3],
+                     float coefficient, int scale) {
+	API_BEGIN
+	MFX_CHECK(flags && density && vel && gravity, "addBuoyancy: flags, density, vel and gravity are required");
+	Grid *f = G(flags), *den = G(density), *v = G(vel);
+	MFX_CHECK(f->isFlags() && den->isReal() && v->isMAC(), "addBuoyancy: needs FlagGrid, RealGrid, MACGrid");
+	same_scene(f, v);
+	same_scene(f, den);
+	Scene* s = f->s;
+	// addBuoyancy (extforces.cpp:83-87): float gridScale = scale ? getDx() : 1;
+	// f = -gravity * dt / gridScale * coefficient (Vec3 float ops)
+	int mx = s->d.nx > s->d.ny ? s->d.nx : s->d.ny;
+	if (s->d.nz > mx) mx = s->d.nz;
+	const float dx = (float)(1.0 / mx);
+	volatile float gridScale = scale ? dx : 1.f;
+	float fv[3];
+	for (int c = 0; c < 3; c++) {
+		volatile float a = -gravity[c];
+		volatile float b = a * s->dt;
+		volatile float q = b / gridScale;
+		fv[c] = q * coefficient;
+	}
+	op_add_buoyancy(s, f, den, v, fv);
+	API_END
+}
+
+}  // extern "C"

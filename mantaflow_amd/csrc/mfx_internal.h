@@ -1,0 +1,213 @@
+// Internal declarations of libmfx: scene/grid objects, launch helpers and
+// the device-side cell-flag helpers shared by all kernel files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/mfx.h"
+
+namespace mfx {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+struct Error : public std::runtime_error {
+	explicit Error(const std::string& s) : std::runtime_error(s) {}
+};
+
+void set_last_error(const std::string& s);
+
+#define MFX_HIP(call)                                                                  \
+	do {                                                                               \
+		hipError_t _e = (call);                                                        \
+		if (_e != hipSuccess)                                                          \
+			throw ::mfx::Error(std::string("HIP error: ") + hipGetErrorString(_e) +   \
+			                   " at " + __FILE__ + ":" + std::to_string(__LINE__));   \
+	} while (0)
+
+#define MFX_CHECK(cond, msg)                      \
+	do {                                          \
+		if (!(cond)) throw ::mfx::Error(msg);     \
+	} while (0)
+
+// ---------------------------------------------------------------------------
+// cell types (grid.h:292-304); the device keeps a uint8 copy of bits 0..6
+// ---------------------------------------------------------------------------
+enum : int {
+	kFluid = 1, kObstacle = 2, kEmpty = 4, kInflow = 8, kOutflow = 16, kOpen = 32, kStick = 64
+};
+
+// grid type bits (grid.h:29)
+enum : int { kTypeReal = 1, kTypeInt = 2, kTypeVec3 = 4, kTypeMAC = 8, kTypeLevelset = 16, kTypeFlags = 32 };
+
+// ---------------------------------------------------------------------------
+// geometry passed by value to kernels
+// ---------------------------------------------------------------------------
+struct Dims {
+	int nx, ny, nz;
+	int is3d;
+	long long n;   // cells
+	long long sz;  // z stride: nx*ny in 3D, 0 in 2D (grid.cpp:55)
+	__host__ __device__ inline long long idx(int i, int j, int k) const {
+		return (long long)i + (long long)nx * j + sz * k;
+	}
+};
+
+// ---------------------------------------------------------------------------
+// scene / grid objects
+// ---------------------------------------------------------------------------
+struct Grid;
+struct MgHierarchy;
+
+struct Scene {
+	Dims d;
+	int dim;
+	float dt = 1.0f;
+	hipStream_t stream = nullptr;
+	int device = 0;
+	size_t cstride = 0;  // per-component stride of Vec3/MAC grids (floats)
+	// pooled temporaries (mirror of FluidSolver::GridStorage)
+	std::vector<float*> freeReal;
+	std::vector<float*> freeVec;
+	std::vector<void*> owned;
+	// small device workspace for reductions
+	double* dpart = nullptr;    // block partials (doubles)
+	size_t dpartCap = 0;
+	float* fpart = nullptr;     // block partials (floats, min/max)
+	size_t fpartCap = 0;
+	void* dscal = nullptr;      // device scalar block
+	void* hscal = nullptr;      // pinned host mirror
+	std::unique_ptr<MgHierarchy> mgStatic;
+	uint8_t* code = nullptr;    // per-cell Laplace matrix code of the current solve
+	std::vector<Grid*> grids;
+	~Scene();
+	float* getReal();
+	void putReal(float* p);
+	float* getVec();
+	void putVec(float* p);
+	void ensurePartials(size_t nd, size_t nf);
+};
+
+struct Grid {
+	Scene* s;
+	int type;
+	void* data = nullptr;           // float[n] / int32[n] / float[3*cstride] (SoA)
+	uint8_t* f8 = nullptr;          // flag grids: packed bits 0..6
+	bool external = false;
+	float* comp(int c) const { return (float*)data + (size_t)c * s->cstride; }
+	float* real() const { return (float*)data; }
+	int* ints() const { return (int*)data; }
+	bool isVec() const { return (type & kTypeVec3) != 0; }
+	bool isMAC() const { return (type & kTypeMAC) != 0; }
+	bool isFlags() const { return (type & kTypeFlags) != 0; }
+	bool isInt() const { return (type & kTypeInt) != 0; }
+	bool isReal() const { return (type & kTypeReal) != 0; }
+};
+
+// ---------------------------------------------------------------------------
+// launch helpers
+// ---------------------------------------------------------------------------
+constexpr int kTX = 64;  // threads along x per tile (one wavefront row)
+constexpr int kTY = 4;   // rows per tile
+constexpr int kBlock = kTX * kTY;
+
+struct Tiles {
+	int tx, ty, tz;  // tiles per axis (tz = nz)
+	long long count() const { return (long long)tx * ty * tz; }
+};
+inline Tiles tiles_for(const Dims& d) {
+	Tiles t;
+	t.tx = (d.nx + kTX - 1) / kTX;
+	t.ty = (d.ny + kTY - 1) / kTY;
+	t.tz = d.nz;
+	return t;
+}
+
+// Map the hardware workgroup id to a logical tile so that each of the 8 XCDs
+// (dispatched round-robin by workgroup id) processes one contiguous run of
+// tiles: neighbouring tiles then share halo lines in the same L2.
+__device__ inline long long xcd_tile(long long wg, long long nwg) {
+	const long long q = nwg >> 3, r = nwg & 7;
+	const long long x = wg & 7, slot = wg >> 3;
+	return x < r ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
+}
+
+// Decode the calling thread's cell (i,j,k) from a 1-D tile launch.
+struct CellPos {
+	int i, j, k;
+};
+__device__ inline CellPos tile_cell(const Tiles& t) {
+	long long tile = xcd_tile(blockIdx.x, gridDim.x);
+	int bx = (int)(tile % t.tx);
+	long long rest = tile / t.tx;
+	int by = (int)(rest % t.ty);
+	int k = (int)(rest / t.ty);
+	CellPos p;
+	p.i = bx * kTX + (threadIdx.x & (kTX - 1));
+	p.j = by * kTY + (threadIdx.x >> 6);
+	p.k = k;
+	return p;
+}
+
+// flat launch over n elements with a grid-stride loop
+inline unsigned flat_blocks(long long n, int per = 256, unsigned cap = 65536u) {
+	long long b = (n + per - 1) / per;
+	if (b < 1) b = 1;
+	return (unsigned)(b > cap ? cap : b);
+}
+
+// ---------------------------------------------------------------------------
+// reductions (block level, deterministic)
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ inline T wave_sum(T v) {
+	for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+	return v;
+}
+__device__ inline float wave_min(float v) {
+	for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+	return v;
+}
+__device__ inline float wave_max(float v) {
+	for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+	return v;
+}
+
+// Sum over a block of kBlock threads; result valid in thread 0.
+template <typename T>
+__device__ inline T block_sum(T v, T* sh) {
+	v = wave_sum(v);
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	if (lane == 0) sh[w] = v;
+	__syncthreads();
+	T r = 0;
+	if (threadIdx.x == 0) {
+		for (int q = 0; q < (int)(blockDim.x >> 6); q++) r += sh[q];
+	}
+	__syncthreads();
+	return r;
+}
+
+// ---------------------------------------------------------------------------
+// host-side API helpers (defined in mfx_grid.hip)
+// ---------------------------------------------------------------------------
+Grid* make_grid(Scene* s, int type);
+void free_grid(Grid* g);
+void flags_repack(Grid* flags);      // int32 -> uint8 mirror
+void sync(Scene* s);
+double reduce_sum_double(Scene* s, const float* a, int mode, long long n);
+
+// kernels / ops implemented in the various .hip files
+void op_set_wall_bcs(Scene* s, Grid* flags, Grid* vel, Grid* obvel);
+void op_add_buoyancy(Scene* s, Grid* flags, Grid* density, Grid* vel, const float f[3]);
+void op_advect(Scene* s, Grid* flags, Grid* vel, Grid* grid, int order, float strength, int clampMode);
+
+}  // namespace mfx

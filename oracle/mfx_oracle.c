@@ -1037,7 +1037,9 @@ int og_solve_pressure_system(og_dims d, og_real* rhs, og_real* pressure, const i
 		og_real dp = (og_real)grid_dot(d, tmp, search);
 		og_real alpha = 0.;
 		if (OG_FABS(dp) > 0.) alpha = sigma / (og_real)dp;
+		OMP_FOR
 		for (long i = 0; i < n; i++) pressure[i] += alpha * search[i];
+		OMP_FOR
 		for (long i = 0; i < n; i++) res[i] += (-alpha) * tmp[i];
 		if (pc == OG_PC_MIC) mic_apply(d, flags, tmp, res, P, Ai, Aj, Ak);
 		else if (mg) og_mg_vcycle(mg, res, tmp);
@@ -1058,6 +1060,7 @@ int og_solve_pressure_system(og_dims d, og_real* rhs, og_real* pressure, const i
 		}
 		og_real sigmaNew = (og_real)grid_dot(d, tmp, res);
 		og_real beta = sigmaNew / sigma;
+		OMP_FOR
 		for (long i = 0; i < n; i++) search[i] = tmp[i] + beta * search[i];
 		sigma = sigmaNew;
 		if (!(resNorm < 1e35)) {

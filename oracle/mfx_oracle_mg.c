@@ -22,6 +22,12 @@
 #include <stdlib.h>
 #include <string.h>
 
+#ifdef _OPENMP
+#define OMP_FOR _Pragma("omp parallel for num_threads(og_get_threads()) if(og_get_threads() > 1) schedule(static)")
+#else
+#define OMP_FOR
+#endif
+
 #define MAXL 32
 
 typedef struct { int x, y, z; } v3i;
@@ -281,6 +287,7 @@ static void gen_coarse_grid(og_mg* m, int l) {
 /* knGenCoarseGridOperator (multigrid.cpp:580-666) */
 static void gen_coarse_op(og_mg* m, int l) {
 	const int S = m->S, S0 = m->S0;
+	OMP_FOR
 	for (long idx = 0; idx < m->n[l]; idx++) {
 		if (m->type[l][idx] == vtInactive) continue;
 		og_real* A = m->A[l] + idx * S;
@@ -421,6 +428,7 @@ static void smooth_gs(og_mg* m, int l, int reversed) {
 	const og_real* A = m->A[l];
 	for (int c = 0; c < nc; c++) {
 		int color = reversed ? nc - 1 - c : c;
+		OMP_FOR
 		for (long idx = 0; idx < nblocks; idx++) {
 			v3i off = mk((int)(idx % bs.x), (int)((idx % ((long)bs.x * bs.y)) / bs.x), (int)(idx / ((long)bs.x * bs.y)));
 			for (int o = 0; o < ncol[color]; o++) {
@@ -462,6 +470,7 @@ static void calc_residual(og_mg* m, int l) {
 	const int S = m->S, S0 = m->S0;
 	const og_real* A = m->A[l];
 	const og_real* x = m->x[l];
+	OMP_FOR
 	for (long idx = 0; idx < m->n[l]; idx++) {
 		if (m->type[l][idx] == vtInactive) continue;
 		v3i V = vecIdx(m, idx, l);
@@ -568,6 +577,7 @@ static void solve_cg(og_mg* m, int l) {
 static void restrict_l(og_mg* m, int ld) {
 	const int ls = ld - 1;
 	const og_real* src = m->r[ls];
+	OMP_FOR
 	for (long idx = 0; idx < m->n[ld]; idx++) {
 		if (m->type[ld][idx] == vtInactive) continue;
 		v3i V = vecIdx(m, idx, ld);
@@ -591,6 +601,7 @@ static void restrict_l(og_mg* m, int ld) {
 static void interpolate_l(og_mg* m, int ld) {
 	const int ls = ld + 1;
 	const og_real* src = m->x[ls];
+	OMP_FOR
 	for (long idx = 0; idx < m->n[ld]; idx++) {
 		if (m->type[ld][idx] == vtInactive) continue;
 		v3i V = vecIdx(m, idx, ld);

@@ -1,0 +1,302 @@
+"""Parity of the HIP path (through libmfx's C ABI, via the manta module)
+against the CPU oracle on the same seeded inputs.
+
+Flag handling, shapes, advection, boundary conditions, buoyancy, RHS and
+velocity correction must match bit for bit (same float operation order,
+no FMA contraction on either side).  Pressure solves are compared within the
+CG tolerance, see the individual tests."""
+import numpy as np
+import pytest
+
+from oracle import EMPTY, FLUID, OBSTACLE, OUTFLOW, PC_MGSTATIC, PC_MIC, PC_NONE, STICK, TYPE_MAC, TYPE_REAL, TYPE_VEC3, TYPE_INT
+from tests.helpers import GpuScene, assert_bitexact, make_flags, rand_vel, smooth_vel
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(24, 20, 1), (17, 13, 11), (32, 32, 32)]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("bw,wall,open_,inflow,outflow", [
+    (0, "xXyYzZ", "      ", "      ", "      "),
+    (1, "xXyYzZ", "      ", "      ", "      "),
+    (1, "xXyzZ", "Y", "      ", "      "),
+    (2, "xyz", "X", "y", "Z"),
+])
+def test_init_domain_fill(manta, oracle, shape, bw, wall, open_, inflow, outflow):
+    ref = oracle.init_domain(shape, bw, wall, open_, inflow, outflow)
+    sc = GpuScene(manta, shape)
+    f = sc.s.create(manta.FlagGrid)
+    f.initDomain(boundaryWidth=bw, wall=wall, open=open_, inflow=inflow, outflow=outflow)
+    assert_bitexact(f.toNumpy(), ref, "initDomain")
+    oracle.fill_grid(shape, ref)
+    f.fillGrid()
+    assert_bitexact(f.toNumpy(), ref, "fillGrid")
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("ob", ["Y", "yY", "xXz", "xXyYzZ"])
+def test_open_bound_reset_outflow(manta, oracle, rng, shape, ob):
+    ref = make_flags(oracle, shape, bw=1)
+    sc = GpuScene(manta, shape)
+    f = sc.flags(ref)
+    oracle.set_open_bound(shape, ref, 1, ob)
+    manta.setOpenBound(f, 1, ob, manta.FlagOutflow | manta.FlagEmpty)
+    assert_bitexact(f.toNumpy(), ref, "setOpenBound")
+    n = int(np.prod(shape))
+    dens = rng.uniform(0, 1, n).astype(np.float32)
+    # make some outflow cells fluid first, like a step after fillGrid
+    ref[(ref & OUTFLOW) != 0] |= FLUID
+    f.fromNumpy(ref)
+    d = sc.real(dens)
+    oracle.reset_outflow(shape, ref, None, dens)
+    manta.resetOutflow(flags=f, real=d)
+    assert_bitexact(f.toNumpy(), ref, "resetOutflow flags")
+    assert_bitexact(d.toNumpy(), dens, "resetOutflow real")
+
+
+@pytest.mark.parametrize("shape", [(24, 20, 1), (17, 13, 11)])
+def test_shapes(manta, oracle, shape):
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    sc = GpuScene(manta, shape)
+    is3 = nz > 1
+    cases = [
+        (1, [nx * 0.2, ny * 0.3, 0.0], [nx * 0.6, ny * 0.55, nz], 0.0, manta.Box(sc.s, p0=manta.vec3(nx * 0.2, ny * 0.3, 0.0), p1=manta.vec3(nx * 0.6, ny * 0.55, nz))),
+        (2, [nx * 0.5, ny * 0.4, nz * 0.5], [1, 1.5, 1], nx * 0.21, manta.Sphere(sc.s, center=manta.vec3(nx * 0.5, ny * 0.4, nz * 0.5), radius=nx * 0.21, scale=manta.vec3(1, 1.5, 1))),
+        (3, [nx * 0.5, ny * 0.1, nz * 0.5], [0, ny * 0.02, 0], nx * 0.14, manta.Cylinder(sc.s, center=manta.vec3(nx * 0.5, ny * 0.1, nz * 0.5), radius=nx * 0.14, z=manta.vec3(0, ny * 0.02, 0))),
+    ]
+    for kind, a, b, r, shp in cases:
+        g = sc.real()
+        ref = np.zeros(n, np.float32)
+        oracle.apply_shape(shape, kind, a, b, r, ref, TYPE_REAL, 0.75)
+        shp.applyToGrid(grid=g, value=0.75)
+        assert_bitexact(g.toNumpy(), ref, "shape %d real" % kind)
+        mg = sc.mac()
+        refm = np.zeros((n, 3), np.float32)
+        oracle.apply_shape(shape, kind, a, b, r, refm, TYPE_MAC, [0.1, 0.2, 0.3])
+        shp.applyToGrid(grid=mg, value=manta.vec3(0.1, 0.2, 0.3))
+        assert_bitexact(mg.toNumpy(), refm, "shape %d mac" % kind)
+
+
+def _adv_inputs(oracle, rng, shape, obstacles=3, open_bound=""):
+    n = int(np.prod(shape))
+    flags = make_flags(oracle, shape, bw=1, open_bound=open_bound, obstacles=obstacles, rng=rng)
+    if open_bound:
+        oracle.reset_outflow(shape, flags)
+    vel = rand_vel(rng, n, 1.6)
+    if shape[2] == 1:
+        vel[:, 2] = 0
+    return flags, vel
+
+
+@pytest.mark.parametrize("shape", SHAPES[:2] + [(20, 24, 18)])
+@pytest.mark.parametrize("order,clamp", [(1, 2), (2, 1), (2, 2)])
+@pytest.mark.parametrize("open_bound", ["", "Y"])
+def test_advect_real(manta, oracle, rng, shape, order, clamp, open_bound):
+    n = int(np.prod(shape))
+    flags, vel = _adv_inputs(oracle, rng, shape, open_bound=open_bound)
+    dens = rng.uniform(0, 1, n).astype(np.float32)
+    dt = 1.2
+    sc = GpuScene(manta, shape, dt)
+    f, v, d = sc.flags(flags), sc.mac(vel), sc.real(dens)
+    ref = dens.copy()
+    oracle.advect(shape, np.float32(dt), flags, vel, ref, TYPE_REAL, order, 1.0, clamp)
+    manta.advectSemiLagrange(flags=f, vel=v, grid=d, order=order, clampMode=clamp)
+    assert_bitexact(d.toNumpy(), ref, "advect real")
+
+
+@pytest.mark.parametrize("shape", SHAPES[:2] + [(20, 24, 18)])
+@pytest.mark.parametrize("order,clamp", [(1, 2), (2, 1), (2, 2)])
+@pytest.mark.parametrize("open_bound", ["", "Y", "xXyY"])
+def test_advect_mac_self(manta, oracle, rng, shape, order, clamp, open_bound):
+    flags, vel = _adv_inputs(oracle, rng, shape, open_bound=open_bound)
+    dt = 1.2
+    sc = GpuScene(manta, shape, dt)
+    f, v = sc.flags(flags), sc.mac(vel)
+    ref = vel.copy()
+    oracle.advect(shape, np.float32(dt), flags, ref, ref, TYPE_MAC, order, 1.0, clamp)
+    manta.advectSemiLagrange(flags=f, vel=v, grid=v, order=order, strength=1.0, clampMode=clamp)
+    assert_bitexact(v.toNumpy(), ref, "advect mac")
+
+
+@pytest.mark.parametrize("shape", SHAPES[:2])
+@pytest.mark.parametrize("order,clamp", [(1, 2), (2, 1), (2, 2)])
+def test_advect_vec3(manta, oracle, rng, shape, order, clamp):
+    n = int(np.prod(shape))
+    flags, vel = _adv_inputs(oracle, rng, shape)
+    g = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+    dt = 0.9
+    sc = GpuScene(manta, shape, dt)
+    f, v, gg = sc.flags(flags), sc.mac(vel), sc.vec(g)
+    ref = g.copy()
+    oracle.advect(shape, np.float32(dt), flags, vel, ref, TYPE_VEC3, order, 0.7, clamp)
+    manta.advectSemiLagrange(flags=f, vel=v, grid=gg, order=order, strength=0.7, clampMode=clamp)
+    assert_bitexact(gg.toNumpy(), ref, "advect vec3")
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("stick", [False, True])
+def test_wall_bcs_buoyancy(manta, oracle, rng, shape, stick):
+    n = int(np.prod(shape))
+    flags = make_flags(oracle, shape, bw=1, obstacles=4, rng=rng, stick=stick)
+    vel = rand_vel(rng, n)
+    obv = rand_vel(rng, n, 0.3)
+    dens = rng.uniform(0, 1, n).astype(np.float32)
+    sc = GpuScene(manta, shape, 0.8)
+    f, v, ob, d = sc.flags(flags), sc.mac(vel), sc.mac(obv), sc.real(dens)
+    ref = vel.copy()
+    oracle.set_wall_bcs(shape, flags, ref)
+    manta.setWallBcs(flags=f, vel=v)
+    assert_bitexact(v.toNumpy(), ref, "setWallBcs")
+    oracle.set_wall_bcs(shape, flags, ref, obv)
+    manta.setWallBcs(flags=f, vel=v, obvel=ob)
+    assert_bitexact(v.toNumpy(), ref, "setWallBcs obvel")
+    oracle.add_buoyancy(shape, np.float32(0.8), flags, dens, ref, (0.0, -4e-3, 1e-3), 1.3, True)
+    manta.addBuoyancy(flags=f, density=d, vel=v, gravity=manta.vec3(0, -4e-3, 1e-3), coefficient=1.3)
+    assert_bitexact(v.toNumpy(), ref, "addBuoyancy")
+    oracle.add_buoyancy(shape, np.float32(0.8), flags, dens, ref, (0.1, 0.2, -0.3), 1.0, False)
+    manta.addBuoyancy(flags=f, density=d, vel=v, gravity=manta.vec3(0.1, 0.2, -0.3), scale=False)
+    assert_bitexact(v.toNumpy(), ref, "addBuoyancy noscale")
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_rhs_and_correct(manta, oracle, rng, shape):
+    n = int(np.prod(shape))
+    flags = make_flags(oracle, shape, bw=1, open_bound="Y", obstacles=3, rng=rng)
+    oracle.reset_outflow(shape, flags)
+    vel = rand_vel(rng, n)
+    p = rng.standard_normal(n).astype(np.float32)
+    sc = GpuScene(manta, shape)
+    f, v, pr, rhs = sc.flags(flags), sc.mac(vel), sc.real(p), sc.real()
+    ref_rhs, cnt, s = oracle.make_rhs(shape, flags, vel)
+    manta.computePressureRhs(rhs, v, pr, f)
+    assert_bitexact(rhs.toNumpy(), ref_rhs, "MakeRhs")
+    ref = vel.copy()
+    oracle.correct_velocity(shape, flags, ref, p)
+    manta.correctVelocity(vel=v, pressure=pr, flags=f)
+    assert_bitexact(v.toNumpy(), ref, "correctVelocity")
+
+
+def _next_to_outflow(flags, shape):
+    nx, ny, nz = shape
+    out = (flags.reshape(nz, ny, nx) & OUTFLOW) != 0
+    m = np.zeros_like(out)
+    m[:, :, :-1] |= out[:, :, 1:]
+    m[:, :-1, :] |= out[:, 1:, :]
+    if nz > 1:
+        m[:-1, :, :] |= out[1:, :, :]
+    return m.reshape(-1)
+
+
+def _solve_case(oracle, rng, shape, open_bound, obstacles):
+    n = int(np.prod(shape))
+    flags = make_flags(oracle, shape, bw=1 if open_bound else 0, open_bound=open_bound,
+                       obstacles=obstacles, rng=rng)
+    if open_bound:
+        oracle.reset_outflow(shape, flags)
+    vel = smooth_vel(rng, shape, 2, 1.0)
+    oracle.set_wall_bcs(shape, flags, vel)
+    return flags, vel
+
+
+@pytest.mark.parametrize("shape", [(48, 40, 1), (24, 20, 18), (33, 31, 29)])
+@pytest.mark.parametrize("open_bound", ["", "Y"])
+def test_solve_plain_cg_matches_oracle(manta, oracle, rng, shape, open_bound):
+    """PcNone: same operations as GridCg; only the double dot-product
+    summation order differs, so iterates agree to float rounding."""
+    flags, vel = _solve_case(oracle, rng, shape, open_bound, 2)
+    n = int(np.prod(shape))
+    sc = GpuScene(manta, shape)
+    f, v, pr = sc.flags(flags), sc.mac(vel), sc.real()
+    ref_v, ref_p = vel.copy(), np.zeros(n, np.float32)
+    st = oracle.solve_pressure(shape, ref_v, ref_p, flags, cgAccuracy=1e-4, preconditioner=PC_NONE,
+                               cgMaxIterFac=20)
+    manta.solvePressure(flags=f, vel=v, pressure=pr, cgAccuracy=1e-4, preconditioner=manta.PcNone,
+                        cgMaxIterFac=20)
+    gst = sc.s._solve_stats
+    assert abs(gst["iterations"] - st["iterations"]) <= 1
+    gp = pr.toNumpy()
+    scale = max(np.abs(ref_p).max(), 1e-6)
+    assert np.abs(gp - ref_p).max() <= 1e-4 * scale
+    assert np.abs(v.toNumpy() - ref_v).max() <= 1e-4
+
+
+@pytest.mark.parametrize("shape", [(48, 40, 1), (24, 20, 18), (40, 36, 34)])
+@pytest.mark.parametrize("pc", ["mgstatic", "mic"])
+@pytest.mark.parametrize("open_bound", ["", "Y"])
+def test_solve_mg_within_tolerance(manta, oracle, rng, shape, pc, open_bound):
+    """Multigrid-preconditioned CG on the GPU against the oracle's CG run to a
+    tight tolerance: the GPU result must meet the stopping criterion
+    ||r||_inf < cgAccuracy measured independently by the oracle's matrix."""
+    flags, vel = _solve_case(oracle, rng, shape, open_bound, 3)
+    n = int(np.prod(shape))
+    acc = 1e-4
+    sc = GpuScene(manta, shape)
+    f, v, pr = sc.flags(flags), sc.mac(vel), sc.real()
+    pcv = manta.PcMGStatic if pc == "mgstatic" else manta.PcMIC
+    manta.solvePressure(flags=f, vel=v, pressure=pr, cgAccuracy=acc, preconditioner=pcv, cgMaxIterFac=20)
+    gst = sc.s._solve_stats
+    assert gst["converged"], gst
+    # residual of the GPU pressure with the oracle's matrix
+    rhs, _, _ = oracle.make_rhs(shape, flags, vel)
+    A = oracle.make_laplace(shape, flags)
+    gp = pr.toNumpy()
+    r = rhs - oracle.apply_matrix(shape, flags, gp, A)
+    assert np.abs(r).max() < 2 * acc
+    # divergence after correction equals -r, except in fluid cells whose upper
+    # face borders an outflow cell (correctVelocity leaves outflow cells alone)
+    rhs2, _, _ = oracle.make_rhs(shape, flags, v.toNumpy())
+    keep = ~_next_to_outflow(flags, shape)
+    assert np.abs(rhs2[keep]).max() < 2 * acc
+    # velocity against a tight oracle solve
+    ref_v, ref_p = vel.copy(), np.zeros(n, np.float32)
+    oracle.solve_pressure(shape, ref_v, ref_p, flags, cgAccuracy=1e-6, preconditioner=PC_NONE, cgMaxIterFac=50)
+    assert np.abs(v.toNumpy() - ref_v).max() < 50 * acc
+
+
+def test_solve_errors(manta, oracle, rng):
+    shape = (16, 16, 16)
+    flags, vel = _solve_case(oracle, rng, shape, "", 0)
+    sc = GpuScene(manta, shape)
+    f, v, pr = sc.flags(flags), sc.mac(vel), sc.real()
+    with pytest.raises(RuntimeError, match="Argument 'foo' unknown"):
+        manta.solvePressure(flags=f, vel=v, pressure=pr, foo=1)
+    bad = vel.copy()
+    bad[8 + 16 * 8 + 256 * 8, 0] = np.nan  # an interior fluid face
+    v.fromNumpy(bad)
+    with pytest.raises(RuntimeError, match="The CG solver diverged"):
+        manta.solvePressure(flags=f, vel=v, pressure=pr, preconditioner=manta.PcNone)
+
+
+@pytest.mark.parametrize("steps", [12])
+def test_plume2d_steps(manta, oracle, steps):
+    """Config-1 style 2D plume (64^2, first-order SL, plain CG at 1e-3) for a
+    few steps, GPU vs oracle after every step."""
+    shape = (64, 64, 1)
+    n = 64 * 64
+    flags = oracle.init_domain(shape)
+    oracle.fill_grid(shape, flags)
+    vel = np.zeros((n, 3), np.float32)
+    dens = np.zeros(n, np.float32)
+    p = np.zeros(n, np.float32)
+    sc = GpuScene(manta, shape)
+    f, v, d, pr = sc.flags(flags), sc.mac(vel), sc.real(dens), sc.real()
+    src = manta.Box(sc.s, p0=manta.vec3(25.6, 3.2, 0), p1=manta.vec3(38.4, 6.4, 1))
+    for t in range(steps):
+        oracle.apply_shape(shape, 1, [25.6, 3.2, 0], [38.4, 6.4, 1], 0, dens, TYPE_REAL, 1.0)
+        oracle.advect(shape, np.float32(1.0), flags, vel, dens, TYPE_REAL, 1)
+        oracle.advect(shape, np.float32(1.0), flags, vel, vel, TYPE_MAC, 1)
+        oracle.set_wall_bcs(shape, flags, vel)
+        oracle.add_buoyancy(shape, np.float32(1.0), flags, dens, vel, (0, -4e-3, 0))
+        oracle.solve_pressure(shape, vel, p, flags, cgAccuracy=1e-3, preconditioner=PC_NONE)
+        src.applyToGrid(grid=d, value=1.0)
+        manta.advectSemiLagrange(flags=f, vel=v, grid=d, order=1)
+        manta.advectSemiLagrange(flags=f, vel=v, grid=v, order=1)
+        manta.setWallBcs(flags=f, vel=v)
+        manta.addBuoyancy(density=d, vel=v, gravity=manta.vec3(0, -4e-3, 0), flags=f)
+        manta.solvePressure(flags=f, vel=v, pressure=pr, cgAccuracy=1e-3, preconditioner=manta.PcNone)
+        sc.s.step()
+        gd, gv = d.toNumpy(), v.toNumpy()
+        assert np.abs(gd - dens).max() <= 1e-5 * max(1.0, np.abs(dens).max()), t
+        assert np.abs(gv - vel).max() <= 1e-5 * max(1.0, np.abs(vel).max()), t
