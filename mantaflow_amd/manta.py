@@ -26,6 +26,8 @@ import numpy as np
 from . import _abi
 from ._abi import check, lib
 
+lib()  # the GPU path is the only path: fail at import if libmfx.so is missing
+
 # ---------------------------------------------------------------------------
 # constants (source/python/defines.py:15-50, pwrapper/registry.cpp:398-420)
 # ---------------------------------------------------------------------------

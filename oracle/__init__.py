@@ -74,6 +74,7 @@ class Oracle:
             "og_apply_matrix": (None, [D, P, P, P, P, P, P, P]),
             "og_find_fix_cell": (I, [D, P]),
             "og_correct_velocity": (None, [D, P, P, P]),
+            "og_mic_precondition": (None, [D, P, P, P, P, P, P, P]),
             "og_solve_pressure": (I, [D, P, P, P, P, P, P]),
             "og_solve_pressure_system": (I, [D, P, P, P, P, P]),
             "og_pressure_opts_default": (None, [P]),
@@ -85,6 +86,7 @@ class Oracle:
             "og_mg_num_levels": (I, [P]),
             "og_mg_level_size": (None, [P, I, C.POINTER(I), C.POINTER(I), C.POINTER(I)]),
             "og_mg_level_types": (I, [P, I, P]),
+            "og_mg_level_matrix": (I, [P, I, P]),
             "og_apply_shape": (None, [D, I, P, P, R, P, I, P, P]),
         }
         for k, (res, args) in sig.items():
@@ -176,6 +178,12 @@ class Oracle:
                                  self._arr(src), *[self._arr(a) for a in A])
         return dst
 
+    def mic_precondition(self, shape, flags, A, r):
+        z = np.zeros_like(r)
+        self.lib.og_mic_precondition(self.dims(shape), self._arr(flags, np.int32),
+                                     *[self._arr(a) for a in A], self._arr(r), self._arr(z))
+        return z
+
     def find_fix_cell(self, shape, flags):
         return self.lib.og_find_fix_cell(self.dims(shape), self._arr(flags, np.int32))
 
@@ -213,6 +221,33 @@ class Oracle:
     # -- multigrid -----------------------------------------------------------
     def mg_create(self, shape):
         return self.lib.og_mg_create(self.dims(shape))
+
+    def mg_levels(self, mg):
+        """[(size, types int8[n], stencil storage [n, S])] for every level."""
+        out = []
+        for l in range(self.lib.og_mg_num_levels(mg)):
+            a, b, c = C.c_int(), C.c_int(), C.c_int()
+            self.lib.og_mg_level_size(mg, l, C.byref(a), C.byref(b), C.byref(c))
+            n = a.value * b.value * c.value
+            t = np.zeros(n, np.int8)
+            self.lib.og_mg_level_types(mg, l, t.ctypes.data_as(C.c_void_p))
+            m = np.zeros(n * 14, self.dtype)
+            ss = self.lib.og_mg_level_matrix(mg, l, m.ctypes.data_as(C.c_void_p))
+            out.append(((a.value, b.value, c.value), t, m[:n * ss].reshape(n, ss)))
+        return out
+
+    def mg_set_a(self, mg, A):
+        self.lib.og_mg_set_a(mg, *[self._arr(a) for a in A])
+
+    def mg_vcycle(self, mg, rhs, acc=None):
+        if acc is not None:
+            self.lib.og_mg_set_coarsest_accuracy(mg, acc)
+        z = np.zeros_like(rhs)
+        self.lib.og_mg_vcycle(mg, self._arr(rhs), self._arr(z))
+        return z
+
+    def mg_destroy(self, mg):
+        self.lib.og_mg_destroy(mg)
 
     # -- shapes ---------------------------------------------------------------
     def apply_shape(self, shape, kind, a, b, r, grid, gridType, value, respectFlags=None):

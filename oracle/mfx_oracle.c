@@ -949,6 +949,16 @@ static void mic_apply(og_dims d, const int* flags, og_real* dst, const og_real* 
 			}
 }
 
+/* z = M^-1 r for the mIC preconditioner built from A (init + apply) */
+void og_mic_precondition(og_dims d, const int* flags, const og_real* A0, const og_real* Ai,
+                         const og_real* Aj, const og_real* Ak, const og_real* r, og_real* z) {
+	long n = NCELLS(d);
+	og_real* P = (og_real*)calloc((size_t)n, sizeof(og_real));
+	mic_init(d, flags, P, A0, Ai, Aj, Ak);
+	mic_apply(d, flags, z, r, P, Ai, Aj, Ak);
+	free(P);
+}
+
 /* ||b - A x||_2 / ||b||_2 in FP64 (extension used for the relative criterion
  * and for reporting); A applied as in ApplyMatrix with double accumulation */
 static double rel_residual(og_dims d, const int* flags, const og_real* b, const og_real* x,

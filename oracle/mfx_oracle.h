@@ -127,6 +127,10 @@ int og_find_fix_cell(og_dims d, const int* flags);
 void og_fix_pressure(og_dims d, int fixPidx, og_real value, og_real* rhs, og_real* A0,
                      og_real* Ai, og_real* Aj, og_real* Ak);
 void og_correct_velocity(og_dims d, const int* flags, og_real* vel, const og_real* pressure);
+/* z = M^-1 r with the modified incomplete Cholesky preconditioner of A
+ * (conjugategrad.cpp:66-97,135-159); z must be zero-initialised */
+void og_mic_precondition(og_dims d, const int* flags, const og_real* A0, const og_real* Ai,
+                         const og_real* Aj, const og_real* Ak, const og_real* r, og_real* z);
 /* full solvePressure: rhs -> solve -> correct.  rhsOut (optional) gets the rhs.
  * Returns 0, or 1 if the reference would raise the divergence error. */
 int og_solve_pressure(og_dims d, og_real* vel, og_real* pressure, const int* flags,
@@ -149,6 +153,9 @@ int og_mg_num_levels(const og_mg* mg);
 void og_mg_level_size(const og_mg* mg, int l, int* sx, int* sy, int* sz);
 /* vertex types of a level (0 inactive, 1 active, 2 active trivial); returns count */
 int og_mg_level_types(const og_mg* mg, int l, int8_t* out);
+/* copies level l's stencil storage (n*4 / n*3 on level 0, n*14 / n*5 on
+ * coarser levels, row-major per vertex); returns the entries per vertex */
+int og_mg_level_matrix(const og_mg* mg, int l, og_real* out);
 
 /* ---- shapes (shapes.cpp:41-84,151-154,240,324-329) ---- */
 /* shape: 1 box (a=p0, b=p1), 2 sphere (a=center, b=scale, r=radius),

@@ -186,6 +186,12 @@ int og_mg_level_types(const og_mg* m, int l, int8_t* out) {
 	return cnt;
 }
 void og_mg_set_coarsest_accuracy(og_mg* m, og_real acc) { m->coarsestAcc = acc; }
+/* stencil storage of level l: n*S0 (level 0) or n*S (coarse) values */
+int og_mg_level_matrix(const og_mg* m, int l, og_real* out) {
+	const int ss = l == 0 ? m->S0 : m->S;
+	memcpy(out, m->A[l], sizeof(og_real) * (size_t)(m->n[l] * ss));
+	return ss;
+}
 
 /* ---- NKMinHeap (multigrid.cpp:59-203) ---- */
 typedef struct { int key, prev, next; } hent;

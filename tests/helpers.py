@@ -15,10 +15,10 @@ def make_flags(o, shape, bw=0, open_bound="", obstacles=0, rng=None, stick=False
     nx, ny, nz = shape
     g = f.reshape(nz, ny, nx)
     for _ in range(obstacles):
-        x0, y0 = rng.integers(2, nx - 4), rng.integers(2, ny - 4)
+        x0, y0 = rng.integers(2, max(3, nx - 4)), rng.integers(2, max(3, ny - 4))
         sx, sy = rng.integers(1, 4), rng.integers(1, 4)
         if nz > 1:
-            z0, sz = rng.integers(2, nz - 4), rng.integers(1, 4)
+            z0, sz = rng.integers(2, max(3, nz - 4)), rng.integers(1, 4)
         else:
             z0, sz = 0, 1
         g[z0:z0 + sz, y0:y0 + sy, x0:x0 + sx] = OBSTACLE | (64 if stick else 0)

@@ -1,0 +1,535 @@
+"""Pins the CPU oracle (the C restatement of the reference's plugins) against
+facts that do not depend on it: hand-derived flag patterns, closed forms,
+invariants, independent numpy/scipy formulations and brute force on small
+grids.  The reference ships no golden vectors (tools/testdata is empty) and
+cannot be built from a few sources (its plugins need the `prep` code
+generator), so these checks are the oracle's anchor.  CPU only."""
+import numpy as np
+import pytest
+from scipy import ndimage
+
+from oracle import (EMPTY, FLUID, OBSTACLE, OPEN, OUTFLOW, PC_MGSTATIC, PC_MIC, PC_NONE, TYPE_INT,
+                    TYPE_MAC, TYPE_REAL, TYPE_VEC3)
+from tests.helpers import make_flags, smooth_vel
+
+
+def grid2(rows):
+    """rows listed top (last j) first, as drawn; returns x-fastest flat int32"""
+    return np.array(rows[::-1], np.int32).reshape(-1)
+
+
+# ---------------------------------------------------------------------------
+# flags (grid.cpp:732-861, extforces.cpp:105-160) -- hand-derived patterns
+# ---------------------------------------------------------------------------
+def test_init_domain_default_2d(oracle):
+    f = oracle.init_domain((5, 4, 1))
+    exp = grid2([[2, 2, 2, 2, 2],
+                 [2, 4, 4, 4, 2],
+                 [2, 4, 4, 4, 2],
+                 [2, 2, 2, 2, 2]])
+    assert np.array_equal(f, exp)
+    oracle.fill_grid((5, 4, 1), f)
+    assert np.array_equal(f, np.where(exp == 4, 1, exp))
+
+
+def test_init_domain_open_top_bw1(oracle):
+    shape = (6, 5, 1)
+    f = oracle.init_domain(shape, boundaryWidth=1, wall="xXy", open="Y")
+    exp = grid2([[32] * 6,
+                 [32] * 6,
+                 [2, 2, 4, 4, 2, 2],
+                 [2] * 6,
+                 [2] * 6])
+    assert np.array_equal(f, exp)
+    oracle.fill_grid(shape, f)
+    assert np.array_equal(f, np.where(exp == 4, 1, exp))  # open cells keep their type
+
+
+def test_init_domain_3d_sides(oracle):
+    shape = (4, 5, 6)
+    f = oracle.init_domain(shape, 0, "xXyY", "z", "", "Z").reshape(6, 5, 4)
+    assert np.all(f[0, :, :] == OPEN)        # z low open (assigned after x/y)
+    assert np.all(f[5, :, :] == OUTFLOW)     # z high outflow
+    assert np.all(f[1:5, 0, :] == OBSTACLE) and np.all(f[1:5, :, 0] == OBSTACLE)
+    assert np.all(f[1:5, 1:4, 1:3] == EMPTY)
+
+
+def test_set_open_bound_2d_corners(oracle):
+    shape = (6, 6, 1)
+    f = oracle.init_domain(shape, boundaryWidth=1)
+    oracle.fill_grid(shape, f)
+    oracle.set_open_bound(shape, f, 1, "Y")
+    exp = grid2([[2, 2, 20, 20, 2, 2],
+                 [2, 2, 20, 20, 2, 2],
+                 [2, 2, 1, 1, 2, 2],
+                 [2, 2, 1, 1, 2, 2],
+                 [2] * 6,
+                 [2] * 6])
+    assert np.array_equal(f, exp)
+    oracle.reset_outflow(shape, f)
+    assert np.array_equal(f, exp)
+
+
+def test_set_open_bound_3d_and_reset(oracle, rng):
+    shape = (5, 5, 5)
+    f = oracle.init_domain(shape)
+    oracle.fill_grid(shape, f)
+    oracle.set_open_bound(shape, f, 0, "Y")
+    g = f.reshape(5, 5, 5)
+    exp = np.full((5, 5, 5), OBSTACLE, np.int32)
+    exp[1:4, 1:4, 1:4] = FLUID
+    exp[1:4, 4, 1:4] = OUTFLOW | EMPTY       # top face without its edges
+    assert np.array_equal(g, exp)
+    g[1:4, 4, 1:4] |= FLUID                  # fluid flag set by some other plugin
+    dens = rng.uniform(0, 1, 125).astype(np.float32)
+    keep = dens.copy()
+    oracle.reset_outflow(shape, f, None, dens)
+    assert np.array_equal(g, exp)
+    out = (exp == (OUTFLOW | EMPTY)).reshape(-1)
+    assert np.all(dens[out] == 0) and np.array_equal(dens[~out], keep[~out])
+
+
+# ---------------------------------------------------------------------------
+# interpolation (interpol.h:50-90)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("shape", [(9, 7, 1), (9, 7, 6)])
+def test_interpolation_matches_scipy_linear_nearest(oracle, rng, shape):
+    nx, ny, nz = shape
+    data = rng.uniform(-1, 1, nx * ny * nz).astype(np.float32)
+    arr = data.reshape(nz, ny, nx).astype(np.float64)
+    for _ in range(300):
+        p = rng.uniform(-2, [nx + 2, ny + 2, nz + 2])
+        if nz == 1:
+            p[2] = 0.5
+        v = oracle.interpol(shape, data, p)
+        coords = [[p[2] - 0.5], [p[1] - 0.5], [p[0] - 0.5]]
+        if nz == 1:
+            ref = ndimage.map_coordinates(arr[0], coords[1:], order=1, mode="nearest")[0]
+        else:
+            ref = ndimage.map_coordinates(arr, coords, order=1, mode="nearest")[0]
+        assert abs(v - ref) < 2e-6, (p, v, ref)
+
+
+def test_interpolation_reproduces_linear_fields(oracle, rng):
+    shape = (10, 9, 8)
+    nx, ny, nz = shape
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    a = (0.3 + 0.1 * (i + 0.5) - 0.05 * (j + 0.5) + 0.02 * (k + 0.5)).astype(np.float32).reshape(-1)
+    for _ in range(200):
+        p = rng.uniform([0.5, 0.5, 0.5], [nx - 0.5, ny - 0.5, nz - 0.5])
+        v = oracle.interpol(shape, a, p)
+        assert abs(v - (0.3 + 0.1 * p[0] - 0.05 * p[1] + 0.02 * p[2])) < 2e-6
+
+
+# ---------------------------------------------------------------------------
+# semi-Lagrangian / MacCormack (advection.cpp)
+# ---------------------------------------------------------------------------
+def _interior(shape):
+    nx, ny, nz = shape
+    m = np.zeros((nz, ny, nx), bool)
+    if nz > 1:
+        m[1:-1, 1:-1, 1:-1] = True
+    else:
+        m[0, 1:-1, 1:-1] = True
+    return m.reshape(-1)
+
+
+@pytest.mark.parametrize("shape", [(12, 10, 1), (12, 10, 9)])
+def test_sl_zero_velocity_is_identity(oracle, rng, shape):
+    n = int(np.prod(shape))
+    flags = make_flags(oracle, shape, bw=1)
+    src = rng.uniform(0, 1, n).astype(np.float32)
+    vel = np.zeros((n, 3), np.float32)
+    d = src.copy()
+    oracle.advect(shape, np.float32(1.0), flags, vel, d, TYPE_REAL, 1)
+    inner = _interior(shape)
+    assert np.array_equal(d[inner], src[inner])
+    assert np.all(d[~inner] == 0)  # zeroed outer layer (bnd=1 into a fresh temp)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_uniform_integer_shift_is_exact_translation(oracle, rng, order):
+    shape = (14, 8, 7)
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    flags = oracle.init_domain(shape)
+    oracle.fill_grid(shape, flags)
+    src = rng.uniform(0, 1, n).astype(np.float32)
+    vel = np.zeros((n, 3), np.float32)
+    vel[:, 0] = 1.0
+    d = src.copy()
+    oracle.advect(shape, np.float32(1.0), flags, vel, d, TYPE_REAL, order, 1.0, 2)
+    d3, s3 = d.reshape(nz, ny, nx), src.reshape(nz, ny, nx)
+    # deep interior: exact copy of the upwind neighbour
+    assert np.array_equal(d3[1:-1, 1:-1, 2:-2], s3[1:-1, 1:-1, 1:-3])
+
+
+def test_sl_reproduces_linear_field(oracle, rng):
+    shape = (16, 15, 14)
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    flags = oracle.init_domain(shape)
+    oracle.fill_grid(shape, flags)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    f = lambda x, y, z: 0.2 + 0.03 * x - 0.02 * y + 0.01 * z
+    src = f(i + 0.5, j + 0.5, k + 0.5).astype(np.float32).reshape(-1)
+    vel = (rng.uniform(-0.9, 0.9, (n, 3))).astype(np.float32)
+    d = src.copy()
+    oracle.advect(shape, np.float32(1.0), flags, vel, d, TYPE_REAL, 1)
+    v3 = vel.reshape(nz, ny, nx, 3)
+    cx = 0.5 * (v3[..., 0] + np.roll(v3[..., 0], -1, 2))
+    cy = 0.5 * (v3[..., 1] + np.roll(v3[..., 1], -1, 1))
+    cz = 0.5 * (v3[..., 2] + np.roll(v3[..., 2], -1, 0))
+    exp = f(i + 0.5 - cx, j + 0.5 - cy, k + 0.5 - cz)
+    sl = (slice(2, -2),) * 3
+    assert np.abs(d.reshape(nz, ny, nx)[sl] - exp[sl]).max() < 2e-6
+
+
+@pytest.mark.parametrize("clamp", [1, 2])
+def test_maccormack_bounded_by_source(oracle, rng, clamp):
+    shape = (16, 14, 12)
+    n = int(np.prod(shape))
+    flags = make_flags(oracle, shape, bw=1, obstacles=3, rng=rng)
+    src = rng.uniform(0.2, 0.8, n).astype(np.float32)
+    vel = (rng.uniform(-1.5, 1.5, (n, 3))).astype(np.float32)
+    d = src.copy()
+    oracle.advect(shape, np.float32(1.0), flags, vel, d, TYPE_REAL, 2, 1.0, clamp)
+    assert d.min() >= 0.0 and d.max() <= src.max()
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+@pytest.mark.parametrize("order", [1, 2])
+def test_mirror_symmetry(oracle, rng, axis, order):
+    """test_2005_symmAdv.py: advection commutes with mirroring the domain."""
+    shape = (18, 16, 14)
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    flags = make_flags(oracle, shape, bw=1, obstacles=2, rng=rng).reshape(nz, ny, nx)
+    src = rng.uniform(0, 1, (nz, ny, nx)).astype(np.float32)
+    vel = rng.uniform(-1.2, 1.2, (nz, ny, nx, 3)).astype(np.float32)
+    ax = 2 - axis  # numpy axis of the mirrored grid axis
+    m = lambda a: np.flip(a, ax)
+    vm = np.flip(vel, ax).copy()
+    # faces of the mirrored axis: u'[j] = -u[N - j] for j >= 1
+    comp = vm[..., axis]
+    moved = np.zeros_like(comp)
+    idx = [slice(None)] * 3
+    src_idx = [slice(None)] * 3
+    idx[ax] = slice(1, None)
+    src_idx[ax] = slice(None, -1)
+    moved[tuple(idx)] = -comp[tuple(src_idx)]
+    vm[..., axis] = moved
+    fl_m = np.ascontiguousarray(m(flags)).reshape(-1)
+    a = src.reshape(-1).copy()
+    b = np.ascontiguousarray(m(src)).reshape(-1)
+    oracle.advect(shape, np.float32(0.8), flags.reshape(-1).copy(), vel.reshape(-1, 3).copy(), a, TYPE_REAL, order, 1.0, 1)
+    oracle.advect(shape, np.float32(0.8), fl_m, np.ascontiguousarray(vm).reshape(-1, 3), b, TYPE_REAL, order, 1.0, 1)
+    assert np.abs(np.ascontiguousarray(m(a.reshape(nz, ny, nx))).reshape(-1) - b).max() < 1e-5
+
+
+# ---------------------------------------------------------------------------
+# forces and boundary conditions (extforces.cpp:72-87,186-217)
+# ---------------------------------------------------------------------------
+def test_wall_bcs_2d_closed_form(oracle, rng):
+    shape = (5, 5, 1)
+    f = oracle.init_domain(shape)
+    oracle.fill_grid(shape, f)
+    f.reshape(5, 5)[2, 2] = OBSTACLE  # obstacle in the middle
+    vel = rng.uniform(0.5, 1.0, (25, 3)).astype(np.float32)
+    v = vel.copy()
+    oracle.set_wall_bcs(shape, f, v)
+    v3, o3 = v.reshape(5, 5, 3), vel.reshape(5, 5, 3)
+    assert np.all(v3[..., 2][f.reshape(5, 5) & (FLUID | OBSTACLE) != 0] == 0)  # 2D: w = 0
+    # faces touching the centre obstacle: u at (2,2) and (3,2), v at (2,2) and (2,3)
+    assert v3[2, 2, 0] == 0 and v3[2, 3, 0] == 0 and v3[2, 2, 1] == 0 and v3[3, 2, 1] == 0
+    # a fluid face between two fluid cells is untouched
+    assert v3[2, 1, 1] == o3[2, 1, 1] or v3[1, 2, 0] == o3[1, 2, 0]
+    assert v3[1, 2, 1] == 0  # j=1 cell: lower neighbour is the wall
+
+
+def test_buoyancy_closed_form(oracle, rng):
+    shape = (6, 7, 5)
+    n = 6 * 7 * 5
+    f = oracle.init_domain(shape)
+    oracle.fill_grid(shape, f)
+    dens = rng.uniform(0, 1, n).astype(np.float32)
+    vel = np.zeros((n, 3), np.float32)
+    g, dt = (0.0, -4e-3, 0.0), np.float32(0.5)
+    oracle.add_buoyancy(shape, dt, f, dens, vel, g, 1.0, True)
+    d3, v3 = dens.reshape(5, 7, 6), vel.reshape(5, 7, 6, 3)
+    fy = np.float32(np.float32(np.float32(4e-3) * dt) / np.float32(1.0 / 7))
+    # the density sum is a float sum, the face update is evaluated in double
+    exp = 0.5 * np.float64(fy) * np.float64(np.float32(d3[2, 3, 3] + d3[2, 2, 3]))
+    assert v3[2, 3, 3, 1] == np.float32(exp)
+    assert v3[2, 1, 3, 1] == 0        # lower neighbour is a wall cell
+    assert np.all(v3[..., 0] == 0) and np.all(v3[..., 2] == 0)
+
+
+# ---------------------------------------------------------------------------
+# pressure: RHS, Laplacian, correction, CG, mIC, multigrid
+# ---------------------------------------------------------------------------
+def _laplacian_dense(flags, shape):
+    """Brute-force matrix of MakeLaplaceMatrix + ApplyMatrix semantics: fluid
+    rows (interior) count non-obstacle neighbours on the diagonal and -1 per
+    fluid neighbour; every other row is the identity."""
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    A = np.zeros((n, n))
+    f = flags
+    sz = nx * ny if nz > 1 else 0
+    for c in range(n):
+        i, j, k = c % nx, (c // nx) % ny, c // (nx * ny)
+        inner = 1 <= i < nx - 1 and 1 <= j < ny - 1 and (1 <= k < nz - 1 if nz > 1 else True)
+        if not (f[c] & FLUID):
+            A[c, c] = 1
+            continue
+        if not inner:
+            continue
+        nbs = [c - 1, c + 1, c - nx, c + nx] + ([c - sz, c + sz] if nz > 1 else [])
+        for nb in nbs:
+            if not (f[nb] & OBSTACLE):
+                A[c, c] += 1
+            if f[nb] & FLUID:
+                A[c, nb] = -1
+    return A
+
+
+@pytest.mark.parametrize("shape", [(9, 8, 1), (7, 6, 5)])
+def test_laplacian_matches_brute_force(oracle, rng, shape):
+    n = int(np.prod(shape))
+    flags = make_flags(oracle, shape, bw=1, open_bound="Y", obstacles=2, rng=rng)
+    A = oracle.make_laplace(shape, flags)
+    x = rng.standard_normal(n).astype(np.float32)
+    y = oracle.apply_matrix(shape, flags, x, A)
+    D = _laplacian_dense(flags, shape)
+    assert np.abs(y - D @ x.astype(np.float64)).max() < 1e-5
+    fl = (flags & FLUID) != 0
+    assert np.allclose(D[np.ix_(fl, fl)], D[np.ix_(fl, fl)].T)  # symmetric on fluid cells
+
+
+@pytest.mark.parametrize("shape", [(9, 8, 1), (8, 7, 6)])
+def test_rhs_is_negative_divergence(oracle, rng, shape):
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    flags = make_flags(oracle, shape, bw=1, obstacles=2, rng=rng)
+    vel = rng.standard_normal((n, 3)).astype(np.float32)
+    rhs, cnt, s = oracle.make_rhs(shape, flags, vel)
+    v = vel.reshape(nz, ny, nx, 3).astype(np.float64)
+    div = (np.roll(v[..., 0], -1, 2) - v[..., 0]) + (np.roll(v[..., 1], -1, 1) - v[..., 1])
+    if nz > 1:
+        div += np.roll(v[..., 2], -1, 0) - v[..., 2]
+    fl = ((flags & FLUID) != 0) & _interior(shape)
+    assert np.abs(rhs[fl] + div.reshape(-1)[fl]).max() < 1e-5
+    assert np.all(rhs[~fl] == 0) and cnt == fl.sum()
+
+
+def _upper_outflow(flags, shape):
+    nx, ny, nz = shape
+    out = (flags.reshape(nz, ny, nx) & OUTFLOW) != 0
+    m = np.zeros_like(out)
+    m[:, :, :-1] |= out[:, :, 1:]
+    m[:, :-1, :] |= out[:, 1:, :]
+    if nz > 1:
+        m[:-1, :, :] |= out[1:, :, :]
+    return m.reshape(-1)
+
+
+@pytest.mark.parametrize("shape", [(10, 9, 1), (9, 8, 7)])
+def test_correct_velocity_divergence_identity(oracle, rng, shape):
+    """rhs(correct(u, p)) = rhs(u) - A p on fluid cells (except next to outflow
+    cells, which correctVelocity leaves alone): pins the correction against the
+    independently written RHS and Laplacian."""
+    n = int(np.prod(shape))
+    flags = make_flags(oracle, shape, bw=1, open_bound="Y", obstacles=2, rng=rng)
+    oracle.reset_outflow(shape, flags)
+    vel = rng.standard_normal((n, 3)).astype(np.float32)
+    oracle.set_wall_bcs(shape, flags, vel)
+    p = rng.standard_normal(n).astype(np.float32)
+    p[(flags & FLUID) == 0] = 0
+    r0, _, _ = oracle.make_rhs(shape, flags, vel)
+    v2 = vel.copy()
+    oracle.correct_velocity(shape, flags, v2, p)
+    r1, _, _ = oracle.make_rhs(shape, flags, v2)
+    Ap = _laplacian_dense(flags, shape) @ p.astype(np.float64)
+    keep = ((flags & FLUID) != 0) & ~_upper_outflow(flags, shape)
+    assert np.abs(r1[keep] - (r0[keep] - Ap[keep])).max() < 2e-5
+
+
+def test_cg_solution_matches_dense_solve(oracle, rng):
+    shape = (12, 10, 1)
+    n = 120
+    flags = make_flags(oracle, shape, bw=0, obstacles=2, rng=rng)
+    vel = smooth_vel(rng, shape)
+    oracle.set_wall_bcs(shape, flags, vel)
+    rhs, _, _ = oracle.make_rhs(shape, flags, vel)
+    p = np.zeros(n, np.float32)
+    rc, st = oracle.solve_pressure_system(shape, rhs.copy(), p, flags, cgAccuracy=1e-6,
+                                          preconditioner=PC_NONE, zeroPressureFixing=True, cgMaxIterFac=50)
+    assert rc == 0 and st["fixPidx"] >= 0
+    D = _laplacian_dense(flags, shape)
+    fix = st["fixPidx"]
+    b = rhs.astype(np.float64)
+    # pin the fixed cell independently: row/col of `fix` become identity, rhs 0
+    D[fix, :] = 0
+    D[:, fix] = 0
+    D[fix, fix] = 1
+    b[fix] = 0
+    fl = ((flags & FLUID) != 0)
+    ref = np.zeros(n)
+    sub = np.ix_(fl, fl)
+    ref[fl] = np.linalg.lstsq(D[sub], b[fl], rcond=None)[0]
+    assert np.abs(p[fl] - ref[fl]).max() < 1e-3 * max(1.0, np.abs(ref).max())
+
+
+def test_mic_reduces_to_jacobi_without_couplings(oracle, rng):
+    shape = (7, 7, 7)
+    n = 343
+    f = oracle.init_domain(shape)
+    # checkerboard of fluid cells: no fluid-fluid couplings
+    k, j, i = np.meshgrid(range(7), range(7), range(7), indexing="ij")
+    g = f.reshape(7, 7, 7)
+    g[1:-1, 1:-1, 1:-1] = np.where(((i + j + k) % 2 == 0)[1:-1, 1:-1, 1:-1], FLUID, EMPTY)
+    A = oracle.make_laplace(shape, f)
+    r = rng.standard_normal(n).astype(np.float32)
+    z = oracle.mic_precondition(shape, f, A, r)
+    fl = (f & FLUID) != 0
+    assert np.allclose(z[fl], r[fl] / A[0][fl], rtol=2e-6)
+
+
+def test_preconditioners_converge_to_same_solution(oracle, rng):
+    shape = (20, 18, 16)
+    n = int(np.prod(shape))
+    flags = make_flags(oracle, shape, bw=1, open_bound="Y", obstacles=3, rng=rng)
+    oracle.reset_outflow(shape, flags)
+    vel = smooth_vel(rng, shape)
+    oracle.set_wall_bcs(shape, flags, vel)
+    res = {}
+    for pc in (PC_NONE, PC_MIC, PC_MGSTATIC):
+        v, p = vel.copy(), np.zeros(n, np.float32)
+        st = oracle.solve_pressure(shape, v, p, flags, cgAccuracy=1e-5, preconditioner=pc, cgMaxIterFac=30)
+        res[pc] = (st["iterations"], v, st["resNorm"])
+        assert st["resNorm"] < 1e-5
+    assert res[PC_MGSTATIC][0] < res[PC_MIC][0] < res[PC_NONE][0]
+    for pc in (PC_MIC, PC_MGSTATIC):
+        assert np.abs(res[pc][1] - res[PC_NONE][1]).max() < 2e-3
+
+
+@pytest.mark.parametrize("shape", [(12, 12, 12), (40, 36, 1)])
+def test_mg_galerkin_operator_is_RAI(oracle, rng, shape):
+    """Level-1 operator of GridMg equals R A_0 I built as explicit matrices
+    from the interpolation weights 1/2^(#odd coords) and the coarse types."""
+    flags = make_flags(oracle, shape, bw=1, open_bound="Y", obstacles=2, rng=rng)
+    A = oracle.make_laplace(shape, flags)
+    mg = oracle.mg_create(shape)
+    oracle.mg_set_a(mg, A)
+    levels = oracle.mg_levels(mg)
+    oracle.mg_destroy(mg)
+    assert len(levels) >= 2
+    (fs, ft, fm), (cs, ct, cm) = levels[0], levels[1]
+    is3 = shape[2] > 1
+    nf, nc = int(np.prod(fs)), int(np.prod(cs))
+    # fine matrix from level-0 storage (A0, Ai, Aj[, Ak]) with trivial scaling
+    D = np.zeros((nf, nf))
+    sx, sy = fs[0], fs[0] * fs[1]
+    for v in range(nf):
+        if ft[v] == 0:
+            continue
+        D[v, v] = fm[v, 0]
+        for d, st in ((1, 1), (2, sx)) + (((3, sy),) if is3 else ()):
+            if fm[v, d] != 0 and v + st < nf:
+                D[v, v + st] = D[v + st, v] = fm[v, d]
+    I = np.zeros((nf, nc))
+    for v in range(nf):
+        if ft[v] == 0:
+            continue
+        x, y, z = v % fs[0], (v // fs[0]) % fs[1], v // sy
+        w = 1.0 / (1 << ((x & 1) + (y & 1) + (z & 1)))
+        for cz in range(z // 2, (z + 1) // 2 + 1):
+            for cy in range(y // 2, (y + 1) // 2 + 1):
+                for cx in range(x // 2, (x + 1) // 2 + 1):
+                    c = cx + cs[0] * cy + cs[0] * cs[1] * cz
+                    if ct[c]:
+                        I[v, c] += w
+    G = I.T @ D @ I
+    S = 14 if is3 else 5
+    C = S - 1
+    for v in np.nonzero(ct)[0]:
+        x, y, z = v % cs[0], (v // cs[0]) % cs[1], v // (cs[0] * cs[1])
+        for dz in ((-1, 0, 1) if is3 else (0,)):
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    s = (dx + 1) + 3 * (dy + 1) + (9 * (dz + 1) if is3 else 0)
+                    if s < C:
+                        continue
+                    nx_, ny_, nz_ = x + dx, y + dy, z + dz
+                    if not (0 <= nx_ < cs[0] and 0 <= ny_ < cs[1] and 0 <= nz_ < cs[2]):
+                        continue
+                    nb = nx_ + cs[0] * ny_ + cs[0] * cs[1] * nz_
+                    assert abs(cm[v, s - C] - G[v, nb]) < 1e-5, (v, s)
+
+
+def test_mg_vcycle_symmetric_positive_and_exact_on_one_level(oracle, rng):
+    shape = (18, 17, 16)
+    n = int(np.prod(shape))
+    flags = make_flags(oracle, shape, bw=1, open_bound="Y", obstacles=2, rng=rng)
+    oracle.reset_outflow(shape, flags)
+    A = oracle.make_laplace(shape, flags)
+    mg = oracle.mg_create(shape)
+    oracle.mg_set_a(mg, A)
+    fl = (A[0] != 0)
+    a = np.where(fl, rng.standard_normal(n), 0).astype(np.float32)
+    b = np.where(fl, rng.standard_normal(n), 0).astype(np.float32)
+    za, zb = oracle.mg_vcycle(mg, a, 1e-10), oracle.mg_vcycle(mg, b, 1e-10)
+    oracle.mg_destroy(mg)
+    lhs, rhs = float(np.dot(za.astype(np.float64), b)), float(np.dot(a.astype(np.float64), zb))
+    assert abs(lhs - rhs) < 1e-4 * max(abs(lhs), 1.0)
+    assert float(np.dot(za.astype(np.float64), a)) > 0
+    # <= 1000 cells: a single level, the V-cycle is the coarsest FP64 CG solve
+    small = (10, 10, 9)
+    f2 = make_flags(oracle, small, bw=1, open_bound="Y")
+    oracle.reset_outflow(small, f2)
+    A2 = oracle.make_laplace(small, f2)
+    mg2 = oracle.mg_create(small)
+    oracle.mg_set_a(mg2, A2)
+    r = np.where(A2[0] != 0, rng.standard_normal(900), 0).astype(np.float32)
+    z = oracle.mg_vcycle(mg2, r, 1e-12)
+    oracle.mg_destroy(mg2)
+    res = r - oracle.apply_matrix(small, f2, z, A2)
+    assert np.abs(res[A2[0] != 0]).max() < 1e-4
+
+
+# ---------------------------------------------------------------------------
+# shapes (shapes.cpp)
+# ---------------------------------------------------------------------------
+def test_shapes_against_numpy(oracle):
+    shape = (12, 11, 10)
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    x, y, z = i + 0.5, j + 0.5, k + 0.5
+    g = np.zeros(n, np.float32)
+    oracle.apply_shape(shape, 1, [2.2, 3.1, 1.0], [7.3, 8.0, 6.6], 0, g, TYPE_REAL, 2.0)
+    box = (x >= 2.2) & (x <= 7.3) & (y >= 3.1) & (y <= 8.0) & (z >= 1.0) & (z <= 6.6)
+    assert np.array_equal(g == 2.0, box.reshape(-1))
+    g[:] = 0
+    oracle.apply_shape(shape, 2, [6.1, 5.2, 4.9], [1, 1, 1], 3.3, g, TYPE_REAL, 1.0)
+    sph = (x - 6.1) ** 2 + (y - 5.2) ** 2 + (z - 4.9) ** 2 <= 3.3 ** 2
+    assert np.array_equal(g == 1.0, sph.reshape(-1))
+    g[:] = 0
+    oracle.apply_shape(shape, 3, [6.0, 2.2, 5.0], [0, 0.4, 0], 2.6, g, TYPE_REAL, 1.0)
+    cyl = (np.abs(y - 2.2) <= 0.4) & ((x - 6.0) ** 2 + (z - 5.0) ** 2 < 2.6 ** 2)
+    assert np.array_equal(g == 1.0, cyl.reshape(-1))
+
+
+def test_double_build_agrees(oracle, oracle64, rng):
+    shape = (16, 14, 12)
+    n = int(np.prod(shape))
+    flags = make_flags(oracle, shape, bw=1, open_bound="Y", obstacles=2, rng=rng)
+    oracle.reset_outflow(shape, flags)
+    vel = smooth_vel(rng, shape)
+    dens = rng.uniform(0, 1, n).astype(np.float32)
+    v32, d32 = vel.copy(), dens.copy()
+    v64, d64 = vel.astype(np.float64), dens.astype(np.float64)
+    oracle.advect(shape, np.float32(1.0), flags, v32, d32, TYPE_REAL, 2, 1.0, 2)
+    oracle64.advect(shape, 1.0, flags, v64, d64, TYPE_REAL, 2, 1.0, 2)
+    assert np.abs(d32 - d64).max() < 1e-5
