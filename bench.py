@@ -38,8 +38,8 @@ HBM_PEAK_GBS = 8000.0
 KERNEL_BYTES = {
     "cg_matvec": 17.0,         # read z 4, p 4, code 1; write p 4, q 4
     "cg_update": 24.0,         # read x, p, r, q; write x, r
-    "mg_l0_smooth": 9.0,       # first sweep: read r 4, code 1; write x 4
-    "mg_l0_residual": 13.0,    # read r 4, code 1, x 4; write r0 4
+    "mg_l0_down": 13.0,        # pre-smooth from 0 + residual: read r 4, code 1; write x 4, r0 4
+    "mg_l0_up": 13.5,          # prolong + post-smooth: read r 4, x 4, code 1, coarse x 0.5; write z 4
     "advect_sl_mac": 24.0,     # read vel 12 (gathers cached); write fwd 12
     "advect_maccormack_mac": 37.0,  # read vel 12, fwd 12 (gathers), flags 1; write 12
 }

@@ -21,7 +21,8 @@ struct MgLevel {
 	long long n = 0;
 	long long sy = 0, sz = 0;   // pitches (sz = nx*ny, also in 2D where nz == 1)
 	float* A = nullptr;          // S arrays of n floats (levels >= 1), symmetric upper half
-	float* x = nullptr;
+	float* x = nullptr;          // pre-smoothed iterate (level 0: before the up leg)
+	float* xp = nullptr;         // post-smoothed iterate (levels 1..L-2)
 	float* b = nullptr;
 	float* r = nullptr;
 	uint8_t* type = nullptr;     // 1 active, 0 inactive
@@ -34,8 +35,10 @@ public:
 	// level-0 matrix code
 	void build(Scene* s, const uint8_t* code, int precision);
 	void setTrivial(const uint8_t*) {}
-	// z = V-cycle(r) from z = 0; kernels skip their work once st->done is set
-	void vcycle(Scene* s, const uint8_t* code, const float* r, float* z, const CgState* st);
+	// z = V-cycle(r) from z = 0; kernels skip their work once st->done is set.
+	// Returns the number of partial sums of fl(z*r) the last kernel wrote into
+	// s->dpart (stride 3), or 0 if the caller has to compute dot(z, r) itself.
+	int vcycle(Scene* s, const uint8_t* code, const float* r, float* z, const CgState* st);
 	int levels() const { return (int)L.size(); }
 
 private:

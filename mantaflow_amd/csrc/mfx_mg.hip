@@ -331,6 +331,277 @@ __global__ void k_mg_prolong(const CgState* __restrict__ st, LvDev f, LvDev c, i
 	}
 }
 
+// ---------------------------------------------------------------------------
+// LDS-brick kernels.  A workgroup owns a brick of cells/vertices and stages
+// the brick plus a halo in LDS, so that a whole smoothing sweep (all
+// colours) is one launch.
+// ---------------------------------------------------------------------------
+template <int BX, int BY, int BZ>
+struct Brick {
+	static constexpr int H = 2;
+	static constexpr int RX = BX + 2 * H, RY = BY + 2 * H, RZ = BZ > 1 ? BZ + 2 * H : 1;
+	static constexpr int RN = RX * RY * RZ;
+	static constexpr int HZ = BZ > 1 ? H : 0;
+};
+
+__device__ inline void brick_origin(int nx, int ny, int BX, int BY, int BZ, int& ox, int& oy, int& oz) {
+	const int nbx = (nx + BX - 1) / BX, nby = (ny + BY - 1) / BY;
+	const long long blk = xcd_tile(blockIdx.x, gridDim.x);
+	ox = (int)(blk % nbx) * BX;
+	const long long rest = blk / nbx;
+	oy = (int)(rest % nby) * BY;
+	oz = (int)(rest / nby) * BZ;
+}
+
+// sum of x over the fluid neighbours coupled to cell q (coefficient -1 each),
+// for a cell in the LDS region with code cs and region strides 1/RX/RXY
+template <bool IS3D>
+__device__ inline float l0_nb_sum(const uint8_t* cs, const float* xs, int q, int RX, int RXY) {
+	const uint8_t cc = cs[q];
+	float sum = 0.f;
+	if (cs[q - 1] & 8) sum += xs[q - 1];
+	if (cc & 8) sum += xs[q + 1];
+	if (cs[q - RX] & 16) sum += xs[q - RX];
+	if (cc & 16) sum += xs[q + RX];
+	if (IS3D) {
+		if (cs[q - RXY] & 32) sum += xs[q - RXY];
+		if (cc & 32) sum += xs[q + RXY];
+	}
+	return sum;
+}
+
+// Level-0 down leg: x = 0, red sweep, black sweep (exact red-black
+// Gauss-Seidel: the black halo ring is recomputed from a 2-cell halo of red
+// values), then the residual r0 = b - A x of the owned cells.
+// Reads b (4 B) and the matrix code (1 B), writes x and r0: 13 B/cell.
+template <int BX, int BY, int BZ, bool IS3D>
+__global__ void __launch_bounds__(1024) k_l0_down(Dims d, const CgState* __restrict__ st,
+                                                   const uint8_t* __restrict__ code, const float* __restrict__ rin,
+                                                   float* __restrict__ z, float* __restrict__ r0) {
+	using B = Brick<BX, BY, BZ>;
+	__shared__ float xs[B::RN];
+	__shared__ float bs[B::RN];
+	__shared__ uint8_t cs[B::RN];
+	if (st->done) return;
+	int ox, oy, oz;
+	brick_origin(d.nx, d.ny, BX, BY, BZ, ox, oy, oz);
+	ox -= B::H;
+	oy -= B::H;
+	oz -= B::HZ;
+	constexpr int RXY = B::RX * B::RY;
+	for (int q = threadIdx.x; q < B::RN; q += blockDim.x) {
+		const int gx = ox + q % B::RX, gy = oy + (q / B::RX) % B::RY, gz = oz + q / RXY;
+		uint8_t cc = 0;
+		float b = 0.f;
+		if (gx >= 0 && gy >= 0 && gz >= 0 && gx < d.nx && gy < d.ny && gz < d.nz) {
+			const long long g = d.idx(gx, gy, gz);
+			cc = code[g];
+			b = rin[g] * l0_scale(cc);
+		}
+		cs[q] = cc;
+		bs[q] = b;
+		xs[q] = (l0_active(cc) && ((gx + gy + gz) & 1) == 0) ? b / l0_diag(cc) : 0.f;
+	}
+	__syncthreads();
+	// black cells of the region without its outermost ring
+	constexpr int IX = B::RX - 2, IY = B::RY - 2, IZ = IS3D ? B::RZ - 2 : 1;
+	for (int t = threadIdx.x; t < IX * IY * IZ; t += blockDim.x) {
+		const int lx = 1 + t % IX, ly = 1 + (t / IX) % IY, lz = (IS3D ? 1 : 0) + t / (IX * IY);
+		const int q = lx + B::RX * ly + RXY * lz;
+		const int gx = ox + lx, gy = oy + ly, gz = oz + lz;
+		const uint8_t cc = cs[q];
+		if (((gx + gy + gz) & 1) == 1 && l0_active(cc))
+			xs[q] = (bs[q] + l0_nb_sum<IS3D>(cs, xs, q, B::RX, RXY)) / l0_diag(cc);
+	}
+	__syncthreads();
+	for (int t = threadIdx.x; t < BX * BY * BZ; t += blockDim.x) {
+		const int lx = B::H + t % BX, ly = B::H + (t / BX) % BY, lz = B::HZ + t / (BX * BY);
+		const int gx = ox + lx, gy = oy + ly, gz = oz + lz;
+		if (gx >= d.nx || gy >= d.ny || gz >= d.nz) continue;
+		const int q = lx + B::RX * ly + RXY * lz;
+		const long long g = d.idx(gx, gy, gz);
+		const uint8_t cc = cs[q];
+		const float x = xs[q];
+		z[g] = x;
+		r0[g] = l0_active(cc) ? (bs[q] + l0_nb_sum<IS3D>(cs, xs, q, B::RX, RXY)) - l0_diag(cc) * x : 0.f;
+	}
+}
+
+// Level-0 up leg: x = xpre + I x_1 (prolongation on load), black sweep, red
+// sweep (reversed colour order), write z, and the partial sums of fl(z*r) for
+// the CG scalar sigma = dot(z, r).  Reads xpre, b, code (9 B), writes z (4 B).
+// Input and output are different arrays: the halo of one brick is owned by
+// its neighbours, which write their result concurrently.
+template <int BX, int BY, int BZ, bool IS3D>
+__global__ void __launch_bounds__(1024) k_l0_up(Dims d, const CgState* __restrict__ st,
+                                                 const uint8_t* __restrict__ code, const float* __restrict__ rin,
+                                                 const float* __restrict__ xpre, float* __restrict__ z, LvDev c1,
+                                                 const uint8_t* __restrict__ ctype, const float* __restrict__ xc,
+                                                 double* __restrict__ part) {
+	using B = Brick<BX, BY, BZ>;
+	// coarse values covering the region's parents (inactive coarse vertices
+	// always hold 0, so no type lookups are needed)
+	constexpr int CRX = B::RX / 2 + 2, CRY = B::RY / 2 + 2, CRZ = IS3D ? B::RZ / 2 + 2 : 1;
+	__shared__ float xs[B::RN];
+	__shared__ float bs[B::RN];
+	__shared__ uint8_t cs[B::RN];
+	__shared__ float xcs[CRX * CRY * CRZ];
+	__shared__ double sh[16];
+	if (st->done) return;
+	int ox, oy, oz;
+	brick_origin(d.nx, d.ny, BX, BY, BZ, ox, oy, oz);
+	ox -= B::H;
+	oy -= B::H;
+	oz -= B::HZ;
+	const int cx0 = ox >> 1, cy0 = oy >> 1, cz0 = oz >> 1;  // floor division
+	for (int q = threadIdx.x; q < CRX * CRY * CRZ; q += blockDim.x) {
+		const int px = cx0 + q % CRX, py = cy0 + (q / CRX) % CRY, pz = cz0 + q / (CRX * CRY);
+		float v = 0.f;
+		if (px >= 0 && py >= 0 && pz >= 0 && px < c1.nx && py < c1.ny && pz < c1.nz) v = xc[px + c1.sy * py + c1.sz * pz];
+		xcs[q] = v;
+	}
+	__syncthreads();
+	constexpr int RXY = B::RX * B::RY;
+	for (int q = threadIdx.x; q < B::RN; q += blockDim.x) {
+		const int gx = ox + q % B::RX, gy = oy + (q / B::RX) % B::RY, gz = oz + q / RXY;
+		uint8_t cc = 0;
+		float b = 0.f, x = 0.f;
+		if (gx >= 0 && gy >= 0 && gz >= 0 && gx < d.nx && gy < d.ny && gz < d.nz) {
+			const long long g = d.idx(gx, gy, gz);
+			cc = code[g];
+			b = rin[g] * l0_scale(cc);
+			x = xpre[g];
+			if (l0_active(cc)) {
+				float sum = 0.f;
+				for (int pz = gz >> 1; pz <= (gz + 1) >> 1; pz++)
+					for (int py = gy >> 1; py <= (gy + 1) >> 1; py++)
+						for (int px = gx >> 1; px <= (gx + 1) >> 1; px++)
+							sum += xcs[(px - cx0) + CRX * (py - cy0) + CRX * CRY * (pz - cz0)];
+				x += wgt(gx, gy, gz) * sum;
+			}
+		}
+		cs[q] = cc;
+		bs[q] = b;
+		xs[q] = x;
+	}
+	__syncthreads();
+	constexpr int IX = B::RX - 2, IY = B::RY - 2, IZ = IS3D ? B::RZ - 2 : 1;
+	for (int t = threadIdx.x; t < IX * IY * IZ; t += blockDim.x) {
+		const int lx = 1 + t % IX, ly = 1 + (t / IX) % IY, lz = (IS3D ? 1 : 0) + t / (IX * IY);
+		const int q = lx + B::RX * ly + RXY * lz;
+		const int gx = ox + lx, gy = oy + ly, gz = oz + lz;
+		const uint8_t cc = cs[q];
+		if (((gx + gy + gz) & 1) == 1 && l0_active(cc))
+			xs[q] = (bs[q] + l0_nb_sum<IS3D>(cs, xs, q, B::RX, RXY)) / l0_diag(cc);
+	}
+	__syncthreads();
+	double acc = 0.0;
+	for (int t = threadIdx.x; t < BX * BY * BZ; t += blockDim.x) {
+		const int lx = B::H + t % BX, ly = B::H + (t / BX) % BY, lz = B::HZ + t / (BX * BY);
+		const int gx = ox + lx, gy = oy + ly, gz = oz + lz;
+		if (gx >= d.nx || gy >= d.ny || gz >= d.nz) continue;
+		const int q = lx + B::RX * ly + RXY * lz;
+		const long long g = d.idx(gx, gy, gz);
+		const uint8_t cc = cs[q];
+		float x = xs[q];
+		if (((gx + gy + gz) & 1) == 0 && l0_active(cc)) x = (bs[q] + l0_nb_sum<IS3D>(cs, xs, q, B::RX, RXY)) / l0_diag(cc);
+		z[g] = x;
+		acc += (double)(x * rin[g]);
+	}
+	const double s = block_sum(acc, sh);
+	if (threadIdx.x == 0) part[3 * blockIdx.x] = s;
+}
+
+// coarse-level brick: 32x4x4 vertices in 3D, 64x8 in 2D (512 threads; rows of
+// 32/64 vertices keep the per-vertex stencil loads in whole cache lines)
+constexpr int kLcBX3 = 32, kLcBY3 = 4, kLcBZ3 = 4, kLcBX2 = 64, kLcBY2 = 8;
+static unsigned lc_blocks(const MgLevel& c, bool is3d) {
+	const long long bx = is3d ? kLcBX3 : kLcBX2, by = is3d ? kLcBY3 : kLcBY2, bz = is3d ? kLcBZ3 : 1;
+	return (unsigned)(((c.nx + bx - 1) / bx) * ((c.ny + by - 1) / by) * ((c.nz + bz - 1) / bz));
+}
+
+// Coarse-level block Gauss-Seidel: one sweep over all 8 (4 in 2D) colours of
+// the 27-point (9-point) Galerkin stencil inside an LDS brick; halo values
+// are the pre-sweep ones (block-hybrid smoother).  Forward colour order for
+// pre-smoothing, reversed for post-smoothing, which keeps the V-cycle
+// symmetric.  first: start from x = 0 (xin unused).  xcp != null: prolongate
+// from the next coarser level on load (x = xin + I x_{l+1}).  The result goes
+// to xout, never to xin: neighbouring bricks read xin as their halo.
+template <bool IS3D>
+__global__ void __launch_bounds__(512) k_lc_smooth_blk(const CgState* __restrict__ st, LvDev c, const float* __restrict__ A,
+                                                       const uint8_t* __restrict__ type, const float* __restrict__ b,
+                                                       const float* __restrict__ xin, float* __restrict__ xout, int first,
+                                                       int reverse, LvDev cp, const uint8_t* __restrict__ ctp,
+                                                       const float* __restrict__ xcp) {
+	constexpr int BX = IS3D ? kLcBX3 : kLcBX2, BY = IS3D ? kLcBY3 : kLcBY2, BZ = IS3D ? kLcBZ3 : 1;
+	constexpr int RX = BX + 2, RY = BY + 2, RZ = IS3D ? BZ + 2 : 1, RN = RX * RY * RZ, RXY = RX * RY;
+	constexpr int S = IS3D ? 14 : 5, C = S - 1, NCOL = IS3D ? 8 : 4;
+	__shared__ float xs[RN];
+	if (st->done) return;
+	int ox, oy, oz;
+	brick_origin(c.nx, c.ny, BX, BY, BZ, ox, oy, oz);
+	const int hz = IS3D ? 1 : 0;
+	for (int q = threadIdx.x; q < RN; q += blockDim.x) {
+		const int gx = ox - 1 + q % RX, gy = oy - 1 + (q / RX) % RY, gz = oz - hz + q / RXY;
+		float v = 0.f;
+		if (!first && gx >= 0 && gy >= 0 && gz >= 0 && gx < c.nx && gy < c.ny && gz < c.nz) {
+			const long long g = gx + c.sy * gy + c.sz * gz;
+			v = xin[g];
+			if (xcp && type[g]) {
+				float sum = 0.f;
+				for (int pz = gz / 2; pz <= (gz + 1) / 2; pz++)
+					for (int py = gy / 2; py <= (gy + 1) / 2; py++)
+						for (int px = gx / 2; px <= (gx + 1) / 2; px++) {
+							const long long pi = px + cp.sy * py + cp.sz * pz;
+							if (ctp[pi]) sum += xcp[pi];
+						}
+				v += wgt(gx, gy, gz) * sum;
+			}
+		}
+		xs[q] = v;
+	}
+	__syncthreads();
+	const int tx = threadIdx.x % BX, ty = (threadIdx.x / BX) % BY, tz = threadIdx.x / (BX * BY);
+	const int gx = ox + tx, gy = oy + ty, gz = oz + tz;
+	const bool valid = gx < c.nx && gy < c.ny && gz < c.nz;
+	const long long v = valid ? gx + c.sy * gy + c.sz * gz : 0;
+	const bool act = valid && type[v];
+	const int me = (tx + 1) + RX * (ty + 1) + RXY * (tz + hz);
+	float a[IS3D ? 27 : 9];
+	float bv = 0.f, diag = 1.f;
+	if (act) {
+		bv = b[v];
+		diag = A[v];
+		int s = 0;
+		for (int dz = (IS3D ? -1 : 0); dz <= (IS3D ? 1 : 0); dz++)
+			for (int dy = -1; dy <= 1; dy++)
+				for (int dx = -1; dx <= 1; dx++, s++) {
+					float coef = 0.f;
+					const int nx = gx + dx, ny = gy + dy, nz = gz + dz;
+					if (s != C && nx >= 0 && ny >= 0 && nz >= 0 && nx < c.nx && ny < c.ny && nz < c.nz) {
+						const long long nn = nx + c.sy * ny + c.sz * nz;
+						coef = s > C ? A[(long long)(s - C) * c.n + v] : A[(long long)(C - s) * c.n + nn];
+					}
+					a[s] = coef;
+				}
+	}
+	const int par = (gx & 1) | ((gy & 1) << 1) | ((gz & 1) << 2);
+	for (int ph = 0; ph < NCOL; ph++) {
+		const int col = reverse ? NCOL - 1 - ph : ph;
+		if (act && par == col) {
+			float sum = bv;
+			int s = 0;
+			for (int dz = (IS3D ? -1 : 0); dz <= (IS3D ? 1 : 0); dz++)
+				for (int dy = -1; dy <= 1; dy++)
+					for (int dx = -1; dx <= 1; dx++, s++)
+						if (s != C) sum -= a[s] * xs[me + dx + RX * dy + RXY * dz];
+			xs[me] = sum / diag;
+		}
+		__syncthreads();
+	}
+	if (valid) xout[v] = xs[me];
+}
+
 __global__ void k_fill_zero(const CgState* __restrict__ st, float* __restrict__ x, long long n) {
 	if (st->done) return;
 	for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < n; v += (long long)gridDim.x * blockDim.x)
@@ -408,10 +679,12 @@ void MgHierarchy::build(Scene* s, const uint8_t* code, int precision) {
 		L.push_back(c);
 	}
 	L[0].r = (float*)alloc(sizeof(float) * (size_t)d.n);
+	L[0].x = (float*)alloc(sizeof(float) * (size_t)d.n);
 	for (size_t l = 1; l < L.size(); l++) {
 		MgLevel& c = L[l];
 		c.A = (float*)alloc(sizeof(float) * (size_t)(c.n * S));
 		c.x = (float*)alloc(sizeof(float) * (size_t)c.n);
+		c.xp = (float*)alloc(sizeof(float) * (size_t)c.n);
 		c.b = (float*)alloc(sizeof(float) * (size_t)c.n);
 		c.r = (float*)alloc(sizeof(float) * (size_t)c.n);
 		c.type = (uint8_t*)alloc((size_t)c.n);
@@ -546,29 +819,27 @@ void MgHierarchy::build_coarsest(Scene* s, const uint8_t* code0) {
 	sync(s);
 }
 
-void MgHierarchy::vcycle(Scene* s, const uint8_t* code, const float* r, float* z, const CgState* st) {
+int MgHierarchy::vcycle(Scene* s, const uint8_t* code, const float* r, float* z, const CgState* st) {
 	const Dims& d = s->d;
-	const Tiles t = tiles_for(d);
-	const dim3 tg((unsigned)t.count()), tb(kBlock);
 	const int nl = (int)L.size();
-	const int cx[8] = {0, 1, 0, 1, 0, 1, 0, 1}, cy[8] = {0, 0, 1, 1, 0, 0, 1, 1}, cz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
-	const int ncol = is3d ? 8 : 4;
 	if (nl == 1) {
 		hipLaunchKernelGGL(k_fill_zero, dim3(nblk(d.n)), dim3(256), 0, s->stream, st, z, d.n);
 		const unsigned rows = (unsigned)((nc + 3) / 4);
 		hipLaunchKernelGGL(k_mg_coarsest, dim3(rows > 0 ? rows : 1), dim3(256), 0, s->stream, st, nc,
 		                   (const int*)cidx, (const double*)G, r, code, z);
-		return;
+		return 0;
 	}
+	// level-0 brick geometry: 64x8x8 cells (3D), 64x16 (2D)
+	const int BX = 64, BY = is3d ? 8 : 16, BZ = is3d ? 8 : 1;
+	const unsigned nb0 = (unsigned)(((d.nx + BX - 1) / BX) * (long long)((d.ny + BY - 1) / BY) * ((d.nz + BZ - 1) / BZ));
 	// down
-	hipLaunchKernelGGL(k_l0_smooth, tg, tb, 0, s->stream, d, t, st, code, r, z, 0, 1);
 	{
-		KtScope kt("mg_l0_smooth", s->stream);
-		hipLaunchKernelGGL(k_l0_smooth, tg, tb, 0, s->stream, d, t, st, code, r, z, 1, 0);
-	}
-	{
-		KtScope kt("mg_l0_residual", s->stream);
-		hipLaunchKernelGGL(k_l0_residual, tg, tb, 0, s->stream, d, t, st, code, r, (const float*)z, L[0].r);
+		KtScope kt("mg_l0_down", s->stream);
+		if (is3d)
+			hipLaunchKernelGGL((k_l0_down<64, 8, 8, true>), dim3(nb0), dim3(1024), 0, s->stream, d, st, code, r, L[0].x, L[0].r);
+		else
+			hipLaunchKernelGGL((k_l0_down<64, 16, 1, false>), dim3(nb0), dim3(1024), 0, s->stream, d, st, code, r, L[0].x,
+			                   L[0].r);
 	}
 	for (int l = 1; l < nl; l++) {
 		MgLevel& c = L[l];
@@ -576,12 +847,15 @@ void MgHierarchy::vcycle(Scene* s, const uint8_t* code, const float* r, float* z
 		hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(c.n)), dim3(256), 0, s->stream, st, lvdev(c), lvdev(f), (int)is3d,
 		                   (const uint8_t*)c.type, (const float*)f.r, c.b);
 		if (l == nl - 1) break;
-		hipLaunchKernelGGL(k_lc_smooth, dim3(nblk(c.n)), dim3(256), 0, s->stream, st, lvdev(c), (int)is3d, S,
-		                   (const float*)c.A, (const uint8_t*)c.type, (const float*)c.b, c.x, 0, 0, 0, 1);
-		for (int col = 1; col < ncol; col++)
-			hipLaunchKernelGGL(k_lc_smooth, dim3(nblk(c.n / 8 + 1)), dim3(256), 0, s->stream, st, lvdev(c), (int)is3d,
-			                   S, (const float*)c.A, (const uint8_t*)c.type, (const float*)c.b, c.x, cx[col], cy[col],
-			                   cz[col], 0);
+		const unsigned nbl = lc_blocks(c, is3d);
+		if (is3d)
+			hipLaunchKernelGGL(k_lc_smooth_blk<true>, dim3(nbl), dim3(512), 0, s->stream, st, lvdev(c), (const float*)c.A,
+			                   (const uint8_t*)c.type, (const float*)c.b, (const float*)nullptr, c.x, 1, 0, lvdev(c),
+			                   (const uint8_t*)nullptr, (const float*)nullptr);
+		else
+			hipLaunchKernelGGL(k_lc_smooth_blk<false>, dim3(nbl), dim3(512), 0, s->stream, st, lvdev(c), (const float*)c.A,
+			                   (const uint8_t*)c.type, (const float*)c.b, (const float*)nullptr, c.x, 1, 0, lvdev(c),
+			                   (const uint8_t*)nullptr, (const float*)nullptr);
 		hipLaunchKernelGGL(k_lc_residual, dim3(nblk(c.n)), dim3(256), 0, s->stream, st, lvdev(c), (int)is3d, S,
 		                   (const float*)c.A, (const uint8_t*)c.type, (const float*)c.b, (const float*)c.x, c.r);
 	}
@@ -592,24 +866,32 @@ void MgHierarchy::vcycle(Scene* s, const uint8_t* code, const float* r, float* z
 		hipLaunchKernelGGL(k_mg_coarsest, dim3(rows > 0 ? rows : 1), dim3(256), 0, s->stream, st, nc,
 		                   (const int*)cidx, (const double*)G, (const float*)c.b, (const uint8_t*)nullptr, c.x);
 	}
-	// up
-	for (int l = nl - 2; l >= 0; l--) {
+	// up: prolongation fused into each post-smoothing sweep
+	for (int l = nl - 2; l >= 1; l--) {
 		const MgLevel& c = L[l + 1];
 		MgLevel& f = L[l];
-		float* xf = l == 0 ? z : f.x;
-		hipLaunchKernelGGL(k_mg_prolong, dim3(nblk(f.n)), dim3(256), 0, s->stream, st, lvdev(f), lvdev(c), (int)is3d,
-		                   l == 0 ? code : (const uint8_t*)nullptr, (const uint8_t*)f.type, (const uint8_t*)c.type,
-		                   (const float*)c.x, xf);
-		if (l == 0) {
-			hipLaunchKernelGGL(k_l0_smooth, tg, tb, 0, s->stream, d, t, st, code, r, z, 1, 0);
-			hipLaunchKernelGGL(k_l0_smooth, tg, tb, 0, s->stream, d, t, st, code, r, z, 0, 0);
-		} else {
-			for (int col = ncol - 1; col >= 0; col--)
-				hipLaunchKernelGGL(k_lc_smooth, dim3(nblk(f.n / 8 + 1)), dim3(256), 0, s->stream, st, lvdev(f),
-				                   (int)is3d, S, (const float*)f.A, (const uint8_t*)f.type, (const float*)f.b, f.x,
-				                   cx[col], cy[col], cz[col], 0);
-		}
+		const unsigned nbl = lc_blocks(f, is3d);
+		if (is3d)
+			hipLaunchKernelGGL(k_lc_smooth_blk<true>, dim3(nbl), dim3(512), 0, s->stream, st, lvdev(f), (const float*)f.A,
+			                   (const uint8_t*)f.type, (const float*)f.b, (const float*)f.x, f.xp, 0, 1, lvdev(c),
+			                   (const uint8_t*)c.type, (const float*)(l + 1 == nl - 1 ? c.x : c.xp));
+		else
+			hipLaunchKernelGGL(k_lc_smooth_blk<false>, dim3(nbl), dim3(512), 0, s->stream, st, lvdev(f), (const float*)f.A,
+			                   (const uint8_t*)f.type, (const float*)f.b, (const float*)f.x, f.xp, 0, 1, lvdev(c),
+			                   (const uint8_t*)c.type, (const float*)(l + 1 == nl - 1 ? c.x : c.xp));
 	}
+	{
+		KtScope kt("mg_l0_up", s->stream);
+		const MgLevel& c = L[1];
+		const float* xc1 = nl == 2 ? c.x : c.xp;
+		if (is3d)
+			hipLaunchKernelGGL((k_l0_up<64, 8, 8, true>), dim3(nb0), dim3(1024), 0, s->stream, d, st, code, r,
+			                   (const float*)L[0].x, z, lvdev(c), (const uint8_t*)c.type, (const float*)xc1, s->dpart);
+		else
+			hipLaunchKernelGGL((k_l0_up<64, 16, 1, false>), dim3(nb0), dim3(1024), 0, s->stream, d, st, code, r,
+			                   (const float*)L[0].x, z, lvdev(c), (const uint8_t*)c.type, (const float*)xc1, s->dpart);
+	}
+	return (int)nb0;
 }
 
 }  // namespace mfx

@@ -213,32 +213,104 @@ __global__ void __launch_bounds__(kBlock) k_dot_zr(Dims d, const CgState* __rest
 	if (threadIdx.x == 0) part[3 * blockIdx.x] = s;
 }
 
-// A: p_new = z + beta*p_old ; q = A p_new ; partial fl(q*p_new)
-__global__ void __launch_bounds__(kBlock) k_cg_matvec(Dims d, Tiles t, const CgState* __restrict__ st,
-                                                      const uint8_t* __restrict__ code, const float* __restrict__ z,
-                                                      const float* __restrict__ pold, float* __restrict__ pnew,
-                                                      float* __restrict__ q, double* __restrict__ part) {
-	__shared__ double sh[8];
+// A: p_new = z + beta*p_old ; q = A p_new ; partial sums of fl(q*p_new).
+// z-marching form: a 64 x TY tile of columns walks a chunk of planes.  The
+// search direction of the current plane (tile + 1-cell ring) is staged in
+// LDS, the planes below and above stay in registers, so every value is read
+// from HBM once.  Row arithmetic and term order follow apply_row; neighbours
+// outside the grid contribute 0 (their coefficient is 0 there).
+constexpr int kMvTY = 8;
+template <int TY>
+__global__ void __launch_bounds__(64 * TY) k_cg_matvec25(Dims d, int kchunk, const CgState* __restrict__ st,
+                                                         const uint8_t* __restrict__ code, const float* __restrict__ z,
+                                                         const float* __restrict__ pold, float* __restrict__ pnew,
+                                                         float* __restrict__ q, double* __restrict__ part) {
+	constexpr int TX = 64, RX = TX + 2, RY = TY + 2, NH = 2 * RX + 2 * (RY - 2);
+	__shared__ float ps[RX * RY];
+	__shared__ uint8_t cs[RX * RY];
+	__shared__ double sh[16];
 	if (st->done) return;
 	const float beta = st->beta;
 	const bool first = st->first;
-	CellPos p = tile_cell(t);
-	double acc = 0;
-	if (p.i < d.nx && p.j < d.ny) {
-		const long long c = d.idx(p.i, p.j, p.k);
-		auto pv = [&](long long m) -> float { return first ? z[m] : z[m] + beta * pold[m]; };
-		const uint8_t cc = code[c];
-		const float pc = pv(c);
-		const float qv = apply_row(d, code, c, cc, pv);
-		pnew[c] = pc;
-		q[c] = qv;
-		acc = (double)(qv * pc);
+	const int ntx = (d.nx + TX - 1) / TX, nty = (d.ny + TY - 1) / TY;
+	const long long blk = xcd_tile(blockIdx.x, gridDim.x);
+	const int ox = (int)(blk % ntx) * TX;
+	const long long rest = blk / ntx;
+	const int oy = (int)(rest % nty) * TY;
+	const int k0 = (int)(rest / nty) * kchunk, k1 = min(d.nz, k0 + kchunk);
+	const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+	const int gx = ox + tx, gy = oy + ty;
+	const bool in = gx < d.nx && gy < d.ny;
+	auto pn = [&](long long m) -> float { return first ? z[m] : z[m] + beta * pold[m]; };
+	float pPrev = 0.f, pCur = 0.f, pNext = 0.f;
+	uint8_t cPrev = 0, cCur = 0, cNext = 0;
+	if (in && k0 > 0) {
+		const long long m = d.idx(gx, gy, k0 - 1);
+		pPrev = pn(m);
+		cPrev = code[m];
+	}
+	if (in) {
+		const long long m = d.idx(gx, gy, k0);
+		pCur = pn(m);
+		cCur = code[m];
+	}
+	const int L = (tx + 1) + RX * (ty + 1);
+	double acc = 0.0;
+	for (int k = k0; k < k1; k++) {
+		if (in && k + 1 < d.nz) {
+			const long long m = d.idx(gx, gy, k + 1);
+			pNext = pn(m);
+			cNext = code[m];
+		} else {
+			pNext = 0.f;
+			cNext = 0;
+		}
+		ps[L] = pCur;
+		cs[L] = cCur;
+		for (int h = threadIdx.x; h < NH; h += blockDim.x) {
+			int hx, hy;
+			if (h < RX) { hx = h; hy = 0; }
+			else if (h < 2 * RX) { hx = h - RX; hy = RY - 1; }
+			else if (h < 2 * RX + RY - 2) { hx = 0; hy = 1 + h - 2 * RX; }
+			else { hx = RX - 1; hy = 1 + h - 2 * RX - (RY - 2); }
+			const int hgx = ox - 1 + hx, hgy = oy - 1 + hy;
+			float v = 0.f;
+			uint8_t cv = 0;
+			if (hgx >= 0 && hgy >= 0 && hgx < d.nx && hgy < d.ny) {
+				const long long m = d.idx(hgx, hgy, k);
+				v = pn(m);
+				cv = code[m];
+			}
+			ps[hx + RX * hy] = v;
+			cs[hx + RX * hy] = cv;
+		}
+		__syncthreads();
+		if (in) {
+			float qv;
+			if (!(cCur & 64)) qv = pCur;
+			else if (d.is3d)
+				qv = pCur * cA0(cCur) + ps[L - 1] * cAi(cs[L - 1]) + ps[L + 1] * cAi(cCur) + ps[L - RX] * cAj(cs[L - RX]) +
+				     ps[L + RX] * cAj(cCur) + pPrev * cAk(cPrev) + pNext * cAk(cCur);
+			else
+				qv = pCur * cA0(cCur) + ps[L - 1] * cAi(cs[L - 1]) + ps[L + 1] * cAi(cCur) + ps[L - RX] * cAj(cs[L - RX]) +
+				     ps[L + RX] * cAj(cCur);
+			const long long m = d.idx(gx, gy, k);
+			pnew[m] = pCur;
+			q[m] = qv;
+			acc += (double)(qv * pCur);
+		}
+		__syncthreads();
+		pPrev = pCur;
+		cPrev = cCur;
+		pCur = pNext;
+		cCur = cNext;
 	}
 	const double s = block_sum(acc, sh);
 	if (threadIdx.x == 0) part[3 * blockIdx.x] = s;
 }
 
-// B: x += alpha p ; r += (-alpha) q ; min/max(r), sum fl(r*r), sum r^2
+// B: x += alpha p ; r += (-alpha) q ; min/max(r), sum fl(r*r), sum r^2.
+// A bounded grid walks the arrays four elements per thread and step.
 __global__ void __launch_bounds__(kBlock) k_cg_update(Dims d, const CgState* __restrict__ st, float* __restrict__ x,
                                                       float* __restrict__ r, const float* __restrict__ p,
                                                       const float* __restrict__ q, int withDot,
@@ -247,18 +319,42 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(Dims d, const CgState* __r
 	__shared__ float shf[8];
 	if (st->done) return;
 	const float alpha = st->alpha, malpha = -alpha;
-	const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
 	float mn = 3.402823466e+38f, mx = -3.402823466e+38f;
 	double zr = 0, rr = 0;
-	if (c < d.n) {
+	const long long n4 = d.n >> 2;
+	const long long stride = (long long)gridDim.x * blockDim.x;
+	for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < n4; v += stride) {
+		float4 xv = reinterpret_cast<const float4*>(x)[v];
+		float4 rv = reinterpret_cast<const float4*>(r)[v];
+		const float4 pv = reinterpret_cast<const float4*>(p)[v];
+		const float4 qv = reinterpret_cast<const float4*>(q)[v];
+		xv.x = xv.x + alpha * pv.x;
+		xv.y = xv.y + alpha * pv.y;
+		xv.z = xv.z + alpha * pv.z;
+		xv.w = xv.w + alpha * pv.w;
+		rv.x = rv.x + malpha * qv.x;
+		rv.y = rv.y + malpha * qv.y;
+		rv.z = rv.z + malpha * qv.z;
+		rv.w = rv.w + malpha * qv.w;
+		reinterpret_cast<float4*>(x)[v] = xv;
+		reinterpret_cast<float4*>(r)[v] = rv;
+		const float e[4] = {rv.x, rv.y, rv.z, rv.w};
+		for (int u = 0; u < 4; u++) {
+			if (e[u] < mn) mn = e[u];
+			if (e[u] > mx) mx = e[u];
+			zr += (double)(e[u] * e[u]);
+			rr += (double)e[u] * (double)e[u];
+		}
+	}
+	for (long long c = (n4 << 2) + blockIdx.x * (long long)blockDim.x + threadIdx.x; c < d.n; c += stride) {
 		const float xn = x[c] + alpha * p[c];
 		const float rn = r[c] + malpha * q[c];
 		x[c] = xn;
 		r[c] = rn;
 		if (rn < mn) mn = rn;
 		if (rn > mx) mx = rn;
-		zr = (double)(rn * rn);
-		rr = (double)rn * (double)rn;
+		zr += (double)(rn * rn);
+		rr += (double)rn * (double)rn;
 	}
 	const double s0 = block_sum(zr, sh);
 	const double s1 = block_sum(rr, sh);
@@ -492,6 +588,14 @@ static void solve_system(Scene* s, Grid* flags, float* rhs, float* x, const mfx_
 	const unsigned nbF = flat_count(d.n);
 	const unsigned nbMax = nbT > nbF ? nbT : nbF;
 	s->ensurePartials(3 * (size_t)nbMax + 16, 2 * (size_t)nbMax + 16);
+	// z-marching matvec: 64 x kMvTY column tiles, chunks of planes sized so
+	// that about 1024 workgroups cover the grid
+	const long long cols = (long long)((d.nx + 63) / 64) * ((d.ny + kMvTY - 1) / kMvTY);
+	int kchunk = d.nz;
+	while (kchunk > 8 && cols * ((d.nz + kchunk - 1) / kchunk) < 1024) kchunk = (kchunk + 1) / 2;
+	const unsigned nbM = (unsigned)(cols * ((d.nz + kchunk - 1) / kchunk));
+	// update: grid-stride over float4 groups, bounded partial count
+	const unsigned nbU = (unsigned)std::max(1LL, std::min<long long>(2048, (d.n / 4 + kBlock - 1) / kBlock));
 
 	// zero-pressure fixing (pressure.cpp:349-388)
 	long long fix = -1;
@@ -563,12 +667,15 @@ static void solve_system(Scene* s, Grid* flags, float* rhs, float* x, const mfx_
 		// bnorm from the init partials, sigma from dot(z, r) after one V-cycle
 		hipLaunchKernelGGL(k_cg_finalize, dim3(1), dim3(1024), 0, s->stream, st, (const double*)s->dpart,
 		                   (const float*)s->fpart, (int)nbF, (int)FIN_INIT, 0);
-		mg->vcycle(s, code, r, z, st);
-		hipLaunchKernelGGL(k_dot_zr, dim3(nbF), dim3(kBlock), 0, s->stream, d, (const CgState*)st, (const float*)z,
-		                   (const float*)r, s->dpart);
+		int np = mg->vcycle(s, code, r, z, st);
+		if (np == 0) {
+			hipLaunchKernelGGL(k_dot_zr, dim3(nbF), dim3(kBlock), 0, s->stream, d, (const CgState*)st,
+			                   (const float*)z, (const float*)r, s->dpart);
+			np = (int)nbF;
+		}
 		// sigma = dot(z, r); keep bnorm2
 		hipLaunchKernelGGL(k_cg_finalize, dim3(1), dim3(1024), 0, s->stream, st, (const double*)s->dpart,
-		                   (const float*)s->fpart, (int)nbF, (int)FIN_BETA, 0);
+		                   (const float*)s->fpart, np, (int)FIN_BETA, 0);
 	} else {
 		hipLaunchKernelGGL(k_cg_finalize, dim3(1), dim3(1024), 0, s->stream, st, (const double*)s->dpart,
 		                   (const float*)s->fpart, (int)nbF, (int)FIN_INIT, 0);
@@ -590,24 +697,27 @@ static void solve_system(Scene* s, Grid* flags, float* rhs, float* x, const mfx_
 		for (int it = 0; it < nIt; it++) {
 			{
 				KtScope kt("cg_matvec", s->stream);
-				hipLaunchKernelGGL(k_cg_matvec, dim3(nbT), dim3(kBlock), 0, s->stream, d, t, (const CgState*)st, code,
-				                   (const float*)z, (const float*)pcur, pnext, q, s->dpart);
+				hipLaunchKernelGGL(k_cg_matvec25<kMvTY>, dim3(nbM), dim3(64 * kMvTY), 0, s->stream, d, kchunk,
+				                   (const CgState*)st, code, (const float*)z, (const float*)pcur, pnext, q, s->dpart);
 			}
 			hipLaunchKernelGGL(k_cg_finalize, dim3(1), dim3(1024), 0, s->stream, st, (const double*)s->dpart,
-			                   (const float*)s->fpart, (int)nbT, (int)FIN_ALPHA, pcNone ? 1 : 0);
+			                   (const float*)s->fpart, (int)nbM, (int)FIN_ALPHA, pcNone ? 1 : 0);
 			{
 				KtScope kt("cg_update", s->stream);
-				hipLaunchKernelGGL(k_cg_update, dim3(nbF), dim3(kBlock), 0, s->stream, d, (const CgState*)st, x, r,
+				hipLaunchKernelGGL(k_cg_update, dim3(nbU), dim3(kBlock), 0, s->stream, d, (const CgState*)st, x, r,
 				                   (const float*)pnext, (const float*)q, pcNone ? 1 : 0, s->dpart, s->fpart);
 			}
 			hipLaunchKernelGGL(k_cg_finalize, dim3(1), dim3(1024), 0, s->stream, st, (const double*)s->dpart,
-			                   (const float*)s->fpart, (int)nbF, (int)FIN_RES, pcNone ? 1 : 0);
+			                   (const float*)s->fpart, (int)nbU, (int)FIN_RES, pcNone ? 1 : 0);
 			if (!pcNone) {
-				mg->vcycle(s, code, r, z, st);
-				hipLaunchKernelGGL(k_dot_zr, dim3(nbF), dim3(kBlock), 0, s->stream, d, (const CgState*)st,
-				                   (const float*)z, (const float*)r, s->dpart);
+				int np = mg->vcycle(s, code, r, z, st);
+				if (np == 0) {
+					hipLaunchKernelGGL(k_dot_zr, dim3(nbF), dim3(kBlock), 0, s->stream, d, (const CgState*)st,
+					                   (const float*)z, (const float*)r, s->dpart);
+					np = (int)nbF;
+				}
 				hipLaunchKernelGGL(k_cg_finalize, dim3(1), dim3(1024), 0, s->stream, st, (const double*)s->dpart,
-				                   (const float*)s->fpart, (int)nbF, (int)FIN_BETA, 0);
+				                   (const float*)s->fpart, np, (int)FIN_BETA, 0);
 			}
 			std::swap(pcur, pnext);
 		}
