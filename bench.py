@@ -138,7 +138,7 @@ def main():
     ap.add_argument("--res", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-res", type=int, default=128)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=40)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build runs the grid path on a single MI355X "

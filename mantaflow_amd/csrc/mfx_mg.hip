@@ -512,6 +512,261 @@ __global__ void __launch_bounds__(1024) k_l0_up(Dims d, const CgState* __restric
 	if (threadIdx.x == 0) part[3 * blockIdx.x] = s;
 }
 
+// ---------------------------------------------------------------------------
+// z-marching level-0 kernels (3D).  A workgroup owns a 64 x TY column tile
+// and a chunk of planes [k0, k1).  It walks the planes once, staging each
+// plane of the tile plus a 2-cell ring in a 5-slot LDS ring buffer, and runs
+// the three dependent stages of red-black Gauss-Seidel as a pipeline:
+//   A(p)   load plane p (code, rhs, x), first colour pointwise
+//   B(p-1) second colour on plane p-1 (tile + 1-cell ring)
+//   C(p-2) final output on the tile's cells of plane p-2
+// so every input element is read from HBM once plus 4 halo planes per
+// chunk.  Arithmetic and operation order equal the brick kernels above.
+// ---------------------------------------------------------------------------
+template <int TY>
+struct ZTile {
+	static constexpr int TX = 64, H = 2, RX = TX + 2 * H, RY = TY + 2 * H, RP = RX * RY, NS = 5, NT = TX * TY;
+};
+
+__device__ inline void zm_origin(const Dims& d, int TY, int kchunk, int& ox, int& oy, int& k0, int& k1) {
+	const int ntx = (d.nx + 63) / 64, nty = (d.ny + TY - 1) / TY;
+	const long long blk = xcd_tile(blockIdx.x, gridDim.x);
+	ox = (int)(blk % ntx) * 64;
+	const long long rest = blk / ntx;
+	oy = (int)(rest % nty) * TY;
+	k0 = (int)(rest / nty) * kchunk;
+	k1 = min(d.nz, k0 + kchunk);
+}
+
+// l0_nb_sum with the plane below (cM, xM) and above (xU) in separate slots
+__device__ inline float zm_nb_sum(const uint8_t* cP, const float* xP, const uint8_t* cM, const float* xM,
+                                  const float* xU, int q, int RX) {
+	const uint8_t cc = cP[q];
+	float sum = 0.f;
+	if (cP[q - 1] & 8) sum += xP[q - 1];
+	if (cc & 8) sum += xP[q + 1];
+	if (cP[q - RX] & 16) sum += xP[q - RX];
+	if (cc & 16) sum += xP[q + RX];
+	if (cM[q] & 32) sum += xM[q];
+	if (cc & 32) sum += xU[q];
+	return sum;
+}
+
+__device__ inline int zslot(int p) { return (p + 10) % 5; }  // p >= -2
+
+// The (at most two) region cells of one thread: q = tid and tid + NT.
+template <typename T>
+struct ZmCells {
+	static_assert(T::RP <= 2 * T::NT, "two region cells per thread");
+	long long g[2];  // in-plane offset i + nx*j
+	bool ok[2];      // inside the region and the grid
+	int par[2];      // (i + j) & 1
+	__device__ ZmCells(const Dims& d, int rx0, int ry0) {
+#pragma unroll
+		for (int u = 0; u < 2; u++) {
+			const int q = threadIdx.x + u * T::NT;
+			const int gx = rx0 + q % T::RX, gy = ry0 + q / T::RX;
+			ok[u] = q < T::RP && gx >= 0 && gy >= 0 && gx < d.nx && gy < d.ny;
+			g[u] = ok[u] ? (long long)gx + (long long)d.nx * gy : 0;
+			par[u] = (gx + gy) & 1;
+		}
+	}
+};
+
+// Down leg (same result as k_l0_down): red from x = 0, black, residual.
+template <int TY>
+__global__ void __launch_bounds__(64 * TY) k_l0_down_zm(Dims d, int kchunk, const CgState* __restrict__ st,
+                                                         const uint8_t* __restrict__ code,
+                                                         const float* __restrict__ rin, float* __restrict__ xout,
+                                                         float* __restrict__ r0) {
+	using T = ZTile<TY>;
+	__shared__ float xs[T::NS][T::RP];
+	__shared__ float bs[T::NS][T::RP];
+	__shared__ uint8_t cs[T::NS][T::RP];
+	if (st->done) return;
+	int ox, oy, k0, k1;
+	zm_origin(d, TY, kchunk, ox, oy, k0, k1);
+	const int rx0 = ox - T::H, ry0 = oy - T::H;
+	ZmCells<T> cells(d, rx0, ry0);
+	uint8_t pc[2];
+	float pr[2];
+	// plane p's inputs are fetched into registers one plane ahead, so the
+	// loads are in flight during the barriers and the sweeps of plane p-1
+	auto fetch = [&](int p) {
+#pragma unroll
+		for (int u = 0; u < 2; u++) {
+			pc[u] = 0;
+			pr[u] = 0.f;
+			if (cells.ok[u] && p >= 0 && p < d.nz) {
+				const long long g = cells.g[u] + d.sz * p;
+				pc[u] = code[g];
+				pr[u] = rin[g];
+			}
+		}
+	};
+	fetch(k0 - 2);
+	for (int p = k0 - 2; p <= k1 + 1; p++) {
+		const int sp = zslot(p);
+#pragma unroll
+		for (int u = 0; u < 2; u++) {
+			const int q = threadIdx.x + u * T::NT;
+			if (q >= T::RP) continue;
+			const uint8_t cc = pc[u];
+			const float b = pr[u] * l0_scale(cc);
+			cs[sp][q] = cc;
+			bs[sp][q] = b;
+			xs[sp][q] = (l0_active(cc) && ((cells.par[u] + p) & 1) == 0) ? b / l0_diag(cc) : 0.f;
+		}
+		if (p <= k1) fetch(p + 1);
+		__syncthreads();
+		const int pb = p - 1;
+		if (pb >= k0 - 1 && pb <= k1 && pb >= 0 && pb < d.nz) {
+			const int s0 = zslot(pb), sm = zslot(pb - 1), su = zslot(pb + 1);
+			constexpr int IX = T::RX - 2, IN = IX * (T::RY - 2);
+			for (int t = threadIdx.x; t < IN; t += T::NT) {
+				const int lx = 1 + t % IX, ly = 1 + t / IX;
+				const int q = lx + T::RX * ly;
+				const uint8_t cc = cs[s0][q];
+				if (((rx0 + lx + ry0 + ly + pb) & 1) == 1 && l0_active(cc))
+					xs[s0][q] = (bs[s0][q] + zm_nb_sum(cs[s0], xs[s0], cs[sm], xs[sm], xs[su], q, T::RX)) / l0_diag(cc);
+			}
+		}
+		__syncthreads();
+		const int pc = p - 2;
+		if (pc >= k0 && pc < k1) {
+			const int s0 = zslot(pc), sm = zslot(pc - 1), su = zslot(pc + 1);
+			const int lx = T::H + (threadIdx.x & 63), ly = T::H + (threadIdx.x >> 6);
+			const int gx = rx0 + lx, gy = ry0 + ly;
+			if (gx < d.nx && gy < d.ny) {
+				const int q = lx + T::RX * ly;
+				const long long g = d.idx(gx, gy, pc);
+				const uint8_t cc = cs[s0][q];
+				const float x = xs[s0][q];
+				xout[g] = x;
+				r0[g] = l0_active(cc)
+				            ? (bs[s0][q] + zm_nb_sum(cs[s0], xs[s0], cs[sm], xs[sm], xs[su], q, T::RX)) - l0_diag(cc) * x
+				            : 0.f;
+			}
+		}
+	}
+}
+
+// Up leg (same result as k_l0_up): x = xpre + I x_1, black, red, write z and
+// the partial sums of fl(z*r).  Coarse planes are staged in a 2-slot ring:
+// fine plane p needs coarse planes p>>1 and (p+1)>>1.
+template <int TY>
+__global__ void __launch_bounds__(64 * TY) k_l0_up_zm(Dims d, int kchunk, const CgState* __restrict__ st,
+                                                       const uint8_t* __restrict__ code,
+                                                       const float* __restrict__ rin, const float* __restrict__ xpre,
+                                                       float* __restrict__ z, LvDev c1, const float* __restrict__ xc,
+                                                       double* __restrict__ part) {
+	using T = ZTile<TY>;
+	constexpr int CRX = T::RX / 2 + 2, CRY = T::RY / 2 + 2, CRP = CRX * CRY;
+	__shared__ float xs[T::NS][T::RP];
+	__shared__ float rs[T::NS][T::RP];
+	__shared__ uint8_t cs[T::NS][T::RP];
+	__shared__ float xcs[2][CRP];
+	__shared__ double sh[16];
+	if (st->done) return;
+	int ox, oy, k0, k1;
+	zm_origin(d, TY, kchunk, ox, oy, k0, k1);
+	const int rx0 = ox - T::H, ry0 = oy - T::H;
+	const int cx0 = rx0 >> 1, cy0 = ry0 >> 1;  // floor division
+	static_assert(CRP <= T::NT, "one coarse value per thread");
+	// this thread's vertex of the coarse-plane region
+	const int cq = threadIdx.x;
+	const int cpx = cx0 + cq % CRX, cpy = cy0 + cq / CRX;
+	const bool cok = cq < CRP && cpx >= 0 && cpy >= 0 && cpx < c1.nx && cpy < c1.ny;
+	const long long cg = cok ? (long long)cpx + c1.sy * cpy : 0;
+	auto cfetch = [&](int P) -> float { return (cok && P >= 0 && P < c1.nz) ? xc[cg + c1.sz * P] : 0.f; };
+	ZmCells<T> cells(d, rx0, ry0);
+	uint8_t pc[2];
+	float pr[2], px[2];
+	auto fetch = [&](int p) {
+#pragma unroll
+		for (int u = 0; u < 2; u++) {
+			pc[u] = 0;
+			pr[u] = 0.f;
+			px[u] = 0.f;
+			if (cells.ok[u] && p >= 0 && p < d.nz) {
+				const long long g = cells.g[u] + d.sz * p;
+				pc[u] = code[g];
+				pr[u] = rin[g];
+				px[u] = xpre[g];
+			}
+		}
+	};
+	// coarse plane (k0-2)/2 goes to LDS now; later planes are fetched into a
+	// register two fine planes before they are needed (k0 is even)
+	if (cq < CRP) xcs[((k0 - 2) >> 1) & 1][cq] = cfetch((k0 - 2) >> 1);
+	float creg = cfetch(k0 >> 1);
+	fetch(k0 - 2);
+	__syncthreads();
+	double acc = 0.0;
+	for (int p = k0 - 2; p <= k1 + 1; p++) {
+		const int sp = zslot(p);
+#pragma unroll
+		for (int u = 0; u < 2; u++) {
+			const int q = threadIdx.x + u * T::NT;
+			if (q >= T::RP) continue;
+			const uint8_t cc = pc[u];
+			float x = px[u];
+			if (l0_active(cc)) {
+				const int gx = rx0 + q % T::RX, gy = ry0 + q / T::RX;
+				float sum = 0.f;
+				for (int pz = p >> 1; pz <= (p + 1) >> 1; pz++)
+					for (int py = gy >> 1; py <= (gy + 1) >> 1; py++)
+						for (int qx = gx >> 1; qx <= (gx + 1) >> 1; qx++)
+							sum += xcs[pz & 1][(qx - cx0) + CRX * (py - cy0)];
+				x += wgt(gx, gy, p) * sum;
+			}
+			cs[sp][q] = cc;
+			rs[sp][q] = pr[u];
+			xs[sp][q] = x;
+		}
+		if (!(p & 1)) {
+			// coarse plane p/2+1 is first needed by plane p+1
+			if (cq < CRP) xcs[((p >> 1) + 1) & 1][cq] = creg;
+			creg = cfetch((p >> 1) + 2);
+		}
+		if (p <= k1) fetch(p + 1);
+		__syncthreads();
+		const int pb = p - 1;
+		if (pb >= k0 - 1 && pb <= k1 && pb >= 0 && pb < d.nz) {
+			const int s0 = zslot(pb), sm = zslot(pb - 1), su = zslot(pb + 1);
+			constexpr int IX = T::RX - 2, IN = IX * (T::RY - 2);
+			for (int t = threadIdx.x; t < IN; t += T::NT) {
+				const int lx = 1 + t % IX, ly = 1 + t / IX;
+				const int q = lx + T::RX * ly;
+				const uint8_t cc = cs[s0][q];
+				if (((rx0 + lx + ry0 + ly + pb) & 1) == 1 && l0_active(cc))
+					xs[s0][q] = (rs[s0][q] * l0_scale(cc) + zm_nb_sum(cs[s0], xs[s0], cs[sm], xs[sm], xs[su], q, T::RX)) /
+					            l0_diag(cc);
+			}
+		}
+		__syncthreads();
+		const int pc = p - 2;
+		if (pc >= k0 && pc < k1) {
+			const int s0 = zslot(pc), sm = zslot(pc - 1), su = zslot(pc + 1);
+			const int lx = T::H + (threadIdx.x & 63), ly = T::H + (threadIdx.x >> 6);
+			const int gx = rx0 + lx, gy = ry0 + ly;
+			if (gx < d.nx && gy < d.ny) {
+				const int q = lx + T::RX * ly;
+				const long long g = d.idx(gx, gy, pc);
+				const uint8_t cc = cs[s0][q];
+				float x = xs[s0][q];
+				if (((gx + gy + pc) & 1) == 0 && l0_active(cc))
+					x = (rs[s0][q] * l0_scale(cc) + zm_nb_sum(cs[s0], xs[s0], cs[sm], xs[sm], xs[su], q, T::RX)) /
+					    l0_diag(cc);
+				z[g] = x;
+				acc += (double)(x * rs[s0][q]);
+			}
+		}
+	}
+	const double s = block_sum(acc, sh);
+	if (threadIdx.x == 0) part[3 * blockIdx.x] = s;
+}
+
 // coarse-level brick: 32x4x4 vertices in 3D, 64x8 in 2D (512 threads; rows of
 // 32/64 vertices keep the per-vertex stencil loads in whole cache lines)
 constexpr int kLcBX3 = 32, kLcBY3 = 4, kLcBZ3 = 4, kLcBX2 = 64, kLcBY2 = 8;
@@ -829,14 +1084,20 @@ int MgHierarchy::vcycle(Scene* s, const uint8_t* code, const float* r, float* z,
 		                   (const int*)cidx, (const double*)G, r, code, z);
 		return 0;
 	}
-	// level-0 brick geometry: 64x8x8 cells (3D), 64x16 (2D)
-	const int BX = 64, BY = is3d ? 8 : 16, BZ = is3d ? 8 : 1;
-	const unsigned nb0 = (unsigned)(((d.nx + BX - 1) / BX) * (long long)((d.ny + BY - 1) / BY) * ((d.nz + BZ - 1) / BZ));
+	// level-0 geometry: 3D z-marching 64x8 column tiles over chunks of an even
+	// number of planes (about 1024 workgroups); 2D bricks of 64x16 cells
+	constexpr int kZmTY = 8;
+	const int BX = 64, BY = is3d ? kZmTY : 16;
+	const long long cols = (long long)((d.nx + BX - 1) / BX) * ((d.ny + BY - 1) / BY);
+	int kchunk = (d.nz + 1) & ~1;
+	while (is3d && kchunk > 16 && cols * ((d.nz + kchunk - 1) / kchunk) < 1024) kchunk = ((kchunk / 2) + 1) & ~1;
+	const unsigned nb0 = (unsigned)(is3d ? cols * ((d.nz + kchunk - 1) / kchunk) : cols);
 	// down
 	{
 		KtScope kt("mg_l0_down", s->stream);
 		if (is3d)
-			hipLaunchKernelGGL((k_l0_down<64, 8, 8, true>), dim3(nb0), dim3(1024), 0, s->stream, d, st, code, r, L[0].x, L[0].r);
+			hipLaunchKernelGGL(k_l0_down_zm<kZmTY>, dim3(nb0), dim3(64 * kZmTY), 0, s->stream, d, kchunk, st, code, r,
+			                   L[0].x, L[0].r);
 		else
 			hipLaunchKernelGGL((k_l0_down<64, 16, 1, false>), dim3(nb0), dim3(1024), 0, s->stream, d, st, code, r, L[0].x,
 			                   L[0].r);
@@ -885,8 +1146,8 @@ int MgHierarchy::vcycle(Scene* s, const uint8_t* code, const float* r, float* z,
 		const MgLevel& c = L[1];
 		const float* xc1 = nl == 2 ? c.x : c.xp;
 		if (is3d)
-			hipLaunchKernelGGL((k_l0_up<64, 8, 8, true>), dim3(nb0), dim3(1024), 0, s->stream, d, st, code, r,
-			                   (const float*)L[0].x, z, lvdev(c), (const uint8_t*)c.type, (const float*)xc1, s->dpart);
+			hipLaunchKernelGGL(k_l0_up_zm<kZmTY>, dim3(nb0), dim3(64 * kZmTY), 0, s->stream, d, kchunk, st, code, r,
+			                   (const float*)L[0].x, z, lvdev(c), xc1, s->dpart);
 		else
 			hipLaunchKernelGGL((k_l0_up<64, 16, 1, false>), dim3(nb0), dim3(1024), 0, s->stream, d, st, code, r,
 			                   (const float*)L[0].x, z, lvdev(c), (const uint8_t*)c.type, (const float*)xc1, s->dpart);

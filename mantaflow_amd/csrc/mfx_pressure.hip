@@ -242,48 +242,54 @@ __global__ void __launch_bounds__(64 * TY) k_cg_matvec25(Dims d, int kchunk, con
 	const int gx = ox + tx, gy = oy + ty;
 	const bool in = gx < d.nx && gy < d.ny;
 	auto pn = [&](long long m) -> float { return first ? z[m] : z[m] + beta * pold[m]; };
-	float pPrev = 0.f, pCur = 0.f, pNext = 0.f;
-	uint8_t cPrev = 0, cCur = 0, cNext = 0;
-	if (in && k0 > 0) {
-		const long long m = d.idx(gx, gy, k0 - 1);
-		pPrev = pn(m);
-		cPrev = code[m];
-	}
-	if (in) {
-		const long long m = d.idx(gx, gy, k0);
-		pCur = pn(m);
-		cCur = code[m];
-	}
+	// own column: planes k-1, k, k+1 in registers, plane k+2 prefetched
+	const long long gcol = in ? (long long)gx + (long long)d.nx * gy : 0;
+	auto fcol = [&](int kk, float& v, uint8_t& c) {
+		v = 0.f;
+		c = 0;
+		if (in && kk >= 0 && kk < d.nz) {
+			const long long m = gcol + d.sz * kk;
+			v = pn(m);
+			c = code[m];
+		}
+	};
+	// one ring cell per thread (threads < NH), prefetched one plane ahead
+	static_assert(NH <= 64 * TY, "one ring cell per thread");
+	const int h = threadIdx.x;
+	int hx = 0, hy = 0;
+	if (h < RX) { hx = h; hy = 0; }
+	else if (h < 2 * RX) { hx = h - RX; hy = RY - 1; }
+	else if (h < 2 * RX + RY - 2) { hx = 0; hy = 1 + h - 2 * RX; }
+	else { hx = RX - 1; hy = 1 + h - 2 * RX - (RY - 2); }
+	const int hgx = ox - 1 + hx, hgy = oy - 1 + hy;
+	const bool hin = h < NH && hgx >= 0 && hgy >= 0 && hgx < d.nx && hgy < d.ny;
+	const long long hcol = hin ? (long long)hgx + (long long)d.nx * hgy : 0;
+	auto fhalo = [&](int kk, float& v, uint8_t& c) {
+		v = 0.f;
+		c = 0;
+		if (hin && kk < d.nz) {
+			const long long m = hcol + d.sz * kk;
+			v = pn(m);
+			c = code[m];
+		}
+	};
+	float pPrev, pCur, pNext, pN2, hv;
+	uint8_t cPrev, cCur, cNext, cN2, hc;
+	fcol(k0 - 1, pPrev, cPrev);
+	fcol(k0, pCur, cCur);
+	fcol(k0 + 1, pNext, cNext);
+	fhalo(k0, hv, hc);
 	const int L = (tx + 1) + RX * (ty + 1);
 	double acc = 0.0;
 	for (int k = k0; k < k1; k++) {
-		if (in && k + 1 < d.nz) {
-			const long long m = d.idx(gx, gy, k + 1);
-			pNext = pn(m);
-			cNext = code[m];
-		} else {
-			pNext = 0.f;
-			cNext = 0;
-		}
 		ps[L] = pCur;
 		cs[L] = cCur;
-		for (int h = threadIdx.x; h < NH; h += blockDim.x) {
-			int hx, hy;
-			if (h < RX) { hx = h; hy = 0; }
-			else if (h < 2 * RX) { hx = h - RX; hy = RY - 1; }
-			else if (h < 2 * RX + RY - 2) { hx = 0; hy = 1 + h - 2 * RX; }
-			else { hx = RX - 1; hy = 1 + h - 2 * RX - (RY - 2); }
-			const int hgx = ox - 1 + hx, hgy = oy - 1 + hy;
-			float v = 0.f;
-			uint8_t cv = 0;
-			if (hgx >= 0 && hgy >= 0 && hgx < d.nx && hgy < d.ny) {
-				const long long m = d.idx(hgx, hgy, k);
-				v = pn(m);
-				cv = code[m];
-			}
-			ps[hx + RX * hy] = v;
-			cs[hx + RX * hy] = cv;
+		if (h < NH) {
+			ps[hx + RX * hy] = hv;
+			cs[hx + RX * hy] = hc;
 		}
+		fcol(k + 2, pN2, cN2);
+		fhalo(k + 1, hv, hc);
 		__syncthreads();
 		if (in) {
 			float qv;
@@ -304,6 +310,8 @@ __global__ void __launch_bounds__(64 * TY) k_cg_matvec25(Dims d, int kchunk, con
 		cPrev = cCur;
 		pCur = pNext;
 		cCur = cNext;
+		pNext = pN2;
+		cNext = cN2;
 	}
 	const double s = block_sum(acc, sh);
 	if (threadIdx.x == 0) part[3 * blockIdx.x] = s;
