@@ -171,6 +171,17 @@ void mfx_pressure_opts_default(mfx_pressure_opts* o);
  * plain CG in 2D, where the reference also disables it (conjugategrad.cpp:315-321). */
 int mfx_solve_pressure(mfx_grid* vel, mfx_grid* pressure, const mfx_grid* flags,
                        const mfx_pressure_opts* opts, mfx_solve_stats* stats, mfx_grid* retRhs);
+/* Batched solvePressure: count independent problems (vel[i], pressure[i],
+ * flags[i]), each on its own scene (any mix of sizes, 2D and 3D), solved
+ * together.  Every CG/multigrid phase is one launch whose work list holds the
+ * tiles of all problems; two iterations are captured in a hipGraph and
+ * replayed until every problem has met its stopping rule.  Results equal
+ * per-problem mfx_solve_pressure calls up to the order of FP64 partial sums.
+ * stats may be NULL or point to count entries.  No counterpart in the
+ * reference (its callers loop over solvePressure, plugin/pressure.cpp:480). */
+int mfx_solve_pressure_batch(int count, mfx_grid* const* vel, mfx_grid* const* pressure,
+                             const mfx_grid* const* flags, const mfx_pressure_opts* opts, mfx_solve_stats* stats);
+
 /* computePressureRhs (source/plugin/pressure.cpp:277-299) */
 int mfx_compute_pressure_rhs(mfx_grid* rhs, const mfx_grid* vel, const mfx_grid* flags,
                              const mfx_pressure_opts* opts);

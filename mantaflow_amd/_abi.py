@@ -79,6 +79,8 @@ SIGNATURES = {
     "mfx_solve_pressure": (_I, [_P, _P, _P, C.POINTER(PressureOpts), C.POINTER(SolveStats), _P]),
     "mfx_compute_pressure_rhs": (_I, [_P, _P, _P, C.POINTER(PressureOpts)]),
     "mfx_solve_pressure_system": (_I, [_P, _P, _P, C.POINTER(PressureOpts), C.POINTER(SolveStats)]),
+    "mfx_solve_pressure_batch": (_I, [C.c_int, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P),
+                                      C.POINTER(PressureOpts), C.POINTER(SolveStats)]),
     "mfx_correct_velocity": (_I, [_P, _P, _P]),
     "mfx_release_mg": (_I, [_P]),
     "mfx_kernel_timing": (_I, [_I]),

@@ -1,0 +1,216 @@
+// Batched launches: one kernel launch covers work items of many independent
+// problems (pressure systems of different sizes).  Every loop kernel of the
+// CG and the multigrid V-cycle is written as a __device__ body taking a
+// parameter struct, a logical tile index and a partial-sum slot.  Two
+// wrappers instantiate it:
+//   k_one   - one problem; tile = slot = XCD-aware remap of blockIdx.x
+//   k_many  - a work list; block b runs item[b] = (problem, tile, slot) with
+//             that problem's parameter struct
+// so the batched path runs exactly the single-problem arithmetic.
+#pragma once
+
+#include <vector>
+
+#include "mfx_internal.h"
+#include "mfx_mg.h"
+
+namespace mfx {
+
+struct WorkItem {
+	int prob;
+	int tile;
+	int slot;
+	int pad;
+};
+
+// The partial-sum slot is the logical tile in both wrappers, so a problem's
+// partials are reduced in the same order alone and in a batch (bit-identical
+// results).
+template <typename P, void (*BODY)(const P&, long long, long long), int NT>
+__global__ void __launch_bounds__(NT) k_one(P p) {
+	const long long t = xcd_tile(blockIdx.x, gridDim.x);
+	BODY(p, t, t);
+}
+
+template <typename P, void (*BODY)(const P&, long long, long long), int NT>
+__global__ void __launch_bounds__(NT) k_many(const WorkItem* __restrict__ items, const P* __restrict__ params) {
+	const WorkItem w = items[blockIdx.x];
+	BODY(params[w.prob], w.tile, w.slot);
+}
+
+// launch helpers (instantiated in the file that defines BODY)
+template <typename P, void (*BODY)(const P&, long long, long long), int NT>
+inline void launch_one(hipStream_t s, long long nb, const P& p) {
+	if (nb > 0) hipLaunchKernelGGL((k_one<P, BODY, NT>), dim3((unsigned)nb), dim3(NT), 0, s, p);
+}
+template <typename P, void (*BODY)(const P&, long long, long long), int NT>
+inline void launch_many(hipStream_t s, long long nb, const WorkItem* items, const P* p) {
+	if (nb > 0) hipLaunchKernelGGL((k_many<P, BODY, NT>), dim3((unsigned)nb), dim3(NT), 0, s, items, p);
+}
+
+// ---------------------------------------------------------------------------
+// CG loop kernels (mfx_pressure.hip)
+// ---------------------------------------------------------------------------
+// A: p_new = z + beta p_old, q = A p_new, partial sums of fl(q p_new)
+struct MatvecP {
+	Dims d;
+	int kchunk;
+	const CgState* st;
+	const uint8_t* code;
+	const float* z;
+	const float* pold;
+	float* pnew;
+	float* q;
+	double* part;
+};
+// B: x += alpha p, r -= alpha q, tiles striding over the float4 groups
+struct UpdateP {
+	Dims d;
+	long long per;  // number of tiles of this problem (stride in 256-thread units)
+	const CgState* st;
+	float* x;
+	float* r;
+	const float* p;
+	const float* q;
+	int withDot;
+	double* part;
+	float* fpart;
+};
+// single-block reduction of one problem's partials + scalar update
+struct FinP {
+	CgState* st;
+	const double* part;
+	const float* fpart;
+	int nb;
+	int phase;
+	int pcNone;
+};
+// sum fl(z r) over a range of 256 cells
+struct DotP {
+	Dims d;
+	const CgState* st;
+	const float* z;
+	const float* r;
+	double* part;
+};
+
+enum { FIN_INIT = 0, FIN_ALPHA = 1, FIN_RES = 2, FIN_BETA = 3 };
+constexpr int kMvTY = 8;
+
+// tiles of each CG kernel for one problem (and the plane chunk / range size)
+long long matvec_tiles(const Dims& d, int* kchunk);
+long long update_tiles(const Dims& d, long long* per);
+inline long long dot_tiles(const Dims& d) { return (d.n + kBlock - 1) / kBlock; }
+
+// launches: *_one with a host parameter struct, *_many over a device work list
+void run_matvec_one(hipStream_t s, unsigned nb, const MatvecP& p);
+void run_matvec_many(hipStream_t s, unsigned nb, const WorkItem* items, const MatvecP* p);
+void run_update_one(hipStream_t s, unsigned nb, const UpdateP& p);
+void run_update_many(hipStream_t s, unsigned nb, const WorkItem* items, const UpdateP* p);
+void run_finalize_one(hipStream_t s, const FinP& p);
+void run_finalize_many(hipStream_t s, unsigned nprob, const WorkItem* items, const FinP* p);
+void run_dot_one(hipStream_t s, unsigned nb, const DotP& p);
+void run_dot_many(hipStream_t s, unsigned nb, const WorkItem* items, const DotP* p);
+
+// ---------------------------------------------------------------------------
+// V-cycle kernels (mfx_mg.hip)
+// ---------------------------------------------------------------------------
+struct L0DownP {  // level-0 pre-smooth from zero + residual
+	Dims d;
+	int kchunk;
+	const CgState* st;
+	const uint8_t* code;
+	const float* rin;
+	float* xout;
+	float* r0;
+};
+struct L0UpP {  // prolongation + level-0 post-smooth + partial sums of fl(z r)
+	Dims d;
+	int kchunk;
+	const CgState* st;
+	const uint8_t* code;
+	const float* rin;
+	const float* xpre;
+	float* z;
+	LvDev c1;
+	const float* xc;
+	double* part;
+};
+struct RestrictP {
+	const CgState* st;
+	LvDev c, f;
+	const uint8_t* ctype;
+	const float* rf;
+	float* bc;
+};
+struct SmoothP {
+	const CgState* st;
+	LvDev c;
+	const void* A;  // float or BF16 by the kernel's template argument
+	const uint8_t* type;
+	const float* b;
+	const float* xin;
+	float* xout;
+	int first, reverse;
+	LvDev cp;
+	const float* xcp;
+	Canon cn;
+};
+struct ResidP {
+	const CgState* st;
+	LvDev c;
+	const void* A;
+	const uint8_t* type;
+	const float* b;
+	const float* x;
+	float* r;
+	Canon cn;
+};
+struct CoarsestP {
+	const CgState* st;
+	int nc;
+	const int* cidx;
+	const double* G;
+	const float* b;
+	const uint8_t* code;
+	float* x;
+};
+struct FillP {
+	const CgState* st;
+	float* x;
+	long long n;
+};
+
+// parameters of every step of one problem's V-cycle
+struct MgHierarchy::Steps {
+	int nl = 0;
+	int kchunk = 1;
+	long long l0Tiles = 0;
+	L0DownP down{};
+	L0UpP up{};
+	struct Level {  // levels 1..nl-1
+		RestrictP restrict_{};
+		SmoothP pre{}, post{};  // not used on the coarsest level
+		ResidP resid{};
+		long long tiles = 0, blkTiles = 0;
+	};
+	std::vector<Level> lv;
+	CoarsestP coarsest{};
+	FillP fill{};  // single-level hierarchies: z = 0 before the coarsest solve
+};
+
+// tiles of the V-cycle kernels for one problem/level
+long long l0_tiles(const Dims& d, int* kchunk);  // z-marching (3D) or brick (2D)
+long long lc_tiles(const LvDev& c);               // 64x4 vertex tiles per plane
+long long lcblk_tiles(const LvDev& c, bool is3d); // smoother bricks
+inline long long coarsest_tiles(int nc) { return (nc + 3) / 4; }
+
+void run_l0_down_many(hipStream_t s, bool is3d, unsigned nb, const WorkItem* items, const L0DownP* p);
+void run_l0_up_many(hipStream_t s, bool is3d, unsigned nb, const WorkItem* items, const L0UpP* p);
+void run_restrict_many(hipStream_t s, bool is3d, unsigned nb, const WorkItem* items, const RestrictP* p);
+void run_smooth_many(hipStream_t s, bool is3d, bool bf16, unsigned nb, const WorkItem* items, const SmoothP* p);
+void run_resid_many(hipStream_t s, bool is3d, bool bf16, unsigned nb, const WorkItem* items, const ResidP* p);
+void run_coarsest_many(hipStream_t s, unsigned nb, const WorkItem* items, const CoarsestP* p);
+void run_fill_many(hipStream_t s, unsigned nb, const WorkItem* items, const FillP* p);
+
+}  // namespace mfx

@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "mfx_batch.h"
 #include "mfx_mg.h"
 #include "mfx_prof.h"
 
@@ -32,10 +33,6 @@ namespace mfx {
 __device__ __host__ inline int oddc(int i, int j, int k) { return (i & 1) + (j & 1) + (k & 1); }
 __device__ inline float wgt(int i, int j, int k) { return 1.0f / (float)(1 << oddc(i, j, k)); }
 
-struct LvDev {
-	int nx, ny, nz;
-	long long sy, sz, n;
-};
 static inline LvDev lvdev(const MgLevel& l) { return LvDev{l.nx, l.ny, l.nz, l.sy, l.sz, l.n}; }
 
 __device__ inline bool l0_active(uint8_t c) { return (c & 7) != 0; }
@@ -162,173 +159,191 @@ __global__ void k_mg_drop(LvDev c, uint8_t* __restrict__ ctype, const float* __r
 	}
 }
 
+// Full 27-point (9-point in 2D) stencil of vertex v from the symmetric
+// upper-half storage, entry C = S-1 being the diagonal.  Returns false when
+// a neighbour lies outside the level or is inactive.
+__device__ inline bool mg_full_stencil(const LvDev& c, int is3d, int S, const float* __restrict__ A,
+                                       const uint8_t* __restrict__ type, long long v, float* out) {
+	const int C = S - 1;
+	const int i = (int)(v % c.nx), j = (int)((v / c.nx) % c.ny), k = (int)(v / c.sz);
+	bool ok = true;
+	int s = 0;
+	for (int dz = is3d ? -1 : 0; dz <= (is3d ? 1 : 0); dz++)
+		for (int dy = -1; dy <= 1; dy++)
+			for (int dx = -1; dx <= 1; dx++, s++) {
+				const int ni = i + dx, nj = j + dy, nk = k + dz;
+				if (ni < 0 || nj < 0 || nk < 0 || ni >= c.nx || nj >= c.ny || nk >= c.nz) {
+					ok = false;
+					out[s] = 0.f;
+					continue;
+				}
+				const long long nn = ni + c.sy * nj + c.sz * nk;
+				if (!type[nn]) ok = false;
+				out[s] = s >= C ? A[(long long)(s - C) * c.n + v] : A[(long long)(C - s) * c.n + nn];
+			}
+	return ok;
+}
+
+// full stencils of a few sample vertices (for picking the canonical one)
+__global__ void k_mg_sample(LvDev c, int is3d, int S, const float* __restrict__ A, const uint8_t* __restrict__ type,
+                            const long long* __restrict__ samples, int ns, float* __restrict__ out,
+                            int* __restrict__ ok) {
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= ns) return;
+	const long long v = samples[t];
+	float st[27];
+	for (int s = 0; s < 27; s++) st[s] = 0.f;
+	const bool good = type[v] && mg_full_stencil(c, is3d, S, A, type, v, st);
+	for (int s = 0; s < 27; s++) out[27 * t + s] = st[s];
+	ok[t] = good ? 1 : 0;
+}
+
+// type 2 = active, all neighbours active, stencil bitwise equal to cn.  Only
+// nonzero type values are ever written, so concurrent readers testing
+// type != 0 are unaffected.
+__global__ void k_mg_classify(LvDev c, int is3d, int S, const float* __restrict__ A, uint8_t* __restrict__ type,
+                              Canon cn) {
+	const int ns = is3d ? 27 : 9;
+	for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < c.n;
+	     v += (long long)gridDim.x * blockDim.x) {
+		if (!type[v]) continue;
+		float st[27];
+		if (!mg_full_stencil(c, is3d, S, A, type, v, st)) continue;
+		bool same = true;
+		for (int s = 0; s < ns; s++) same = same && (__float_as_uint(st[s]) == __float_as_uint(cn.a[s]));
+		if (same) type[v] = 2;
+	}
+}
+
 // ---------------------------------------------------------------------------
 // V-cycle kernels
 // ---------------------------------------------------------------------------
-// level 0 red-black Gauss-Seidel (colour = parity of i+j+k).  first: x = 0
-// everywhere, then the colour-0 sweep (which then only reads zeros).
-__global__ void __launch_bounds__(kBlock) k_l0_smooth(Dims d, Tiles t, const CgState* __restrict__ st,
-                                                      const uint8_t* __restrict__ code, const float* __restrict__ rin,
-                                                      float* __restrict__ x, int color, int first) {
-	if (st->done) return;
-	CellPos p = tile_cell(t);
-	if (p.i >= d.nx || p.j >= d.ny) return;
-	const long long c = d.idx(p.i, p.j, p.k);
-	const int col = (p.i + p.j + p.k) & 1;
-	const uint8_t cc = code[c];
-	if (first) {
-		x[c] = (col == 0 && l0_active(cc)) ? (rin[c] * l0_scale(cc)) / l0_diag(cc) : 0.f;
-		return;
-	}
-	if (col != color || !l0_active(cc)) return;
-	float sum = rin[c] * l0_scale(cc);
-	if (p.i > 0 && (code[c - 1] & 8)) sum += x[c - 1];
-	if (p.i < d.nx - 1 && (cc & 8)) sum += x[c + 1];
-	if (p.j > 0 && (code[c - d.nx] & 16)) sum += x[c - d.nx];
-	if (p.j < d.ny - 1 && (cc & 16)) sum += x[c + d.nx];
-	if (d.is3d) {
-		if (p.k > 0 && (code[c - d.sz] & 32)) sum += x[c - d.sz];
-		if (p.k < d.nz - 1 && (cc & 32)) sum += x[c + d.sz];
-	}
-	x[c] = sum / l0_diag(cc);
+// Coarse-level pointwise kernels run on tiles of 64x4 vertices of one plane
+// (a thread per vertex, rows along x).
+long long lc_tiles(const LvDev& c) { return (long long)((c.nx + 63) / 64) * ((c.ny + 3) / 4) * c.nz; }
+struct VtxPos {
+	int i, j, k;
+	bool in;
+};
+__device__ inline VtxPos lc_vertex(const LvDev& c, long long blk) {
+	const int nbx = (c.nx + 63) / 64, nby = (c.ny + 3) / 4;
+	const int b = (int)blk;
+	VtxPos p;
+	p.i = (b % nbx) * 64 + (threadIdx.x & 63);
+	p.j = ((b / nbx) % nby) * 4 + (threadIdx.x >> 6);
+	p.k = b / (nbx * nby);
+	p.in = p.i < c.nx && p.j < c.ny;
+	return p;
 }
 
-// level 0 residual r0 = b - A x (0 on inactive vertices)
-__global__ void __launch_bounds__(kBlock) k_l0_residual(Dims d, Tiles t, const CgState* __restrict__ st,
-                                                        const uint8_t* __restrict__ code,
-                                                        const float* __restrict__ rin, const float* __restrict__ x,
-                                                        float* __restrict__ r0) {
-	if (st->done) return;
-	CellPos p = tile_cell(t);
-	if (p.i >= d.nx || p.j >= d.ny) return;
-	const long long c = d.idx(p.i, p.j, p.k);
-	const uint8_t cc = code[c];
-	if (!l0_active(cc)) {
-		r0[c] = 0.f;
-		return;
-	}
-	float sum = rin[c] * l0_scale(cc);
-	if (p.i > 0 && (code[c - 1] & 8)) sum += x[c - 1];
-	if (p.i < d.nx - 1 && (cc & 8)) sum += x[c + 1];
-	if (p.j > 0 && (code[c - d.nx] & 16)) sum += x[c - d.nx];
-	if (p.j < d.ny - 1 && (cc & 16)) sum += x[c + d.nx];
-	if (d.is3d) {
-		if (p.k > 0 && (code[c - d.sz] & 32)) sum += x[c - d.sz];
-		if (p.k < d.nz - 1 && (cc & 32)) sum += x[c + d.sz];
-	}
-	sum -= l0_diag(cc) * x[c];
-	r0[c] = sum;
+// Coarse-level stencil coefficients are FP32, or BF16 (upper 16 bits of the
+// FP32 value, round-to-nearest-even) in the mixed-precision mode; the
+// arithmetic is FP32 either way.
+__device__ inline float ldA(const float* __restrict__ A, long long i) { return A[i]; }
+__device__ inline float ldA(const uint16_t* __restrict__ A, long long i) {
+	return __uint_as_float((unsigned)A[i] << 16);
+}
+__host__ __device__ inline uint16_t to_bf16(float f) {
+	unsigned u;
+	memcpy(&u, &f, 4);
+	u += 0x7FFFu + ((u >> 16) & 1u);
+	return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float from_bf16(uint16_t h) {
+	const unsigned u = (unsigned)h << 16;
+	float f;
+	memcpy(&f, &u, 4);
+	return f;
+}
+__global__ void k_to_bf16(const float* __restrict__ in, uint16_t* __restrict__ out, long long n) {
+	for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+		out[i] = to_bf16(in[i]);
 }
 
-// b_c[V] = sum_{U in [2V-1,2V+1]} w(U) r_f[U]   (r_f is 0 on inactive vertices)
-__global__ void k_mg_restrict(const CgState* __restrict__ st, LvDev c, LvDev f, int is3d,
-                              const uint8_t* __restrict__ ctype, const float* __restrict__ rf, float* __restrict__ bc) {
+// b_c[V] = sum_{U in [2V-1,2V+1]} w(U) r_f[U]   (r_f is 0 on inactive vertices).
+// Fixed 3x3x3 (3x3) trip counts with predicates: the loops unroll and all
+// fine loads are in flight together; skipped terms are exact zeros.
+template <bool IS3D>
+__device__ void body_restrict(const RestrictP& P, long long blk, long long) {
+	const CgState* __restrict__ st = P.st;
+	const LvDev& c = P.c;
+	const LvDev& f = P.f;
+	const uint8_t* __restrict__ ctype = P.ctype;
+	const float* __restrict__ rf = P.rf;
+	float* __restrict__ bc = P.bc;
 	if (st->done) return;
-	for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < c.n;
-	     v += (long long)gridDim.x * blockDim.x) {
-		if (!ctype[v]) {
-			bc[v] = 0.f;
-			continue;
-		}
-		const int i = (int)(v % c.nx), j = (int)((v / c.nx) % c.ny), k = (int)(v / c.sz);
-		float sum = 0.f;
-		const int k0 = is3d ? max(0, 2 * k - 1) : 0, k1 = is3d ? min(f.nz - 1, 2 * k + 1) : 0;
-		for (int uz = k0; uz <= k1; uz++)
-			for (int uy = max(0, 2 * j - 1); uy <= min(f.ny - 1, 2 * j + 1); uy++)
-				for (int ux = max(0, 2 * i - 1); ux <= min(f.nx - 1, 2 * i + 1); ux++)
-					sum += wgt(ux, uy, uz) * rf[ux + f.sy * uy + f.sz * uz];
-		bc[v] = sum;
-	}
-}
-
-// coarse-level Gauss-Seidel on one colour (parity triple); first: zero x and
-// sweep colour 0 from zeros
-__global__ void k_lc_smooth(const CgState* __restrict__ st, LvDev c, int is3d, int S, const float* __restrict__ A,
-                            const uint8_t* __restrict__ type, const float* __restrict__ b, float* __restrict__ x,
-                            int cx, int cy, int cz, int first) {
-	if (st->done) return;
-	const int C = S - 1;
-	if (first) {
-		for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < c.n;
-		     v += (long long)gridDim.x * blockDim.x) {
-			const int i = (int)(v % c.nx), j = (int)((v / c.nx) % c.ny), k = (int)(v / c.sz);
-			const bool col0 = ((i & 1) == 0) && ((j & 1) == 0) && ((k & 1) == 0);
-			x[v] = (col0 && type[v]) ? b[v] / A[v] : 0.f;
-		}
-		return;
-	}
-	const int hx = (c.nx - cx + 1) / 2, hy = (c.ny - cy + 1) / 2, hz = (c.nz - cz + 1) / 2;
-	const long long m = (long long)hx * hy * hz;
-	for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < m; q += (long long)gridDim.x * blockDim.x) {
-		const int i = 2 * (int)(q % hx) + cx, j = 2 * (int)((q / hx) % hy) + cy, k = 2 * (int)(q / ((long long)hx * hy)) + cz;
-		const long long v = i + c.sy * j + c.sz * k;
-		if (!type[v]) continue;
-		float sum = b[v];
-		const int dz0 = is3d ? -1 : 0, dz1 = is3d ? 1 : 0;
-		for (int dz = dz0; dz <= dz1; dz++)
+	const VtxPos p = lc_vertex(c, blk);
+	if (!p.in) return;
+	const int i = p.i, j = p.j, k = p.k;
+	const long long v = i + c.sy * j + c.sz * k;
+	float sum = 0.f;
+	if (ctype[v]) {
+#pragma unroll
+		for (int dz = IS3D ? -1 : 0; dz <= (IS3D ? 1 : 0); dz++)
+#pragma unroll
 			for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
 				for (int dx = -1; dx <= 1; dx++) {
-					const int s = is3d ? (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1) : (dx + 1) + 3 * (dy + 1);
-					if (s == C) continue;
-					const int ni = i + dx, nj = j + dy, nk = k + dz;
-					if (ni < 0 || nj < 0 || nk < 0 || ni >= c.nx || nj >= c.ny || nk >= c.nz) continue;
-					const long long nn = ni + c.sy * nj + c.sz * nk;
-					if (!type[nn]) continue;
-					const float a = s > C ? A[(long long)(s - C) * c.n + v] : A[(long long)(C - s) * c.n + nn];
-					sum -= a * x[nn];
+					const int ux = 2 * i + dx, uy = 2 * j + dy, uz = 2 * k + dz;
+					if (ux >= 0 && uy >= 0 && uz >= 0 && ux < f.nx && uy < f.ny && uz < f.nz)
+						sum += wgt(ux, uy, uz) * rf[ux + f.sy * uy + f.sz * uz];
 				}
-		x[v] = sum / A[v];
 	}
+	bc[v] = sum;
 }
 
-__global__ void k_lc_residual(const CgState* __restrict__ st, LvDev c, int is3d, int S, const float* __restrict__ A,
-                              const uint8_t* __restrict__ type, const float* __restrict__ b,
-                              const float* __restrict__ x, float* __restrict__ r) {
+// r = b - A x on a coarse level (0 on inactive vertices)
+template <bool IS3D, typename AT>
+__device__ void body_resid(const ResidP& P, long long blk, long long) {
+	const CgState* __restrict__ st = P.st;
+	const LvDev& c = P.c;
+	const AT* __restrict__ A = (const AT*)P.A;
+	const uint8_t* __restrict__ type = P.type;
+	const float* __restrict__ b = P.b;
+	const float* __restrict__ x = P.x;
+	float* __restrict__ r = P.r;
+	const Canon& cn = P.cn;
+	constexpr int C = IS3D ? 13 : 4;
 	if (st->done) return;
-	const int C = S - 1;
-	for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < c.n;
-	     v += (long long)gridDim.x * blockDim.x) {
-		if (!type[v]) {
-			r[v] = 0.f;
-			continue;
-		}
-		const int i = (int)(v % c.nx), j = (int)((v / c.nx) % c.ny), k = (int)(v / c.sz);
-		float sum = b[v];
-		const int dz0 = is3d ? -1 : 0, dz1 = is3d ? 1 : 0;
-		for (int dz = dz0; dz <= dz1; dz++)
+	const VtxPos p = lc_vertex(c, blk);
+	if (!p.in) return;
+	const int i = p.i, j = p.j, k = p.k;
+	const long long v = i + c.sy * j + c.sz * k;
+	const uint8_t tv = type[v];
+	if (!tv) {
+		r[v] = 0.f;
+		return;
+	}
+	float sum = b[v];
+	if (tv == 2) {
+		// canonical stencil, every neighbour inside and active
+#pragma unroll
+		for (int dz = IS3D ? -1 : 0; dz <= (IS3D ? 1 : 0); dz++)
+#pragma unroll
 			for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
 				for (int dx = -1; dx <= 1; dx++) {
-					const int s = is3d ? (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1) : (dx + 1) + 3 * (dy + 1);
-					const int ni = i + dx, nj = j + dy, nk = k + dz;
-					if (ni < 0 || nj < 0 || nk < 0 || ni >= c.nx || nj >= c.ny || nk >= c.nz) continue;
-					const long long nn = ni + c.sy * nj + c.sz * nk;
-					if (!type[nn]) continue;
-					const float a = s >= C ? A[(long long)(s - C) * c.n + v] : A[(long long)(C - s) * c.n + nn];
-					sum -= a * x[nn];
+					const int s = (dx + 1) + 3 * (dy + 1) + (IS3D ? 9 * (dz + 1) : 0);
+					sum -= cn.a[s] * x[v + dx + c.sy * dy + c.sz * dz];
 				}
 		r[v] = sum;
+		return;
 	}
-}
-
-// x_f[V] += w(V) * sum_{I in [V/2,(V+1)/2], active} x_c[I] on active fine vertices
-__global__ void k_mg_prolong(const CgState* __restrict__ st, LvDev f, LvDev c, int is3d,
-                             const uint8_t* __restrict__ fcode, const uint8_t* __restrict__ ftype,
-                             const uint8_t* __restrict__ ctype, const float* __restrict__ xc, float* __restrict__ xf) {
-	if (st->done) return;
-	for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < f.n;
-	     v += (long long)gridDim.x * blockDim.x) {
-		const bool act = fcode ? l0_active(fcode[v]) : ftype[v] != 0;
-		if (!act) continue;
-		const int i = (int)(v % f.nx), j = (int)((v / f.nx) % f.ny), k = (int)(v / f.sz);
-		float sum = 0.f;
-		for (int nz = k / 2; nz <= (k + 1) / 2; nz++)
-			for (int ny = j / 2; ny <= (j + 1) / 2; ny++)
-				for (int nx = i / 2; nx <= (i + 1) / 2; nx++) {
-					const long long q = nx + c.sy * ny + c.sz * nz;
-					if (ctype[q]) sum += xc[q];
-				}
-		xf[v] += wgt(i, j, k) * sum;
-	}
+#pragma unroll
+	for (int dz = IS3D ? -1 : 0; dz <= (IS3D ? 1 : 0); dz++)
+#pragma unroll
+		for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+			for (int dx = -1; dx <= 1; dx++) {
+				const int s = (dx + 1) + 3 * (dy + 1) + (IS3D ? 9 * (dz + 1) : 0);
+				const int ni = i + dx, nj = j + dy, nk = k + dz;
+				if (ni < 0 || nj < 0 || nk < 0 || ni >= c.nx || nj >= c.ny || nk >= c.nz) continue;
+				const long long nn = ni + c.sy * nj + c.sz * nk;
+				if (!type[nn]) continue;
+				const float a = s >= C ? ldA(A, (long long)(s - C) * c.n + v) : ldA(A, (long long)(C - s) * c.n + nn);
+				sum -= a * x[nn];
+			}
+	r[v] = sum;
 }
 
 // ---------------------------------------------------------------------------
@@ -344,9 +359,9 @@ struct Brick {
 	static constexpr int HZ = BZ > 1 ? H : 0;
 };
 
-__device__ inline void brick_origin(int nx, int ny, int BX, int BY, int BZ, int& ox, int& oy, int& oz) {
+__device__ inline void brick_origin(long long blk, int nx, int ny, int BX, int BY, int BZ, int& ox, int& oy,
+                                    int& oz) {
 	const int nbx = (nx + BX - 1) / BX, nby = (ny + BY - 1) / BY;
-	const long long blk = xcd_tile(blockIdx.x, gridDim.x);
 	ox = (int)(blk % nbx) * BX;
 	const long long rest = blk / nbx;
 	oy = (int)(rest % nby) * BY;
@@ -375,16 +390,20 @@ __device__ inline float l0_nb_sum(const uint8_t* cs, const float* xs, int q, int
 // values), then the residual r0 = b - A x of the owned cells.
 // Reads b (4 B) and the matrix code (1 B), writes x and r0: 13 B/cell.
 template <int BX, int BY, int BZ, bool IS3D>
-__global__ void __launch_bounds__(1024) k_l0_down(Dims d, const CgState* __restrict__ st,
-                                                   const uint8_t* __restrict__ code, const float* __restrict__ rin,
-                                                   float* __restrict__ z, float* __restrict__ r0) {
+__device__ void body_l0_down_brick(const L0DownP& P, long long blk, long long) {
+	const Dims& d = P.d;
+	const CgState* __restrict__ st = P.st;
+	const uint8_t* __restrict__ code = P.code;
+	const float* __restrict__ rin = P.rin;
+	float* __restrict__ z = P.xout;
+	float* __restrict__ r0 = P.r0;
 	using B = Brick<BX, BY, BZ>;
 	__shared__ float xs[B::RN];
 	__shared__ float bs[B::RN];
 	__shared__ uint8_t cs[B::RN];
 	if (st->done) return;
 	int ox, oy, oz;
-	brick_origin(d.nx, d.ny, BX, BY, BZ, ox, oy, oz);
+	brick_origin(blk, d.nx, d.ny, BX, BY, BZ, ox, oy, oz);
 	ox -= B::H;
 	oy -= B::H;
 	oz -= B::HZ;
@@ -433,11 +452,16 @@ __global__ void __launch_bounds__(1024) k_l0_down(Dims d, const CgState* __restr
 // Input and output are different arrays: the halo of one brick is owned by
 // its neighbours, which write their result concurrently.
 template <int BX, int BY, int BZ, bool IS3D>
-__global__ void __launch_bounds__(1024) k_l0_up(Dims d, const CgState* __restrict__ st,
-                                                 const uint8_t* __restrict__ code, const float* __restrict__ rin,
-                                                 const float* __restrict__ xpre, float* __restrict__ z, LvDev c1,
-                                                 const uint8_t* __restrict__ ctype, const float* __restrict__ xc,
-                                                 double* __restrict__ part) {
+__device__ void body_l0_up_brick(const L0UpP& P, long long blk, long long slot) {
+	const Dims& d = P.d;
+	const CgState* __restrict__ st = P.st;
+	const uint8_t* __restrict__ code = P.code;
+	const float* __restrict__ rin = P.rin;
+	const float* __restrict__ xpre = P.xpre;
+	float* __restrict__ z = P.z;
+	const LvDev& c1 = P.c1;
+	const float* __restrict__ xc = P.xc;
+	double* __restrict__ part = P.part;
 	using B = Brick<BX, BY, BZ>;
 	// coarse values covering the region's parents (inactive coarse vertices
 	// always hold 0, so no type lookups are needed)
@@ -449,7 +473,7 @@ __global__ void __launch_bounds__(1024) k_l0_up(Dims d, const CgState* __restric
 	__shared__ double sh[16];
 	if (st->done) return;
 	int ox, oy, oz;
-	brick_origin(d.nx, d.ny, BX, BY, BZ, ox, oy, oz);
+	brick_origin(blk, d.nx, d.ny, BX, BY, BZ, ox, oy, oz);
 	ox -= B::H;
 	oy -= B::H;
 	oz -= B::HZ;
@@ -509,7 +533,7 @@ __global__ void __launch_bounds__(1024) k_l0_up(Dims d, const CgState* __restric
 		acc += (double)(x * rin[g]);
 	}
 	const double s = block_sum(acc, sh);
-	if (threadIdx.x == 0) part[3 * blockIdx.x] = s;
+	if (threadIdx.x == 0) part[3 * slot] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -528,9 +552,9 @@ struct ZTile {
 	static constexpr int TX = 64, H = 2, RX = TX + 2 * H, RY = TY + 2 * H, RP = RX * RY, NS = 5, NT = TX * TY;
 };
 
-__device__ inline void zm_origin(const Dims& d, int TY, int kchunk, int& ox, int& oy, int& k0, int& k1) {
+__device__ inline void zm_origin(long long blk, const Dims& d, int TY, int kchunk, int& ox, int& oy, int& k0,
+                                 int& k1) {
 	const int ntx = (d.nx + 63) / 64, nty = (d.ny + TY - 1) / TY;
-	const long long blk = xcd_tile(blockIdx.x, gridDim.x);
 	ox = (int)(blk % ntx) * 64;
 	const long long rest = blk / ntx;
 	oy = (int)(rest % nty) * TY;
@@ -575,17 +599,21 @@ struct ZmCells {
 
 // Down leg (same result as k_l0_down): red from x = 0, black, residual.
 template <int TY>
-__global__ void __launch_bounds__(64 * TY) k_l0_down_zm(Dims d, int kchunk, const CgState* __restrict__ st,
-                                                         const uint8_t* __restrict__ code,
-                                                         const float* __restrict__ rin, float* __restrict__ xout,
-                                                         float* __restrict__ r0) {
+__device__ void body_l0_down_zm(const L0DownP& P, long long blk, long long) {
+	const Dims& d = P.d;
+	const int kchunk = P.kchunk;
+	const CgState* __restrict__ st = P.st;
+	const uint8_t* __restrict__ code = P.code;
+	const float* __restrict__ rin = P.rin;
+	float* __restrict__ xout = P.xout;
+	float* __restrict__ r0 = P.r0;
 	using T = ZTile<TY>;
 	__shared__ float xs[T::NS][T::RP];
 	__shared__ float bs[T::NS][T::RP];
 	__shared__ uint8_t cs[T::NS][T::RP];
 	if (st->done) return;
 	int ox, oy, k0, k1;
-	zm_origin(d, TY, kchunk, ox, oy, k0, k1);
+	zm_origin(blk, d, TY, kchunk, ox, oy, k0, k1);
 	const int rx0 = ox - T::H, ry0 = oy - T::H;
 	ZmCells<T> cells(d, rx0, ry0);
 	uint8_t pc[2];
@@ -655,11 +683,17 @@ __global__ void __launch_bounds__(64 * TY) k_l0_down_zm(Dims d, int kchunk, cons
 // the partial sums of fl(z*r).  Coarse planes are staged in a 2-slot ring:
 // fine plane p needs coarse planes p>>1 and (p+1)>>1.
 template <int TY>
-__global__ void __launch_bounds__(64 * TY) k_l0_up_zm(Dims d, int kchunk, const CgState* __restrict__ st,
-                                                       const uint8_t* __restrict__ code,
-                                                       const float* __restrict__ rin, const float* __restrict__ xpre,
-                                                       float* __restrict__ z, LvDev c1, const float* __restrict__ xc,
-                                                       double* __restrict__ part) {
+__device__ void body_l0_up_zm(const L0UpP& P, long long blk, long long slot) {
+	const Dims& d = P.d;
+	const int kchunk = P.kchunk;
+	const CgState* __restrict__ st = P.st;
+	const uint8_t* __restrict__ code = P.code;
+	const float* __restrict__ rin = P.rin;
+	const float* __restrict__ xpre = P.xpre;
+	float* __restrict__ z = P.z;
+	const LvDev& c1 = P.c1;
+	const float* __restrict__ xc = P.xc;
+	double* __restrict__ part = P.part;
 	using T = ZTile<TY>;
 	constexpr int CRX = T::RX / 2 + 2, CRY = T::RY / 2 + 2, CRP = CRX * CRY;
 	__shared__ float xs[T::NS][T::RP];
@@ -669,7 +703,7 @@ __global__ void __launch_bounds__(64 * TY) k_l0_up_zm(Dims d, int kchunk, const 
 	__shared__ double sh[16];
 	if (st->done) return;
 	int ox, oy, k0, k1;
-	zm_origin(d, TY, kchunk, ox, oy, k0, k1);
+	zm_origin(blk, d, TY, kchunk, ox, oy, k0, k1);
 	const int rx0 = ox - T::H, ry0 = oy - T::H;
 	const int cx0 = rx0 >> 1, cy0 = ry0 >> 1;  // floor division
 	static_assert(CRP <= T::NT, "one coarse value per thread");
@@ -764,39 +798,48 @@ __global__ void __launch_bounds__(64 * TY) k_l0_up_zm(Dims d, int kchunk, const 
 		}
 	}
 	const double s = block_sum(acc, sh);
-	if (threadIdx.x == 0) part[3 * blockIdx.x] = s;
+	if (threadIdx.x == 0) part[3 * slot] = s;
 }
 
 // coarse-level brick: 32x4x4 vertices in 3D, 64x8 in 2D (512 threads; rows of
 // 32/64 vertices keep the per-vertex stencil loads in whole cache lines)
 constexpr int kLcBX3 = 32, kLcBY3 = 4, kLcBZ3 = 4, kLcBX2 = 64, kLcBY2 = 8;
-static unsigned lc_blocks(const MgLevel& c, bool is3d) {
-	const long long bx = is3d ? kLcBX3 : kLcBX2, by = is3d ? kLcBY3 : kLcBY2, bz = is3d ? kLcBZ3 : 1;
-	return (unsigned)(((c.nx + bx - 1) / bx) * ((c.ny + by - 1) / by) * ((c.nz + bz - 1) / bz));
-}
 
 // Coarse-level block Gauss-Seidel: one sweep over all 8 (4 in 2D) colours of
 // the 27-point (9-point) Galerkin stencil inside an LDS brick; halo values
 // are the pre-sweep ones (block-hybrid smoother).  Forward colour order for
 // pre-smoothing, reversed for post-smoothing, which keeps the V-cycle
 // symmetric.  first: start from x = 0 (xin unused).  xcp != null: prolongate
-// from the next coarser level on load (x = xin + I x_{l+1}).  The result goes
-// to xout, never to xin: neighbouring bricks read xin as their halo.
-template <bool IS3D>
-__global__ void __launch_bounds__(512) k_lc_smooth_blk(const CgState* __restrict__ st, LvDev c, const float* __restrict__ A,
-                                                       const uint8_t* __restrict__ type, const float* __restrict__ b,
-                                                       const float* __restrict__ xin, float* __restrict__ xout, int first,
-                                                       int reverse, LvDev cp, const uint8_t* __restrict__ ctp,
-                                                       const float* __restrict__ xcp) {
+// from the next coarser level on load (x = xin + I x_{l+1}); inactive coarse
+// vertices always hold 0 there, so no coarse type lookups are needed.  The
+// result goes to xout, never to xin: neighbouring bricks read xin as their
+// halo.
+template <bool IS3D, typename AT>
+__device__ void body_smooth(const SmoothP& P, long long blk, long long) {
+	const CgState* __restrict__ st = P.st;
+	const LvDev& c = P.c;
+	const AT* __restrict__ A = (const AT*)P.A;
+	const uint8_t* __restrict__ type = P.type;
+	const float* __restrict__ b = P.b;
+	const float* __restrict__ xin = P.xin;
+	float* __restrict__ xout = P.xout;
+	const int first = P.first, reverse = P.reverse;
+	const LvDev& cp = P.cp;
+	const float* __restrict__ xcp = P.xcp;
+	const Canon& cn = P.cn;
 	constexpr int BX = IS3D ? kLcBX3 : kLcBX2, BY = IS3D ? kLcBY3 : kLcBY2, BZ = IS3D ? kLcBZ3 : 1;
 	constexpr int RX = BX + 2, RY = BY + 2, RZ = IS3D ? BZ + 2 : 1, RN = RX * RY * RZ, RXY = RX * RY;
 	constexpr int S = IS3D ? 14 : 5, C = S - 1, NCOL = IS3D ? 8 : 4;
 	__shared__ float xs[RN];
 	if (st->done) return;
 	int ox, oy, oz;
-	brick_origin(c.nx, c.ny, BX, BY, BZ, ox, oy, oz);
+	brick_origin(blk, c.nx, c.ny, BX, BY, BZ, ox, oy, oz);
 	const int hz = IS3D ? 1 : 0;
-	for (int q = threadIdx.x; q < RN; q += blockDim.x) {
+	// fixed trip count (512 threads): the region loads of all rounds issue together
+#pragma unroll
+	for (int it = 0; it < (RN + 511) / 512; it++) {
+		const int q = threadIdx.x + 512 * it;
+		if (q >= RN) break;
 		const int gx = ox - 1 + q % RX, gy = oy - 1 + (q / RX) % RY, gz = oz - hz + q / RXY;
 		float v = 0.f;
 		if (!first && gx >= 0 && gy >= 0 && gz >= 0 && gx < c.nx && gy < c.ny && gz < c.nz) {
@@ -804,11 +847,15 @@ __global__ void __launch_bounds__(512) k_lc_smooth_blk(const CgState* __restrict
 			v = xin[g];
 			if (xcp && type[g]) {
 				float sum = 0.f;
-				for (int pz = gz / 2; pz <= (gz + 1) / 2; pz++)
-					for (int py = gy / 2; py <= (gy + 1) / 2; py++)
-						for (int px = gx / 2; px <= (gx + 1) / 2; px++) {
-							const long long pi = px + cp.sy * py + cp.sz * pz;
-							if (ctp[pi]) sum += xcp[pi];
+#pragma unroll
+				for (int ez = 0; ez < (IS3D ? 2 : 1); ez++)
+#pragma unroll
+					for (int ey = 0; ey < 2; ey++)
+#pragma unroll
+						for (int ex = 0; ex < 2; ex++) {
+							const int px = gx / 2 + ex, py = gy / 2 + ey, pz = gz / 2 + ez;
+							if (px <= (gx + 1) / 2 && py <= (gy + 1) / 2 && pz <= (gz + 1) / 2)
+								sum += xcp[px + cp.sy * py + cp.sz * pz];
 						}
 				v += wgt(gx, gy, gz) * sum;
 			}
@@ -820,13 +867,19 @@ __global__ void __launch_bounds__(512) k_lc_smooth_blk(const CgState* __restrict
 	const int gx = ox + tx, gy = oy + ty, gz = oz + tz;
 	const bool valid = gx < c.nx && gy < c.ny && gz < c.nz;
 	const long long v = valid ? gx + c.sy * gy + c.sz * gz : 0;
-	const bool act = valid && type[v];
+	const uint8_t tv = valid ? type[v] : 0;
+	const bool act = tv != 0;
 	const int me = (tx + 1) + RX * (ty + 1) + RXY * (tz + hz);
 	float a[IS3D ? 27 : 9];
 	float bv = 0.f, diag = 1.f;
-	if (act) {
+	if (tv == 2) {
 		bv = b[v];
-		diag = A[v];
+		diag = cn.a[C];
+#pragma unroll
+		for (int s = 0; s < (IS3D ? 27 : 9); s++) a[s] = s == C ? 0.f : cn.a[s];
+	} else if (act) {
+		bv = b[v];
+		diag = ldA(A, v);
 		int s = 0;
 		for (int dz = (IS3D ? -1 : 0); dz <= (IS3D ? 1 : 0); dz++)
 			for (int dy = -1; dy <= 1; dy++)
@@ -835,7 +888,7 @@ __global__ void __launch_bounds__(512) k_lc_smooth_blk(const CgState* __restrict
 					const int nx = gx + dx, ny = gy + dy, nz = gz + dz;
 					if (s != C && nx >= 0 && ny >= 0 && nz >= 0 && nx < c.nx && ny < c.ny && nz < c.nz) {
 						const long long nn = nx + c.sy * ny + c.sz * nz;
-						coef = s > C ? A[(long long)(s - C) * c.n + v] : A[(long long)(C - s) * c.n + nn];
+						coef = s > C ? ldA(A, (long long)(s - C) * c.n + v) : ldA(A, (long long)(C - s) * c.n + nn);
 					}
 					a[s] = coef;
 				}
@@ -857,22 +910,28 @@ __global__ void __launch_bounds__(512) k_lc_smooth_blk(const CgState* __restrict
 	if (valid) xout[v] = xs[me];
 }
 
-__global__ void k_fill_zero(const CgState* __restrict__ st, float* __restrict__ x, long long n) {
-	if (st->done) return;
-	for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < n; v += (long long)gridDim.x * blockDim.x)
-		x[v] = 0.f;
+__device__ void body_fill(const FillP& P, long long blk, long long) {
+	if (P.st->done) return;
+	const long long v = blk * 256 + threadIdx.x;
+	if (v < P.n) P.x[v] = 0.f;
 }
 
 // coarsest level: x[cidx[i]] = sum_j G[i][j] * b[cidx[j]] (FP64), one wave per row;
 // for a single-level hierarchy the rhs is the trivially scaled level-0 input
-__global__ void k_mg_coarsest(const CgState* __restrict__ st, int nc, const int* __restrict__ cidx,
-                              const double* __restrict__ G, const float* __restrict__ b,
-                              const uint8_t* __restrict__ code, float* __restrict__ x) {
+__device__ void body_coarsest(const CoarsestP& P, long long blk, long long) {
+	const CgState* __restrict__ st = P.st;
+	const int nc = P.nc;
+	const int* __restrict__ cidx = P.cidx;
+	const double* __restrict__ G = P.G;
+	const float* __restrict__ b = P.b;
+	const uint8_t* __restrict__ code = P.code;
+	float* __restrict__ x = P.x;
 	if (st->done) return;
 	const int lane = threadIdx.x & 63;
-	const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+	const int row = (int)blk * 4 + (threadIdx.x >> 6);
 	if (row >= nc) return;
 	double acc = 0.0;
+#pragma unroll 4
 	for (int jj = lane; jj < nc; jj += 64) {
 		const int v = cidx[jj];
 		const double bv = code ? (double)(b[v] * l0_scale(code[v])) : (double)b[v];
@@ -904,7 +963,7 @@ static unsigned nblk(long long n) {
 }
 
 void MgHierarchy::build(Scene* s, const uint8_t* code, int precision) {
-	(void)precision;
+	prec = precision;
 	const Dims& d = s->d;
 	is3d = d.is3d != 0;
 	S = is3d ? 14 : 5;
@@ -961,6 +1020,72 @@ void MgHierarchy::build(Scene* s, const uint8_t* code, int precision) {
 		                   (const float*)c.r);
 	}
 	build_coarsest(s, code);
+	// the coarsest level is solved through G; the others get a canonical stencil
+	for (int l = 1; l + 1 < (int)L.size(); l++) {
+		classify(s, l);
+		if (prec == 1) {
+			// mixed precision: smoother and residual read BF16 coefficients; the
+			// canonical stencil is rounded the same way so that every coupling
+			// has one value seen from both of its vertices (symmetric V-cycle)
+			MgLevel& c = L[l];
+			c.Ah = (uint16_t*)alloc(sizeof(uint16_t) * (size_t)(c.n * S));
+			hipLaunchKernelGGL(k_to_bf16, dim3(nblk(c.n * S)), dim3(256), 0, s->stream, (const float*)c.A, c.Ah,
+			                   c.n * S);
+			for (int q = 0; q < 27; q++) c.canon.a[q] = from_bf16(to_bf16(c.canon.a[q]));
+		}
+	}
+}
+
+// Pick the most frequent full stencil among a lattice of sample vertices
+// (all neighbours active) and mark the vertices that share it bitwise.
+void MgHierarchy::classify(Scene* s, int l) {
+	MgLevel& c = L[l];
+	c.hasCanon = false;
+	const int per = is3d ? 4 : 8;
+	std::vector<long long> smp;
+	for (int a = 0; a < per; a++)
+		for (int b = 0; b < per; b++)
+			for (int e = 0; e < (is3d ? per : 1); e++) {
+				const int i = (int)((c.nx - 2) * (a + 0.5) / per) + 1, j = (int)((c.ny - 2) * (b + 0.5) / per) + 1;
+				const int k = is3d ? (int)((c.nz - 2) * (e + 0.5) / per) + 1 : 0;
+				if (i < 1 || j < 1 || i >= c.nx - 1 || j >= c.ny - 1 || (is3d && (k < 1 || k >= c.nz - 1))) continue;
+				smp.push_back(i + c.sy * j + c.sz * k);
+			}
+	const int ns = (int)smp.size();
+	if (ns == 0) return;
+	long long* dsmp = nullptr;
+	float* dout = nullptr;
+	int* dok = nullptr;
+	MFX_HIP(hipMalloc((void**)&dsmp, sizeof(long long) * ns));
+	MFX_HIP(hipMalloc((void**)&dout, sizeof(float) * 27 * ns));
+	MFX_HIP(hipMalloc((void**)&dok, sizeof(int) * ns));
+	MFX_HIP(hipMemcpyAsync(dsmp, smp.data(), sizeof(long long) * ns, hipMemcpyHostToDevice, s->stream));
+	hipLaunchKernelGGL(k_mg_sample, dim3((ns + 63) / 64), dim3(64), 0, s->stream, lvdev(c), (int)is3d, S,
+	                   (const float*)c.A, (const uint8_t*)c.type, (const long long*)dsmp, ns, dout, dok);
+	std::vector<float> out((size_t)27 * ns);
+	std::vector<int> ok(ns);
+	MFX_HIP(hipMemcpyAsync(out.data(), dout, sizeof(float) * 27 * ns, hipMemcpyDeviceToHost, s->stream));
+	MFX_HIP(hipMemcpyAsync(ok.data(), dok, sizeof(int) * ns, hipMemcpyDeviceToHost, s->stream));
+	sync(s);
+	MFX_HIP(hipFree(dsmp));
+	MFX_HIP(hipFree(dout));
+	MFX_HIP(hipFree(dok));
+	int best = -1, bestCount = 0;
+	for (int a = 0; a < ns; a++) {
+		if (!ok[a]) continue;
+		int cnt = 0;
+		for (int b = 0; b < ns; b++)
+			if (ok[b] && !memcmp(&out[27 * (size_t)a], &out[27 * (size_t)b], sizeof(float) * 27)) cnt++;
+		if (cnt > bestCount) {
+			bestCount = cnt;
+			best = a;
+		}
+	}
+	if (best < 0) return;
+	memcpy(c.canon.a, &out[27 * (size_t)best], sizeof(float) * 27);
+	c.hasCanon = true;
+	hipLaunchKernelGGL(k_mg_classify, dim3(nblk(c.n)), dim3(256), 0, s->stream, lvdev(c), (int)is3d, S,
+	                   (const float*)c.A, c.type, c.canon);
 }
 
 // dense generalised inverse of the coarsest operator (host, FP64)
@@ -1074,85 +1199,143 @@ void MgHierarchy::build_coarsest(Scene* s, const uint8_t* code0) {
 	sync(s);
 }
 
+// ---------------------------------------------------------------------------
+// launches (single problem and work lists)
+// ---------------------------------------------------------------------------
+constexpr int kZmTY = 8;
+long long l0_tiles(const Dims& d, int* kchunk) {
+	// 3D: z-marching 64x8 column tiles over chunks of an even number of planes
+	// (about 1024 workgroups); 2D: bricks of 64x16 cells
+	const int BY = d.is3d ? kZmTY : 16;
+	const long long cols = (long long)((d.nx + 63) / 64) * ((d.ny + BY - 1) / BY);
+	int kc = (d.nz + 1) & ~1;
+	while (d.is3d && kc > 16 && cols * ((d.nz + kc - 1) / kc) < 1024) kc = ((kc / 2) + 1) & ~1;
+	*kchunk = kc;
+	return d.is3d ? cols * ((d.nz + kc - 1) / kc) : cols;
+}
+long long lcblk_tiles(const LvDev& c, bool is3d) {
+	const long long bx = is3d ? kLcBX3 : kLcBX2, by = is3d ? kLcBY3 : kLcBY2, bz = is3d ? kLcBZ3 : 1;
+	return ((c.nx + bx - 1) / bx) * ((c.ny + by - 1) / by) * ((c.nz + bz - 1) / bz);
+}
+void run_l0_down_many(hipStream_t s, bool is3d, unsigned nb, const WorkItem* items, const L0DownP* p) {
+	if (is3d) launch_many<L0DownP, body_l0_down_zm<kZmTY>, 64 * kZmTY>(s, nb, items, p);
+	else launch_many<L0DownP, body_l0_down_brick<64, 16, 1, false>, 1024>(s, nb, items, p);
+}
+void run_l0_up_many(hipStream_t s, bool is3d, unsigned nb, const WorkItem* items, const L0UpP* p) {
+	if (is3d) launch_many<L0UpP, body_l0_up_zm<kZmTY>, 64 * kZmTY>(s, nb, items, p);
+	else launch_many<L0UpP, body_l0_up_brick<64, 16, 1, false>, 1024>(s, nb, items, p);
+}
+void run_restrict_many(hipStream_t s, bool is3d, unsigned nb, const WorkItem* items, const RestrictP* p) {
+	if (is3d) launch_many<RestrictP, body_restrict<true>, 256>(s, nb, items, p);
+	else launch_many<RestrictP, body_restrict<false>, 256>(s, nb, items, p);
+}
+void run_smooth_many(hipStream_t s, bool is3d, bool bf16, unsigned nb, const WorkItem* items, const SmoothP* p) {
+	if (is3d && bf16) launch_many<SmoothP, body_smooth<true, uint16_t>, 512>(s, nb, items, p);
+	else if (is3d) launch_many<SmoothP, body_smooth<true, float>, 512>(s, nb, items, p);
+	else if (bf16) launch_many<SmoothP, body_smooth<false, uint16_t>, 512>(s, nb, items, p);
+	else launch_many<SmoothP, body_smooth<false, float>, 512>(s, nb, items, p);
+}
+void run_resid_many(hipStream_t s, bool is3d, bool bf16, unsigned nb, const WorkItem* items, const ResidP* p) {
+	if (is3d && bf16) launch_many<ResidP, body_resid<true, uint16_t>, 256>(s, nb, items, p);
+	else if (is3d) launch_many<ResidP, body_resid<true, float>, 256>(s, nb, items, p);
+	else if (bf16) launch_many<ResidP, body_resid<false, uint16_t>, 256>(s, nb, items, p);
+	else launch_many<ResidP, body_resid<false, float>, 256>(s, nb, items, p);
+}
+void run_coarsest_many(hipStream_t s, unsigned nb, const WorkItem* items, const CoarsestP* p) {
+	launch_many<CoarsestP, body_coarsest, 256>(s, nb, items, p);
+}
+void run_fill_many(hipStream_t s, unsigned nb, const WorkItem* items, const FillP* p) {
+	launch_many<FillP, body_fill, 256>(s, nb, items, p);
+}
+
+// Parameters of every V-cycle step of this hierarchy (shared by the single
+// launch sequence below and by batched work lists).
+MgHierarchy::Steps MgHierarchy::steps(const Dims& d, const uint8_t* code, const float* r, float* z, const CgState* st,
+                                      double* part) const {
+	Steps S_;
+	const int nl = (int)L.size();
+	S_.nl = nl;
+	S_.l0Tiles = l0_tiles(d, &S_.kchunk);
+	const void* noA = nullptr;
+	if (nl == 1) {
+		S_.fill = FillP{st, z, d.n};
+		S_.coarsest = CoarsestP{st, nc, cidx, G, r, code, z};
+		return S_;
+	}
+	S_.down = L0DownP{d, S_.kchunk, st, code, r, L[0].x, L[0].r};
+	for (int l = 1; l < nl; l++) {
+		const MgLevel& c = L[l];
+		const MgLevel& f = L[l - 1];
+		Steps::Level q;
+		q.restrict_ = RestrictP{st, lvdev(c), lvdev(f), c.type, f.r, c.b};
+		q.tiles = lc_tiles(lvdev(c));
+		q.blkTiles = lcblk_tiles(lvdev(c), is3d);
+		if (l < nl - 1) {
+			const void* A = c.Ah ? (const void*)c.Ah : (const void*)c.A;
+			q.pre = SmoothP{st, lvdev(c), A, c.type, c.b, nullptr, c.x, 1, 0, lvdev(c), nullptr, c.canon};
+			q.resid = ResidP{st, lvdev(c), A, c.type, c.b, c.x, c.r, c.canon};
+			const MgLevel& cc = L[l + 1];
+			q.post = SmoothP{st, lvdev(c), A, c.type, c.b, c.x, c.xp, 0, 1, lvdev(cc), l + 1 == nl - 1 ? cc.x : cc.xp,
+			                 c.canon};
+		}
+		S_.lv.push_back(q);
+	}
+	(void)noA;
+	S_.coarsest = CoarsestP{st, nc, cidx, G, L[nl - 1].b, nullptr, L[nl - 1].x};
+	const MgLevel& c1 = L[1];
+	S_.up = L0UpP{d, S_.kchunk, st, code, r, L[0].x, z, lvdev(c1), nl == 2 ? c1.x : c1.xp, part};
+	return S_;
+}
+
 int MgHierarchy::vcycle(Scene* s, const uint8_t* code, const float* r, float* z, const CgState* st) {
 	const Dims& d = s->d;
-	const int nl = (int)L.size();
-	if (nl == 1) {
-		hipLaunchKernelGGL(k_fill_zero, dim3(nblk(d.n)), dim3(256), 0, s->stream, st, z, d.n);
-		const unsigned rows = (unsigned)((nc + 3) / 4);
-		hipLaunchKernelGGL(k_mg_coarsest, dim3(rows > 0 ? rows : 1), dim3(256), 0, s->stream, st, nc,
-		                   (const int*)cidx, (const double*)G, r, code, z);
+	const Steps S_ = steps(d, code, r, z, st, s->dpart);
+	hipStream_t q = s->stream;
+	const bool bf = prec == 1;
+	if (S_.nl == 1) {
+		launch_one<FillP, body_fill, 256>(q, (d.n + 255) / 256, S_.fill);
+		launch_one<CoarsestP, body_coarsest, 256>(q, coarsest_tiles(nc), S_.coarsest);
 		return 0;
 	}
-	// level-0 geometry: 3D z-marching 64x8 column tiles over chunks of an even
-	// number of planes (about 1024 workgroups); 2D bricks of 64x16 cells
-	constexpr int kZmTY = 8;
-	const int BX = 64, BY = is3d ? kZmTY : 16;
-	const long long cols = (long long)((d.nx + BX - 1) / BX) * ((d.ny + BY - 1) / BY);
-	int kchunk = (d.nz + 1) & ~1;
-	while (is3d && kchunk > 16 && cols * ((d.nz + kchunk - 1) / kchunk) < 1024) kchunk = ((kchunk / 2) + 1) & ~1;
-	const unsigned nb0 = (unsigned)(is3d ? cols * ((d.nz + kchunk - 1) / kchunk) : cols);
-	// down
 	{
-		KtScope kt("mg_l0_down", s->stream);
-		if (is3d)
-			hipLaunchKernelGGL(k_l0_down_zm<kZmTY>, dim3(nb0), dim3(64 * kZmTY), 0, s->stream, d, kchunk, st, code, r,
-			                   L[0].x, L[0].r);
-		else
-			hipLaunchKernelGGL((k_l0_down<64, 16, 1, false>), dim3(nb0), dim3(1024), 0, s->stream, d, st, code, r, L[0].x,
-			                   L[0].r);
+		KtScope kt("mg_l0_down", q);
+		if (is3d) launch_one<L0DownP, body_l0_down_zm<kZmTY>, 64 * kZmTY>(q, S_.l0Tiles, S_.down);
+		else launch_one<L0DownP, body_l0_down_brick<64, 16, 1, false>, 1024>(q, S_.l0Tiles, S_.down);
 	}
-	for (int l = 1; l < nl; l++) {
-		MgLevel& c = L[l];
-		const MgLevel& f = L[l - 1];
-		hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(c.n)), dim3(256), 0, s->stream, st, lvdev(c), lvdev(f), (int)is3d,
-		                   (const uint8_t*)c.type, (const float*)f.r, c.b);
-		if (l == nl - 1) break;
-		const unsigned nbl = lc_blocks(c, is3d);
-		if (is3d)
-			hipLaunchKernelGGL(k_lc_smooth_blk<true>, dim3(nbl), dim3(512), 0, s->stream, st, lvdev(c), (const float*)c.A,
-			                   (const uint8_t*)c.type, (const float*)c.b, (const float*)nullptr, c.x, 1, 0, lvdev(c),
-			                   (const uint8_t*)nullptr, (const float*)nullptr);
-		else
-			hipLaunchKernelGGL(k_lc_smooth_blk<false>, dim3(nbl), dim3(512), 0, s->stream, st, lvdev(c), (const float*)c.A,
-			                   (const uint8_t*)c.type, (const float*)c.b, (const float*)nullptr, c.x, 1, 0, lvdev(c),
-			                   (const uint8_t*)nullptr, (const float*)nullptr);
-		hipLaunchKernelGGL(k_lc_residual, dim3(nblk(c.n)), dim3(256), 0, s->stream, st, lvdev(c), (int)is3d, S,
-		                   (const float*)c.A, (const uint8_t*)c.type, (const float*)c.b, (const float*)c.x, c.r);
+	const int nlv = (int)S_.lv.size();
+	for (int i = 0; i < nlv; i++) {
+		const Steps::Level& v = S_.lv[i];
+		if (is3d) launch_one<RestrictP, body_restrict<true>, 256>(q, v.tiles, v.restrict_);
+		else launch_one<RestrictP, body_restrict<false>, 256>(q, v.tiles, v.restrict_);
+		if (i == nlv - 1) break;
+		if (is3d && bf) {
+			launch_one<SmoothP, body_smooth<true, uint16_t>, 512>(q, v.blkTiles, v.pre);
+			launch_one<ResidP, body_resid<true, uint16_t>, 256>(q, v.tiles, v.resid);
+		} else if (is3d) {
+			launch_one<SmoothP, body_smooth<true, float>, 512>(q, v.blkTiles, v.pre);
+			launch_one<ResidP, body_resid<true, float>, 256>(q, v.tiles, v.resid);
+		} else if (bf) {
+			launch_one<SmoothP, body_smooth<false, uint16_t>, 512>(q, v.blkTiles, v.pre);
+			launch_one<ResidP, body_resid<false, uint16_t>, 256>(q, v.tiles, v.resid);
+		} else {
+			launch_one<SmoothP, body_smooth<false, float>, 512>(q, v.blkTiles, v.pre);
+			launch_one<ResidP, body_resid<false, float>, 256>(q, v.tiles, v.resid);
+		}
 	}
-	// coarsest
-	{
-		MgLevel& c = L[nl - 1];
-		const unsigned rows = (unsigned)((nc + 3) / 4);
-		hipLaunchKernelGGL(k_mg_coarsest, dim3(rows > 0 ? rows : 1), dim3(256), 0, s->stream, st, nc,
-		                   (const int*)cidx, (const double*)G, (const float*)c.b, (const uint8_t*)nullptr, c.x);
-	}
-	// up: prolongation fused into each post-smoothing sweep
-	for (int l = nl - 2; l >= 1; l--) {
-		const MgLevel& c = L[l + 1];
-		MgLevel& f = L[l];
-		const unsigned nbl = lc_blocks(f, is3d);
-		if (is3d)
-			hipLaunchKernelGGL(k_lc_smooth_blk<true>, dim3(nbl), dim3(512), 0, s->stream, st, lvdev(f), (const float*)f.A,
-			                   (const uint8_t*)f.type, (const float*)f.b, (const float*)f.x, f.xp, 0, 1, lvdev(c),
-			                   (const uint8_t*)c.type, (const float*)(l + 1 == nl - 1 ? c.x : c.xp));
-		else
-			hipLaunchKernelGGL(k_lc_smooth_blk<false>, dim3(nbl), dim3(512), 0, s->stream, st, lvdev(f), (const float*)f.A,
-			                   (const uint8_t*)f.type, (const float*)f.b, (const float*)f.x, f.xp, 0, 1, lvdev(c),
-			                   (const uint8_t*)c.type, (const float*)(l + 1 == nl - 1 ? c.x : c.xp));
+	launch_one<CoarsestP, body_coarsest, 256>(q, coarsest_tiles(nc), S_.coarsest);
+	for (int i = nlv - 2; i >= 0; i--) {
+		const Steps::Level& v = S_.lv[i];
+		if (is3d && bf) launch_one<SmoothP, body_smooth<true, uint16_t>, 512>(q, v.blkTiles, v.post);
+		else if (is3d) launch_one<SmoothP, body_smooth<true, float>, 512>(q, v.blkTiles, v.post);
+		else if (bf) launch_one<SmoothP, body_smooth<false, uint16_t>, 512>(q, v.blkTiles, v.post);
+		else launch_one<SmoothP, body_smooth<false, float>, 512>(q, v.blkTiles, v.post);
 	}
 	{
-		KtScope kt("mg_l0_up", s->stream);
-		const MgLevel& c = L[1];
-		const float* xc1 = nl == 2 ? c.x : c.xp;
-		if (is3d)
-			hipLaunchKernelGGL(k_l0_up_zm<kZmTY>, dim3(nb0), dim3(64 * kZmTY), 0, s->stream, d, kchunk, st, code, r,
-			                   (const float*)L[0].x, z, lvdev(c), xc1, s->dpart);
-		else
-			hipLaunchKernelGGL((k_l0_up<64, 16, 1, false>), dim3(nb0), dim3(1024), 0, s->stream, d, st, code, r,
-			                   (const float*)L[0].x, z, lvdev(c), (const uint8_t*)c.type, (const float*)xc1, s->dpart);
+		KtScope kt("mg_l0_up", q);
+		if (is3d) launch_one<L0UpP, body_l0_up_zm<kZmTY>, 64 * kZmTY>(q, S_.l0Tiles, S_.up);
+		else launch_one<L0UpP, body_l0_up_brick<64, 16, 1, false>, 1024>(q, S_.l0Tiles, S_.up);
 	}
-	return (int)nb0;
+	return (int)S_.l0Tiles;
 }
 
 }  // namespace mfx

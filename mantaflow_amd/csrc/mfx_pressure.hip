@@ -31,6 +31,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "mfx_batch.h"
 #include "mfx_internal.h"
 #include "mfx_mg.h"
 #include "mfx_prof.h"
@@ -201,16 +202,15 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(Dims d, const float* __restr
 	}
 }
 
-// sum fl(z*r) (GridDotProduct(mTmp, mResidual)) into part[3*b]
-__global__ void __launch_bounds__(kBlock) k_dot_zr(Dims d, const CgState* __restrict__ st, const float* __restrict__ z,
-                                                   const float* __restrict__ r, double* __restrict__ part) {
+// sum fl(z*r) (GridDotProduct(mTmp, mResidual)) of 256 cells into part[3*slot]
+__device__ void body_dot(const DotP& P, long long blk, long long slot) {
 	__shared__ double sh[8];
-	if (st->done) return;
-	const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+	if (P.st->done) return;
+	const long long c = blk * kBlock + threadIdx.x;
 	double a = 0;
-	if (c < d.n) a = (double)(z[c] * r[c]);
+	if (c < P.d.n) a = (double)(P.z[c] * P.r[c]);
 	const double s = block_sum(a, sh);
-	if (threadIdx.x == 0) part[3 * blockIdx.x] = s;
+	if (threadIdx.x == 0) P.part[3 * slot] = s;
 }
 
 // A: p_new = z + beta*p_old ; q = A p_new ; partial sums of fl(q*p_new).
@@ -219,21 +219,24 @@ __global__ void __launch_bounds__(kBlock) k_dot_zr(Dims d, const CgState* __rest
 // LDS, the planes below and above stay in registers, so every value is read
 // from HBM once.  Row arithmetic and term order follow apply_row; neighbours
 // outside the grid contribute 0 (their coefficient is 0 there).
-constexpr int kMvTY = 8;
 template <int TY>
-__global__ void __launch_bounds__(64 * TY) k_cg_matvec25(Dims d, int kchunk, const CgState* __restrict__ st,
-                                                         const uint8_t* __restrict__ code, const float* __restrict__ z,
-                                                         const float* __restrict__ pold, float* __restrict__ pnew,
-                                                         float* __restrict__ q, double* __restrict__ part) {
+__device__ void body_matvec(const MatvecP& P, long long blk, long long slot) {
 	constexpr int TX = 64, RX = TX + 2, RY = TY + 2, NH = 2 * RX + 2 * (RY - 2);
 	__shared__ float ps[RX * RY];
 	__shared__ uint8_t cs[RX * RY];
 	__shared__ double sh[16];
+	const Dims& d = P.d;
+	const int kchunk = P.kchunk;
+	const CgState* __restrict__ st = P.st;
+	const uint8_t* __restrict__ code = P.code;
+	const float* __restrict__ z = P.z;
+	const float* __restrict__ pold = P.pold;
+	float* __restrict__ pnew = P.pnew;
+	float* __restrict__ q = P.q;
 	if (st->done) return;
 	const float beta = st->beta;
 	const bool first = st->first;
 	const int ntx = (d.nx + TX - 1) / TX, nty = (d.ny + TY - 1) / TY;
-	const long long blk = xcd_tile(blockIdx.x, gridDim.x);
 	const int ox = (int)(blk % ntx) * TX;
 	const long long rest = blk / ntx;
 	const int oy = (int)(rest % nty) * TY;
@@ -314,24 +317,29 @@ __global__ void __launch_bounds__(64 * TY) k_cg_matvec25(Dims d, int kchunk, con
 		cNext = cN2;
 	}
 	const double s = block_sum(acc, sh);
-	if (threadIdx.x == 0) part[3 * blockIdx.x] = s;
+	if (threadIdx.x == 0) P.part[3 * slot] = s;
 }
 
 // B: x += alpha p ; r += (-alpha) q ; min/max(r), sum fl(r*r), sum r^2.
-// A bounded grid walks the arrays four elements per thread and step.
-__global__ void __launch_bounds__(kBlock) k_cg_update(Dims d, const CgState* __restrict__ st, float* __restrict__ x,
-                                                      float* __restrict__ r, const float* __restrict__ p,
-                                                      const float* __restrict__ q, int withDot,
-                                                      double* __restrict__ part, float* __restrict__ fpart) {
+// Tile blk of per (= tile count) tiles walks the float4 groups with a stride
+// of per*256 (grid-stride order: concurrent tiles touch adjacent memory); the
+// n % 4 trailing cells are spread the same way.
+__device__ void body_update(const UpdateP& P, long long blk, long long slot) {
 	__shared__ double sh[8];
 	__shared__ float shf[8];
+	const Dims& d = P.d;
+	const CgState* __restrict__ st = P.st;
+	float* __restrict__ x = P.x;
+	float* __restrict__ r = P.r;
+	const float* __restrict__ p = P.p;
+	const float* __restrict__ q = P.q;
 	if (st->done) return;
 	const float alpha = st->alpha, malpha = -alpha;
 	float mn = 3.402823466e+38f, mx = -3.402823466e+38f;
 	double zr = 0, rr = 0;
 	const long long n4 = d.n >> 2;
-	const long long stride = (long long)gridDim.x * blockDim.x;
-	for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < n4; v += stride) {
+	const long long stride = P.per * kBlock;
+	for (long long v = blk * kBlock + threadIdx.x; v < n4; v += stride) {
 		float4 xv = reinterpret_cast<const float4*>(x)[v];
 		float4 rv = reinterpret_cast<const float4*>(r)[v];
 		const float4 pv = reinterpret_cast<const float4*>(p)[v];
@@ -354,7 +362,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(Dims d, const CgState* __r
 			rr += (double)e[u] * (double)e[u];
 		}
 	}
-	for (long long c = (n4 << 2) + blockIdx.x * (long long)blockDim.x + threadIdx.x; c < d.n; c += stride) {
+	for (long long c = (n4 << 2) + blk * kBlock + threadIdx.x; c < d.n; c += stride) {
 		const float xn = x[c] + alpha * p[c];
 		const float rn = r[c] + malpha * q[c];
 		x[c] = xn;
@@ -380,10 +388,10 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(Dims d, const CgState* __r
 			a = fminf(a, shf[k]);
 			b = fmaxf(b, shf[4 + k]);
 		}
-		part[3 * blockIdx.x] = withDot ? s0 : 0.0;
-		part[3 * blockIdx.x + 1] = s1;
-		fpart[2 * blockIdx.x] = a;
-		fpart[2 * blockIdx.x + 1] = b;
+		P.part[3 * slot] = P.withDot ? s0 : 0.0;
+		P.part[3 * slot + 1] = s1;
+		P.fpart[2 * slot] = a;
+		P.fpart[2 * slot + 1] = b;
 	}
 }
 
@@ -402,10 +410,11 @@ __device__ inline void reduce_parts(const double* __restrict__ part, int nb, int
 	__syncthreads();
 }
 
-enum { FIN_INIT = 0, FIN_ALPHA = 1, FIN_RES = 2, FIN_BETA = 3 };
-
-__global__ void __launch_bounds__(1024) k_cg_finalize(CgState* st, const double* __restrict__ part,
-                                                      const float* __restrict__ fpart, int nb, int phase, int pcNone) {
+__device__ void body_finalize(const FinP& P, long long, long long) {
+	CgState* st = P.st;
+	const double* __restrict__ part = P.part;
+	const float* __restrict__ fpart = P.fpart;
+	const int nb = P.nb, phase = P.phase, pcNone = P.pcNone;
 	__shared__ double sh[16];
 	__shared__ double r0, r1;
 	__shared__ float smn[16], smx[16];
@@ -528,6 +537,112 @@ __global__ void __launch_bounds__(kBlock) k_true_residual(Dims d, Tiles t, const
 	if (threadIdx.x == 0) part[3 * blockIdx.x] = s;
 }
 
+// Residual replacement: r = b - A x with FP64 accumulation of the row sum,
+// rounded once to float, in place of the recursively updated residual.  Emits
+// the same partials as k_cg_update (sum fl(r*r) when z = r, sum r^2, min/max).
+__global__ void __launch_bounds__(kBlock) k_cg_replace_residual(Dims d, Tiles t, const CgState* __restrict__ st,
+                                                                const uint8_t* __restrict__ code,
+                                                                const float* __restrict__ b,
+                                                                const float* __restrict__ x, float* __restrict__ r,
+                                                                int withDot, double* __restrict__ part,
+                                                                float* __restrict__ fpart) {
+	__shared__ double sh[8];
+	__shared__ float shf[8];
+	if (st->done) return;
+	CellPos p = tile_cell(t);
+	float mn = 3.402823466e+38f, mx = -3.402823466e+38f;
+	double zr = 0, rr = 0;
+	if (p.i < d.nx && p.j < d.ny) {
+		const long long c = d.idx(p.i, p.j, p.k);
+		const uint8_t cc = code[c];
+		double ax;
+		if (!(cc & 64)) ax = x[c];
+		else {
+			const long long X = 1, Y = d.nx, Z = d.sz, n = d.n;
+			ax = (double)x[c] * (double)(cc & 7);
+			if (c - X >= 0 && (code[c - X] & 8)) ax -= x[c - X];
+			if (c + X < n && (cc & 8)) ax -= x[c + X];
+			if (c - Y >= 0 && (code[c - Y] & 16)) ax -= x[c - Y];
+			if (c + Y < n && (cc & 16)) ax -= x[c + Y];
+			if (d.is3d) {
+				if (c - Z >= 0 && (code[c - Z] & 32)) ax -= x[c - Z];
+				if (c + Z < n && (cc & 32)) ax -= x[c + Z];
+			}
+		}
+		const float rn = (float)((double)b[c] - ax);
+		r[c] = rn;
+		mn = rn;
+		mx = rn;
+		zr = (double)(rn * rn);
+		rr = (double)rn * (double)rn;
+	}
+	const double s0 = block_sum(zr, sh);
+	const double s1 = block_sum(rr, sh);
+	mn = wave_min(mn);
+	mx = wave_max(mx);
+	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+	if (lane == 0) {
+		shf[wv] = mn;
+		shf[4 + wv] = mx;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		float a = shf[0], bb = shf[4];
+		for (int k = 1; k < 4; k++) {
+			a = fminf(a, shf[k]);
+			bb = fmaxf(bb, shf[4 + k]);
+		}
+		part[3 * blockIdx.x] = withDot ? s0 : 0.0;
+		part[3 * blockIdx.x + 1] = s1;
+		fpart[2 * blockIdx.x] = a;
+		fpart[2 * blockIdx.x + 1] = bb;
+	}
+}
+
+// ---------------------------------------------------------------------------
+// launches of the CG loop kernels (single problem and work lists)
+// ---------------------------------------------------------------------------
+long long matvec_tiles(const Dims& d, int* kchunk) {
+	// 64 x kMvTY column tiles, chunks of planes sized so that about 1024
+	// workgroups cover the grid
+	const long long cols = (long long)((d.nx + 63) / 64) * ((d.ny + kMvTY - 1) / kMvTY);
+	int kc = d.nz;
+	while (kc > 8 && cols * ((d.nz + kc - 1) / kc) < 1024) kc = (kc + 1) / 2;
+	*kchunk = kc;
+	return cols * ((d.nz + kc - 1) / kc);
+}
+long long update_tiles(const Dims& d, long long* per) {
+	// at most 2048 tiles; *per = tile count (the stride in 256-thread units)
+	long long nt = std::min<long long>(2048, ((d.n >> 2) + kBlock - 1) / kBlock);
+	if (nt < 1) nt = 1;
+	*per = nt;
+	return nt;
+}
+void run_matvec_one(hipStream_t s, unsigned nb, const MatvecP& p) {
+	hipLaunchKernelGGL((k_one<MatvecP, body_matvec<kMvTY>, 64 * kMvTY>), dim3(nb), dim3(64 * kMvTY), 0, s, p);
+}
+void run_matvec_many(hipStream_t s, unsigned nb, const WorkItem* items, const MatvecP* p) {
+	hipLaunchKernelGGL((k_many<MatvecP, body_matvec<kMvTY>, 64 * kMvTY>), dim3(nb), dim3(64 * kMvTY), 0, s, items, p);
+}
+void run_update_one(hipStream_t s, unsigned nb, const UpdateP& p) {
+	hipLaunchKernelGGL((k_one<UpdateP, body_update, kBlock>), dim3(nb), dim3(kBlock), 0, s, p);
+}
+void run_update_many(hipStream_t s, unsigned nb, const WorkItem* items, const UpdateP* p) {
+	hipLaunchKernelGGL((k_many<UpdateP, body_update, kBlock>), dim3(nb), dim3(kBlock), 0, s, items, p);
+}
+void run_finalize_one(hipStream_t s, const FinP& p) {
+	hipLaunchKernelGGL((k_one<FinP, body_finalize, 1024>), dim3(1), dim3(1024), 0, s, p);
+}
+void run_finalize_many(hipStream_t s, unsigned nprob, const WorkItem* items, const FinP* p) {
+	hipLaunchKernelGGL((k_many<FinP, body_finalize, 1024>), dim3(nprob), dim3(1024), 0, s, items, p);
+}
+void run_dot_one(hipStream_t s, unsigned nb, const DotP& p) {
+	hipLaunchKernelGGL((k_one<DotP, body_dot, kBlock>), dim3(nb), dim3(kBlock), 0, s, p);
+}
+void run_dot_many(hipStream_t s, unsigned nb, const WorkItem* items, const DotP* p) {
+	hipLaunchKernelGGL((k_many<DotP, body_dot, kBlock>), dim3(nb), dim3(kBlock), 0, s, items, p);
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -581,159 +696,228 @@ static void compute_rhs(Scene* s, Grid* flags, Grid* vel, float* rhs, int enforc
 	}
 }
 
-struct SolveWork {
-	float *r = nullptr, *p0 = nullptr, *p1 = nullptr, *q = nullptr, *z = nullptr;
+// One pressure-system solve in three phases.  The batched solver runs the
+// same set-up and finish per problem (each problem owns its scene, hence its
+// partial-sum buffers and device CgState) and drives the iterations of all
+// problems through work lists.
+struct CgRun {
+	Scene* s = nullptr;
+	Grid* flags = nullptr;
+	float* rhs = nullptr;
+	float* x = nullptr;
+	mfx_pressure_opts o{};
+	std::chrono::steady_clock::time_point t0;
+	Tiles t{};
+	unsigned nbT = 0, nbF = 0, nbM = 0, nbU = 0;
+	int kchunk = 1;
+	long long per = 0;
+	long long fix = -1;
 	uint8_t* code = nullptr;
+	int pc = 0, maxIter = 0;
+	bool pcNone = true;
+	MgHierarchy* mg = nullptr;
+	std::unique_ptr<MgHierarchy> mgDyn;
+	float *r = nullptr, *pa = nullptr, *pb = nullptr, *q = nullptr, *z = nullptr;
+	CgState* st = nullptr;  // device
+	int launched = 0;
+	MatvecP mvp(bool odd) const {
+		MatvecP p{s->d, kchunk, st, code, z, odd ? pb : pa, odd ? pa : pb, q, s->dpart};
+		return p;
+	}
+	UpdateP upd(bool odd) const {
+		UpdateP p{s->d, per, st, x, r, odd ? pa : pb, q, pcNone ? 1 : 0, s->dpart, s->fpart};
+		return p;
+	}
+	FinP fin(int nb, int phase) const {
+		FinP p{st, s->dpart, s->fpart, nb, phase, pcNone ? 1 : 0};
+		return p;
+	}
 };
 
-// pressure system solve: rhs (modified by fixPressure), pressure out
-static void solve_system(Scene* s, Grid* flags, float* rhs, float* x, const mfx_pressure_opts& o,
-                         mfx_solve_stats* stats) {
-	auto t0 = std::chrono::steady_clock::now();
+// matrix, preconditioner, buffers, CgState and GridCg::doInit
+// (conjugategrad.cpp:210-235, pressure.cpp:349-433)
+static void cg_setup(CgRun& R, Scene* s, Grid* flags, float* rhs, float* x, const mfx_pressure_opts& o) {
+	R.t0 = std::chrono::steady_clock::now();
+	R.s = s;
+	R.flags = flags;
+	R.rhs = rhs;
+	R.x = x;
+	R.o = o;
 	const Dims& d = s->d;
-	const Tiles t = tiles_for(d);
-	const unsigned nbT = (unsigned)t.count();
-	const unsigned nbF = flat_count(d.n);
-	const unsigned nbMax = nbT > nbF ? nbT : nbF;
+	R.t = tiles_for(d);
+	R.nbT = (unsigned)R.t.count();
+	R.nbF = flat_count(d.n);
+	R.nbM = (unsigned)matvec_tiles(d, &R.kchunk);
+	R.nbU = (unsigned)update_tiles(d, &R.per);
+	const unsigned nbMax = std::max(R.nbT, R.nbF);
 	s->ensurePartials(3 * (size_t)nbMax + 16, 2 * (size_t)nbMax + 16);
-	// z-marching matvec: 64 x kMvTY column tiles, chunks of planes sized so
-	// that about 1024 workgroups cover the grid
-	const long long cols = (long long)((d.nx + 63) / 64) * ((d.ny + kMvTY - 1) / kMvTY);
-	int kchunk = d.nz;
-	while (kchunk > 8 && cols * ((d.nz + kchunk - 1) / kchunk) < 1024) kchunk = (kchunk + 1) / 2;
-	const unsigned nbM = (unsigned)(cols * ((d.nz + kchunk - 1) / kchunk));
-	// update: grid-stride over float4 groups, bounded partial count
-	const unsigned nbU = (unsigned)std::max(1LL, std::min<long long>(2048, (d.n / 4 + kBlock - 1) / kBlock));
 
 	// zero-pressure fixing (pressure.cpp:349-388)
-	long long fix = -1;
 	if (o.zeroPressureFixing || o.cgAccuracy < 1e-07f) {
-		fix = find_fix_cell(s, flags);
-		if (fix >= 0) {
+		R.fix = find_fix_cell(s, flags);
+		if (R.fix >= 0) {
 			const float zero = 0.f;
-			MFX_HIP(hipMemcpyAsync(rhs + fix, &zero, sizeof(float), hipMemcpyHostToDevice, s->stream));
+			MFX_HIP(hipMemcpyAsync(rhs + R.fix, &zero, sizeof(float), hipMemcpyHostToDevice, s->stream));
 		}
 	}
 	// matrix
 	if (!s->code) MFX_HIP(hipMalloc((void**)&s->code, (size_t)d.n));
-	uint8_t* code = s->code;
-	hipLaunchKernelGGL(k_build_code, dim3(nbT), dim3(kBlock), 0, s->stream, d, t, flags->f8, code, fix);
+	R.code = s->code;
+	hipLaunchKernelGGL(k_build_code, dim3(R.nbT), dim3(kBlock), 0, s->stream, d, R.t, flags->f8, R.code, R.fix);
 
 	// preconditioner choice (pressure.cpp:404-433, conjugategrad.cpp:302-336)
 	int pc = o.preconditioner;
 	int mx = d.nx > d.ny ? d.nx : d.ny;
 	if (d.nz > mx) mx = d.nz;
-	int maxIter;
 	if (pc == MFX_PC_NONE || pc == MFX_PC_MIC) {
-		maxIter = (int)(o.cgMaxIterFac * (float)mx) * (d.is3d ? 1 : 4);
+		R.maxIter = (int)(o.cgMaxIterFac * (float)mx) * (d.is3d ? 1 : 4);
 		// mIC only exists in 3D in the reference; the GPU runs the parallel
 		// multigrid preconditioner in its place
 		pc = (pc == MFX_PC_MIC && d.is3d) ? MFX_PC_MGSTATIC : MFX_PC_NONE;
 	} else {
-		maxIter = 100;
+		R.maxIter = 100;
 	}
-	if (o.maxIterOverride > 0) maxIter = o.maxIterOverride;
-	const bool pcNone = pc == MFX_PC_NONE;
+	if (o.maxIterOverride > 0) R.maxIter = o.maxIterOverride;
+	R.pc = pc;
+	R.pcNone = pc == MFX_PC_NONE;
 
-	MgHierarchy* mg = nullptr;
-	std::unique_ptr<MgHierarchy> mgDyn;
-	if (!pcNone) {
+	if (!R.pcNone) {
 		if (o.preconditioner == MFX_PC_MGDYNAMIC) {
 			s->mgStatic.reset();
-			mgDyn.reset(new MgHierarchy());
-			mgDyn->build(s, code, o.precision);
-			mg = mgDyn.get();
+			R.mgDyn.reset(new MgHierarchy());
+			R.mgDyn->build(s, R.code, o.precision);
+			R.mg = R.mgDyn.get();
 		} else {
-			if (!s->mgStatic) {
+			if (!s->mgStatic || s->mgStatic->precision() != o.precision) {
 				s->mgStatic.reset(new MgHierarchy());
-				s->mgStatic->build(s, code, o.precision);
+				s->mgStatic->build(s, R.code, o.precision);
 			}
-			mg = s->mgStatic.get();
+			R.mg = s->mgStatic.get();
 		}
-		mg->setTrivial(code);
 	}
 
-	float* r = s->getReal();
-	float* pa = s->getReal();
-	float* pb = s->getReal();
-	float* q = s->getReal();
-	float* z = pcNone ? r : s->getReal();
+	R.r = s->getReal();
+	R.pa = s->getReal();
+	R.pb = s->getReal();
+	R.q = s->getReal();
+	R.z = R.pcNone ? R.r : s->getReal();
 
 	CgState hst;
 	memset(&hst, 0, sizeof(hst));
 	hst.acc = o.cgAccuracy;
-	hst.maxIter = maxIter;
+	hst.maxIter = R.maxIter;
 	hst.useL2 = o.useL2Norm;
 	hst.relTol = o.relTol;
-	CgState* st = (CgState*)s->dscal;
-	MFX_HIP(hipMemcpyAsync(st, &hst, sizeof(CgState), hipMemcpyHostToDevice, s->stream));
+	R.st = (CgState*)s->dscal;
+	MFX_HIP(hipMemcpyAsync(R.st, &hst, sizeof(CgState), hipMemcpyHostToDevice, s->stream));
 
-	// doInit (conjugategrad.cpp:210-235)
-	hipLaunchKernelGGL(k_cg_init, dim3(nbF), dim3(kBlock), 0, s->stream, d, (const float*)rhs, x, r, pcNone ? 1 : 0,
-	                   s->dpart);
-	if (!pcNone) {
-		// bnorm from the init partials, sigma from dot(z, r) after one V-cycle
-		hipLaunchKernelGGL(k_cg_finalize, dim3(1), dim3(1024), 0, s->stream, st, (const double*)s->dpart,
-		                   (const float*)s->fpart, (int)nbF, (int)FIN_INIT, 0);
-		int np = mg->vcycle(s, code, r, z, st);
+	// doInit: x = 0, r = b, bnorm; preconditioned: z = M r, sigma = dot(z, r)
+	hipLaunchKernelGGL(k_cg_init, dim3(R.nbF), dim3(kBlock), 0, s->stream, d, (const float*)rhs, x, R.r,
+	                   R.pcNone ? 1 : 0, s->dpart);
+	run_finalize_one(s->stream, FinP{R.st, s->dpart, s->fpart, (int)R.nbF, FIN_INIT, 0});
+	if (!R.pcNone) {
+		int np = R.mg->vcycle(s, R.code, R.r, R.z, R.st);
 		if (np == 0) {
-			hipLaunchKernelGGL(k_dot_zr, dim3(nbF), dim3(kBlock), 0, s->stream, d, (const CgState*)st,
-			                   (const float*)z, (const float*)r, s->dpart);
-			np = (int)nbF;
+			run_dot_one(s->stream, R.nbF, DotP{d, R.st, R.z, R.r, s->dpart});
+			np = (int)R.nbF;
 		}
-		// sigma = dot(z, r); keep bnorm2
-		hipLaunchKernelGGL(k_cg_finalize, dim3(1), dim3(1024), 0, s->stream, st, (const double*)s->dpart,
-		                   (const float*)s->fpart, np, (int)FIN_BETA, 0);
-	} else {
-		hipLaunchKernelGGL(k_cg_finalize, dim3(1), dim3(1024), 0, s->stream, st, (const double*)s->dpart,
-		                   (const float*)s->fpart, (int)nbF, (int)FIN_INIT, 0);
-	}
-	if (!pcNone) {
-		// FIN_BETA above set sigma = dot(z,r); restart the search direction
+		run_finalize_one(s->stream, FinP{R.st, s->dpart, s->fpart, np, FIN_BETA, 0});
+		// FIN_BETA set sigma = dot(z, r); restart the search direction
 		static const int one = 1;
-		MFX_HIP(hipMemcpyAsync(&st->first, &one, sizeof(int), hipMemcpyHostToDevice, s->stream));
+		MFX_HIP(hipMemcpyAsync(&R.st->first, &one, sizeof(int), hipMemcpyHostToDevice, s->stream));
 	}
+	R.launched = 0;
+}
 
-	// iterations in batches; the kernels stop on st->done
-	int launched = 0;
+// GridCg::iterate (conjugategrad.cpp:238-299), queued on the scene stream
+static void cg_iteration(CgRun& R) {
+	Scene* s = R.s;
+	const bool odd = (R.launched & 1) != 0;
+	{
+		KtScope kt("cg_matvec", s->stream);
+		run_matvec_one(s->stream, R.nbM, R.mvp(odd));
+	}
+	run_finalize_one(s->stream, R.fin((int)R.nbM, FIN_ALPHA));
+	{
+		KtScope kt("cg_update", s->stream);
+		run_update_one(s->stream, R.nbU, R.upd(odd));
+	}
+	int nbRes = (int)R.nbU;
+	if (R.o.trueResidualEvery > 0 && (R.launched + 1) % R.o.trueResidualEvery == 0) {
+		// FP64 residual replacement; its partials replace the update's
+		hipLaunchKernelGGL(k_cg_replace_residual, dim3(R.nbT), dim3(kBlock), 0, s->stream, s->d, R.t,
+		                   (const CgState*)R.st, R.code, (const float*)R.rhs, (const float*)R.x, R.r,
+		                   R.pcNone ? 1 : 0, s->dpart, s->fpart);
+		nbRes = (int)R.nbT;
+	}
+	run_finalize_one(s->stream, R.fin(nbRes, FIN_RES));
+	if (!R.pcNone) {
+		int np = R.mg->vcycle(s, R.code, R.r, R.z, R.st);
+		if (np == 0) {
+			run_dot_one(s->stream, R.nbF, DotP{s->d, R.st, R.z, R.r, s->dpart});
+			np = (int)R.nbF;
+		}
+		run_finalize_one(s->stream, R.fin(np, FIN_BETA));
+	}
+	R.launched++;
+}
+
+// FP64 true residual, buffers back to the pool, statistics, divergence error
+static void cg_finish(CgRun& R, mfx_solve_stats* stats) {
+	Scene* s = R.s;
+	const Dims& d = s->d;
+	// true relative residual (FP64)
+	hipLaunchKernelGGL(k_true_residual, dim3(R.nbT), dim3(kBlock), 0, s->stream, d, R.t, R.code,
+	                   (const float*)R.rhs, (const float*)R.x, s->dpart);
+	std::vector<double> h(3 * (size_t)R.nbT);
+	CgState* hs = (CgState*)s->hscal;
+	MFX_HIP(hipMemcpyAsync(h.data(), s->dpart, sizeof(double) * 3 * R.nbT, hipMemcpyDeviceToHost, s->stream));
+	MFX_HIP(hipMemcpyAsync(hs, R.st, sizeof(CgState), hipMemcpyDeviceToHost, s->stream));
+	sync(s);
+	double rr = 0;
+	for (unsigned b = 0; b < R.nbT; b++) rr += h[3 * b];
+	const double bb = hs->bnorm2;
+
+	s->putReal(R.r);
+	s->putReal(R.pa);
+	s->putReal(R.pb);
+	s->putReal(R.q);
+	if (!R.pcNone) s->putReal(R.z);
+
+	if (stats) {
+		stats->iterations = hs->iter;
+		stats->resNorm = hs->resNorm;
+		stats->relResidual = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+		stats->fixPidx = (int)R.fix;
+		stats->maxIter = R.maxIter;
+		stats->preconditionerUsed = R.pc;
+		stats->converged = hs->converged;
+		stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - R.t0).count();
+	}
+	if (hs->diverged) {
+		if (!R.pcNone)
+			fprintf(stderr,
+			        "GridCg::iterate: Warning - this diverging solve can be caused by the 'static' mode of the MG "
+			        "preconditioner. If the static mode is active, try switching to dynamic.\n");
+		throw Error("GridCg::iterate: The CG solver diverged, residual norm > 1e30, stopping.");
+	}
+}
+
+// pressure system solve: rhs (modified by fixPressure), pressure out.
+// Iterations are queued in batches whose size is predicted from the
+// residual history; kernels stop on st->done.
+static void solve_system(Scene* s, Grid* flags, float* rhs, float* x, const mfx_pressure_opts& o,
+                         mfx_solve_stats* stats) {
+	CgRun R;
+	cg_setup(R, s, flags, rhs, x, o);
 	int batch = 4;
-	float* pcur = pa;
-	float* pnext = pb;
 	CgState* hs = (CgState*)s->hscal;
 	while (true) {
-		const int nIt = std::min(batch, maxIter - launched);
-		for (int it = 0; it < nIt; it++) {
-			{
-				KtScope kt("cg_matvec", s->stream);
-				hipLaunchKernelGGL(k_cg_matvec25<kMvTY>, dim3(nbM), dim3(64 * kMvTY), 0, s->stream, d, kchunk,
-				                   (const CgState*)st, code, (const float*)z, (const float*)pcur, pnext, q, s->dpart);
-			}
-			hipLaunchKernelGGL(k_cg_finalize, dim3(1), dim3(1024), 0, s->stream, st, (const double*)s->dpart,
-			                   (const float*)s->fpart, (int)nbM, (int)FIN_ALPHA, pcNone ? 1 : 0);
-			{
-				KtScope kt("cg_update", s->stream);
-				hipLaunchKernelGGL(k_cg_update, dim3(nbU), dim3(kBlock), 0, s->stream, d, (const CgState*)st, x, r,
-				                   (const float*)pnext, (const float*)q, pcNone ? 1 : 0, s->dpart, s->fpart);
-			}
-			hipLaunchKernelGGL(k_cg_finalize, dim3(1), dim3(1024), 0, s->stream, st, (const double*)s->dpart,
-			                   (const float*)s->fpart, (int)nbU, (int)FIN_RES, pcNone ? 1 : 0);
-			if (!pcNone) {
-				int np = mg->vcycle(s, code, r, z, st);
-				if (np == 0) {
-					hipLaunchKernelGGL(k_dot_zr, dim3(nbF), dim3(kBlock), 0, s->stream, d, (const CgState*)st,
-					                   (const float*)z, (const float*)r, s->dpart);
-					np = (int)nbF;
-				}
-				hipLaunchKernelGGL(k_cg_finalize, dim3(1), dim3(1024), 0, s->stream, st, (const double*)s->dpart,
-				                   (const float*)s->fpart, np, (int)FIN_BETA, 0);
-			}
-			std::swap(pcur, pnext);
-		}
-		launched += nIt;
-		MFX_HIP(hipMemcpyAsync(hs, st, sizeof(CgState), hipMemcpyDeviceToHost, s->stream));
+		const int nIt = std::min(batch, R.maxIter - R.launched);
+		for (int it = 0; it < nIt; it++) cg_iteration(R);
+		MFX_HIP(hipMemcpyAsync(hs, R.st, sizeof(CgState), hipMemcpyDeviceToHost, s->stream));
 		sync(s);
-		if (hs->done || launched >= maxIter) break;
-		// predict the remaining iterations from the residual history
+		if (hs->done || R.launched >= R.maxIter) break;
 		int itNow = hs->iter;
 		int predict = batch;
 		if (itNow >= 2 && itNow <= 64) {
@@ -748,49 +932,309 @@ static void solve_system(Scene* s, Grid* flags, float* rhs, float* x, const mfx_
 		}
 		batch = std::max(1, std::min(predict, 64));
 	}
-	if (!hs->done) {
-		// cap reached without the kernels flagging it (maxIter launched)
-	}
-	// true relative residual (FP64)
-	hipLaunchKernelGGL(k_true_residual, dim3(nbT), dim3(kBlock), 0, s->stream, d, t, code, (const float*)rhs,
-	                   (const float*)x, s->dpart);
-	std::vector<double> h(3 * (size_t)nbT);
-	MFX_HIP(hipMemcpyAsync(h.data(), s->dpart, sizeof(double) * 3 * nbT, hipMemcpyDeviceToHost, s->stream));
-	MFX_HIP(hipMemcpyAsync(hs, st, sizeof(CgState), hipMemcpyDeviceToHost, s->stream));
-	sync(s);
-	double rr = 0;
-	for (unsigned b = 0; b < nbT; b++) rr += h[3 * b];
-	const double bb = hs->bnorm2;
-
-	s->putReal(r);
-	s->putReal(pa);
-	s->putReal(pb);
-	s->putReal(q);
-	if (!pcNone) s->putReal(z);
-
-	if (stats) {
-		stats->iterations = hs->iter;
-		stats->resNorm = hs->resNorm;
-		stats->relResidual = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
-		stats->fixPidx = (int)fix;
-		stats->maxIter = maxIter;
-		stats->preconditionerUsed = pc;
-		stats->converged = hs->converged;
-		stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-	}
-	if (hs->diverged) {
-		if (!pcNone)
-			fprintf(stderr,
-			        "GridCg::iterate: Warning - this diverging solve can be caused by the 'static' mode of the MG "
-			        "preconditioner. If the static mode is active, try switching to dynamic.\n");
-		throw Error("GridCg::iterate: The CG solver diverged, residual norm > 1e30, stopping.");
-	}
+	cg_finish(R, stats);
 }
+
 
 static void correct_velocity(Scene* s, Grid* flags, Grid* vel, const float* pressure) {
 	const Tiles t = tiles_for(s->d);
 	hipLaunchKernelGGL(k_correct_velocity, dim3((unsigned)t.count()), dim3(kBlock), 0, s->stream, s->d, t, flags->f8,
 	                   pressure, vel->comp(0), vel->comp(1), vel->comp(2));
+}
+
+// ---------------------------------------------------------------------------
+// Batched solve: many independent pressure systems (one per scene, any mix
+// of sizes and 2D/3D).  Set-up and finish run per problem; every iteration
+// is one launch per CG/MG phase over a work list that holds the tiles of all
+// problems, so small problems share launches with large ones.  Two
+// iterations (both parities of the search-direction buffers) are captured in
+// a hipGraph and replayed until every problem has stopped; each problem keeps
+// its own device CgState, so converged problems drop out at once (their
+// workgroups return on st->done).
+// ---------------------------------------------------------------------------
+namespace {
+
+// work list in hardware order: logical items are grouped per XCD so that the
+// tiles of one problem stay on one L2
+std::vector<WorkItem> xcd_order(const std::vector<WorkItem>& logical) {
+	const long long n = (long long)logical.size();
+	std::vector<WorkItem> hw((size_t)n);
+	const long long q = n >> 3, r = n & 7;
+	for (long long wg = 0; wg < n; wg++) {
+		const long long x = wg & 7, slot = wg >> 3;
+		const long long lt = x < r ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
+		hw[(size_t)wg] = logical[(size_t)lt];
+	}
+	return hw;
+}
+
+struct DevArena {
+	std::vector<uint8_t> host;
+	size_t add(const void* p, size_t bytes) {
+		const size_t off = (host.size() + 255) & ~(size_t)255;
+		host.resize(off + bytes);
+		if (bytes) memcpy(host.data() + off, p, bytes);
+		return off;
+	}
+};
+
+template <typename P>
+struct Step {
+	std::vector<WorkItem> items;
+	std::vector<P> params;
+	size_t itemOff = 0, paramOff = 0;
+	const WorkItem* dItems = nullptr;
+	const P* dParams = nullptr;
+	explicit Step(size_t nprob = 0) : params(nprob) {}
+	void add_tiles(int prob, long long tiles) {
+		for (long long t = 0; t < tiles; t++) items.push_back(WorkItem{prob, (int)t, (int)t, 0});
+	}
+	void place(DevArena& A) {
+		items = xcd_order(items);
+		itemOff = A.add(items.data(), sizeof(WorkItem) * items.size());
+		paramOff = A.add(params.data(), sizeof(P) * params.size());
+	}
+	void bind(uint8_t* base) {
+		dItems = (const WorkItem*)(base + itemOff);
+		dParams = (const P*)(base + paramOff);
+	}
+	unsigned n() const { return (unsigned)items.size(); }
+};
+
+__global__ void k_gather_done(int np, CgState* const* __restrict__ st, int* __restrict__ done) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < np) done[i] = st[i]->done;
+}
+
+struct BatchSolve {
+	int P = 0;
+	bool bf16 = false;
+	// CG
+	Step<MatvecP> mv[2];
+	Step<UpdateP> up[2];
+	Step<FinP> fa, fr, fb;
+	// V-cycle
+	Step<FillP> fill;
+	Step<L0DownP> down[2];  // [is3d]
+	std::vector<Step<RestrictP>> rs[2];
+	std::vector<Step<SmoothP>> pre[2], post[2];
+	std::vector<Step<ResidP>> res[2];
+	Step<CoarsestP> crs;
+	Step<L0UpP> l0up[2];
+	Step<DotP> dot;
+	bool anyMg = false;
+	uint8_t* dmem = nullptr;
+	CgState** dStates = nullptr;
+	int* dDone = nullptr;
+
+	void enqueue(hipStream_t s, int parity) {
+		run_matvec_many(s, mv[parity].n(), mv[parity].dItems, mv[parity].dParams);
+		run_finalize_many(s, fa.n(), fa.dItems, fa.dParams);
+		run_update_many(s, up[parity].n(), up[parity].dItems, up[parity].dParams);
+		run_finalize_many(s, fr.n(), fr.dItems, fr.dParams);
+		if (!anyMg) return;
+		run_fill_many(s, fill.n(), fill.dItems, fill.dParams);
+		for (int d3 = 0; d3 < 2; d3++) run_l0_down_many(s, d3, down[d3].n(), down[d3].dItems, down[d3].dParams);
+		const size_t nl = std::max(rs[0].size(), rs[1].size());
+		for (size_t l = 0; l < nl; l++)
+			for (int d3 = 0; d3 < 2; d3++) {
+				if (l < rs[d3].size()) run_restrict_many(s, d3, rs[d3][l].n(), rs[d3][l].dItems, rs[d3][l].dParams);
+				if (l < pre[d3].size()) {
+					run_smooth_many(s, d3, bf16, pre[d3][l].n(), pre[d3][l].dItems, pre[d3][l].dParams);
+					run_resid_many(s, d3, bf16, res[d3][l].n(), res[d3][l].dItems, res[d3][l].dParams);
+				}
+			}
+		run_coarsest_many(s, crs.n(), crs.dItems, crs.dParams);
+		for (size_t l = nl; l-- > 0;)
+			for (int d3 = 0; d3 < 2; d3++)
+				if (l < post[d3].size())
+					run_smooth_many(s, d3, bf16, post[d3][l].n(), post[d3][l].dItems, post[d3][l].dParams);
+		for (int d3 = 0; d3 < 2; d3++) run_l0_up_many(s, d3, l0up[d3].n(), l0up[d3].dItems, l0up[d3].dParams);
+		run_dot_many(s, dot.n(), dot.dItems, dot.dParams);
+		run_finalize_many(s, fb.n(), fb.dItems, fb.dParams);
+	}
+	~BatchSolve() {
+		if (dmem) hipFree(dmem);
+	}
+};
+
+}  // namespace
+
+static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>& pres,
+                        const std::vector<Grid*>& flg, const mfx_pressure_opts& o, mfx_solve_stats* stats) {
+	const int P = (int)vel.size();
+	std::vector<float*> rhs((size_t)P, nullptr);
+	std::vector<CgRun> R((size_t)P);
+	auto release = [&]() {
+		for (int i = 0; i < P; i++)
+			if (rhs[(size_t)i]) vel[(size_t)i]->s->putReal(rhs[(size_t)i]);
+	};
+	try {
+		for (int i = 0; i < P; i++) {
+			Scene* s = vel[(size_t)i]->s;
+			rhs[(size_t)i] = s->getReal();
+			compute_rhs(s, flg[(size_t)i], vel[(size_t)i], rhs[(size_t)i], o.enforceCompatibility);
+			cg_setup(R[(size_t)i], s, flg[(size_t)i], rhs[(size_t)i], pres[(size_t)i]->real(), o);
+		}
+		for (int i = 0; i < P; i++) sync(vel[(size_t)i]->s);
+
+		BatchSolve B;
+		B.P = P;
+		B.bf16 = o.precision == 1;
+		for (int par = 0; par < 2; par++) {
+			B.mv[par] = Step<MatvecP>((size_t)P);
+			B.up[par] = Step<UpdateP>((size_t)P);
+		}
+		B.fa = Step<FinP>((size_t)P);
+		B.fr = Step<FinP>((size_t)P);
+		B.fb = Step<FinP>((size_t)P);
+		B.fill = Step<FillP>((size_t)P);
+		B.crs = Step<CoarsestP>((size_t)P);
+		B.dot = Step<DotP>((size_t)P);
+		for (int d3 = 0; d3 < 2; d3++) {
+			B.down[d3] = Step<L0DownP>((size_t)P);
+			B.l0up[d3] = Step<L0UpP>((size_t)P);
+		}
+		int maxIt = 0;
+		for (int i = 0; i < P; i++) {
+			CgRun& r = R[(size_t)i];
+			const Dims& d = r.s->d;
+			maxIt = std::max(maxIt, r.maxIter);
+			for (int par = 0; par < 2; par++) {
+				B.mv[par].params[(size_t)i] = r.mvp(par == 1);
+				B.mv[par].add_tiles(i, r.nbM);
+				B.up[par].params[(size_t)i] = r.upd(par == 1);
+				B.up[par].add_tiles(i, r.nbU);
+			}
+			B.fa.params[(size_t)i] = r.fin((int)r.nbM, FIN_ALPHA);
+			B.fa.items.push_back(WorkItem{i, 0, 0, 0});
+			B.fr.params[(size_t)i] = r.fin((int)r.nbU, FIN_RES);
+			B.fr.items.push_back(WorkItem{i, 0, 0, 0});
+			if (r.pcNone) continue;
+			B.anyMg = true;
+			const MgHierarchy::Steps S = r.mg->steps(d, r.code, r.r, r.z, r.st, r.s->dpart);
+			const int d3 = d.is3d ? 1 : 0;
+			int np;
+			if (S.nl == 1) {
+				B.fill.params[(size_t)i] = S.fill;
+				B.fill.add_tiles(i, (d.n + 255) / 256);
+				B.crs.params[(size_t)i] = S.coarsest;
+				B.crs.add_tiles(i, coarsest_tiles(S.coarsest.nc));
+				B.dot.params[(size_t)i] = DotP{d, r.st, r.z, r.r, r.s->dpart};
+				B.dot.add_tiles(i, dot_tiles(d));
+				np = (int)dot_tiles(d);
+			} else {
+				B.down[d3].params[(size_t)i] = S.down;
+				B.down[d3].add_tiles(i, S.l0Tiles);
+				const int nlv = (int)S.lv.size();
+				while ((int)B.rs[d3].size() < nlv) B.rs[d3].emplace_back((size_t)P);
+				while ((int)B.pre[d3].size() < nlv - 1) {
+					B.pre[d3].emplace_back((size_t)P);
+					B.res[d3].emplace_back((size_t)P);
+					B.post[d3].emplace_back((size_t)P);
+				}
+				for (int l = 0; l < nlv; l++) {
+					B.rs[d3][(size_t)l].params[(size_t)i] = S.lv[(size_t)l].restrict_;
+					B.rs[d3][(size_t)l].add_tiles(i, S.lv[(size_t)l].tiles);
+					if (l == nlv - 1) break;
+					B.pre[d3][(size_t)l].params[(size_t)i] = S.lv[(size_t)l].pre;
+					B.pre[d3][(size_t)l].add_tiles(i, S.lv[(size_t)l].blkTiles);
+					B.res[d3][(size_t)l].params[(size_t)i] = S.lv[(size_t)l].resid;
+					B.res[d3][(size_t)l].add_tiles(i, S.lv[(size_t)l].tiles);
+					B.post[d3][(size_t)l].params[(size_t)i] = S.lv[(size_t)l].post;
+					B.post[d3][(size_t)l].add_tiles(i, S.lv[(size_t)l].blkTiles);
+				}
+				B.crs.params[(size_t)i] = S.coarsest;
+				B.crs.add_tiles(i, coarsest_tiles(S.coarsest.nc));
+				B.l0up[d3].params[(size_t)i] = S.up;
+				B.l0up[d3].add_tiles(i, S.l0Tiles);
+				np = (int)S.l0Tiles;
+			}
+			B.fb.params[(size_t)i] = r.fin(np, FIN_BETA);
+			B.fb.items.push_back(WorkItem{i, 0, 0, 0});
+		}
+		// one device block for every list and parameter array
+		DevArena A;
+		auto placeAll = [&](auto&& f) {
+			for (int par = 0; par < 2; par++) {
+				f(B.mv[par]);
+				f(B.up[par]);
+			}
+			f(B.fa);
+			f(B.fr);
+			f(B.fb);
+			f(B.fill);
+			f(B.crs);
+			f(B.dot);
+			for (int d3 = 0; d3 < 2; d3++) {
+				f(B.down[d3]);
+				f(B.l0up[d3]);
+				for (auto& x : B.rs[d3]) f(x);
+				for (auto& x : B.pre[d3]) f(x);
+				for (auto& x : B.res[d3]) f(x);
+				for (auto& x : B.post[d3]) f(x);
+			}
+		};
+		placeAll([&](auto& st) { st.place(A); });
+		std::vector<CgState*> states((size_t)P);
+		for (int i = 0; i < P; i++) states[(size_t)i] = R[(size_t)i].st;
+		const size_t stOff = A.add(states.data(), sizeof(CgState*) * (size_t)P);
+		std::vector<int> zeros((size_t)P, 0);
+		const size_t doneOff = A.add(zeros.data(), sizeof(int) * (size_t)P);
+		Scene* s0 = vel[0]->s;
+		hipStream_t q = s0->stream;
+		MFX_HIP(hipMalloc((void**)&B.dmem, A.host.size()));
+		MFX_HIP(hipMemcpyAsync(B.dmem, A.host.data(), A.host.size(), hipMemcpyHostToDevice, q));
+		placeAll([&](auto& st) { st.bind(B.dmem); });
+		B.dStates = (CgState**)(B.dmem + stOff);
+		B.dDone = (int*)(B.dmem + doneOff);
+		sync(s0);
+
+		// capture two iterations and replay
+		hipGraph_t graph = nullptr;
+		hipGraphExec_t exec = nullptr;
+		MFX_HIP(hipStreamBeginCapture(q, hipStreamCaptureModeThreadLocal));
+		B.enqueue(q, 0);
+		B.enqueue(q, 1);
+		MFX_HIP(hipStreamEndCapture(q, &graph));
+		MFX_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+		std::vector<int> done((size_t)P, 0);
+		int its = 0;
+		int* hdone = (int*)s0->hscal;  // pinned, 4 KiB
+		const bool pinned = (size_t)P * sizeof(int) <= 4096;
+		while (its < maxIt) {
+			MFX_HIP(hipGraphLaunch(exec, q));
+			its += 2;
+			hipLaunchKernelGGL(k_gather_done, dim3((P + 255) / 256), dim3(256), 0, q, P, (CgState* const*)B.dStates,
+			                   B.dDone);
+			MFX_HIP(hipMemcpyAsync(pinned ? hdone : done.data(), B.dDone, sizeof(int) * (size_t)P,
+			                       hipMemcpyDeviceToHost, q));
+			sync(s0);
+			const int* dn = pinned ? hdone : done.data();
+			bool all = true;
+			for (int i = 0; i < P && all; i++) all = dn[i] != 0;
+			if (all) break;
+		}
+		MFX_HIP(hipGraphExecDestroy(exec));
+		MFX_HIP(hipGraphDestroy(graph));
+		for (int i = 0; i < P; i++) R[(size_t)i].launched = its;
+		// finish, correct velocities (each problem on its own stream)
+		std::string err;
+		for (int i = 0; i < P; i++) {
+			try {
+				cg_finish(R[(size_t)i], stats ? &stats[i] : nullptr);
+			} catch (const Error& e) {
+				if (err.empty()) err = e.what();
+			}
+			correct_velocity(vel[(size_t)i]->s, flg[(size_t)i], vel[(size_t)i], pres[(size_t)i]->real());
+		}
+		for (int i = 0; i < P; i++) sync(vel[(size_t)i]->s);
+		release();
+		if (!err.empty()) throw Error(err);
+	} catch (...) {
+		release();
+		throw;
+	}
 }
 
 }  // namespace mfx
@@ -864,6 +1308,23 @@ int mfx_solve_pressure(mfx_grid* vel, mfx_grid* pressure, const mfx_grid* flags,
 		throw;
 	}
 	s->putReal(rhs);
+	API_END
+}
+
+int mfx_solve_pressure_batch(int count, mfx_grid* const* vel, mfx_grid* const* pressure,
+                             const mfx_grid* const* flags, const mfx_pressure_opts* opts, mfx_solve_stats* stats) {
+	API_BEGIN
+	MFX_CHECK(count >= 1 && vel && pressure && flags, "solvePressureBatch: needs count >= 1 and the grid arrays");
+	std::vector<Grid*> v((size_t)count), p((size_t)count), f((size_t)count);
+	for (int i = 0; i < count; i++) {
+		v[(size_t)i] = G(vel[i]);
+		p[(size_t)i] = G(pressure[i]);
+		f[(size_t)i] = G(flags[i]);
+		check_pressure_args(v[(size_t)i], p[(size_t)i], f[(size_t)i]);
+		for (int j = 0; j < i; j++)
+			MFX_CHECK(v[(size_t)j]->s != v[(size_t)i]->s, "solvePressureBatch: each problem needs its own solver");
+	}
+	solve_batch(v, p, f, opts_or_default(opts), stats);
 	API_END
 }
 

@@ -258,9 +258,11 @@ class FluidSolver:
         self._grids = []
         self._solve_stats = None
         # pressure-solve extensions (not kwargs of the reference): relative
-        # stopping criterion and precision mode
+        # stopping criterion, precision mode (1 = BF16 multigrid smoother
+        # coefficients) and FP64 residual replacement every k iterations
         self.mfxRelTol = 0.0
         self.mfxPrecision = 0
+        self.mfxTrueResidualEvery = 0
 
     # timestep property keeps the device copy in sync (Real = float32)
     @property
@@ -760,6 +762,7 @@ def _opts(a, solver):
     o.zeroPressureFixing = int(bool(a["zeroPressureFixing"]))
     o.relTol = float(getattr(solver, "mfxRelTol", 0.0) or 0.0)
     o.precision = int(getattr(solver, "mfxPrecision", 0) or 0)
+    o.trueResidualEvery = int(getattr(solver, "mfxTrueResidualEvery", 0) or 0)
     return o
 
 
@@ -782,6 +785,28 @@ def solvePressure(*args, **kwargs):
     check(lib().mfx_solve_pressure(_h(a["vel"]), _h(a["pressure"]), _h(a["flags"]), C.byref(o), C.byref(st),
                                    _h(a["retRhs"])))
     _report(solver, st)
+
+
+def solvePressureBatch(problems, **kwargs):
+    """Extension (no reference counterpart): solvePressure for many
+    independent (vel, pressure, flags) problems at once, each on its own
+    Solver; the keyword arguments are solvePressure's and apply to all.  The
+    first problem's solver supplies mfxRelTol / mfxPrecision.  Sets every
+    solver's lastSolveStats."""
+    problems = list(problems)
+    if not problems:
+        return
+    a = _parse(_PRESSURE_SIG, (), kwargs)
+    solver = problems[0][0].parent
+    o = _opts(a, solver)
+    n = len(problems)
+    V = (C.c_void_p * n)(*[_h(v) for v, _, _ in problems])
+    Pp = (C.c_void_p * n)(*[_h(p) for _, p, _ in problems])
+    F = (C.c_void_p * n)(*[_h(f) for _, _, f in problems])
+    st = (_abi.SolveStats * n)()
+    check(lib().mfx_solve_pressure_batch(n, V, Pp, F, C.byref(o), st))
+    for i, (v, _, _) in enumerate(problems):
+        _report(v.parent, st[i])
 
 
 @_plugin

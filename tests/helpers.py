@@ -85,3 +85,47 @@ def assert_bitexact(a, b, what=""):
         raise AssertionError("%s: %d of %d differ, first %s: gpu=%s oracle=%s" % (
             what, int((~same).sum()), same.size, idx.tolist(),
             [a[tuple(i)] for i in idx], [b[tuple(i)] for i in idx]))
+
+
+def obstacle_flags(o, shape, frac, rng):
+    """initDomain() walls plus seeded boxes and spheres stamped into the
+    interior until `frac` of the interior cells are obstacles, then fillGrid
+    (the 256^3 mixed-precision projection workload, at any size)."""
+    f = o.init_domain(shape)
+    nx, ny, nz = shape
+    g = f.reshape(nz, ny, nx)
+    zz, yy, xx = np.meshgrid(np.arange(nz) + 0.5, np.arange(ny) + 0.5, np.arange(nx) + 0.5, indexing="ij")
+    is3d = nz > 1
+    inner = np.zeros(g.shape, bool)
+    if is3d:
+        inner[1:-1, 1:-1, 1:-1] = True
+    else:
+        inner[:, 1:-1, 1:-1] = True
+    target = frac * inner.sum()
+    rmax = max(2.0, min(s for s in shape if s > 1) / 8.0)
+    while ((g & OBSTACLE) != 0)[inner].sum() < target:
+        c = rng.uniform(1, np.array([nx - 1, ny - 1, max(nz - 1, 1.0001)]))
+        dz = (zz - c[2]) if is3d else 0.0
+        if rng.random() < 0.5:
+            h = rng.uniform(1.0, rmax, 3)
+            m = (np.abs(xx - c[0]) <= h[0]) & (np.abs(yy - c[1]) <= h[1]) & (np.abs(dz) <= h[2])
+        else:
+            r = rng.uniform(1.5, rmax)
+            m = (xx - c[0]) ** 2 + (yy - c[1]) ** 2 + dz ** 2 < r * r
+        g[m & inner] = OBSTACLE
+    o.fill_grid(shape, f)
+    return f
+
+
+def component_demean(p, flags, shape):
+    """Pressure modulo one constant per connected fluid region (6-connected)."""
+    from scipy import ndimage
+    nx, ny, nz = shape
+    fl = (flags.reshape(nz, ny, nx) & FLUID) != 0
+    lab, n = ndimage.label(fl)
+    q = p.reshape(nz, ny, nx).astype(np.float64).copy()
+    for c in range(1, n + 1):
+        m = lab == c
+        q[m] -= q[m].mean()
+    q[~fl] = 0.0
+    return q

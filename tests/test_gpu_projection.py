@@ -1,0 +1,83 @@
+"""Projection on a domain with ~15 % seeded box/sphere obstacles and a
+smoothed random divergent velocity (the 256^3 mixed-precision workload at
+48^3, where the FP64 oracle runs in a second).  The GPU solve in FP32 and in
+mixed precision (BF16 multigrid smoother coefficients, FP64 dots/scalars,
+FP64 residual replacement) is compared with the double-precision oracle
+solved to 1e-12: velocity error after the correction, and pressure error
+modulo one constant per fluid region."""
+import numpy as np
+import pytest
+
+from helpers import GpuScene, component_demean, obstacle_flags, smooth_vel
+from oracle import OBSTACLE, PC_MGSTATIC, PC_NONE
+
+pytestmark = pytest.mark.gpu
+
+SHAPE = (48, 48, 48)
+
+
+@pytest.fixture(scope="module")
+def case(oracle64):
+    rng = np.random.default_rng(3)
+    flags = obstacle_flags(oracle64, SHAPE, 0.15, rng)
+    vel = smooth_vel(np.random.default_rng(4), SHAPE, passes=3)
+    v64 = vel.astype(np.float64)
+    oracle64.set_wall_bcs(SHAPE, flags, v64)
+    vel = v64.astype(np.float32)
+    ref_v = vel.astype(np.float64)
+    ref_p = np.zeros(int(np.prod(SHAPE)), np.float64)
+    st = oracle64.solve_pressure(SHAPE, ref_v, ref_p, flags, cgAccuracy=1e-12, preconditioner=PC_NONE,
+                                 maxIterOverride=20000)
+    assert st["relResidual"] < 1e-10
+    return flags, vel, ref_v, ref_p
+
+
+def test_obstacle_fraction(case):
+    flags = case[0]
+    nx, ny, nz = SHAPE
+    g = flags.reshape(nz, ny, nx)[1:-1, 1:-1, 1:-1]
+    frac = ((g & OBSTACLE) != 0).mean()
+    assert 0.15 <= frac < 0.25
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("tol", [1e-4, 1e-6])
+def test_projection_vs_fp64_oracle(manta, case, precision, tol):
+    flags, vel, ref_v, ref_p = case
+    g = GpuScene(manta, SHAPE)
+    f = g.flags(flags)
+    v = g.mac(vel)
+    p = g.real()
+    g.s.mfxRelTol = tol
+    g.s.mfxPrecision = precision
+    g.s.mfxTrueResidualEvery = 5
+    manta.solvePressure(flags=f, vel=v, pressure=p, preconditioner=manta.PcMGStatic)
+    st = manta.lastSolveStats(g.s)
+    assert st["converged"] and st["relResidual"] <= tol, st
+    uv = v.toNumpy().reshape(-1, 3).astype(np.float64)
+    scale = np.abs(ref_v).max()
+    verr = np.abs(uv - ref_v).max() / scale
+    perr = np.abs(component_demean(p.toNumpy().ravel(), flags, SHAPE) -
+                  component_demean(ref_p, flags, SHAPE)).max() / np.abs(component_demean(ref_p, flags, SHAPE)).max()
+    print("precision=%d tol=%g: iterations %d, rel.res %.3g, velocity error %.3g, pressure error %.3g "
+          "(relative to max |ref|)" % (precision, tol, st["iterations"], st["relResidual"], verr, perr))
+    # the error is bounded by the conditioning times the residual; FP32
+    # storage of u and p puts a floor of a few 1e-7
+    assert verr < (3e-2 if tol >= 1e-4 else 1e-3)
+    assert perr < (3e-2 if tol >= 1e-4 else 1e-3)
+
+
+def test_mixed_precision_keeps_iteration_count(manta, case):
+    """BF16 smoother coefficients change the preconditioner only slightly."""
+    flags, vel = case[0], case[1]
+    its = []
+    for precision in (0, 1):
+        g = GpuScene(manta, SHAPE)
+        f = g.flags(flags)
+        v = g.mac(vel)
+        p = g.real()
+        g.s.mfxRelTol = 1e-6
+        g.s.mfxPrecision = precision
+        manta.solvePressure(flags=f, vel=v, pressure=p, preconditioner=manta.PcMGStatic)
+        its.append(manta.lastSolveStats(g.s)["iterations"])
+    assert its[1] <= its[0] + 2, its
