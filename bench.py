@@ -45,6 +45,24 @@ KERNEL_BYTES = {
 }
 
 
+def measured_traffic(kernel, n):
+    """HBM bytes per launch of `kernel` at n^3 from the committed PMC summary
+    (profiles/plume256_traffic.md: FETCH_SIZE and WRITE_SIZE per dispatch,
+    each scaled by the same counter's reading for a plain copy of known
+    size), or None when no summary covers this kernel and size."""
+    path = os.path.join(ROOT, "profiles", "plume256_traffic.md")
+    if n != 256 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.strip().strip("|").split("|")
+        if len(parts) >= 5 and parts[0].strip() == "`%s`" % kernel:
+            try:
+                return float(parts[4])
+            except ValueError:
+                return None
+    return None
+
+
 def plume_setup(manta, n, seed=1):
     s = manta.Solver(name="bench", gridSize=manta.vec3(n, n, n), dim=3)
     s.timestep = 1.0
@@ -130,6 +148,66 @@ def cpu_baseline(n_sample=128, steps=3, threads=None, warm=10):
                       "the reference), %.1f s timed" % (steps, n, warm, dt)}
 
 
+def bench_batched(manta, n2d=32, n3d=32):
+    """64 uneven pressure problems: one batched solve vs the problems solved
+    one after another on the plain path (same inputs, rel. residual 1e-4)."""
+    from mantaflow_amd import workloads
+    probs = workloads.batched_problems(n2d, n3d)
+    kw = dict(cgAccuracy=1e-4, preconditioner=manta.PcMGStatic)
+    for p in probs:  # build the static multigrid hierarchies
+        manta.solvePressure(flags=p.flags, vel=p.vel, pressure=p.pressure, **kw)
+    for p in probs:
+        p.reset()
+        p.s.sync()
+    t0 = time.perf_counter()
+    manta.solvePressureBatch([p.problem() for p in probs], **kw)
+    tb = time.perf_counter() - t0
+    its = [manta.lastSolveStats(p.s)["iterations"] for p in probs]
+    rel = max(manta.lastSolveStats(p.s)["relResidual"] for p in probs)
+    for p in probs:
+        p.reset()
+        p.s.sync()
+    t0 = time.perf_counter()
+    for p in probs:
+        manta.solvePressure(flags=p.flags, vel=p.vel, pressure=p.pressure, **kw)
+        p.s.sync()
+    ts = time.perf_counter() - t0
+    cells = sum(p.cells for p in probs)
+    return {"problems": len(probs), "cells": cells, "batched_ms": round(tb * 1e3, 3),
+            "serial_ms": round(ts * 1e3, 3), "batched_solves_per_s": round(len(probs) / tb, 1),
+            "serial_solves_per_s": round(len(probs) / ts, 1), "speedup": round(ts / tb, 2),
+            "iterations_max": int(max(its)), "rel_residual_max": rel}
+
+
+def bench_serving(n_sessions=24, n_req=240, rate=200.0):
+    """Poisson stream of plume-step / pressure-solve requests on resident
+    sessions (64^2 .. 192^3 cells, log-uniform): served (continuous batching)
+    vs the plain path one request at a time; in real time at `rate` req/s
+    (latency) and with every request queued at t=0 (throughput)."""
+    from mantaflow_amd import serve
+    sessions = [serve.PlumeScene(sh, seed=20 + i) for i, sh in enumerate(serve.make_sessions(n_sessions))]
+    for sess in sessions:  # warm: multigrid hierarchies, pools
+        sess.run_single(serve.REQ_STEP)
+    trace = serve.make_trace(n_req, n_sessions, rate=rate)
+    single = []
+    for sess in sessions[:8]:
+        t0 = time.perf_counter()
+        sess.run_single(serve.REQ_SOLVE)
+        single.append(time.perf_counter() - t0)
+    out = {"sessions": n_sessions, "requests": n_req, "rate_req_per_s": rate,
+           "isolated_solve_ms_median": round(float(np.median(single)) * 1e3, 3)}
+    for name, realtime in (("realtime", True), ("saturated", False)):
+        srv = serve.Server(sessions)
+        lat, span = srv.run(trace, realtime=realtime)
+        out["served_" + name] = {k: round(v, 3) for k, v in serve.summarize(lat, span).items()}
+        out["served_" + name]["mean_batch"] = round(float(np.mean(srv.batch_sizes)), 2)
+        lat, span = serve.serve_serial(sessions, trace, realtime=realtime)
+        out["serial_" + name] = {k: round(v, 3) for k, v in serve.summarize(lat, span).items()}
+    out["throughput_speedup_saturated"] = round(out["served_saturated"]["solves_per_s"] /
+                                                out["serial_saturated"]["solves_per_s"], 2)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,6 +217,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-res", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-problem batched-solve leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the served request-stream leg")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run 5 plain 64 MiB device copies first (PMC byte-counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build runs the grid path on a single MI355X "
@@ -152,6 +234,13 @@ def main():
     n = a.res
     st = plume_setup(manta, n)
     s = st["s"]
+    if a.calibrate_copy:
+        src = s.create(manta.RealGrid)
+        dst = s.create(manta.RealGrid)
+        src.setConst(1.0)
+        for _ in range(5):
+            dst.copyFrom(src)  # n^3 floats read + written (64 MiB each at 256^3)
+        s.sync()
     for _ in range(a.warmup):
         plume_step(manta, st)
     s.sync()
@@ -189,11 +278,27 @@ def main():
         algo = KERNEL_BYTES[best] * cells
         achieved = algo / avg_s / 1e9
         roof = {"kernel": best, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(best, n),
                 "algo_bytes_per_launch": algo, "avg_launch_ms": round(kstats[best]["avg_ms"], 4)}
+
+    batched = served = None
+    if not a.no_batched:
+        print("bench: batched solves ...", file=sys.stderr, flush=True)
+        try:
+            batched = bench_batched(manta)
+        except Exception as e:
+            batched = {"error": str(e)}
+    if not a.no_serving:
+        print("bench: serving ...", file=sys.stderr, flush=True)
+        try:
+            served = bench_serving()
+        except Exception as e:
+            served = {"error": str(e)}
 
     cpu = None
     if not a.no_cpu_baseline:
+        print("bench: CPU baseline ...", file=sys.stderr, flush=True)
         try:
             cpu = cpu_baseline(a.cpu_res, a.cpu_steps)
         except Exception as e:  # the baseline must not hide the GPU number
@@ -211,6 +316,8 @@ def main():
                    "rel_residual_max": float(np.max(rel)), "parallelism": "single GPU"},
         "roofline": roof,
         "kernels": kstats,
+        "batched_solves": batched,
+        "serving": served,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

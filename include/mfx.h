@@ -56,6 +56,9 @@ enum { MFX_SHAPE_BOX = 1, MFX_SHAPE_SPHERE = 2, MFX_SHAPE_CYLINDER = 3 };
  * the reference's wrappers turn into RuntimeError, source/pwrapper/pclass.cpp:57-61) */
 const char* mfx_last_error(void);
 int mfx_device_count(int* out);
+/* Free and total device memory of the current GPU (hipMemGetInfo); the
+ * serving loop admits work by it.  No counterpart in the reference. */
+int mfx_device_memory(size_t* freeBytes, size_t* totalBytes);
 
 /* ---------------------------------------------------------------- scenes */
 /* FluidSolver(gridSize, dim) (source/fluidsolver.cpp:107-115) */

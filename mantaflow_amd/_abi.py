@@ -36,6 +36,7 @@ _FP = C.POINTER(C.c_float)
 SIGNATURES = {
     "mfx_last_error": (C.c_char_p, []),
     "mfx_device_count": (_I, [C.POINTER(_I)]),
+    "mfx_device_memory": (_I, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "mfx_scene_create": (_I, [_I, _I, _I, _I, C.POINTER(_P)]),
     "mfx_scene_destroy": (_I, [_P]),
     "mfx_scene_set_dt": (_I, [_P, _F]),

@@ -37,6 +37,8 @@ Scene::~Scene() {
 	if (dscal) hipFree(dscal);
 	if (code) hipFree(code);
 	if (hscal) hipHostFree(hscal);
+	if (ev) hipEventDestroy(ev);
+	if (arena) hipFree(arena);
 	if (stream) hipStreamDestroy(stream);
 }
 
@@ -677,6 +679,15 @@ int mfx_device_count(int* out) {
 	int c = 0;
 	if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
 	*out = c;
+	API_END
+}
+
+int mfx_device_memory(size_t* freeBytes, size_t* totalBytes) {
+	API_BEGIN
+	size_t f = 0, t = 0;
+	MFX_HIP(hipMemGetInfo(&f, &t));
+	if (freeBytes) *freeBytes = f;
+	if (totalBytes) *totalBytes = t;
 	API_END
 }
 

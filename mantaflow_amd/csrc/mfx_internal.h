@@ -85,6 +85,9 @@ struct Scene {
 	size_t fpartCap = 0;
 	void* dscal = nullptr;      // device scalar block
 	void* hscal = nullptr;      // pinned host mirror
+	hipEvent_t ev = nullptr;    // cross-stream ordering (batched solves)
+	void* arena = nullptr;      // work lists of batched solves led by this scene
+	size_t arenaCap = 0;
 	std::unique_ptr<MgHierarchy> mgStatic;
 	uint8_t* code = nullptr;    // per-cell Laplace matrix code of the current solve
 	std::vector<Grid*> grids;

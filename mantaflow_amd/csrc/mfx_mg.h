@@ -14,6 +14,7 @@ struct CgState {
 	float sigma, alpha, beta, resNorm, acc;
 	int iter, maxIter, done, converged, diverged, useL2, first, pad;
 	float hist[64];  // residual norm per iteration (batch sizing)
+	double trueRR;   // ||b - A x||^2 with FP64 row sums, after the solve
 };
 
 // canonical full stencil of a coarse level (27 entries in 3D, 9 in 2D,

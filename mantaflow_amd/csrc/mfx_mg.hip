@@ -42,25 +42,58 @@ __device__ inline float l0_scale(uint8_t c) { return (c & 128) ? 1e-6f : 1.0f; }
 // ---------------------------------------------------------------------------
 // hierarchy construction
 // ---------------------------------------------------------------------------
-// coarse vertex active iff an active fine vertex lies in [2V-1, 2V+1]
+// Coarse vertices, a parallel restatement of genCoarseGrid
+// (multigrid.cpp:520-578).  There, a fine vertex with one undecided parent
+// left activates it, and a fine vertex popped with several free parents
+// activates the first and removes the others, so each fine vertex adds at
+// most one parent beyond the "standard" ones.  Here, in two passes:
+//  1. standard: V is active iff its injection point, fine vertex 2V, is active;
+//  2. every active fine vertex activates its first (loop-order) parent that
+//     is not standard.
+// Parents of walls and obstacles (fine 2V inactive) are thereby active where
+// a fine vertex needs them, but a fine vertex never turns on several weak,
+// nearly dependent parents - those make the Galerkin operator ill-conditioned.
+__device__ inline bool fine_active(const uint8_t* code, const uint8_t* ftype, long long u) {
+	return code ? l0_active(code[u]) : (ftype[u] != 0);
+}
 __global__ void k_mg_activate(LvDev c, LvDev f, int is3d, const uint8_t* __restrict__ code,
                               const uint8_t* __restrict__ ftype, uint8_t* __restrict__ ctype) {
 	for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < c.n;
 	     v += (long long)gridDim.x * blockDim.x) {
 		const int i = (int)(v % c.nx), j = (int)((v / c.nx) % c.ny), k = (int)(v / c.sz);
+		const int ux = 2 * i, uy = 2 * j, uz = is3d ? 2 * k : 0;
 		bool act = false;
-		const int k0 = is3d ? max(0, 2 * k - 1) : 0, k1 = is3d ? min(f.nz - 1, 2 * k + 1) : 0;
-		for (int uz = k0; uz <= k1 && !act; uz++)
-			for (int uy = max(0, 2 * j - 1); uy <= min(f.ny - 1, 2 * j + 1) && !act; uy++)
-				for (int ux = max(0, 2 * i - 1); ux <= min(f.nx - 1, 2 * i + 1); ux++) {
-					const long long u = ux + f.sy * uy + f.sz * uz;
-					if (code ? l0_active(code[u]) : (ftype[u] != 0)) {
-						act = true;
+		if (ux < f.nx && uy < f.ny && uz < f.nz) act = fine_active(code, ftype, ux + f.sy * uy + f.sz * uz);
+		ctype[v] = act ? 1 : 0;
+	}
+}
+__global__ void k_mg_cover(LvDev c, LvDev f, int is3d, const uint8_t* __restrict__ code,
+                           const uint8_t* __restrict__ ftype, uint8_t* __restrict__ ctype) {
+	for (long long u = blockIdx.x * (long long)blockDim.x + threadIdx.x; u < f.n;
+	     u += (long long)gridDim.x * blockDim.x) {
+		if (!fine_active(code, ftype, u)) continue;
+		const int ux = (int)(u % f.nx), uy = (int)((u / f.nx) % f.ny), uz = (int)(u / f.sz);
+		long long pick = -1;
+		for (int iz = uz / 2; iz <= (uz + 1) / 2 && pick < 0; iz++)
+			for (int iy = uy / 2; iy <= (uy + 1) / 2 && pick < 0; iy++)
+				for (int ix = ux / 2; ix <= (ux + 1) / 2; ix++) {
+					if (ix >= c.nx || iy >= c.ny || iz >= c.nz) continue;
+					const long long pi = ix + c.sy * iy + c.sz * iz;
+					// pass-1 marks (1) decide "standard"; pass-2 marks (3) written by
+					// other fine vertices are not standard either, so the choice
+					// does not depend on the order of the writes
+					if (ctype[pi] != 1) {
+						pick = pi;
 						break;
 					}
 				}
-		ctype[v] = act ? 1 : 0;
+		if (pick >= 0) ctype[pick] = 3;
 	}
+}
+__global__ void k_mg_cover_done(LvDev c, uint8_t* __restrict__ ctype) {
+	for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < c.n;
+	     v += (long long)gridDim.x * blockDim.x)
+		if (ctype[v] == 3) ctype[v] = 1;
 }
 
 // fine-level matrix entry A(U, W) for W = U + off, off in [-1,1]^3;
@@ -264,7 +297,7 @@ __global__ void k_to_bf16(const float* __restrict__ in, uint16_t* __restrict__ o
 // Fixed 3x3x3 (3x3) trip counts with predicates: the loops unroll and all
 // fine loads are in flight together; skipped terms are exact zeros.
 template <bool IS3D>
-__device__ void body_restrict(const RestrictP& P, long long blk, long long) {
+__device__ __forceinline__ void body_restrict(const RestrictP& P, long long blk, long long) {
 	const CgState* __restrict__ st = P.st;
 	const LvDev& c = P.c;
 	const LvDev& f = P.f;
@@ -278,23 +311,32 @@ __device__ void body_restrict(const RestrictP& P, long long blk, long long) {
 	const long long v = i + c.sy * j + c.sz * k;
 	float sum = 0.f;
 	if (ctype[v]) {
+		// branch-free: clamped (valid) addresses, every load issued before
+		// the sum; terms outside the level add exactly +0
+		constexpr int NZ = IS3D ? 3 : 1;
+		float rv[NZ * 9];
+		bool ok[NZ * 9];
 #pragma unroll
-		for (int dz = IS3D ? -1 : 0; dz <= (IS3D ? 1 : 0); dz++)
+		for (int e = 0; e < NZ * 9; e++) {
+			const int dx = e % 3 - 1, dy = (e / 3) % 3 - 1, dz = IS3D ? e / 9 - 1 : 0;
+			const int ux = 2 * i + dx, uy = 2 * j + dy, uz = 2 * k + dz;
+			ok[e] = ux >= 0 && uy >= 0 && uz >= 0 && ux < f.nx && uy < f.ny && uz < f.nz;
+			const long long g = ok[e] ? ux + f.sy * uy + f.sz * uz : 0;
+			rv[e] = rf[g];
+		}
 #pragma unroll
-			for (int dy = -1; dy <= 1; dy++)
-#pragma unroll
-				for (int dx = -1; dx <= 1; dx++) {
-					const int ux = 2 * i + dx, uy = 2 * j + dy, uz = 2 * k + dz;
-					if (ux >= 0 && uy >= 0 && uz >= 0 && ux < f.nx && uy < f.ny && uz < f.nz)
-						sum += wgt(ux, uy, uz) * rf[ux + f.sy * uy + f.sz * uz];
-				}
+		for (int e = 0; e < NZ * 9; e++) {
+			const int dx = e % 3 - 1, dy = (e / 3) % 3 - 1, dz = IS3D ? e / 9 - 1 : 0;
+			const int ux = 2 * i + dx, uy = 2 * j + dy, uz = 2 * k + dz;
+			if (ok[e]) sum += wgt(ux, uy, uz) * rv[e];
+		}
 	}
 	bc[v] = sum;
 }
 
 // r = b - A x on a coarse level (0 on inactive vertices)
 template <bool IS3D, typename AT>
-__device__ void body_resid(const ResidP& P, long long blk, long long) {
+__device__ __forceinline__ void body_resid(const ResidP& P, long long blk, long long) {
 	const CgState* __restrict__ st = P.st;
 	const LvDev& c = P.c;
 	const AT* __restrict__ A = (const AT*)P.A;
@@ -329,20 +371,25 @@ __device__ void body_resid(const ResidP& P, long long blk, long long) {
 		r[v] = sum;
 		return;
 	}
+	// general stencil, branch-free: neighbours outside the level or inactive
+	// contribute an exact +0 (every load is issued before the sum)
+	constexpr int NS = IS3D ? 27 : 9;
+	float av[NS], xv[NS];
+	bool ok[NS];
 #pragma unroll
-	for (int dz = IS3D ? -1 : 0; dz <= (IS3D ? 1 : 0); dz++)
+	for (int s = 0; s < NS; s++) {
+		const int dx = s % 3 - 1, dy = (s / 3) % 3 - 1, dz = IS3D ? s / 9 - 1 : 0;
+		const int ni = i + dx, nj = j + dy, nk = k + dz;
+		const bool in = ni >= 0 && nj >= 0 && nk >= 0 && ni < c.nx && nj < c.ny && nk < c.nz;
+		const long long nn = in ? ni + c.sy * nj + c.sz * nk : v;
+		const uint8_t tn = type[nn];  // unconditional: nn is always a valid vertex
+		ok[s] = in & (tn != 0);
+		av[s] = s >= C ? ldA(A, (long long)(s - C) * c.n + v) : ldA(A, (long long)(C - s) * c.n + nn);
+		xv[s] = x[nn];
+	}
 #pragma unroll
-		for (int dy = -1; dy <= 1; dy++)
-#pragma unroll
-			for (int dx = -1; dx <= 1; dx++) {
-				const int s = (dx + 1) + 3 * (dy + 1) + (IS3D ? 9 * (dz + 1) : 0);
-				const int ni = i + dx, nj = j + dy, nk = k + dz;
-				if (ni < 0 || nj < 0 || nk < 0 || ni >= c.nx || nj >= c.ny || nk >= c.nz) continue;
-				const long long nn = ni + c.sy * nj + c.sz * nk;
-				if (!type[nn]) continue;
-				const float a = s >= C ? ldA(A, (long long)(s - C) * c.n + v) : ldA(A, (long long)(C - s) * c.n + nn);
-				sum -= a * x[nn];
-			}
+	for (int s = 0; s < NS; s++)
+		if (ok[s]) sum -= av[s] * xv[s];
 	r[v] = sum;
 }
 
@@ -390,7 +437,7 @@ __device__ inline float l0_nb_sum(const uint8_t* cs, const float* xs, int q, int
 // values), then the residual r0 = b - A x of the owned cells.
 // Reads b (4 B) and the matrix code (1 B), writes x and r0: 13 B/cell.
 template <int BX, int BY, int BZ, bool IS3D>
-__device__ void body_l0_down_brick(const L0DownP& P, long long blk, long long) {
+__device__ __forceinline__ void body_l0_down_brick(const L0DownP& P, long long blk, long long) {
 	const Dims& d = P.d;
 	const CgState* __restrict__ st = P.st;
 	const uint8_t* __restrict__ code = P.code;
@@ -452,7 +499,7 @@ __device__ void body_l0_down_brick(const L0DownP& P, long long blk, long long) {
 // Input and output are different arrays: the halo of one brick is owned by
 // its neighbours, which write their result concurrently.
 template <int BX, int BY, int BZ, bool IS3D>
-__device__ void body_l0_up_brick(const L0UpP& P, long long blk, long long slot) {
+__device__ __forceinline__ void body_l0_up_brick(const L0UpP& P, long long blk, long long slot) {
 	const Dims& d = P.d;
 	const CgState* __restrict__ st = P.st;
 	const uint8_t* __restrict__ code = P.code;
@@ -597,9 +644,9 @@ struct ZmCells {
 	}
 };
 
-// Down leg (same result as k_l0_down): red from x = 0, black, residual.
+// Down leg (same result as body_l0_down_brick): red from x = 0, black, residual.
 template <int TY>
-__device__ void body_l0_down_zm(const L0DownP& P, long long blk, long long) {
+__device__ __forceinline__ void body_l0_down_zm(const L0DownP& P, long long blk, long long) {
 	const Dims& d = P.d;
 	const int kchunk = P.kchunk;
 	const CgState* __restrict__ st = P.st;
@@ -679,11 +726,11 @@ __device__ void body_l0_down_zm(const L0DownP& P, long long blk, long long) {
 	}
 }
 
-// Up leg (same result as k_l0_up): x = xpre + I x_1, black, red, write z and
+// Up leg (same result as body_l0_up_brick): x = xpre + I x_1, black, red, write z and
 // the partial sums of fl(z*r).  Coarse planes are staged in a 2-slot ring:
 // fine plane p needs coarse planes p>>1 and (p+1)>>1.
 template <int TY>
-__device__ void body_l0_up_zm(const L0UpP& P, long long blk, long long slot) {
+__device__ __forceinline__ void body_l0_up_zm(const L0UpP& P, long long blk, long long slot) {
 	const Dims& d = P.d;
 	const int kchunk = P.kchunk;
 	const CgState* __restrict__ st = P.st;
@@ -815,7 +862,7 @@ constexpr int kLcBX3 = 32, kLcBY3 = 4, kLcBZ3 = 4, kLcBX2 = 64, kLcBY2 = 8;
 // result goes to xout, never to xin: neighbouring bricks read xin as their
 // halo.
 template <bool IS3D, typename AT>
-__device__ void body_smooth(const SmoothP& P, long long blk, long long) {
+__device__ __forceinline__ void body_smooth(const SmoothP& P, long long blk, long long) {
 	const CgState* __restrict__ st = P.st;
 	const LvDev& c = P.c;
 	const AT* __restrict__ A = (const AT*)P.A;
@@ -841,24 +888,33 @@ __device__ void body_smooth(const SmoothP& P, long long blk, long long) {
 		const int q = threadIdx.x + 512 * it;
 		if (q >= RN) break;
 		const int gx = ox - 1 + q % RX, gy = oy - 1 + (q / RX) % RY, gz = oz - hz + q / RXY;
+		const bool inb = gx >= 0 && gy >= 0 && gz >= 0 && gx < c.nx && gy < c.ny && gz < c.nz;
 		float v = 0.f;
-		if (!first && gx >= 0 && gy >= 0 && gz >= 0 && gx < c.nx && gy < c.ny && gz < c.nz) {
-			const long long g = gx + c.sy * gy + c.sz * gz;
+		if (!first) {
+			// branch-free: a valid address for every load, results selected
+			const int sx = inb ? gx : 0, sy = inb ? gy : 0, sz = inb ? gz : 0;
+			const long long g = sx + c.sy * sy + c.sz * sz;
 			v = xin[g];
-			if (xcp && type[g]) {
+			if (xcp) {
+				const uint8_t t = type[g];
+				// parents: [s/2, (s+1)/2] per axis (one parent for even s)
+				float pv[IS3D ? 8 : 4];
+#pragma unroll
+				for (int e = 0; e < (IS3D ? 8 : 4); e++) {
+					const int px = min((sx >> 1) + (e & 1), cp.nx - 1), py = min((sy >> 1) + ((e >> 1) & 1), cp.ny - 1);
+					const int pz = IS3D ? min((sz >> 1) + (e >> 2), cp.nz - 1) : 0;
+					pv[e] = xcp[px + cp.sy * py + cp.sz * pz];
+				}
 				float sum = 0.f;
 #pragma unroll
-				for (int ez = 0; ez < (IS3D ? 2 : 1); ez++)
-#pragma unroll
-					for (int ey = 0; ey < 2; ey++)
-#pragma unroll
-						for (int ex = 0; ex < 2; ex++) {
-							const int px = gx / 2 + ex, py = gy / 2 + ey, pz = gz / 2 + ez;
-							if (px <= (gx + 1) / 2 && py <= (gy + 1) / 2 && pz <= (gz + 1) / 2)
-								sum += xcp[px + cp.sy * py + cp.sz * pz];
-						}
-				v += wgt(gx, gy, gz) * sum;
+				for (int e = 0; e < (IS3D ? 8 : 4); e++) {
+					const bool use = ((e & 1) == 0 || (sx & 1)) && (((e >> 1) & 1) == 0 || (sy & 1)) &&
+					                 ((e >> 2) == 0 || (sz & 1));
+					if (use) sum += pv[e];
+				}
+				if (t) v += wgt(sx, sy, sz) * sum;
 			}
+			if (!inb) v = 0.f;
 		}
 		xs[q] = v;
 	}
@@ -910,7 +966,7 @@ __device__ void body_smooth(const SmoothP& P, long long blk, long long) {
 	if (valid) xout[v] = xs[me];
 }
 
-__device__ void body_fill(const FillP& P, long long blk, long long) {
+__device__ __forceinline__ void body_fill(const FillP& P, long long blk, long long) {
 	if (P.st->done) return;
 	const long long v = blk * 256 + threadIdx.x;
 	if (v < P.n) P.x[v] = 0.f;
@@ -918,7 +974,7 @@ __device__ void body_fill(const FillP& P, long long blk, long long) {
 
 // coarsest level: x[cidx[i]] = sum_j G[i][j] * b[cidx[j]] (FP64), one wave per row;
 // for a single-level hierarchy the rhs is the trivially scaled level-0 input
-__device__ void body_coarsest(const CoarsestP& P, long long blk, long long) {
+__device__ __forceinline__ void body_coarsest(const CoarsestP& P, long long blk, long long) {
 	const CgState* __restrict__ st = P.st;
 	const int nc = P.nc;
 	const int* __restrict__ cidx = P.cidx;
@@ -1006,6 +1062,9 @@ void MgHierarchy::build(Scene* s, const uint8_t* code, int precision) {
 		const MgLevel& f = L[l - 1];
 		hipLaunchKernelGGL(k_mg_activate, dim3(nblk(c.n)), dim3(256), 0, s->stream, lvdev(c), lvdev(f), (int)is3d,
 		                   l == 1 ? code : (const uint8_t*)nullptr, f.type, c.type);
+		hipLaunchKernelGGL(k_mg_cover, dim3(nblk(f.n)), dim3(256), 0, s->stream, lvdev(c), lvdev(f), (int)is3d,
+		                   l == 1 ? code : (const uint8_t*)nullptr, f.type, c.type);
+		hipLaunchKernelGGL(k_mg_cover_done, dim3(nblk(c.n)), dim3(256), 0, s->stream, lvdev(c), c.type);
 		FineOp F;
 		F.code = l == 1 ? code : nullptr;
 		F.A = l == 1 ? nullptr : f.A;
@@ -1145,14 +1204,49 @@ void MgHierarchy::build_coarsest(Scene* s, const uint8_t* code0) {
 					M[(size_t)a * m + pos[nn]] = val;
 				}
 	}
-	// Cholesky with dropped (non-positive) pivots
-	std::vector<double> Lm((size_t)m * m, 0.0);
+	// Pure-Neumann blocks (fluid pockets enclosed by obstacles, or a closed
+	// domain) are singular; with float coefficients their last pivot is
+	// round-off rather than zero and would put ~1e7-sized entries into G.
+	// Pin one vertex of every connected component whose rows all sum to ~0.
 	std::vector<char> dropped(m, 0);
+	{
+		std::vector<int> comp(m, -1), stack;
+		for (int a0 = 0; a0 < m; a0++) {
+			if (comp[a0] >= 0) continue;
+			std::vector<int> members;
+			comp[a0] = a0;
+			stack.push_back(a0);
+			while (!stack.empty()) {
+				const int a = stack.back();
+				stack.pop_back();
+				members.push_back(a);
+				for (int b = 0; b < m; b++)
+					if (b != a && comp[b] < 0 && M[(size_t)a * m + b] != 0.0) {
+						comp[b] = a0;
+						stack.push_back(b);
+					}
+			}
+			double maxSum = 0.0, maxDiag = 0.0;
+			for (int a : members) {
+				double rs = 0.0;
+				for (int b : members) rs += M[(size_t)a * m + b];
+				maxSum = std::max(maxSum, std::fabs(rs));
+				maxDiag = std::max(maxDiag, M[(size_t)a * m + a]);
+			}
+			if (members.size() > 1 && maxSum <= 1e-5 * maxDiag) {
+				int last = members[0];
+				for (int a : members) last = std::max(last, a);
+				dropped[last] = 1;
+			}
+		}
+	}
+	// Cholesky with dropped (pinned or non-positive) pivots
+	std::vector<double> Lm((size_t)m * m, 0.0);
 	for (int jj = 0; jj < m; jj++) {
 		double dval = M[(size_t)jj * m + jj];
 		for (int q = 0; q < jj; q++) dval -= Lm[(size_t)jj * m + q] * Lm[(size_t)jj * m + q];
 		const double orig = M[(size_t)jj * m + jj];
-		if (!(dval > 1e-10 * (orig > 0 ? orig : 1.0))) {
+		if (dropped[jj] || !(dval > 1e-10 * (orig > 0 ? orig : 1.0))) {
 			dropped[jj] = 1;
 			for (int q = 0; q < jj; q++) Lm[(size_t)jj * m + q] = 0.0;
 			Lm[(size_t)jj * m + jj] = 1.0;

@@ -202,8 +202,35 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(Dims d, const float* __restr
 	}
 }
 
+// CgState of a new solve (set on the device: no host copy to wait for)
+__global__ void k_cg_state_init(CgState* st, float acc, int maxIter, int useL2, double relTol) {
+	if (threadIdx.x != 0) return;
+	CgState z;
+	memset(&z, 0, sizeof(z));
+	z.acc = acc;
+	z.maxIter = maxIter;
+	z.useL2 = useL2;
+	z.relTol = relTol;
+	*st = z;
+}
+
+// st->trueRR = sum of nb true-residual partials (fixed order)
+__global__ void __launch_bounds__(1024) k_reduce_true_rr(CgState* st, const double* __restrict__ part, int nb) {
+	__shared__ double sh[16];
+	double a = 0;
+	for (int b = threadIdx.x; b < nb; b += blockDim.x) a += part[3 * b];
+	a = wave_sum(a);
+	if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = a;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		double s = 0;
+		for (int q = 0; q < (int)(blockDim.x >> 6); q++) s += sh[q];
+		st->trueRR = s;
+	}
+}
+
 // sum fl(z*r) (GridDotProduct(mTmp, mResidual)) of 256 cells into part[3*slot]
-__device__ void body_dot(const DotP& P, long long blk, long long slot) {
+__device__ __forceinline__ void body_dot(const DotP& P, long long blk, long long slot) {
 	__shared__ double sh[8];
 	if (P.st->done) return;
 	const long long c = blk * kBlock + threadIdx.x;
@@ -220,7 +247,7 @@ __device__ void body_dot(const DotP& P, long long blk, long long slot) {
 // from HBM once.  Row arithmetic and term order follow apply_row; neighbours
 // outside the grid contribute 0 (their coefficient is 0 there).
 template <int TY>
-__device__ void body_matvec(const MatvecP& P, long long blk, long long slot) {
+__device__ __forceinline__ void body_matvec(const MatvecP& P, long long blk, long long slot) {
 	constexpr int TX = 64, RX = TX + 2, RY = TY + 2, NH = 2 * RX + 2 * (RY - 2);
 	__shared__ float ps[RX * RY];
 	__shared__ uint8_t cs[RX * RY];
@@ -324,7 +351,7 @@ __device__ void body_matvec(const MatvecP& P, long long blk, long long slot) {
 // Tile blk of per (= tile count) tiles walks the float4 groups with a stride
 // of per*256 (grid-stride order: concurrent tiles touch adjacent memory); the
 // n % 4 trailing cells are spread the same way.
-__device__ void body_update(const UpdateP& P, long long blk, long long slot) {
+__device__ __forceinline__ void body_update(const UpdateP& P, long long blk, long long slot) {
 	__shared__ double sh[8];
 	__shared__ float shf[8];
 	const Dims& d = P.d;
@@ -410,7 +437,7 @@ __device__ inline void reduce_parts(const double* __restrict__ part, int nb, int
 	__syncthreads();
 }
 
-__device__ void body_finalize(const FinP& P, long long, long long) {
+__device__ __forceinline__ void body_finalize(const FinP& P, long long, long long) {
 	CgState* st = P.st;
 	const double* __restrict__ part = P.part;
 	const float* __restrict__ fpart = P.fpart;
@@ -494,10 +521,11 @@ __device__ void body_finalize(const FinP& P, long long, long long) {
 		}
 		return;
 	}
-	// FIN_BETA (preconditioned): sigmaNew = dot(z, r)
+	// FIN_BETA / FIN_BETA0 (preconditioned): sigmaNew = dot(z, r)
 	const float sigmaNew = (float)r0;
 	st->beta = sigmaNew / st->sigma;
 	st->sigma = sigmaNew;
+	if (phase == FIN_BETA0) st->first = 1;
 	if (!(st->resNorm < 1e35f)) {
 		st->diverged = 1;
 		st->done = 1;
@@ -735,8 +763,11 @@ struct CgRun {
 };
 
 // matrix, preconditioner, buffers, CgState and GridCg::doInit
-// (conjugategrad.cpp:210-235, pressure.cpp:349-433)
-static void cg_setup(CgRun& R, Scene* s, Grid* flags, float* rhs, float* x, const mfx_pressure_opts& o) {
+// (conjugategrad.cpp:210-235, pressure.cpp:349-433).  With batchInit the
+// initial V-cycle and sigma = dot(z, r) are left to the caller's batched
+// launches (FIN_BETA0 afterwards).
+static void cg_setup(CgRun& R, Scene* s, Grid* flags, float* rhs, float* x, const mfx_pressure_opts& o,
+                     bool batchInit = false) {
 	R.t0 = std::chrono::steady_clock::now();
 	R.s = s;
 	R.flags = flags;
@@ -802,29 +833,21 @@ static void cg_setup(CgRun& R, Scene* s, Grid* flags, float* rhs, float* x, cons
 	R.q = s->getReal();
 	R.z = R.pcNone ? R.r : s->getReal();
 
-	CgState hst;
-	memset(&hst, 0, sizeof(hst));
-	hst.acc = o.cgAccuracy;
-	hst.maxIter = R.maxIter;
-	hst.useL2 = o.useL2Norm;
-	hst.relTol = o.relTol;
 	R.st = (CgState*)s->dscal;
-	MFX_HIP(hipMemcpyAsync(R.st, &hst, sizeof(CgState), hipMemcpyHostToDevice, s->stream));
+	hipLaunchKernelGGL(k_cg_state_init, dim3(1), dim3(64), 0, s->stream, R.st, o.cgAccuracy, R.maxIter, o.useL2Norm,
+	                   o.relTol);
 
 	// doInit: x = 0, r = b, bnorm; preconditioned: z = M r, sigma = dot(z, r)
 	hipLaunchKernelGGL(k_cg_init, dim3(R.nbF), dim3(kBlock), 0, s->stream, d, (const float*)rhs, x, R.r,
 	                   R.pcNone ? 1 : 0, s->dpart);
 	run_finalize_one(s->stream, FinP{R.st, s->dpart, s->fpart, (int)R.nbF, FIN_INIT, 0});
-	if (!R.pcNone) {
+	if (!R.pcNone && !batchInit) {
 		int np = R.mg->vcycle(s, R.code, R.r, R.z, R.st);
 		if (np == 0) {
 			run_dot_one(s->stream, R.nbF, DotP{d, R.st, R.z, R.r, s->dpart});
 			np = (int)R.nbF;
 		}
-		run_finalize_one(s->stream, FinP{R.st, s->dpart, s->fpart, np, FIN_BETA, 0});
-		// FIN_BETA set sigma = dot(z, r); restart the search direction
-		static const int one = 1;
-		MFX_HIP(hipMemcpyAsync(&R.st->first, &one, sizeof(int), hipMemcpyHostToDevice, s->stream));
+		run_finalize_one(s->stream, FinP{R.st, s->dpart, s->fpart, np, FIN_BETA0, 0});
 	}
 	R.launched = 0;
 }
@@ -862,28 +885,26 @@ static void cg_iteration(CgRun& R) {
 	R.launched++;
 }
 
-// FP64 true residual, buffers back to the pool, statistics, divergence error
-static void cg_finish(CgRun& R, mfx_solve_stats* stats) {
+// FP64 true residual reduced on the device, CgState to the scene's pinned
+// mirror, buffers back to the pool (queued; nothing waits)
+static void cg_finish_enqueue(CgRun& R) {
 	Scene* s = R.s;
-	const Dims& d = s->d;
-	// true relative residual (FP64)
-	hipLaunchKernelGGL(k_true_residual, dim3(R.nbT), dim3(kBlock), 0, s->stream, d, R.t, R.code,
+	hipLaunchKernelGGL(k_true_residual, dim3(R.nbT), dim3(kBlock), 0, s->stream, s->d, R.t, R.code,
 	                   (const float*)R.rhs, (const float*)R.x, s->dpart);
-	std::vector<double> h(3 * (size_t)R.nbT);
-	CgState* hs = (CgState*)s->hscal;
-	MFX_HIP(hipMemcpyAsync(h.data(), s->dpart, sizeof(double) * 3 * R.nbT, hipMemcpyDeviceToHost, s->stream));
-	MFX_HIP(hipMemcpyAsync(hs, R.st, sizeof(CgState), hipMemcpyDeviceToHost, s->stream));
-	sync(s);
-	double rr = 0;
-	for (unsigned b = 0; b < R.nbT; b++) rr += h[3 * b];
-	const double bb = hs->bnorm2;
-
+	hipLaunchKernelGGL(k_reduce_true_rr, dim3(1), dim3(1024), 0, s->stream, R.st, (const double*)s->dpart,
+	                   (int)R.nbT);
+	MFX_HIP(hipMemcpyAsync(s->hscal, R.st, sizeof(CgState), hipMemcpyDeviceToHost, s->stream));
 	s->putReal(R.r);
 	s->putReal(R.pa);
 	s->putReal(R.pb);
 	s->putReal(R.q);
 	if (!R.pcNone) s->putReal(R.z);
+}
 
+// after the scene stream has drained: statistics and the divergence error
+static void cg_finish_collect(CgRun& R, mfx_solve_stats* stats) {
+	const CgState* hs = (const CgState*)R.s->hscal;
+	const double bb = hs->bnorm2, rr = hs->trueRR;
 	if (stats) {
 		stats->iterations = hs->iter;
 		stats->resNorm = hs->resNorm;
@@ -932,7 +953,9 @@ static void solve_system(Scene* s, Grid* flags, float* rhs, float* x, const mfx_
 		}
 		batch = std::max(1, std::min(predict, 64));
 	}
-	cg_finish(R, stats);
+	cg_finish_enqueue(R);
+	sync(s);
+	cg_finish_collect(R, stats);
 }
 
 
@@ -1012,7 +1035,7 @@ struct BatchSolve {
 	// CG
 	Step<MatvecP> mv[2];
 	Step<UpdateP> up[2];
-	Step<FinP> fa, fr, fb;
+	Step<FinP> fa, fr, fb, fb0;
 	// V-cycle
 	Step<FillP> fill;
 	Step<L0DownP> down[2];  // [is3d]
@@ -1033,6 +1056,11 @@ struct BatchSolve {
 		run_update_many(s, up[parity].n(), up[parity].dItems, up[parity].dParams);
 		run_finalize_many(s, fr.n(), fr.dItems, fr.dParams);
 		if (!anyMg) return;
+		enqueue_vcycle(s);
+		run_finalize_many(s, fb.n(), fb.dItems, fb.dParams);
+	}
+	// z = M r for every preconditioned problem (+ dot(z, r) partials)
+	void enqueue_vcycle(hipStream_t s) {
 		run_fill_many(s, fill.n(), fill.dItems, fill.dParams);
 		for (int d3 = 0; d3 < 2; d3++) run_l0_down_many(s, d3, down[d3].n(), down[d3].dItems, down[d3].dParams);
 		const size_t nl = std::max(rs[0].size(), rs[1].size());
@@ -1051,10 +1079,6 @@ struct BatchSolve {
 					run_smooth_many(s, d3, bf16, post[d3][l].n(), post[d3][l].dItems, post[d3][l].dParams);
 		for (int d3 = 0; d3 < 2; d3++) run_l0_up_many(s, d3, l0up[d3].n(), l0up[d3].dItems, l0up[d3].dParams);
 		run_dot_many(s, dot.n(), dot.dItems, dot.dParams);
-		run_finalize_many(s, fb.n(), fb.dItems, fb.dParams);
-	}
-	~BatchSolve() {
-		if (dmem) hipFree(dmem);
 	}
 };
 
@@ -1074,9 +1098,8 @@ static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>&
 			Scene* s = vel[(size_t)i]->s;
 			rhs[(size_t)i] = s->getReal();
 			compute_rhs(s, flg[(size_t)i], vel[(size_t)i], rhs[(size_t)i], o.enforceCompatibility);
-			cg_setup(R[(size_t)i], s, flg[(size_t)i], rhs[(size_t)i], pres[(size_t)i]->real(), o);
+			cg_setup(R[(size_t)i], s, flg[(size_t)i], rhs[(size_t)i], pres[(size_t)i]->real(), o, true);
 		}
-		for (int i = 0; i < P; i++) sync(vel[(size_t)i]->s);
 
 		BatchSolve B;
 		B.P = P;
@@ -1088,6 +1111,7 @@ static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>&
 		B.fa = Step<FinP>((size_t)P);
 		B.fr = Step<FinP>((size_t)P);
 		B.fb = Step<FinP>((size_t)P);
+		B.fb0 = Step<FinP>((size_t)P);
 		B.fill = Step<FillP>((size_t)P);
 		B.crs = Step<CoarsestP>((size_t)P);
 		B.dot = Step<DotP>((size_t)P);
@@ -1152,6 +1176,8 @@ static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>&
 			}
 			B.fb.params[(size_t)i] = r.fin(np, FIN_BETA);
 			B.fb.items.push_back(WorkItem{i, 0, 0, 0});
+			B.fb0.params[(size_t)i] = r.fin(np, FIN_BETA0);
+			B.fb0.items.push_back(WorkItem{i, 0, 0, 0});
 		}
 		// one device block for every list and parameter array
 		DevArena A;
@@ -1163,6 +1189,7 @@ static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>&
 			f(B.fa);
 			f(B.fr);
 			f(B.fb);
+			f(B.fb0);
 			f(B.fill);
 			f(B.crs);
 			f(B.dot);
@@ -1183,12 +1210,31 @@ static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>&
 		const size_t doneOff = A.add(zeros.data(), sizeof(int) * (size_t)P);
 		Scene* s0 = vel[0]->s;
 		hipStream_t q = s0->stream;
-		MFX_HIP(hipMalloc((void**)&B.dmem, A.host.size()));
+		if (s0->arenaCap < A.host.size()) {
+			// grow (hipFree waits for the device; this happens once per size class)
+			if (s0->arena) MFX_HIP(hipFree(s0->arena));
+			s0->arena = nullptr;
+			s0->arenaCap = 0;
+			MFX_HIP(hipMalloc(&s0->arena, A.host.size() * 2));
+			s0->arenaCap = A.host.size() * 2;
+		}
+		B.dmem = (uint8_t*)s0->arena;
 		MFX_HIP(hipMemcpyAsync(B.dmem, A.host.data(), A.host.size(), hipMemcpyHostToDevice, q));
 		placeAll([&](auto& st) { st.bind(B.dmem); });
 		B.dStates = (CgState**)(B.dmem + stOff);
 		B.dDone = (int*)(B.dmem + doneOff);
-		sync(s0);
+		// the batch stream waits for every problem's set-up
+		for (int i = 1; i < P; i++) {
+			Scene* si = vel[(size_t)i]->s;
+			if (!si->ev) MFX_HIP(hipEventCreateWithFlags(&si->ev, hipEventDisableTiming));
+			MFX_HIP(hipEventRecord(si->ev, si->stream));
+			MFX_HIP(hipStreamWaitEvent(q, si->ev, 0));
+		}
+		// initial V-cycle of every preconditioned problem, sigma = dot(z, r)
+		if (B.anyMg) {
+			B.enqueue_vcycle(q);
+			run_finalize_many(q, B.fb0.n(), B.fb0.dItems, B.fb0.dParams);
+		}
 
 		// capture two iterations and replay
 		hipGraph_t graph = nullptr;
@@ -1218,17 +1264,24 @@ static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>&
 		MFX_HIP(hipGraphExecDestroy(exec));
 		MFX_HIP(hipGraphDestroy(graph));
 		for (int i = 0; i < P; i++) R[(size_t)i].launched = its;
-		// finish, correct velocities (each problem on its own stream)
+		// finish and correct velocities, each problem on its own stream
+		if (!s0->ev) MFX_HIP(hipEventCreateWithFlags(&s0->ev, hipEventDisableTiming));
+		MFX_HIP(hipEventRecord(s0->ev, q));
+		for (int i = 0; i < P; i++) {
+			Scene* si = vel[(size_t)i]->s;
+			if (i > 0) MFX_HIP(hipStreamWaitEvent(si->stream, s0->ev, 0));
+			cg_finish_enqueue(R[(size_t)i]);
+			correct_velocity(si, flg[(size_t)i], vel[(size_t)i], pres[(size_t)i]->real());
+		}
 		std::string err;
 		for (int i = 0; i < P; i++) {
+			sync(vel[(size_t)i]->s);
 			try {
-				cg_finish(R[(size_t)i], stats ? &stats[i] : nullptr);
+				cg_finish_collect(R[(size_t)i], stats ? &stats[i] : nullptr);
 			} catch (const Error& e) {
 				if (err.empty()) err = e.what();
 			}
-			correct_velocity(vel[(size_t)i]->s, flg[(size_t)i], vel[(size_t)i], pres[(size_t)i]->real());
 		}
-		for (int i = 0; i < P; i++) sync(vel[(size_t)i]->s);
 		release();
 		if (!err.empty()) throw Error(err);
 	} catch (...) {

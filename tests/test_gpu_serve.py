@@ -1,0 +1,42 @@
+"""Serving: a mixed stream of plume-step and pressure-solve requests over
+sessions of different sizes (2D and 3D).  Each session's state after the
+served (continuously batched) replay is bit-identical to replaying the same
+requests one at a time on the plain path: a result does not depend on what
+it was batched with."""
+import numpy as np
+import pytest
+
+from mantaflow_amd import serve
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(24, 24, 24), (48, 48, 1), (40, 40, 40), (64, 64, 1), (16, 16, 16), (100, 100, 1), (33, 33, 33)]
+
+
+def state(sess):
+    return [sess.density.toNumpy().copy(), sess.vel.toNumpy().copy(), sess.pressure.toNumpy().copy()]
+
+
+def test_served_equals_serial():
+    a = [serve.PlumeScene(sh, seed=10 + i) for i, sh in enumerate(SHAPES)]
+    b = [serve.PlumeScene(sh, seed=10 + i) for i, sh in enumerate(SHAPES)]
+    trace = serve.make_trace(40, len(SHAPES), rate=1e9, seed=3)
+    lat_s, _ = serve.serve_serial(a, trace, realtime=False)
+    srv = serve.Server(b, max_batch=8)
+    lat_b, _ = srv.run(trace, realtime=False)
+    assert len(lat_s) == len(lat_b) == len(trace)
+    assert srv.ticks < len(trace) and max(srv.batch_sizes) > 1
+    for i, (x, y) in enumerate(zip(a, b)):
+        assert x.steps == y.steps
+        for u, v, name in zip(state(x), state(y), ("density", "vel", "pressure")):
+            np.testing.assert_array_equal(u, v, err_msg="session %d %s" % (i, name))
+
+
+def test_admission_respects_memory():
+    sessions = [serve.PlumeScene((16, 16, 16), seed=1), serve.PlumeScene((16, 16, 16), seed=2)]
+    srv = serve.Server(sessions, max_batch=8, hbm_reserve=0)
+    ready = [serve.Request(0, 0.0, 0, serve.REQ_SOLVE), serve.Request(1, 0.0, 1, serve.REQ_SOLVE),
+             serve.Request(2, 0.0, 0, serve.REQ_STEP)]
+    # one request per session per tick, and nothing beyond the memory budget
+    assert [r.rid for r in srv._admit(ready, 1 << 40)] == [0, 1]
+    assert [r.rid for r in srv._admit(ready, 6 * 4 * 16 ** 3)] == [0]
