@@ -176,6 +176,12 @@ class Server:
         return chosen
 
     def tick(self, batch):
+        if len(batch) == 1:
+            # nothing to share: the plain path has the lowest latency
+            self.sessions[batch[0].session].run_single(batch[0].kind)
+            self.ticks += 1
+            self.batch_sizes.append(1)
+            return
         for r in batch:
             if r.kind == REQ_STEP:
                 self.sessions[r.session].pre()

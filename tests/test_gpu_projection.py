@@ -81,3 +81,17 @@ def test_mixed_precision_keeps_iteration_count(manta, case):
         manta.solvePressure(flags=f, vel=v, pressure=p, preconditioner=manta.PcMGStatic)
         its.append(manta.lastSolveStats(g.s)["iterations"])
     assert its[1] <= its[0] + 2, its
+
+
+@pytest.mark.parametrize("index", [38, 62])
+def test_multigrid_converges_on_obstacle_rich_problems(manta, index):
+    """Two problems of the 64-problem batched workload (116^3 and 131^3 with
+    17 % / 14 % seeded obstacles) on which a coarse grid with weak, nearly
+    dependent interpolation columns made MG-PCG stagnate near 1e-3."""
+    from mantaflow_amd import workloads
+    shapes, fracs = workloads.batched_shapes()
+    p = workloads.Problem(shapes[index], fracs[index], 1000 + index)
+    manta.solvePressure(flags=p.flags, vel=p.vel, pressure=p.pressure, cgAccuracy=1e-4,
+                        preconditioner=manta.PcMGStatic)
+    st = manta.lastSolveStats(p.s)
+    assert st["converged"] and st["relResidual"] <= 1e-4 and st["iterations"] <= 12, st
