@@ -65,7 +65,7 @@ struct Dims {
 // scene / grid objects
 // ---------------------------------------------------------------------------
 struct Grid;
-struct MgHierarchy;
+class MgHierarchy;
 
 struct Scene {
 	Dims d;

@@ -31,7 +31,11 @@
 namespace mfx {
 
 __device__ __host__ inline int oddc(int i, int j, int k) { return (i & 1) + (j & 1) + (k & 1); }
-__device__ inline float wgt(int i, int j, int k) { return 1.0f / (float)(1 << oddc(i, j, k)); }
+// 1/2^(#odd coordinates): exact powers of two (same values as the division)
+__device__ inline float wgt(int i, int j, int k) {
+	const int o = oddc(i, j, k);
+	return o == 0 ? 1.0f : (o == 1 ? 0.5f : (o == 2 ? 0.25f : 0.125f));
+}
 
 static inline LvDev lvdev(const MgLevel& l) { return LvDev{l.nx, l.ny, l.nz, l.sy, l.sz, l.n}; }
 

@@ -269,10 +269,13 @@ def test_solve_errors(manta, oracle, rng):
         manta.solvePressure(flags=f, vel=v, pressure=pr, preconditioner=manta.PcNone)
 
 
-@pytest.mark.parametrize("steps", [12])
+@pytest.mark.parametrize("steps", [12, 200])
 def test_plume2d_steps(manta, oracle, steps):
-    """Config-1 style 2D plume (64^2, first-order SL, plain CG at 1e-3) for a
-    few steps, GPU vs oracle after every step."""
+    """2D plume (64^2, walls on all sides, box source, first-order SL, buoyancy
+    (0,-4e-3), plain CG at cgAccuracy 1e-3) for 12 and for the full 200 steps,
+    GPU vs oracle after every step.  Advection and forces are bit-exact; the
+    CG differs only in the summation order of its FP64 dot products, so the
+    tolerance is 1e-5 of the field scale for 12 steps and 1e-3 over 200."""
     shape = (64, 64, 1)
     n = 64 * 64
     flags = oracle.init_domain(shape)
@@ -298,5 +301,60 @@ def test_plume2d_steps(manta, oracle, steps):
         manta.solvePressure(flags=f, vel=v, pressure=pr, cgAccuracy=1e-3, preconditioner=manta.PcNone)
         sc.s.step()
         gd, gv = d.toNumpy(), v.toNumpy()
-        assert np.abs(gd - dens).max() <= 1e-5 * max(1.0, np.abs(dens).max()), t
-        assert np.abs(gv - vel).max() <= 1e-5 * max(1.0, np.abs(vel).max()), t
+        tol = 1e-5 if steps <= 12 else 1e-3
+        assert np.abs(gd - dens).max() <= tol * max(1.0, np.abs(dens).max()), t
+        assert np.abs(gv - vel).max() <= tol * max(1.0, np.abs(vel).max()), t
+    print("2D plume %d steps: max |density diff| %.3g, max |velocity diff| %.3g" %
+          (steps, np.abs(gd - dens).max(), np.abs(gv - vel).max()))
+
+
+def test_plume3d_open_top_mg(manta, oracle):
+    """3D plume as in the 128^3 configuration, at 48^3: boundary width 1,
+    open top (outflow), cylinder source with seeded density U[0.8,1],
+    MacCormack advection (order 2, strength 1), resetOutflow, setWallBcs,
+    buoyancy (0,-1e-3,0), MG-preconditioned solvePressure at cgAccuracy 1e-4,
+    setWallBcs; 8 steps.  The GPU multigrid differs from the reference's
+    (parallel coarsening, exact coarsest solve), so the CG iterates differ
+    within the solver tolerance: fields agree to 1e-3 of their scale."""
+    n = 48
+    shape = (n, n, n)
+    N = n ** 3
+    flags = oracle.init_domain(shape, boundaryWidth=1)
+    oracle.fill_grid(shape, flags)
+    oracle.set_open_bound(shape, flags, 1, "Y")
+    mask = np.zeros(N, np.float32)
+    oracle.apply_shape(shape, 3, [0.5 * n, 0.1 * n, 0.5 * n], [0, 0.02 * n, 0], 0.14 * n, mask, TYPE_REAL, 1.0)
+    val = np.random.default_rng(1).uniform(0.8, 1.0, N).astype(np.float32) * mask
+    keep = (np.float32(1.0) - mask).astype(np.float32)
+    vel = np.zeros((N, 3), np.float32)
+    dens = np.zeros(N, np.float32)
+    p = np.zeros(N, np.float32)
+    sc = GpuScene(manta, shape)
+    f, v, d, pr = sc.flags(flags), sc.mac(vel), sc.real(dens), sc.real()
+    gkeep, gval = sc.real(keep), sc.real(val)
+    its = []
+    for t in range(8):
+        dens = (dens * keep + val).astype(np.float32)
+        oracle.advect(shape, np.float32(1.0), flags, vel, dens, TYPE_REAL, 2, 1.0, 2)
+        oracle.advect(shape, np.float32(1.0), flags, vel, vel, TYPE_MAC, 2, 1.0, 2)
+        oracle.reset_outflow(shape, flags, None, dens)
+        oracle.set_wall_bcs(shape, flags, vel)
+        oracle.add_buoyancy(shape, np.float32(1.0), flags, dens, vel, (0, -1e-3, 0))
+        st = oracle.solve_pressure(shape, vel, p, flags, cgAccuracy=1e-4, preconditioner=PC_MGSTATIC)
+        oracle.set_wall_bcs(shape, flags, vel)
+        d.mult(gkeep)
+        d.add(gval)
+        manta.advectSemiLagrange(flags=f, vel=v, grid=d, order=2)
+        manta.advectSemiLagrange(flags=f, vel=v, grid=v, order=2, strength=1.0)
+        manta.resetOutflow(flags=f, real=d)
+        manta.setWallBcs(flags=f, vel=v)
+        manta.addBuoyancy(density=d, vel=v, gravity=manta.vec3(0, -1e-3, 0), flags=f)
+        manta.solvePressure(flags=f, vel=v, pressure=pr, cgAccuracy=1e-4, preconditioner=manta.PcMGStatic)
+        manta.setWallBcs(flags=f, vel=v)
+        its.append((st["iterations"], manta.lastSolveStats(sc.s)["iterations"]))
+        gd, gv = d.toNumpy(), v.toNumpy()
+        assert np.abs(gd - dens).max() <= 1e-3 * max(1.0, np.abs(dens).max()), t
+        assert np.abs(gv - vel).max() <= 1e-3 * max(1e-3, np.abs(vel).max()), t
+    print("3D plume 48^3, 8 steps: (oracle, GPU) MG-PCG iterations", its,
+          "max |density diff| %.3g, max |velocity diff| %.3g (max |vel| %.3g)" %
+          (np.abs(gd - dens).max(), np.abs(gv - vel).max(), np.abs(vel).max()))
