@@ -648,7 +648,50 @@ struct ZmCells {
 	}
 };
 
-// Down leg (same result as body_l0_down_brick): red from x = 0, black, residual.
+// x / diag of a level-0 row (A0 = 1..6, or A0 * 1e-6 on trivial rows)
+__device__ inline float l0_div(float v, uint8_t c) { return v / l0_diag(c); }
+
+// Per-thread positions of the z-marching kernels that do not change from
+// plane to plane (the plane loop only rotates ring slots).
+template <typename T>
+struct ZmPos {
+	// load stage: at most two region cells per thread (q = tid, tid + NT)
+	int qa[2];
+	// second-colour stage: region without its outer ring (at most two cells)
+	int qb[2];
+	int parb[2];
+	bool inb[2];
+	// output stage: the thread's tile cell
+	int qc;
+	long long gc;
+	int parc;
+	bool inc;
+	__device__ ZmPos(const Dims& d, int ox, int oy) {
+		const int rx0 = ox - T::H, ry0 = oy - T::H;
+		constexpr int IX = T::RX - 2, IN = IX * (T::RY - 2);
+		static_assert(IN <= 2 * T::NT, "two second-colour cells per thread");
+#pragma unroll
+		for (int u = 0; u < 2; u++) {
+			qa[u] = threadIdx.x + u * T::NT;
+			const int t = threadIdx.x + u * T::NT;
+			const int lx = 1 + t % IX, ly = 1 + t / IX;
+			inb[u] = t < IN;
+			qb[u] = lx + T::RX * ly;
+			parb[u] = (rx0 + lx + ry0 + ly) & 1;
+		}
+		const int lx = T::H + (threadIdx.x & 63), ly = T::H + (threadIdx.x >> 6);
+		const int gx = rx0 + lx, gy = ry0 + ly;
+		qc = lx + T::RX * ly;
+		inc = gx < d.nx && gy < d.ny;
+		gc = inc ? (long long)gx + (long long)d.nx * gy : 0;
+		parc = (gx + gy) & 1;
+	}
+};
+
+__device__ inline int ring_next(int s) { return s == 4 ? 0 : s + 1; }
+__device__ inline int ring_back(int s, int k) { return (s + 5 - k) % 5; }
+
+// Down leg (same sweeps as body_l0_down_brick): red from x = 0, black, residual.
 template <int TY>
 __device__ __forceinline__ void body_l0_down_zm(const L0DownP& P, long long blk, long long) {
 	const Dims& d = P.d;
@@ -667,6 +710,7 @@ __device__ __forceinline__ void body_l0_down_zm(const L0DownP& P, long long blk,
 	zm_origin(blk, d, TY, kchunk, ox, oy, k0, k1);
 	const int rx0 = ox - T::H, ry0 = oy - T::H;
 	ZmCells<T> cells(d, rx0, ry0);
+	const ZmPos<T> pos(d, ox, oy);
 	uint8_t pc[2];
 	float pr[2];
 	// plane p's inputs are fetched into registers one plane ahead, so the
@@ -684,55 +728,53 @@ __device__ __forceinline__ void body_l0_down_zm(const L0DownP& P, long long blk,
 		}
 	};
 	fetch(k0 - 2);
-	for (int p = k0 - 2; p <= k1 + 1; p++) {
-		const int sp = zslot(p);
+	int sp = 0;  // ring slot of plane p
+	for (int p = k0 - 2; p <= k1 + 1; p++, sp = ring_next(sp)) {
 #pragma unroll
 		for (int u = 0; u < 2; u++) {
-			const int q = threadIdx.x + u * T::NT;
+			const int q = pos.qa[u];
 			if (q >= T::RP) continue;
 			const uint8_t cc = pc[u];
 			const float b = pr[u] * l0_scale(cc);
 			cs[sp][q] = cc;
 			bs[sp][q] = b;
-			xs[sp][q] = (l0_active(cc) && ((cells.par[u] + p) & 1) == 0) ? b / l0_diag(cc) : 0.f;
+			xs[sp][q] = (l0_active(cc) && ((cells.par[u] + p) & 1) == 0) ? l0_div(b, cc) : 0.f;
 		}
 		if (p <= k1) fetch(p + 1);
 		__syncthreads();
 		const int pb = p - 1;
 		if (pb >= k0 - 1 && pb <= k1 && pb >= 0 && pb < d.nz) {
-			const int s0 = zslot(pb), sm = zslot(pb - 1), su = zslot(pb + 1);
-			constexpr int IX = T::RX - 2, IN = IX * (T::RY - 2);
-			for (int t = threadIdx.x; t < IN; t += T::NT) {
-				const int lx = 1 + t % IX, ly = 1 + t / IX;
-				const int q = lx + T::RX * ly;
+			const int s0 = ring_back(sp, 1), sm = ring_back(sp, 2);
+#pragma unroll
+			for (int u = 0; u < 2; u++) {
+				if (!pos.inb[u]) continue;
+				const int q = pos.qb[u];
 				const uint8_t cc = cs[s0][q];
-				if (((rx0 + lx + ry0 + ly + pb) & 1) == 1 && l0_active(cc))
-					xs[s0][q] = (bs[s0][q] + zm_nb_sum(cs[s0], xs[s0], cs[sm], xs[sm], xs[su], q, T::RX)) / l0_diag(cc);
+				if (((pos.parb[u] + pb) & 1) == 1 && l0_active(cc))
+					xs[s0][q] = l0_div(bs[s0][q] + zm_nb_sum(cs[s0], xs[s0], cs[sm], xs[sm], xs[sp], q, T::RX), cc);
 			}
 		}
 		__syncthreads();
-		const int pc = p - 2;
-		if (pc >= k0 && pc < k1) {
-			const int s0 = zslot(pc), sm = zslot(pc - 1), su = zslot(pc + 1);
-			const int lx = T::H + (threadIdx.x & 63), ly = T::H + (threadIdx.x >> 6);
-			const int gx = rx0 + lx, gy = ry0 + ly;
-			if (gx < d.nx && gy < d.ny) {
-				const int q = lx + T::RX * ly;
-				const long long g = d.idx(gx, gy, pc);
-				const uint8_t cc = cs[s0][q];
-				const float x = xs[s0][q];
-				xout[g] = x;
-				r0[g] = l0_active(cc)
-				            ? (bs[s0][q] + zm_nb_sum(cs[s0], xs[s0], cs[sm], xs[sm], xs[su], q, T::RX)) - l0_diag(cc) * x
-				            : 0.f;
-			}
+		const int pz = p - 2;
+		if (pz >= k0 && pz < k1 && pos.inc) {
+			const int s0 = ring_back(sp, 2), sm = ring_back(sp, 3), su = ring_back(sp, 1);
+			const int q = pos.qc;
+			const long long g = pos.gc + d.sz * pz;
+			const uint8_t cc = cs[s0][q];
+			const float x = xs[s0][q];
+			xout[g] = x;
+			r0[g] = l0_active(cc)
+			            ? (bs[s0][q] + zm_nb_sum(cs[s0], xs[s0], cs[sm], xs[sm], xs[su], q, T::RX)) - l0_diag(cc) * x
+			            : 0.f;
 		}
 	}
 }
 
-// Up leg (same result as body_l0_up_brick): x = xpre + I x_1, black, red, write z and
-// the partial sums of fl(z*r).  Coarse planes are staged in a 2-slot ring:
-// fine plane p needs coarse planes p>>1 and (p+1)>>1.
+// Up leg (same sweeps as body_l0_up_brick): x = xpre + I x_1, black, red,
+// write z and the partial sums of fl(z*r).  Coarse planes are staged in a
+// 2-slot ring: fine plane p needs coarse planes p>>1 and (p+1)>>1.  The x/y
+// parents of the thread's region cells and their weights are fixed for the
+// whole column, only the z parents change with the plane.
 template <int TY>
 __device__ __forceinline__ void body_l0_up_zm(const L0UpP& P, long long blk, long long slot) {
 	const Dims& d = P.d;
@@ -763,8 +805,22 @@ __device__ __forceinline__ void body_l0_up_zm(const L0UpP& P, long long blk, lon
 	const int cpx = cx0 + cq % CRX, cpy = cy0 + cq / CRX;
 	const bool cok = cq < CRP && cpx >= 0 && cpy >= 0 && cpx < c1.nx && cpy < c1.ny;
 	const long long cg = cok ? (long long)cpx + c1.sy * cpy : 0;
-	auto cfetch = [&](int P) -> float { return (cok && P >= 0 && P < c1.nz) ? xc[cg + c1.sz * P] : 0.f; };
+	auto cfetch = [&](int Pz) -> float { return (cok && Pz >= 0 && Pz < c1.nz) ? xc[cg + c1.sz * Pz] : 0.f; };
 	ZmCells<T> cells(d, rx0, ry0);
+	const ZmPos<T> pos(d, ox, oy);
+	// prolongation offsets (coarse-plane region) and x/y weights per region cell
+	int po[2];
+	bool pxo[2], pyo[2];
+	float pw[2];
+#pragma unroll
+	for (int u = 0; u < 2; u++) {
+		const int q = pos.qa[u];
+		const int gx = rx0 + q % T::RX, gy = ry0 + q / T::RX;
+		pxo[u] = gx & 1;
+		pyo[u] = gy & 1;
+		po[u] = ((gx >> 1) - cx0) + CRX * ((gy >> 1) - cy0);
+		pw[u] = (pxo[u] ? 0.5f : 1.0f) * (pyo[u] ? 0.5f : 1.0f);
+	}
 	uint8_t pc[2];
 	float pr[2], px[2];
 	auto fetch = [&](int p) {
@@ -788,22 +844,35 @@ __device__ __forceinline__ void body_l0_up_zm(const L0UpP& P, long long blk, lon
 	fetch(k0 - 2);
 	__syncthreads();
 	double acc = 0.0;
-	for (int p = k0 - 2; p <= k1 + 1; p++) {
-		const int sp = zslot(p);
+	int sp = 0;  // ring slot of plane p
+	for (int p = k0 - 2; p <= k1 + 1; p++, sp = ring_next(sp)) {
+		const float* c0 = xcs[(p >> 1) & 1];
+		const float* c1p = xcs[((p >> 1) + 1) & 1];
+		const bool zo = p & 1;
 #pragma unroll
 		for (int u = 0; u < 2; u++) {
-			const int q = threadIdx.x + u * T::NT;
+			const int q = pos.qa[u];
 			if (q >= T::RP) continue;
 			const uint8_t cc = pc[u];
 			float x = px[u];
 			if (l0_active(cc)) {
-				const int gx = rx0 + q % T::RX, gy = ry0 + q / T::RX;
-				float sum = 0.f;
-				for (int pz = p >> 1; pz <= (p + 1) >> 1; pz++)
-					for (int py = gy >> 1; py <= (gy + 1) >> 1; py++)
-						for (int qx = gx >> 1; qx <= (gx + 1) >> 1; qx++)
-							sum += xcs[pz & 1][(qx - cx0) + CRX * (py - cy0)];
-				x += wgt(gx, gy, p) * sum;
+				// parents in the reference's loop order: z, then y, then x
+				const int o = po[u];
+				float sum = c0[o];
+				if (pxo[u]) sum += c0[o + 1];
+				if (pyo[u]) {
+					sum += c0[o + CRX];
+					if (pxo[u]) sum += c0[o + CRX + 1];
+				}
+				if (zo) {
+					sum += c1p[o];
+					if (pxo[u]) sum += c1p[o + 1];
+					if (pyo[u]) {
+						sum += c1p[o + CRX];
+						if (pxo[u]) sum += c1p[o + CRX + 1];
+					}
+				}
+				x += (zo ? 0.5f * pw[u] : pw[u]) * sum;
 			}
 			cs[sp][q] = cc;
 			rs[sp][q] = pr[u];
@@ -818,34 +887,29 @@ __device__ __forceinline__ void body_l0_up_zm(const L0UpP& P, long long blk, lon
 		__syncthreads();
 		const int pb = p - 1;
 		if (pb >= k0 - 1 && pb <= k1 && pb >= 0 && pb < d.nz) {
-			const int s0 = zslot(pb), sm = zslot(pb - 1), su = zslot(pb + 1);
-			constexpr int IX = T::RX - 2, IN = IX * (T::RY - 2);
-			for (int t = threadIdx.x; t < IN; t += T::NT) {
-				const int lx = 1 + t % IX, ly = 1 + t / IX;
-				const int q = lx + T::RX * ly;
+			const int s0 = ring_back(sp, 1), sm = ring_back(sp, 2);
+#pragma unroll
+			for (int u = 0; u < 2; u++) {
+				if (!pos.inb[u]) continue;
+				const int q = pos.qb[u];
 				const uint8_t cc = cs[s0][q];
-				if (((rx0 + lx + ry0 + ly + pb) & 1) == 1 && l0_active(cc))
-					xs[s0][q] = (rs[s0][q] * l0_scale(cc) + zm_nb_sum(cs[s0], xs[s0], cs[sm], xs[sm], xs[su], q, T::RX)) /
-					            l0_diag(cc);
+				if (((pos.parb[u] + pb) & 1) == 1 && l0_active(cc))
+					xs[s0][q] = l0_div(rs[s0][q] * l0_scale(cc) + zm_nb_sum(cs[s0], xs[s0], cs[sm], xs[sm], xs[sp], q, T::RX),
+					                   cc);
 			}
 		}
 		__syncthreads();
-		const int pc = p - 2;
-		if (pc >= k0 && pc < k1) {
-			const int s0 = zslot(pc), sm = zslot(pc - 1), su = zslot(pc + 1);
-			const int lx = T::H + (threadIdx.x & 63), ly = T::H + (threadIdx.x >> 6);
-			const int gx = rx0 + lx, gy = ry0 + ly;
-			if (gx < d.nx && gy < d.ny) {
-				const int q = lx + T::RX * ly;
-				const long long g = d.idx(gx, gy, pc);
-				const uint8_t cc = cs[s0][q];
-				float x = xs[s0][q];
-				if (((gx + gy + pc) & 1) == 0 && l0_active(cc))
-					x = (rs[s0][q] * l0_scale(cc) + zm_nb_sum(cs[s0], xs[s0], cs[sm], xs[sm], xs[su], q, T::RX)) /
-					    l0_diag(cc);
-				z[g] = x;
-				acc += (double)(x * rs[s0][q]);
-			}
+		const int pz = p - 2;
+		if (pz >= k0 && pz < k1 && pos.inc) {
+			const int s0 = ring_back(sp, 2), sm = ring_back(sp, 3), su = ring_back(sp, 1);
+			const int q = pos.qc;
+			const long long g = pos.gc + d.sz * pz;
+			const uint8_t cc = cs[s0][q];
+			float x = xs[s0][q];
+			if (((pos.parc + pz) & 1) == 0 && l0_active(cc))
+				x = l0_div(rs[s0][q] * l0_scale(cc) + zm_nb_sum(cs[s0], xs[s0], cs[sm], xs[sm], xs[su], q, T::RX), cc);
+			z[g] = x;
+			acc += (double)(x * rs[s0][q]);
 		}
 	}
 	const double s = block_sum(acc, sh);
