@@ -11,6 +11,8 @@
 // gathers are served from L1/L2.  The MacCormack backward step, correction
 // and clamp are fused into one pass (the backward value of a cell is only
 // needed at that cell), so order-2 advection is two full passes.
+#include <type_traits>
+
 #include "mfx_internal.h"
 #include "mfx_prof.h"
 
@@ -26,37 +28,77 @@ __device__ inline bool interior(const Dims& d, int i, int j, int k) {
 	return i >= 1 && i < d.nx - 1 && j >= 1 && j < d.ny - 1 && (d.is3d ? (k >= 1 && k < d.nz - 1) : true);
 }
 
+// Cell geometry of the advection kernels.  IS3D is the grid's dimension,
+// fixed at compile time (2D grids have z stride 0, as the reference's grids
+// do).  I is the index type: int when the grid has fewer than 2^31 cells and
+// nx and ny*nz are below 2^24 (checked on the host; two 24-bit multiply-adds
+// then give the exact index), long long otherwise.
+template <bool IS3D, typename I>
+struct Geo {
+	int nx, ny, nz;
+	I Y, Z;
+	__device__ explicit Geo(const Dims& d) : nx(d.nx), ny(d.ny), nz(d.nz), Y((I)d.nx), Z(IS3D ? (I)d.sz : (I)0) {}
+	__device__ I at(int i, int j, int k) const {
+		if constexpr (sizeof(I) == 4) {
+			const unsigned row = IS3D ? __umul24((unsigned)k, (unsigned)ny) + (unsigned)j : (unsigned)j;
+			return (I)(__umul24(row, (unsigned)nx) + (unsigned)i);
+		} else {
+			return (I)i + Y * j + Z * k;
+		}
+	}
+	__device__ bool interior(int i, int j, int k) const {
+		return i >= 1 && i < nx - 1 && j >= 1 && j < ny - 1 && (IS3D ? (k >= 1 && k < nz - 1) : true);
+	}
+};
+
+inline bool index32_ok(const Dims& d) {
+	return d.n < (1ll << 31) && d.nx < (1 << 24) && (long long)d.ny * d.nz < (1ll << 24);
+}
+
 // MACGrid::getCentered (grid.h:424-437)
-__device__ inline void getCentered(const Vel& V, const Dims& d, long long ix, float* o) {
+template <bool IS3D, typename I>
+__device__ inline void getCentered(const Vel& V, const Geo<IS3D, I>& g, I ix, float* o) {
 	o[0] = 0.5f * (V.u[ix] + V.u[ix + 1]);
-	o[1] = 0.5f * (V.v[ix] + V.v[ix + d.nx]);
+	o[1] = 0.5f * (V.v[ix] + V.v[ix + g.Y]);
 	o[2] = 0.f;
-	if (d.is3d) o[2] = 0.5f * (V.w[ix] + V.w[ix + d.sz]);
+	if (IS3D) o[2] = 0.5f * (V.w[ix] + V.w[ix + g.Z]);
 }
-// MACGrid::getAtMACX/Y/Z (grid.h:439-470); Z is not guarded by is3D
-__device__ inline void getAtMACX(const Vel& V, const Dims& d, long long ix, float* o) {
-	const long long Y = d.nx, Z = d.sz;
+// MACGrid::getAtMACX/Y/Z (grid.h:439-470); Z is not guarded by is3D (in 2D
+// its z offsets are 0)
+template <bool IS3D, typename I>
+__device__ inline void getAtMACX(const Vel& V, const Geo<IS3D, I>& g, I ix, float* o) {
+	const float* __restrict__ v = V.v + ix;
+	const float* __restrict__ w = V.w + ix;
+	const I Y = g.Y, Z = g.Z;
 	o[0] = V.u[ix];
-	o[1] = 0.25f * (V.v[ix] + V.v[ix - 1] + V.v[ix + Y] + V.v[ix + Y - 1]);
+	o[1] = 0.25f * (v[0] + v[-1] + v[Y] + v[Y - 1]);
 	o[2] = 0.f;
-	if (d.is3d) o[2] = 0.25f * (V.w[ix] + V.w[ix - 1] + V.w[ix + Z] + V.w[ix + Z - 1]);
+	if (IS3D) o[2] = 0.25f * (w[0] + w[-1] + w[Z] + w[Z - 1]);
 }
-__device__ inline void getAtMACY(const Vel& V, const Dims& d, long long ix, float* o) {
-	const long long Y = d.nx, Z = d.sz;
-	o[0] = 0.25f * (V.u[ix] + V.u[ix - Y] + V.u[ix + 1] + V.u[ix + 1 - Y]);
+template <bool IS3D, typename I>
+__device__ inline void getAtMACY(const Vel& V, const Geo<IS3D, I>& g, I ix, float* o) {
+	const float* __restrict__ u = V.u + ix;
+	const float* __restrict__ w = V.w + ix;
+	const I Y = g.Y, Z = g.Z;
+	o[0] = 0.25f * (u[0] + u[-Y] + u[1] + u[1 - Y]);
 	o[1] = V.v[ix];
 	o[2] = 0.f;
-	if (d.is3d) o[2] = 0.25f * (V.w[ix] + V.w[ix - Y] + V.w[ix + Z] + V.w[ix + Z - Y]);
+	if (IS3D) o[2] = 0.25f * (w[0] + w[-Y] + w[Z] + w[Z - Y]);
 }
-__device__ inline void getAtMACZ(const Vel& V, const Dims& d, long long ix, float* o) {
-	const long long Y = d.nx, Z = d.sz;
-	o[0] = 0.25f * (V.u[ix] + V.u[ix - Z] + V.u[ix + 1] + V.u[ix + 1 - Z]);
-	o[1] = 0.25f * (V.v[ix] + V.v[ix - Z] + V.v[ix + Y] + V.v[ix + Y - Z]);
+template <bool IS3D, typename I>
+__device__ inline void getAtMACZ(const Vel& V, const Geo<IS3D, I>& g, I ix, float* o) {
+	const float* __restrict__ u = V.u + ix;
+	const float* __restrict__ v = V.v + ix;
+	const I Y = g.Y, Z = g.Z;
+	o[0] = 0.25f * (u[0] + u[-Z] + u[1] + u[1 - Z]);
+	o[1] = 0.25f * (v[0] + v[-Z] + v[Y] + v[Y - Z]);
 	o[2] = V.w[ix];
 }
 
-// interpol<Real> / interpolComponent (interpol.h:50-90)
-__device__ inline float interp(const float* __restrict__ data, const Dims& d, float posx, float posy,
+// interpol<Real> / interpolComponent (interpol.h:50-90).  The eight corners
+// are read at one base index plus constant strides.
+template <bool IS3D, typename I>
+__device__ inline float interp(const float* __restrict__ data, const Geo<IS3D, I>& g, float posx, float posy,
                                float posz) {
 	float px = posx - 0.5f, py = posy - 0.5f, pz = posz - 0.5f;
 	int xi = (int)px, yi = (int)py, zi = (int)pz;
@@ -66,16 +108,15 @@ __device__ inline float interp(const float* __restrict__ data, const Dims& d, fl
 	if (px < 0.f) { xi = 0; s0 = 1.0f; s1 = 0.0f; }
 	if (py < 0.f) { yi = 0; t0 = 1.0f; t1 = 0.0f; }
 	if (pz < 0.f) { zi = 0; f0 = 1.0f; f1 = 0.0f; }
-	if (xi >= d.nx - 1) { xi = d.nx - 2; s0 = 0.0f; s1 = 1.0f; }
-	if (yi >= d.ny - 1) { yi = d.ny - 2; t0 = 0.0f; t1 = 1.0f; }
-	if (d.nz > 1) {
-		if (zi >= d.nz - 1) { zi = d.nz - 2; f0 = 0.0f; f1 = 1.0f; }
+	if (xi >= g.nx - 1) { xi = g.nx - 2; s0 = 0.0f; s1 = 1.0f; }
+	if (yi >= g.ny - 1) { yi = g.ny - 2; t0 = 0.0f; t1 = 1.0f; }
+	if (g.nz > 1) {
+		if (zi >= g.nz - 1) { zi = g.nz - 2; f0 = 0.0f; f1 = 1.0f; }
 	}
-	const long long X = 1, Y = d.nx, Z = d.sz;
-	const long long idx = (long long)xi + Y * yi + Z * zi;
-	return ((data[idx] * t0 + data[idx + Y] * t1) * s0 + (data[idx + X] * t0 + data[idx + X + Y] * t1) * s1) * f0 +
-	       ((data[idx + Z] * t0 + data[idx + Y + Z] * t1) * s0 +
-	        (data[idx + X + Z] * t0 + data[idx + X + Y + Z] * t1) * s1) * f1;
+	const I Y = g.Y, Z = g.Z;
+	const float* __restrict__ p = data + g.at(xi, yi, IS3D ? zi : 0);
+	return ((p[0] * t0 + p[Y] * t1) * s0 + (p[1] * t0 + p[1 + Y] * t1) * s1) * f0 +
+	       ((p[Z] * t0 + p[Y + Z] * t1) * s0 + (p[1 + Z] * t0 + p[1 + Y + Z] * t1) * s1) * f1;
 }
 
 __device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -84,6 +125,11 @@ __device__ inline float clampf_ref(float v, float lo, float hi) {
 	if (v > hi) return hi;
 	return v;
 }
+// getMinMax (advection.cpp): strict comparisons, first of equal values kept
+__device__ inline void min_max(float v, float& mn, float& mx) {
+	if (v < mn) mn = v;
+	if (v > mx) mx = v;
+}
 
 #define CHECKFLAG(ix) (f8[ix] & (kFluid | kEmpty))
 
@@ -91,15 +137,16 @@ __device__ inline float clampf_ref(float v, float lo, float hi) {
 // forward SL step for centered grids (1 or 3 components): dst = 0 on the
 // outer layer (the reference writes a zeroed temporary with bnd=1)
 // ---------------------------------------------------------------------------
-template <int NC>
+template <int NC, bool IS3D, typename I>
 __global__ void __launch_bounds__(kBlock) k_sl_centered(Vel V, Dims d, Tiles t, float dt, const float* __restrict__ s0,
                                                         const float* __restrict__ s1, const float* __restrict__ s2,
                                                         float* __restrict__ o0, float* __restrict__ o1,
                                                         float* __restrict__ o2) {
-	CellPos p = tile_cell(t);
+	const CellPos p = tile_cell(t);
 	if (p.i >= d.nx || p.j >= d.ny) return;
-	const long long ix = d.idx(p.i, p.j, p.k);
-	if (!interior(d, p.i, p.j, p.k)) {
+	const Geo<IS3D, I> g(d);
+	const I ix = g.at(p.i, p.j, p.k);
+	if (!g.interior(p.i, p.j, p.k)) {
 		o0[ix] = 0.f;
 		if (NC == 3) {
 			o1[ix] = 0.f;
@@ -108,46 +155,49 @@ __global__ void __launch_bounds__(kBlock) k_sl_centered(Vel V, Dims d, Tiles t, 
 		return;
 	}
 	float c[3];
-	getCentered(V, d, ix, c);
+	getCentered(V, g, ix, c);
 	const float px = ((float)p.i + 0.5f) - c[0] * dt;
 	const float py = ((float)p.j + 0.5f) - c[1] * dt;
 	const float pz = ((float)p.k + 0.5f) - c[2] * dt;
-	o0[ix] = interp(s0, d, px, py, pz);
+	o0[ix] = interp(s0, g, px, py, pz);
 	if (NC == 3) {
-		o1[ix] = interp(s1, d, px, py, pz);
-		o2[ix] = interp(s2, d, px, py, pz);
+		o1[ix] = interp(s1, g, px, py, pz);
+		o2[ix] = interp(s2, g, px, py, pz);
 	}
 }
 
-// SemiLagrangeMAC value of one cell (advection.cpp:44-57)
-__device__ inline void sl_mac_cell(const Vel& V, const Dims& d, long long ix, int i, int j, int k, float dt,
-                                   const float* __restrict__ su, const float* __restrict__ sv,
-                                   const float* __restrict__ sw, float* out) {
+// SemiLagrangeMAC (advection.cpp:44-57) of one cell from the face velocities
+// vx/vy/vz (getAtMACX/Y/Z at the cell)
+template <bool IS3D, typename I>
+__device__ inline void sl_mac_cell(const Geo<IS3D, I>& g, int i, int j, int k, float dt, const float* vx,
+                                   const float* vy, const float* vz, const float* __restrict__ su,
+                                   const float* __restrict__ sv, const float* __restrict__ sw, float* out) {
 	const float p0x = (float)i + 0.5f, p0y = (float)j + 0.5f, p0z = (float)k + 0.5f;
-	float v[3];
-	getAtMACX(V, d, ix, v);
-	out[0] = interp(su, d, p0x - v[0] * dt, p0y - v[1] * dt, p0z - v[2] * dt);
-	getAtMACY(V, d, ix, v);
-	out[1] = interp(sv, d, p0x - v[0] * dt, p0y - v[1] * dt, p0z - v[2] * dt);
-	getAtMACZ(V, d, ix, v);
-	out[2] = interp(sw, d, p0x - v[0] * dt, p0y - v[1] * dt, p0z - v[2] * dt);
+	out[0] = interp(su, g, p0x - vx[0] * dt, p0y - vx[1] * dt, p0z - vx[2] * dt);
+	out[1] = interp(sv, g, p0x - vy[0] * dt, p0y - vy[1] * dt, p0z - vy[2] * dt);
+	out[2] = interp(sw, g, p0x - vz[0] * dt, p0y - vz[1] * dt, p0z - vz[2] * dt);
 }
 
+template <bool IS3D, typename I>
 __global__ void __launch_bounds__(kBlock) k_sl_mac(Vel V, Dims d, Tiles t, float dt, const float* __restrict__ su,
                                                    const float* __restrict__ sv, const float* __restrict__ sw,
                                                    float* __restrict__ ou, float* __restrict__ ov,
                                                    float* __restrict__ ow) {
-	CellPos p = tile_cell(t);
+	const CellPos p = tile_cell(t);
 	if (p.i >= d.nx || p.j >= d.ny) return;
-	const long long ix = d.idx(p.i, p.j, p.k);
-	if (!interior(d, p.i, p.j, p.k)) {
+	const Geo<IS3D, I> g(d);
+	const I ix = g.at(p.i, p.j, p.k);
+	if (!g.interior(p.i, p.j, p.k)) {
 		ou[ix] = 0.f;
 		ov[ix] = 0.f;
 		ow[ix] = 0.f;
 		return;
 	}
-	float out[3];
-	sl_mac_cell(V, d, ix, p.i, p.j, p.k, dt, su, sv, sw, out);
+	float vx[3], vy[3], vz[3], out[3];
+	getAtMACX(V, g, ix, vx);
+	getAtMACY(V, g, ix, vy);
+	getAtMACZ(V, g, ix, vz);
+	sl_mac_cell(g, p.i, p.j, p.k, dt, vx, vy, vz, su, sv, sw, out);
 	ou[ix] = out[0];
 	ov[ix] = out[1];
 	ow[ix] = out[2];
@@ -157,7 +207,7 @@ __global__ void __launch_bounds__(kBlock) k_sl_mac(Vel V, Dims d, Tiles t, float
 // MacCormack for centered grids: backward SL (fwd, -dt), correction
 // (MacCormackCorrect, all idx) and clamp (MacCormackClamp, bnd=1) fused.
 // ---------------------------------------------------------------------------
-template <int NC>
+template <int NC, bool IS3D, typename I>
 __global__ void __launch_bounds__(kBlock) k_maccormack_centered(Vel V, Dims d, Tiles t, float dt, float strength,
                                                                 int clampMode, const uint8_t* __restrict__ f8,
                                                                 const float* o0, const float* o1, const float* o2,
@@ -165,17 +215,18 @@ __global__ void __launch_bounds__(kBlock) k_maccormack_centered(Vel V, Dims d, T
                                                                 const float* __restrict__ fw1,
                                                                 const float* __restrict__ fw2, float* __restrict__ n0,
                                                                 float* __restrict__ n1, float* __restrict__ n2) {
-	CellPos p = tile_cell(t);
+	const CellPos p = tile_cell(t);
 	if (p.i >= d.nx || p.j >= d.ny) return;
-	const long long ix = d.idx(p.i, p.j, p.k);
-	const bool inner = interior(d, p.i, p.j, p.k);
+	const Geo<IS3D, I> g(d);
+	const I ix = g.at(p.i, p.j, p.k);
+	const bool inner = g.interior(p.i, p.j, p.k);
 	const float* orig[3] = {o0, o1, o2};
 	const float* fwd[3] = {fw0, fw1, fw2};
 	float* dst[3] = {n0, n1, n2};
 	float vc[3] = {0.f, 0.f, 0.f};
 	float px = 0.f, py = 0.f, pz = 0.f;
 	if (inner) {
-		getCentered(V, d, ix, vc);
+		getCentered(V, g, ix, vc);
 		// bwd position uses -dt: pos - v*(-dt)
 		const float mdt = -dt;
 		px = ((float)p.i + 0.5f) - vc[0] * mdt;
@@ -184,8 +235,9 @@ __global__ void __launch_bounds__(kBlock) k_maccormack_centered(Vel V, Dims d, T
 	}
 	const bool fl = f8[ix] & kFluid;
 	float val[3], fv[3];
+#pragma unroll
 	for (int c = 0; c < NC; c++) {
-		const float bwd = inner ? interp(fwd[c], d, px, py, pz) : 0.f;
+		const float bwd = inner ? interp(fwd[c], g, px, py, pz) : 0.f;
 		fv[c] = fwd[c][ix];
 		float x = fv[c];
 		if (fl) {
@@ -203,29 +255,25 @@ __global__ void __launch_bounds__(kBlock) k_maccormack_centered(Vel V, Dims d, T
 		// doClampComponent (advection.cpp:144-186) with pos = (i,j,k)
 		const float vdt[3] = {vc[0] * dt, vc[1] * dt, vc[2] * dt};
 		const float pos[3] = {(float)p.i, (float)p.j, (float)p.k};
-		const int gux = d.nx - 1, guy = d.ny - 1, guz = d.nz - 1;
+		const int gux = g.nx - 1, guy = g.ny - 1, guz = g.nz - 1;
 		float mn[3] = {3.402823466e+38f, 3.402823466e+38f, 3.402823466e+38f};
 		float mx[3] = {-3.402823466e+38f, -3.402823466e+38f, -3.402823466e+38f};
 		bool haveFl = false;
-		const int numPos = clampMode == 1 ? 2 : 1;
-		for (int l = 0; l < numPos; ++l) {
-			const float sg = l == 0 ? -1.f : 1.f;
+		const I Y = g.Y, Z = g.Z;
+#pragma unroll
+		for (int l = 0; l < 2; ++l) {
+			if (l == 1 && clampMode != 1) break;
 			int P[3];
 			for (int a = 0; a < 3; a++) P[a] = (int)(l == 0 ? pos[a] - vdt[a] : pos[a] + vdt[a]);
-			(void)sg;
 			const int i0 = clampi(P[0], 0, gux - 1), j0 = clampi(P[1], 0, guy - 1);
-			const int k0 = clampi(P[2], 0, d.is3d ? guz - 1 : 1);
-			const int i1 = i0 + 1, j1 = j0 + 1, k1 = d.is3d ? k0 + 1 : k0;
-			const int nc = d.is3d ? 8 : 4;
-			for (int q = 0; q < nc; q++) {
-				const int ci = (q & 1) ? i1 : i0, cj = (q & 2) ? j1 : j0, ck = (q & 4) ? k1 : k0;
-				const long long cx = d.idx(ci, cj, ck);
+			const I c0 = g.at(i0, j0, IS3D ? clampi(P[2], 0, guz - 1) : 0);
+			// corners (i0|i1, j0|j1, k0|k1) in the reference's order
+			const I off[8] = {0, 1, Y, 1 + Y, Z, 1 + Z, Y + Z, 1 + Y + Z};
+#pragma unroll
+			for (int q = 0; q < (IS3D ? 8 : 4); q++) {
+				const I cx = c0 + off[q];
 				if (CHECKFLAG(cx)) {
-					for (int c = 0; c < NC; c++) {
-						const float ov = orig[c][cx];
-						if (ov < mn[c]) mn[c] = ov;
-						if (ov > mx[c]) mx[c] = ov;
-					}
+					for (int c = 0; c < NC; c++) min_max(orig[c][cx], mn[c], mx[c]);
 					haveFl = true;
 				}
 			}
@@ -248,9 +296,9 @@ __global__ void __launch_bounds__(kBlock) k_maccormack_centered(Vel V, Dims d, T
 				pb[a] = (int)(base + vdt[a]);
 			}
 			if (pf[0] < 0 || pf[1] < 0 || pf[2] < 0 || pb[0] < 0 || pb[1] < 0 || pb[2] < 0 || pf[0] > gux ||
-			    pf[1] > guy || ((pf[2] > guz) && d.is3d) || pb[0] > gux || pb[1] > guy ||
-			    ((pb[2] > guz) && d.is3d) || (f8[d.idx(pf[0], pf[1], pf[2])] & kObstacle) ||
-			    (f8[d.idx(pb[0], pb[1], pb[2])] & kObstacle)) {
+			    pf[1] > guy || ((pf[2] > guz) && IS3D) || pb[0] > gux || pb[1] > guy || ((pb[2] > guz) && IS3D) ||
+			    (f8[g.at(pf[0], pf[1], IS3D ? pf[2] : 0)] & kObstacle) ||
+			    (f8[g.at(pb[0], pb[1], IS3D ? pb[2] : 0)] & kObstacle)) {
 				for (int c = 0; c < NC; c++) val[c] = fv[c];
 			}
 		}
@@ -258,32 +306,35 @@ __global__ void __launch_bounds__(kBlock) k_maccormack_centered(Vel V, Dims d, T
 	for (int c = 0; c < NC; c++) dst[c][ix] = val[c];
 }
 
-// doClampComponentMAC<c> (advection.cpp:191-235)
-__device__ inline float clamp_comp_mac(const Dims& d, const uint8_t* __restrict__ f8, int c, float dst,
+// doClampComponentMAC<C> (advection.cpp:191-235); vel = face velocity * dt
+template <int C, bool IS3D, typename I>
+__device__ inline float clamp_comp_mac(const Geo<IS3D, I>& g, const uint8_t* __restrict__ f8, I ix, float dst,
                                        const float* __restrict__ origc, float fwd, int i, int j, int k,
                                        const float* vel, int clampMode) {
-	const int gux = d.nx - 1, guy = d.ny - 1, guz = d.nz - 1;
-	const float pos[3] = {(float)i, (float)j, (float)k};
+	const I Y = g.Y, Z = g.Z;
 	if (clampMode == 2) {
-		const int o[3] = {i, j, k};
-		int nb[3] = {i, j, k};
-		nb[c] -= 1;
-		if (!(CHECKFLAG(d.idx(o[0], o[1], o[2])) && CHECKFLAG(d.idx(nb[0], nb[1], nb[2])))) return fwd;
+		const I nb = ix - (C == 0 ? (I)1 : (C == 1 ? Y : Z));
+		if (!(CHECKFLAG(ix) && CHECKFLAG(nb))) return fwd;
 	}
+	const int gux = g.nx - 1, guy = g.ny - 1, guz = g.nz - 1;
+	const float pos[3] = {(float)i, (float)j, (float)k};
 	float mn = 3.402823466e+38f, mx = -3.402823466e+38f;
-	const int numPos = clampMode == 1 ? 2 : 1;
-	for (int l = 0; l < numPos; ++l) {
+#pragma unroll
+	for (int l = 0; l < 2; ++l) {
+		if (l == 1 && clampMode != 1) break;
 		int P[3];
 		for (int a = 0; a < 3; a++) P[a] = (int)(l == 0 ? pos[a] - vel[a] : pos[a] + vel[a]);
 		const int i0 = clampi(P[0], 0, gux - 1), j0 = clampi(P[1], 0, guy - 1);
-		const int k0 = clampi(P[2], 0, d.is3d ? guz - 1 : 0);
-		const int i1 = i0 + 1, j1 = j0 + 1, k1 = d.is3d ? k0 + 1 : k0;
-		const int nc = d.is3d ? 8 : 4;
-		for (int q = 0; q < nc; q++) {
-			const int ci = (q & 1) ? i1 : i0, cj = (q & 2) ? j1 : j0, ck = (q & 4) ? k1 : k0;
-			const float ov = origc[d.idx(ci, cj, ck)];
-			if (ov < mn) mn = ov;
-			if (ov > mx) mx = ov;
+		const float* __restrict__ o = origc + g.at(i0, j0, IS3D ? clampi(P[2], 0, guz - 1) : 0);
+		min_max(o[0], mn, mx);
+		min_max(o[1], mn, mx);
+		min_max(o[Y], mn, mx);
+		min_max(o[1 + Y], mn, mx);
+		if (IS3D) {
+			min_max(o[Z], mn, mx);
+			min_max(o[1 + Z], mn, mx);
+			min_max(o[Y + Z], mn, mx);
+			min_max(o[1 + Y + Z], mn, mx);
 		}
 	}
 	if (clampMode == 1) return clampf_ref(dst, mn, mx);
@@ -293,45 +344,53 @@ __device__ inline float clamp_comp_mac(const Dims& d, const uint8_t* __restrict_
 
 // MacCormack for MAC grids: backward SemiLagrangeMAC (fwd, -dt), correction
 // (MacCormackCorrectMAC, all ijk) and clamp (MacCormackClampMAC, bnd=1) fused.
+// The face velocities of the cell are read once and serve both the backward
+// trace and the clamp.
+template <bool IS3D, typename I>
 __global__ void __launch_bounds__(kBlock) k_maccormack_mac(Vel V, Dims d, Tiles t, float dt, float strength,
                                                            int clampMode, const uint8_t* __restrict__ f8,
                                                            const float* ou, const float* ov, const float* ow,
                                                            const float* __restrict__ fu, const float* __restrict__ fv,
                                                            const float* __restrict__ fw, float* __restrict__ nu,
                                                            float* __restrict__ nv, float* __restrict__ nw) {
-	CellPos p = tile_cell(t);
+	const CellPos p = tile_cell(t);
 	if (p.i >= d.nx || p.j >= d.ny) return;
 	const int i = p.i, j = p.j, k = p.k;
-	const long long ix = d.idx(i, j, k);
-	const bool inner = interior(d, i, j, k);
+	const Geo<IS3D, I> g(d);
+	const I ix = g.at(i, j, k);
+	const bool inner = g.interior(i, j, k);
+	float vx[3], vy[3], vz[3];
 	float bwd[3] = {0.f, 0.f, 0.f};
-	if (inner) sl_mac_cell(V, d, ix, i, j, k, -dt, fu, fv, fw, bwd);
+	if (inner) {
+		getAtMACX(V, g, ix, vx);
+		getAtMACY(V, g, ix, vy);
+		getAtMACZ(V, g, ix, vz);
+		sl_mac_cell(g, i, j, k, -dt, vx, vy, vz, fu, fv, fw, bwd);
+	}
 	// MacCormackCorrectMAC skip logic (isMAC = true)
 	bool skip[3] = {false, false, false};
 	if (!(f8[ix] & kFluid)) skip[0] = skip[1] = skip[2] = true;
 	if ((i > 0) && !(f8[ix - 1] & kFluid)) skip[0] = true;
-	if ((j > 0) && !(f8[ix - d.nx] & kFluid)) skip[1] = true;
-	if ((k > 0) && !(f8[ix - d.sz] & kFluid)) skip[2] = true;
+	if ((j > 0) && !(f8[ix - g.Y] & kFluid)) skip[1] = true;
+	if (IS3D && (k > 0) && !(f8[ix - g.Z] & kFluid)) skip[2] = true;
 	const float* orig[3] = {ou, ov, ow};
 	const float* fwd[3] = {fu, fv, fw};
 	float val[3], fvv[3];
+#pragma unroll
 	for (int c = 0; c < 3; c++) {
 		fvv[c] = fwd[c][ix];
 		if (skip[c]) val[c] = fvv[c];
 		else val[c] = (float)((double)fvv[c] + (double)strength * 0.5 * (double)(orig[c][ix] - bwd[c]));
 	}
 	if (inner) {
-		float v[3], vdt[3];
-		getAtMACX(V, d, ix, v);
-		for (int a = 0; a < 3; a++) vdt[a] = v[a] * dt;
-		val[0] = clamp_comp_mac(d, f8, 0, val[0], ou, fvv[0], i, j, k, vdt, clampMode);
-		getAtMACY(V, d, ix, v);
-		for (int a = 0; a < 3; a++) vdt[a] = v[a] * dt;
-		val[1] = clamp_comp_mac(d, f8, 1, val[1], ov, fvv[1], i, j, k, vdt, clampMode);
-		if (d.is3d) {
-			getAtMACZ(V, d, ix, v);
-			for (int a = 0; a < 3; a++) vdt[a] = v[a] * dt;
-			val[2] = clamp_comp_mac(d, f8, 2, val[2], ow, fvv[2], i, j, k, vdt, clampMode);
+		float vdt[3];
+		for (int a = 0; a < 3; a++) vdt[a] = vx[a] * dt;
+		val[0] = clamp_comp_mac<0>(g, f8, ix, val[0], ou, fvv[0], i, j, k, vdt, clampMode);
+		for (int a = 0; a < 3; a++) vdt[a] = vy[a] * dt;
+		val[1] = clamp_comp_mac<1>(g, f8, ix, val[1], ov, fvv[1], i, j, k, vdt, clampMode);
+		if (IS3D) {
+			for (int a = 0; a < 3; a++) vdt[a] = vz[a] * dt;
+			val[2] = clamp_comp_mac<2>(g, f8, ix, val[2], ow, fvv[2], i, j, k, vdt, clampMode);
 		}
 	}
 	nu[ix] = val[0];
@@ -516,6 +575,20 @@ void op_add_buoyancy(Scene* s, Grid* flags, Grid* density, Grid* vel, const floa
 	                   (const float*)density->real(), vel->comp(0), vel->comp(1), vel->comp(2), f[0], f[1], f[2]);
 }
 
+// Calls f(std::integral_constant<bool, IS3D>, I{}) with the grid's dimension
+// and the narrowest index type that addresses it.
+template <typename F>
+static void with_geo(const Dims& d, F&& f) {
+	const bool i32 = index32_ok(d);
+	if (d.is3d) {
+		if (i32) f(std::true_type{}, int{});
+		else f(std::true_type{}, (long long){});
+	} else {
+		if (i32) f(std::false_type{}, int{});
+		else f(std::false_type{}, (long long){});
+	}
+}
+
 void op_advect(Scene* s, Grid* flags, Grid* vel, Grid* grid, int order, float strength, int clampMode) {
 	const Dims d = s->d;
 	const Tiles t = tiles_for(d);
@@ -527,17 +600,23 @@ void op_advect(Scene* s, Grid* flags, Grid* vel, Grid* grid, int order, float st
 		float* fw[3] = {fwd, fwd + s->cstride, fwd + 2 * s->cstride};
 		{
 			KtScope kt("advect_sl_mac", s->stream);
-			hipLaunchKernelGGL(k_sl_mac, gb, tb, 0, s->stream, V, d, t, dt, (const float*)grid->comp(0),
-			                   (const float*)grid->comp(1), (const float*)grid->comp(2), fw[0], fw[1], fw[2]);
+			with_geo(d, [&](auto D3, auto ix) {
+				hipLaunchKernelGGL((k_sl_mac<decltype(D3)::value, decltype(ix)>), gb, tb, 0, s->stream, V, d, t, dt,
+				                   (const float*)grid->comp(0), (const float*)grid->comp(1),
+				                   (const float*)grid->comp(2), fw[0], fw[1], fw[2]);
+			});
 		}
 		float* result = fwd;
 		if (order == 2) {
 			float* ng = s->getVec();
 			float* nw[3] = {ng, ng + s->cstride, ng + 2 * s->cstride};
 			KtScope kt("advect_maccormack_mac", s->stream);
-			hipLaunchKernelGGL(k_maccormack_mac, gb, tb, 0, s->stream, V, d, t, dt, strength, clampMode, flags->f8,
-			                   (const float*)grid->comp(0), (const float*)grid->comp(1), (const float*)grid->comp(2),
-			                   (const float*)fw[0], (const float*)fw[1], (const float*)fw[2], nw[0], nw[1], nw[2]);
+			with_geo(d, [&](auto D3, auto ix) {
+				hipLaunchKernelGGL((k_maccormack_mac<decltype(D3)::value, decltype(ix)>), gb, tb, 0, s->stream, V, d,
+				                   t, dt, strength, clampMode, flags->f8, (const float*)grid->comp(0),
+				                   (const float*)grid->comp(1), (const float*)grid->comp(2), (const float*)fw[0],
+				                   (const float*)fw[1], (const float*)fw[2], nw[0], nw[1], nw[2]);
+			});
 			s->putVec(fwd);
 			result = ng;
 		}
@@ -564,24 +643,32 @@ void op_advect(Scene* s, Grid* flags, Grid* vel, Grid* grid, int order, float st
 	float* fwd = nc == 3 ? s->getVec() : s->getReal();
 	float* fw[3] = {fwd, nc == 3 ? fwd + s->cstride : nullptr, nc == 3 ? fwd + 2 * s->cstride : nullptr};
 	const float* src[3] = {grid->comp(0), nc == 3 ? grid->comp(1) : nullptr, nc == 3 ? grid->comp(2) : nullptr};
-	if (nc == 1)
-		hipLaunchKernelGGL(k_sl_centered<1>, gb, tb, 0, s->stream, V, d, t, dt, src[0], src[1], src[2], fw[0], fw[1],
-		                   fw[2]);
-	else
-		hipLaunchKernelGGL(k_sl_centered<3>, gb, tb, 0, s->stream, V, d, t, dt, src[0], src[1], src[2], fw[0], fw[1],
-		                   fw[2]);
+	with_geo(d, [&](auto D3, auto ix) {
+		constexpr bool D = decltype(D3)::value;
+		using I = decltype(ix);
+		if (nc == 1)
+			hipLaunchKernelGGL((k_sl_centered<1, D, I>), gb, tb, 0, s->stream, V, d, t, dt, src[0], src[1], src[2],
+			                   fw[0], fw[1], fw[2]);
+		else
+			hipLaunchKernelGGL((k_sl_centered<3, D, I>), gb, tb, 0, s->stream, V, d, t, dt, src[0], src[1], src[2],
+			                   fw[0], fw[1], fw[2]);
+	});
 	float* result = fwd;
 	if (order == 2) {
 		float* ng = nc == 3 ? s->getVec() : s->getReal();
 		float* nw[3] = {ng, nc == 3 ? ng + s->cstride : nullptr, nc == 3 ? ng + 2 * s->cstride : nullptr};
-		if (nc == 1)
-			hipLaunchKernelGGL(k_maccormack_centered<1>, gb, tb, 0, s->stream, V, d, t, dt, strength, clampMode,
-			                   flags->f8, src[0], src[1], src[2], (const float*)fw[0], (const float*)fw[1],
-			                   (const float*)fw[2], nw[0], nw[1], nw[2]);
-		else
-			hipLaunchKernelGGL(k_maccormack_centered<3>, gb, tb, 0, s->stream, V, d, t, dt, strength, clampMode,
-			                   flags->f8, src[0], src[1], src[2], (const float*)fw[0], (const float*)fw[1],
-			                   (const float*)fw[2], nw[0], nw[1], nw[2]);
+		with_geo(d, [&](auto D3, auto ix) {
+			constexpr bool D = decltype(D3)::value;
+			using I = decltype(ix);
+			if (nc == 1)
+				hipLaunchKernelGGL((k_maccormack_centered<1, D, I>), gb, tb, 0, s->stream, V, d, t, dt, strength,
+				                   clampMode, flags->f8, src[0], src[1], src[2], (const float*)fw[0],
+				                   (const float*)fw[1], (const float*)fw[2], nw[0], nw[1], nw[2]);
+			else
+				hipLaunchKernelGGL((k_maccormack_centered<3, D, I>), gb, tb, 0, s->stream, V, d, t, dt, strength,
+				                   clampMode, flags->f8, src[0], src[1], src[2], (const float*)fw[0],
+				                   (const float*)fw[1], (const float*)fw[2], nw[0], nw[1], nw[2]);
+		});
 		if (nc == 3) s->putVec(fwd);
 		else s->putReal(fwd);
 		result = ng;

@@ -153,7 +153,7 @@ struct SmoothP {
 	const float* b;
 	const float* xin;
 	float* xout;
-	int first, reverse;
+	int first, reverse;  // the mode of the launch: pre-smoothing 1/0, post-smoothing 0/1
 	LvDev cp;
 	const float* xcp;
 	Canon cn;
@@ -210,7 +210,9 @@ inline long long coarsest_tiles(int nc) { return (nc + 3) / 4; }
 void run_l0_down_many(hipStream_t s, bool is3d, unsigned nb, const WorkItem* items, const L0DownP* p);
 void run_l0_up_many(hipStream_t s, bool is3d, unsigned nb, const WorkItem* items, const L0UpP* p);
 void run_restrict_many(hipStream_t s, bool is3d, unsigned nb, const WorkItem* items, const RestrictP* p);
-void run_smooth_many(hipStream_t s, bool is3d, bool bf16, unsigned nb, const WorkItem* items, const SmoothP* p);
+// post: post-smoothing (prolongation on load, reversed colours), else pre-smoothing
+void run_smooth_many(hipStream_t s, bool is3d, bool bf16, bool post, unsigned nb, const WorkItem* items,
+                     const SmoothP* p);
 void run_resid_many(hipStream_t s, bool is3d, bool bf16, unsigned nb, const WorkItem* items, const ResidP* p);
 void run_coarsest_many(hipStream_t s, unsigned nb, const WorkItem* items, const CoarsestP* p);
 void run_fill_many(hipStream_t s, unsigned nb, const WorkItem* items, const FillP* p);

@@ -122,8 +122,31 @@ constexpr int kTX = 64;  // threads along x per tile (one wavefront row)
 constexpr int kTY = 4;   // rows per tile
 constexpr int kBlock = kTX * kTY;
 
+// Unsigned division by a divisor fixed at launch, for dividends below 2^31:
+// q = (mulhi(n, m) + n) >> l with l = ceil(log2 d) and
+// m = floor(2^32 (2^l - d) / d) + 1, i.e. the 33-bit "round-up" multiplier
+// of Granlund & Montgomery, "Division by invariant integers using
+// multiplication" (PLDI 1994), Thm 4.2 (exact for n < 2^32; n < 2^31 keeps
+// the 32-bit sum from overflowing).  Three scalar instructions instead of a
+// 64-bit division expansion in every tile decode.
+struct FastDiv {
+	unsigned d = 1, m = 1, l = 0;
+};
+inline FastDiv fastdiv_make(unsigned d) {
+	if (d == 0) throw std::runtime_error("fastdiv: zero divisor");
+	FastDiv f;
+	f.d = d;
+	while ((1ull << f.l) < d) f.l++;
+	f.m = (unsigned)(((1ull << 32) * ((1ull << f.l) - d)) / d + 1);
+	return f;
+}
+__host__ __device__ inline unsigned fastdiv(unsigned n, const FastDiv& f) {
+	return ((unsigned)(((unsigned long long)n * f.m) >> 32) + n) >> f.l;
+}
+
 struct Tiles {
 	int tx, ty, tz;  // tiles per axis (tz = nz)
+	FastDiv dx, dy;  // division by tx and ty
 	long long count() const { return (long long)tx * ty * tz; }
 };
 inline Tiles tiles_for(const Dims& d) {
@@ -131,15 +154,18 @@ inline Tiles tiles_for(const Dims& d) {
 	t.tx = (d.nx + kTX - 1) / kTX;
 	t.ty = (d.ny + kTY - 1) / kTY;
 	t.tz = d.nz;
+	if (t.count() >= (1ll << 31)) throw std::runtime_error("grid has too many tiles for one launch");
+	t.dx = fastdiv_make((unsigned)t.tx);
+	t.dy = fastdiv_make((unsigned)t.ty);
 	return t;
 }
 
 // Map the hardware workgroup id to a logical tile so that each of the 8 XCDs
 // (dispatched round-robin by workgroup id) processes one contiguous run of
 // tiles: neighbouring tiles then share halo lines in the same L2.
-__device__ inline long long xcd_tile(long long wg, long long nwg) {
-	const long long q = nwg >> 3, r = nwg & 7;
-	const long long x = wg & 7, slot = wg >> 3;
+__device__ inline unsigned xcd_tile(unsigned wg, unsigned nwg) {
+	const unsigned q = nwg >> 3, r = nwg & 7;
+	const unsigned x = wg & 7, slot = wg >> 3;
 	return x < r ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
 }
 
@@ -148,15 +174,15 @@ struct CellPos {
 	int i, j, k;
 };
 __device__ inline CellPos tile_cell(const Tiles& t) {
-	long long tile = xcd_tile(blockIdx.x, gridDim.x);
-	int bx = (int)(tile % t.tx);
-	long long rest = tile / t.tx;
-	int by = (int)(rest % t.ty);
-	int k = (int)(rest / t.ty);
+	const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+	const unsigned rest = fastdiv(tile, t.dx);
+	const unsigned bx = tile - rest * (unsigned)t.tx;
+	const unsigned k = fastdiv(rest, t.dy);
+	const unsigned by = rest - k * (unsigned)t.ty;
 	CellPos p;
-	p.i = bx * kTX + (threadIdx.x & (kTX - 1));
-	p.j = by * kTY + (threadIdx.x >> 6);
-	p.k = k;
+	p.i = (int)bx * kTX + (int)(threadIdx.x & (kTX - 1));
+	p.j = (int)by * kTY + (int)(threadIdx.x >> 6);
+	p.k = (int)k;
 	return p;
 }
 

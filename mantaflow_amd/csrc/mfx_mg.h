@@ -27,6 +27,7 @@ struct Canon {
 struct LvDev {
 	int nx, ny, nz;
 	long long sy, sz, n;
+	FastDiv fdx, fdy;  // division by nx and ny (vertex index decode)
 };
 
 struct MgLevel {

@@ -37,7 +37,9 @@ __device__ inline float wgt(int i, int j, int k) {
 	return o == 0 ? 1.0f : (o == 1 ? 0.5f : (o == 2 ? 0.25f : 0.125f));
 }
 
-static inline LvDev lvdev(const MgLevel& l) { return LvDev{l.nx, l.ny, l.nz, l.sy, l.sz, l.n}; }
+static inline LvDev lvdev(const MgLevel& l) {
+	return LvDev{l.nx, l.ny, l.nz, l.sy, l.sz, l.n, fastdiv_make((unsigned)l.nx), fastdiv_make((unsigned)l.ny)};
+}
 
 __device__ inline bool l0_active(uint8_t c) { return (c & 7) != 0; }
 __device__ inline float l0_diag(uint8_t c) { return (c & 128) ? (float)(c & 7) * 1e-6f : (float)(c & 7); }
@@ -255,21 +257,22 @@ __global__ void k_mg_classify(LvDev c, int is3d, int S, const float* __restrict_
 // ---------------------------------------------------------------------------
 // V-cycle kernels
 // ---------------------------------------------------------------------------
-// Coarse-level pointwise kernels run on tiles of 64x4 vertices of one plane
-// (a thread per vertex, rows along x).
-long long lc_tiles(const LvDev& c) { return (long long)((c.nx + 63) / 64) * ((c.ny + 3) / 4) * c.nz; }
+// Coarse-level pointwise kernels run on runs of 256 consecutive vertices (a
+// thread per vertex; 32-bit indices: coarse levels hold fewer than 2^31
+// vertices).
+long long lc_tiles(const LvDev& c) { return (c.n + 255) / 256; }
 struct VtxPos {
-	int i, j, k;
+	int i, j, k, v;
 	bool in;
 };
 __device__ inline VtxPos lc_vertex(const LvDev& c, long long blk) {
-	const int nbx = (c.nx + 63) / 64, nby = (c.ny + 3) / 4;
-	const int b = (int)blk;
 	VtxPos p;
-	p.i = (b % nbx) * 64 + (threadIdx.x & 63);
-	p.j = ((b / nbx) % nby) * 4 + (threadIdx.x >> 6);
-	p.k = b / (nbx * nby);
-	p.in = p.i < c.nx && p.j < c.ny;
+	p.v = (int)blk * 256 + (int)threadIdx.x;
+	p.in = p.v < c.n;
+	const unsigned v = p.in ? (unsigned)p.v : 0u, row = fastdiv(v, c.fdx), k = fastdiv(row, c.fdy);
+	p.i = (int)(v - row * (unsigned)c.nx);
+	p.j = (int)(row - k * (unsigned)c.ny);
+	p.k = (int)k;
 	return p;
 }
 
@@ -311,22 +314,32 @@ __device__ __forceinline__ void body_restrict(const RestrictP& P, long long blk,
 	if (st->done) return;
 	const VtxPos p = lc_vertex(c, blk);
 	if (!p.in) return;
-	const int i = p.i, j = p.j, k = p.k;
-	const long long v = i + c.sy * j + c.sz * k;
+	const int i = p.i, j = p.j, k = p.k, v = p.v;
 	float sum = 0.f;
 	if (ctype[v]) {
-		// branch-free: clamped (valid) addresses, every load issued before
-		// the sum; terms outside the level add exactly +0
+		// branch-free: per-axis offsets of the fine coordinates 2V-1..2V+1
+		// (0 where outside the level, so every address is valid), every load
+		// issued before the sum; terms outside the level add exactly +0
 		constexpr int NZ = IS3D ? 3 : 1;
+		long long ax[3], ay[3], az[3];
+		bool kx[3], ky[3], kz[3];
+#pragma unroll
+		for (int d = 0; d < 3; d++) {
+			const int ux = 2 * i + d - 1, uy = 2 * j + d - 1, uz = 2 * k + (IS3D ? d - 1 : 0);
+			kx[d] = ux >= 0 && ux < f.nx;
+			ky[d] = uy >= 0 && uy < f.ny;
+			kz[d] = uz >= 0 && uz < f.nz;
+			ax[d] = kx[d] ? ux : 0;
+			ay[d] = ky[d] ? f.sy * uy : 0;
+			az[d] = kz[d] ? f.sz * uz : 0;
+		}
 		float rv[NZ * 9];
 		bool ok[NZ * 9];
 #pragma unroll
 		for (int e = 0; e < NZ * 9; e++) {
-			const int dx = e % 3 - 1, dy = (e / 3) % 3 - 1, dz = IS3D ? e / 9 - 1 : 0;
-			const int ux = 2 * i + dx, uy = 2 * j + dy, uz = 2 * k + dz;
-			ok[e] = ux >= 0 && uy >= 0 && uz >= 0 && ux < f.nx && uy < f.ny && uz < f.nz;
-			const long long g = ok[e] ? ux + f.sy * uy + f.sz * uz : 0;
-			rv[e] = rf[g];
+			const int ex = e % 3, ey = (e / 3) % 3, ez = IS3D ? e / 9 : 1;
+			ok[e] = kx[ex] && ky[ey] && kz[ez];
+			rv[e] = rf[ax[ex] + ay[ey] + az[ez]];
 		}
 #pragma unroll
 		for (int e = 0; e < NZ * 9; e++) {
@@ -353,8 +366,8 @@ __device__ __forceinline__ void body_resid(const ResidP& P, long long blk, long 
 	if (st->done) return;
 	const VtxPos p = lc_vertex(c, blk);
 	if (!p.in) return;
-	const int i = p.i, j = p.j, k = p.k;
-	const long long v = i + c.sy * j + c.sz * k;
+	const int i = p.i, j = p.j, k = p.k, v = p.v;
+	const int sy = (int)c.sy, sz = (int)c.sz;
 	const uint8_t tv = type[v];
 	if (!tv) {
 		r[v] = 0.f;
@@ -362,17 +375,21 @@ __device__ __forceinline__ void body_resid(const ResidP& P, long long blk, long 
 	}
 	float sum = b[v];
 	if (tv == 2) {
-		// canonical stencil, every neighbour inside and active
+		// canonical stencil, every neighbour inside and active: one fused
+		// multiply-add chain per z layer
+		const float* __restrict__ xv = x + v;
+		float acc[IS3D ? 3 : 1];
 #pragma unroll
-		for (int dz = IS3D ? -1 : 0; dz <= (IS3D ? 1 : 0); dz++)
+		for (int lz = 0; lz < (IS3D ? 3 : 1); lz++) {
+			float sacc = 0.f;
 #pragma unroll
 			for (int dy = -1; dy <= 1; dy++)
 #pragma unroll
-				for (int dx = -1; dx <= 1; dx++) {
-					const int s = (dx + 1) + 3 * (dy + 1) + (IS3D ? 9 * (dz + 1) : 0);
-					sum -= cn.a[s] * x[v + dx + c.sy * dy + c.sz * dz];
-				}
-		r[v] = sum;
+				for (int dx = -1; dx <= 1; dx++)
+					sacc = fmaf(cn.a[(dx + 1) + 3 * (dy + 1) + 9 * lz], xv[dx + sy * dy + sz * (IS3D ? lz - 1 : 0)], sacc);
+			acc[lz] = sacc;
+		}
+		r[v] = sum - (IS3D ? (acc[0] + acc[1]) + acc[IS3D ? 2 : 0] : acc[0]);
 		return;
 	}
 	// general stencil, branch-free: neighbours outside the level or inactive
@@ -385,7 +402,7 @@ __device__ __forceinline__ void body_resid(const ResidP& P, long long blk, long 
 		const int dx = s % 3 - 1, dy = (s / 3) % 3 - 1, dz = IS3D ? s / 9 - 1 : 0;
 		const int ni = i + dx, nj = j + dy, nk = k + dz;
 		const bool in = ni >= 0 && nj >= 0 && nk >= 0 && ni < c.nx && nj < c.ny && nk < c.nz;
-		const long long nn = in ? ni + c.sy * nj + c.sz * nk : v;
+		const int nn = in ? ni + sy * nj + sz * nk : v;
 		const uint8_t tn = type[nn];  // unconditional: nn is always a valid vertex
 		ok[s] = in & (tn != 0);
 		av[s] = s >= C ? ldA(A, (long long)(s - C) * c.n + v) : ldA(A, (long long)(C - s) * c.n + nn);
@@ -412,11 +429,11 @@ struct Brick {
 
 __device__ inline void brick_origin(long long blk, int nx, int ny, int BX, int BY, int BZ, int& ox, int& oy,
                                     int& oz) {
-	const int nbx = (nx + BX - 1) / BX, nby = (ny + BY - 1) / BY;
-	ox = (int)(blk % nbx) * BX;
-	const long long rest = blk / nbx;
-	oy = (int)(rest % nby) * BY;
-	oz = (int)(rest / nby) * BZ;
+	const unsigned nbx = (unsigned)(nx + BX - 1) / BX, nby = (unsigned)(ny + BY - 1) / BY;
+	const unsigned b = (unsigned)blk, rest = b / nbx, kz = rest / nby;
+	ox = (int)(b - rest * nbx) * BX;
+	oy = (int)(rest - kz * nby) * BY;
+	oz = (int)kz * BZ;
 }
 
 // sum of x over the fluid neighbours coupled to cell q (coefficient -1 each),
@@ -605,11 +622,11 @@ struct ZTile {
 
 __device__ inline void zm_origin(long long blk, const Dims& d, int TY, int kchunk, int& ox, int& oy, int& k0,
                                  int& k1) {
-	const int ntx = (d.nx + 63) / 64, nty = (d.ny + TY - 1) / TY;
-	ox = (int)(blk % ntx) * 64;
-	const long long rest = blk / ntx;
-	oy = (int)(rest % nty) * TY;
-	k0 = (int)(rest / nty) * kchunk;
+	const unsigned ntx = (unsigned)(d.nx + 63) / 64, nty = (unsigned)(d.ny + TY - 1) / TY;
+	const unsigned b = (unsigned)blk, rest = b / ntx, kz = rest / nty;
+	ox = (int)(b - rest * ntx) * 64;
+	oy = (int)(rest - kz * nty) * TY;
+	k0 = (int)kz * kchunk;
 	k1 = min(d.nz, k0 + kchunk);
 }
 
@@ -916,20 +933,28 @@ __device__ __forceinline__ void body_l0_up_zm(const L0UpP& P, long long blk, lon
 	if (threadIdx.x == 0) part[3 * slot] = s;
 }
 
-// coarse-level brick: 32x4x4 vertices in 3D, 64x8 in 2D (512 threads; rows of
-// 32/64 vertices keep the per-vertex stencil loads in whole cache lines)
+// coarse-level brick: 32x4x4 vertices in 3D, 64x8 in 2D, 512 threads (rows
+// of 32/64 vertices keep the region loads in whole cache lines)
 constexpr int kLcBX3 = 32, kLcBY3 = 4, kLcBZ3 = 4, kLcBX2 = 64, kLcBY2 = 8;
+constexpr int kSmoothNT = 512;
 
 // Coarse-level block Gauss-Seidel: one sweep over all 8 (4 in 2D) colours of
 // the 27-point (9-point) Galerkin stencil inside an LDS brick; halo values
-// are the pre-sweep ones (block-hybrid smoother).  Forward colour order for
-// pre-smoothing, reversed for post-smoothing, which keeps the V-cycle
-// symmetric.  first: start from x = 0 (xin unused).  xcp != null: prolongate
-// from the next coarser level on load (x = xin + I x_{l+1}); inactive coarse
-// vertices always hold 0 there, so no coarse type lookups are needed.  The
-// result goes to xout, never to xin: neighbouring bricks read xin as their
-// halo.
-template <bool IS3D, typename AT>
+// are the pre-sweep ones (block-hybrid smoother).  Two modes (POST):
+//   pre-smoothing:  x starts at 0, forward colour order;
+//   post-smoothing: x = xin + I x_{l+1} (prolongation from the next coarser
+//                   level on load; inactive coarse vertices hold 0 there, so
+//                   no coarse type lookups are needed), reversed colour order,
+// which keeps the V-cycle symmetric.  The result goes to xout, never to xin:
+// neighbouring bricks read xin as their halo.
+// Threads are assigned by colour: the 64 vertices of colour c are the lanes
+// of wavefront c (in 2D, 128 vertices on wavefronts 2c and 2c+1), so each
+// phase runs whole wavefronts.  Every thread loads its vertex's stencil once,
+// before the sweep (canonical vertices take the uniform stencil), and the
+// coarse parents of the brick are staged in LDS.  The stencil sum runs as
+// three fused multiply-add chains, one per z layer.  Vertex indices are
+// 32-bit (coarse levels hold fewer than 2^31 vertices).
+template <bool IS3D, typename AT, bool POST>
 __device__ __forceinline__ void body_smooth(const SmoothP& P, long long blk, long long) {
 	const CgState* __restrict__ st = P.st;
 	const LvDev& c = P.c;
@@ -938,71 +963,42 @@ __device__ __forceinline__ void body_smooth(const SmoothP& P, long long blk, lon
 	const float* __restrict__ b = P.b;
 	const float* __restrict__ xin = P.xin;
 	float* __restrict__ xout = P.xout;
-	const int first = P.first, reverse = P.reverse;
 	const LvDev& cp = P.cp;
 	const float* __restrict__ xcp = P.xcp;
 	const Canon& cn = P.cn;
 	constexpr int BX = IS3D ? kLcBX3 : kLcBX2, BY = IS3D ? kLcBY3 : kLcBY2, BZ = IS3D ? kLcBZ3 : 1;
 	constexpr int RX = BX + 2, RY = BY + 2, RZ = IS3D ? BZ + 2 : 1, RN = RX * RY * RZ, RXY = RX * RY;
-	constexpr int S = IS3D ? 14 : 5, C = S - 1, NCOL = IS3D ? 8 : 4;
+	constexpr int C = IS3D ? 13 : 4, NCOL = IS3D ? 8 : 4, NS = IS3D ? 27 : 9;
+	// one colour's sub-lattice of the brick and the wavefronts that hold it
+	constexpr int HX = BX / 2, HY = BY / 2, NWPC = kSmoothNT / 64 / NCOL;
+	static_assert(BX * BY * BZ == kSmoothNT && HX * HY * (IS3D ? BZ / 2 : 1) == 64 * NWPC, "brick/thread mismatch");
+	// coarse parents of the region: coarse coordinates o/2 - 1 .. (o + B)/2
+	constexpr int CRX = BX / 2 + 2, CRY = BY / 2 + 2, CRZ = IS3D ? BZ / 2 + 2 : 1, CRXY = CRX * CRY;
+	constexpr int CRN = CRXY * CRZ;
 	__shared__ float xs[RN];
+	__shared__ float xc[POST ? CRN : 1];
 	if (st->done) return;
 	int ox, oy, oz;
 	brick_origin(blk, c.nx, c.ny, BX, BY, BZ, ox, oy, oz);
+	const int t = threadIdx.x;
 	const int hz = IS3D ? 1 : 0;
-	// fixed trip count (512 threads): the region loads of all rounds issue together
-#pragma unroll
-	for (int it = 0; it < (RN + 511) / 512; it++) {
-		const int q = threadIdx.x + 512 * it;
-		if (q >= RN) break;
-		const int gx = ox - 1 + q % RX, gy = oy - 1 + (q / RX) % RY, gz = oz - hz + q / RXY;
-		const bool inb = gx >= 0 && gy >= 0 && gz >= 0 && gx < c.nx && gy < c.ny && gz < c.nz;
-		float v = 0.f;
-		if (!first) {
-			// branch-free: a valid address for every load, results selected
-			const int sx = inb ? gx : 0, sy = inb ? gy : 0, sz = inb ? gz : 0;
-			const long long g = sx + c.sy * sy + c.sz * sz;
-			v = xin[g];
-			if (xcp) {
-				const uint8_t t = type[g];
-				// parents: [s/2, (s+1)/2] per axis (one parent for even s)
-				float pv[IS3D ? 8 : 4];
-#pragma unroll
-				for (int e = 0; e < (IS3D ? 8 : 4); e++) {
-					const int px = min((sx >> 1) + (e & 1), cp.nx - 1), py = min((sy >> 1) + ((e >> 1) & 1), cp.ny - 1);
-					const int pz = IS3D ? min((sz >> 1) + (e >> 2), cp.nz - 1) : 0;
-					pv[e] = xcp[px + cp.sy * py + cp.sz * pz];
-				}
-				float sum = 0.f;
-#pragma unroll
-				for (int e = 0; e < (IS3D ? 8 : 4); e++) {
-					const bool use = ((e & 1) == 0 || (sx & 1)) && (((e >> 1) & 1) == 0 || (sy & 1)) &&
-					                 ((e >> 2) == 0 || (sz & 1));
-					if (use) sum += pv[e];
-				}
-				if (t) v += wgt(sx, sy, sz) * sum;
-			}
-			if (!inb) v = 0.f;
-		}
-		xs[q] = v;
-	}
-	__syncthreads();
-	const int tx = threadIdx.x % BX, ty = (threadIdx.x / BX) % BY, tz = threadIdx.x / (BX * BY);
+	const int sy = (int)c.sy, sz = (int)c.sz;
+	// this thread's vertex
+	const int col = (t >> 6) / NWPC, sidx = ((t >> 6) % NWPC) * 64 + (t & 63);
+	const int tx = 2 * (sidx % HX) + (col & 1), ty = 2 * ((sidx / HX) % HY) + ((col >> 1) & 1);
+	const int tz = IS3D ? 2 * (sidx / (HX * HY)) + (col >> 2) : 0;
 	const int gx = ox + tx, gy = oy + ty, gz = oz + tz;
 	const bool valid = gx < c.nx && gy < c.ny && gz < c.nz;
-	const long long v = valid ? gx + c.sy * gy + c.sz * gz : 0;
-	const uint8_t tv = valid ? type[v] : 0;
-	const bool act = tv != 0;
-	const int me = (tx + 1) + RX * (ty + 1) + RXY * (tz + hz);
-	float a[IS3D ? 27 : 9];
-	float bv = 0.f, diag = 1.f;
+	const int v = valid ? gx + sy * gy + sz * gz : 0;
+	const uint8_t tv = valid ? type[v] : (uint8_t)0;
+	const float bv = b[v];
+	float a[NS];
+	float diag = 1.f;
 	if (tv == 2) {
-		bv = b[v];
 		diag = cn.a[C];
 #pragma unroll
-		for (int s = 0; s < (IS3D ? 27 : 9); s++) a[s] = s == C ? 0.f : cn.a[s];
-	} else if (act) {
-		bv = b[v];
+		for (int s = 0; s < NS; s++) a[s] = s == C ? 0.f : cn.a[s];
+	} else if (tv) {
 		diag = ldA(A, v);
 		int s = 0;
 		for (int dz = (IS3D ? -1 : 0); dz <= (IS3D ? 1 : 0); dz++)
@@ -1011,23 +1007,82 @@ __device__ __forceinline__ void body_smooth(const SmoothP& P, long long blk, lon
 					float coef = 0.f;
 					const int nx = gx + dx, ny = gy + dy, nz = gz + dz;
 					if (s != C && nx >= 0 && ny >= 0 && nz >= 0 && nx < c.nx && ny < c.ny && nz < c.nz) {
-						const long long nn = nx + c.sy * ny + c.sz * nz;
+						const int nn = nx + sy * ny + sz * nz;
 						coef = s > C ? ldA(A, (long long)(s - C) * c.n + v) : ldA(A, (long long)(C - s) * c.n + nn);
 					}
 					a[s] = coef;
 				}
 	}
-	const int par = (gx & 1) | ((gy & 1) << 1) | ((gz & 1) << 2);
+	if (POST) {
+		const int psy = (int)cp.sy, psz = (int)cp.sz;
+		const int cox = (ox >> 1) - 1, coy = (oy >> 1) - 1, coz = IS3D ? (oz >> 1) - 1 : 0;
+		for (int q = t; q < CRN; q += kSmoothNT) {
+			const int cx = q % CRX, cy = (q / CRX) % CRY, cz = q / CRXY;
+			const int px = min(max(cox + cx, 0), cp.nx - 1), py = min(max(coy + cy, 0), cp.ny - 1);
+			const int pz = IS3D ? min(max(coz + cz, 0), cp.nz - 1) : 0;
+			xc[q] = xcp[px + psy * py + psz * pz];
+		}
+		__syncthreads();
+	}
+#pragma unroll
+	for (int it = 0; it < (RN + kSmoothNT - 1) / kSmoothNT; it++) {
+		const int q = t + kSmoothNT * it;
+		if (q >= RN) break;
+		const int rx = q % RX, ry = (q / RX) % RY, rz = q / RXY;
+		const int hx = ox - 1 + rx, hy = oy - 1 + ry, hzz = oz - hz + rz;
+		const bool inb = hx >= 0 && hy >= 0 && hzz >= 0 && hx < c.nx && hy < c.ny && hzz < c.nz;
+		float x = 0.f;
+		if (POST) {
+			// branch-free: a valid address for every load, results selected
+			const int g = inb ? hx + sy * hy + sz * hzz : 0;
+			x = xin[g];
+			const uint8_t tp = type[g];
+			// parents [s/2, (s+1)/2] per axis; with o even, s = o - 1 + r has
+			// parity (r + 1) & 1 and its first parent in LDS slot (r + 1) >> 1
+			const int odx = (rx + 1) & 1, ody = (ry + 1) & 1, odz = IS3D ? (rz + 1) & 1 : 0;
+			const int base = ((rx + 1) >> 1) + CRX * ((ry + 1) >> 1) + (IS3D ? CRXY * ((rz + 1) >> 1) : 0);
+			const int ex = odx, ey = CRX * ody, ez = CRXY * odz;
+			float pv[IS3D ? 8 : 4];
+			pv[0] = xc[base];
+			pv[1] = xc[base + ex];
+			pv[2] = xc[base + ey];
+			pv[3] = xc[base + ex + ey];
+			if constexpr (IS3D) {
+				pv[4] = xc[base + ez];
+				pv[5] = xc[base + ex + ez];
+				pv[6] = xc[base + ey + ez];
+				pv[7] = xc[base + ex + ey + ez];
+			}
+			float sum = 0.f;
+#pragma unroll
+			for (int e = 0; e < (IS3D ? 8 : 4); e++) {
+				const bool use = ((e & 1) == 0 || odx) && (((e >> 1) & 1) == 0 || ody) && ((e >> 2) == 0 || odz);
+				if (use) sum += pv[e];
+			}
+			if (tp) x += wgt(hx, hy, hzz) * sum;
+			if (!inb) x = 0.f;
+		}
+		xs[q] = x;
+	}
+	__syncthreads();
+	const int me = (tx + 1) + RX * (ty + 1) + RXY * (tz + hz);
 	for (int ph = 0; ph < NCOL; ph++) {
-		const int col = reverse ? NCOL - 1 - ph : ph;
-		if (act && par == col) {
-			float sum = bv;
-			int s = 0;
-			for (int dz = (IS3D ? -1 : 0); dz <= (IS3D ? 1 : 0); dz++)
+		if (tv && col == (POST ? NCOL - 1 - ph : ph)) {
+			float acc[IS3D ? 3 : 1];
+#pragma unroll
+			for (int lz = 0; lz < (IS3D ? 3 : 1); lz++) {
+				float sacc = 0.f;
+#pragma unroll
 				for (int dy = -1; dy <= 1; dy++)
-					for (int dx = -1; dx <= 1; dx++, s++)
-						if (s != C) sum -= a[s] * xs[me + dx + RX * dy + RXY * dz];
-			xs[me] = sum / diag;
+#pragma unroll
+					for (int dx = -1; dx <= 1; dx++) {
+						const int s = (dx + 1) + 3 * (dy + 1) + 9 * lz;
+						if (s != C) sacc = fmaf(a[s], xs[me + dx + RX * dy + RXY * (lz - hz)], sacc);
+					}
+				acc[lz] = sacc;
+			}
+			const float tot = IS3D ? (acc[0] + acc[1]) + acc[IS3D ? 2 : 0] : acc[0];
+			xs[me] = (bv - tot) / diag;
 		}
 		__syncthreads();
 	}
@@ -1069,7 +1124,7 @@ __device__ __forceinline__ void body_coarsest(const CoarsestP& P, long long blk,
 // host
 // ---------------------------------------------------------------------------
 MgHierarchy::~MgHierarchy() {
-	for (void* p : allocs) hipFree(p);
+	for (void* p : allocs) (void)hipFree(p);
 }
 
 void* MgHierarchy::alloc(size_t bytes) {
@@ -1111,6 +1166,8 @@ void MgHierarchy::build(Scene* s, const uint8_t* code, int precision) {
 		c.ny = (p.ny + 2) / 2;
 		c.nz = (p.nz + 2) / 2;
 		c.n = (long long)c.nx * c.ny * c.nz;
+		// the coarse-level kernels address vertices with 32-bit indices
+		MFX_CHECK(c.n < (1ll << 31), "multigrid: coarse level has too many vertices");
 		c.sy = c.nx;
 		c.sz = (long long)c.nx * c.ny;
 		n = c.n;
@@ -1391,11 +1448,18 @@ void run_restrict_many(hipStream_t s, bool is3d, unsigned nb, const WorkItem* it
 	if (is3d) launch_many<RestrictP, body_restrict<true>, 256>(s, nb, items, p);
 	else launch_many<RestrictP, body_restrict<false>, 256>(s, nb, items, p);
 }
-void run_smooth_many(hipStream_t s, bool is3d, bool bf16, unsigned nb, const WorkItem* items, const SmoothP* p) {
-	if (is3d && bf16) launch_many<SmoothP, body_smooth<true, uint16_t>, 512>(s, nb, items, p);
-	else if (is3d) launch_many<SmoothP, body_smooth<true, float>, 512>(s, nb, items, p);
-	else if (bf16) launch_many<SmoothP, body_smooth<false, uint16_t>, 512>(s, nb, items, p);
-	else launch_many<SmoothP, body_smooth<false, float>, 512>(s, nb, items, p);
+template <bool POST>
+static void run_smooth_mode(hipStream_t s, bool is3d, bool bf16, unsigned nb, const WorkItem* items,
+                            const SmoothP* p) {
+	if (is3d && bf16) launch_many<SmoothP, body_smooth<true, uint16_t, POST>, kSmoothNT>(s, nb, items, p);
+	else if (is3d) launch_many<SmoothP, body_smooth<true, float, POST>, kSmoothNT>(s, nb, items, p);
+	else if (bf16) launch_many<SmoothP, body_smooth<false, uint16_t, POST>, kSmoothNT>(s, nb, items, p);
+	else launch_many<SmoothP, body_smooth<false, float, POST>, kSmoothNT>(s, nb, items, p);
+}
+void run_smooth_many(hipStream_t s, bool is3d, bool bf16, bool post, unsigned nb, const WorkItem* items,
+                     const SmoothP* p) {
+	if (post) run_smooth_mode<true>(s, is3d, bf16, nb, items, p);
+	else run_smooth_mode<false>(s, is3d, bf16, nb, items, p);
 }
 void run_resid_many(hipStream_t s, bool is3d, bool bf16, unsigned nb, const WorkItem* items, const ResidP* p) {
 	if (is3d && bf16) launch_many<ResidP, body_resid<true, uint16_t>, 256>(s, nb, items, p);
@@ -1471,26 +1535,26 @@ int MgHierarchy::vcycle(Scene* s, const uint8_t* code, const float* r, float* z,
 		else launch_one<RestrictP, body_restrict<false>, 256>(q, v.tiles, v.restrict_);
 		if (i == nlv - 1) break;
 		if (is3d && bf) {
-			launch_one<SmoothP, body_smooth<true, uint16_t>, 512>(q, v.blkTiles, v.pre);
+			launch_one<SmoothP, body_smooth<true, uint16_t, false>, kSmoothNT>(q, v.blkTiles, v.pre);
 			launch_one<ResidP, body_resid<true, uint16_t>, 256>(q, v.tiles, v.resid);
 		} else if (is3d) {
-			launch_one<SmoothP, body_smooth<true, float>, 512>(q, v.blkTiles, v.pre);
+			launch_one<SmoothP, body_smooth<true, float, false>, kSmoothNT>(q, v.blkTiles, v.pre);
 			launch_one<ResidP, body_resid<true, float>, 256>(q, v.tiles, v.resid);
 		} else if (bf) {
-			launch_one<SmoothP, body_smooth<false, uint16_t>, 512>(q, v.blkTiles, v.pre);
+			launch_one<SmoothP, body_smooth<false, uint16_t, false>, kSmoothNT>(q, v.blkTiles, v.pre);
 			launch_one<ResidP, body_resid<false, uint16_t>, 256>(q, v.tiles, v.resid);
 		} else {
-			launch_one<SmoothP, body_smooth<false, float>, 512>(q, v.blkTiles, v.pre);
+			launch_one<SmoothP, body_smooth<false, float, false>, kSmoothNT>(q, v.blkTiles, v.pre);
 			launch_one<ResidP, body_resid<false, float>, 256>(q, v.tiles, v.resid);
 		}
 	}
 	launch_one<CoarsestP, body_coarsest, 256>(q, coarsest_tiles(nc), S_.coarsest);
 	for (int i = nlv - 2; i >= 0; i--) {
 		const Steps::Level& v = S_.lv[i];
-		if (is3d && bf) launch_one<SmoothP, body_smooth<true, uint16_t>, 512>(q, v.blkTiles, v.post);
-		else if (is3d) launch_one<SmoothP, body_smooth<true, float>, 512>(q, v.blkTiles, v.post);
-		else if (bf) launch_one<SmoothP, body_smooth<false, uint16_t>, 512>(q, v.blkTiles, v.post);
-		else launch_one<SmoothP, body_smooth<false, float>, 512>(q, v.blkTiles, v.post);
+		if (is3d && bf) launch_one<SmoothP, body_smooth<true, uint16_t, true>, kSmoothNT>(q, v.blkTiles, v.post);
+		else if (is3d) launch_one<SmoothP, body_smooth<true, float, true>, kSmoothNT>(q, v.blkTiles, v.post);
+		else if (bf) launch_one<SmoothP, body_smooth<false, uint16_t, true>, kSmoothNT>(q, v.blkTiles, v.post);
+		else launch_one<SmoothP, body_smooth<false, float, true>, kSmoothNT>(q, v.blkTiles, v.post);
 	}
 	{
 		KtScope kt("mg_l0_up", q);

@@ -263,9 +263,9 @@ __device__ __forceinline__ void body_matvec(const MatvecP& P, long long blk, lon
 	if (st->done) return;
 	const float beta = st->beta;
 	const bool first = st->first;
-	const int ntx = (d.nx + TX - 1) / TX, nty = (d.ny + TY - 1) / TY;
-	const int ox = (int)(blk % ntx) * TX;
-	const long long rest = blk / ntx;
+	const unsigned ntx = (unsigned)(d.nx + TX - 1) / TX, nty = (unsigned)(d.ny + TY - 1) / TY;
+	const unsigned b32 = (unsigned)blk, rest = b32 / ntx;
+	const int ox = (int)(b32 - rest * ntx) * TX;
 	const int oy = (int)(rest % nty) * TY;
 	const int k0 = (int)(rest / nty) * kchunk, k1 = min(d.nz, k0 + kchunk);
 	const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
@@ -1068,7 +1068,7 @@ struct BatchSolve {
 			for (int d3 = 0; d3 < 2; d3++) {
 				if (l < rs[d3].size()) run_restrict_many(s, d3, rs[d3][l].n(), rs[d3][l].dItems, rs[d3][l].dParams);
 				if (l < pre[d3].size()) {
-					run_smooth_many(s, d3, bf16, pre[d3][l].n(), pre[d3][l].dItems, pre[d3][l].dParams);
+					run_smooth_many(s, d3, bf16, false, pre[d3][l].n(), pre[d3][l].dItems, pre[d3][l].dParams);
 					run_resid_many(s, d3, bf16, res[d3][l].n(), res[d3][l].dItems, res[d3][l].dParams);
 				}
 			}
@@ -1076,7 +1076,7 @@ struct BatchSolve {
 		for (size_t l = nl; l-- > 0;)
 			for (int d3 = 0; d3 < 2; d3++)
 				if (l < post[d3].size())
-					run_smooth_many(s, d3, bf16, post[d3][l].n(), post[d3][l].dItems, post[d3][l].dParams);
+					run_smooth_many(s, d3, bf16, true, post[d3][l].n(), post[d3][l].dItems, post[d3][l].dParams);
 		for (int d3 = 0; d3 < 2; d3++) run_l0_up_many(s, d3, l0up[d3].n(), l0up[d3].dItems, l0up[d3].dParams);
 		run_dot_many(s, dot.n(), dot.dItems, dot.dParams);
 	}
