@@ -488,73 +488,129 @@ __global__ void __launch_bounds__(kBlock) k_outflow_copy(Dims d, Tiles t, const 
 }
 
 // ---------------------------------------------------------------------------
-// KnSetWallBcs (extforces.cpp:186-217), all ijk
+// KnSetWallBcs (extforces.cpp:186-217), all ijk.  kLightKZ planes per thread:
+// every flag (and obstacle velocity) of all planes is loaded first, from
+// offsets that are 0 where the reference would not read the neighbour, then
+// each face's final value is written once (the reference's later writes
+// overwrite its earlier ones; no face is read in between).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_set_wall_bcs(Dims d, Tiles t, const uint8_t* __restrict__ f8,
                                                          float* __restrict__ u, float* __restrict__ v,
                                                          float* __restrict__ w, const float* __restrict__ ou,
                                                          const float* __restrict__ ov, const float* __restrict__ ow) {
-	CellPos p = tile_cell(t);
+	const CellPos p = tile_cell(t);
 	if (p.i >= d.nx || p.j >= d.ny) return;
-	const int i = p.i, j = p.j, k = p.k;
-	const long long ix = d.idx(i, j, k);
-	const uint8_t f = f8[ix];
-	const bool curFluid = f & kFluid, curObs = f & kObstacle;
-	if (!curFluid && !curObs) return;
-	float bx = 0.f, by = 0.f, bz = 0.f;
-	if (ou) {
-		bx = ou[ix];
-		by = ov[ix];
-		if (d.is3d) bz = ow[ix];
-	}
-	if (i > 0) {
-		const uint8_t g = f8[ix - 1];
-		if (g & kObstacle) u[ix] = bx;
-		if (curObs && (g & kFluid)) u[ix] = bx;
-	}
-	if (j > 0) {
-		const uint8_t g = f8[ix - d.nx];
-		if (g & kObstacle) v[ix] = by;
-		if (curObs && (g & kFluid)) v[ix] = by;
-	}
-	if (!d.is3d) {
-		w[ix] = 0.f;
-	} else if (k > 0) {
-		const uint8_t g = f8[ix - d.sz];
-		if (g & kObstacle) w[ix] = bz;
-		if (curObs && (g & kFluid)) w[ix] = bz;
-	}
-	if (curFluid) {
-		if ((i > 0 && (f8[ix - 1] & kStick)) || (i < d.nx - 1 && (f8[ix + 1] & kStick))) {
-			v[ix] = 0.f;
-			w[ix] = 0.f;
+	const int i = p.i, j = p.j;
+	const long long oxm = i > 0 ? 1 : 0, oxp = i < d.nx - 1 ? 1 : 0;
+	const long long oym = j > 0 ? d.nx : 0, oyp = j < d.ny - 1 ? d.nx : 0;
+	constexpr int KZ = kLightKZ;
+	uint8_t f[KZ], fxm[KZ], fxp[KZ], fym[KZ], fyp[KZ], fzm[KZ], fzp[KZ];
+	float ob[KZ][3];
+#pragma unroll
+	for (int q = 0; q < KZ; q++) {
+		const int k = min(p.k + q, d.nz - 1);  // planes past the end are loaded, never used
+		const long long ix = d.idx(i, j, k);
+		const long long ozm = (d.is3d && k > 0) ? d.sz : 0, ozp = (d.is3d && k < d.nz - 1) ? d.sz : 0;
+		f[q] = f8[ix];
+		fxm[q] = f8[ix - oxm];
+		fxp[q] = f8[ix + oxp];
+		fym[q] = f8[ix - oym];
+		fyp[q] = f8[ix + oyp];
+		fzm[q] = f8[ix - ozm];
+		fzp[q] = f8[ix + ozp];
+		ob[q][0] = ob[q][1] = ob[q][2] = 0.f;
+		if (ou) {
+			ob[q][0] = ou[ix];
+			ob[q][1] = ov[ix];
+			if (d.is3d) ob[q][2] = ow[ix];
 		}
-		if ((j > 0 && (f8[ix - d.nx] & kStick)) || (j < d.ny - 1 && (f8[ix + d.nx] & kStick))) {
-			u[ix] = 0.f;
-			w[ix] = 0.f;
+	}
+#pragma unroll
+	for (int q = 0; q < KZ; q++) {
+		const int k = p.k + q;
+		if (k >= d.nz) break;
+		const long long ix = d.idx(i, j, k);
+		const bool curFluid = f[q] & kFluid, curObs = f[q] & kObstacle;
+		if (!curFluid && !curObs) continue;
+		bool wu = false, wv = false, ww = false;
+		float nu = 0.f, nv = 0.f, nw = 0.f;
+		if (i > 0 && ((fxm[q] & kObstacle) || (curObs && (fxm[q] & kFluid)))) {
+			nu = ob[q][0];
+			wu = true;
 		}
-		if (d.is3d && ((k > 0 && (f8[ix - d.sz] & kStick)) || (k < d.nz - 1 && (f8[ix + d.sz] & kStick)))) {
-			u[ix] = 0.f;
-			v[ix] = 0.f;
+		if (j > 0 && ((fym[q] & kObstacle) || (curObs && (fym[q] & kFluid)))) {
+			nv = ob[q][1];
+			wv = true;
 		}
+		if (!d.is3d) {
+			nw = 0.f;
+			ww = true;
+		} else if (k > 0 && ((fzm[q] & kObstacle) || (curObs && (fzm[q] & kFluid)))) {
+			nw = ob[q][2];
+			ww = true;
+		}
+		if (curFluid) {
+			if ((i > 0 && (fxm[q] & kStick)) || (i < d.nx - 1 && (fxp[q] & kStick))) {
+				nv = nw = 0.f;
+				wv = ww = true;
+			}
+			if ((j > 0 && (fym[q] & kStick)) || (j < d.ny - 1 && (fyp[q] & kStick))) {
+				nu = nw = 0.f;
+				wu = ww = true;
+			}
+			if (d.is3d && ((k > 0 && (fzm[q] & kStick)) || (k < d.nz - 1 && (fzp[q] & kStick)))) {
+				nu = nv = 0.f;
+				wu = wv = true;
+			}
+		}
+		if (wu) u[ix] = nu;
+		if (wv) v[ix] = nv;
+		if (ww) w[ix] = nw;
 	}
 }
 
-// KnAddBuoyancy (extforces.cpp:72-80), bnd=1; face update evaluated in double
+// KnAddBuoyancy (extforces.cpp:72-80), bnd=1; face update evaluated in
+// double.  kLightKZ planes per thread, all loads issued first.
 __global__ void __launch_bounds__(kBlock) k_add_buoyancy(Dims d, Tiles t, const uint8_t* __restrict__ f8,
                                                          const float* __restrict__ den, float* __restrict__ u,
                                                          float* __restrict__ v, float* __restrict__ w, float fx,
                                                          float fy, float fz) {
-	CellPos p = tile_cell(t);
+	const CellPos p = tile_cell(t);
 	if (p.i >= d.nx || p.j >= d.ny) return;
-	if (!interior(d, p.i, p.j, p.k)) return;
-	const long long ix = d.idx(p.i, p.j, p.k);
-	if (!(f8[ix] & kFluid)) return;
-	if (f8[ix - 1] & kFluid) u[ix] = (float)((double)u[ix] + (0.5 * (double)fx) * (double)(den[ix] + den[ix - 1]));
-	if (f8[ix - d.nx] & kFluid)
-		v[ix] = (float)((double)v[ix] + (0.5 * (double)fy) * (double)(den[ix] + den[ix - d.nx]));
-	if (d.is3d && (f8[ix - d.sz] & kFluid))
-		w[ix] = (float)((double)w[ix] + (0.5 * (double)fz) * (double)(den[ix] + den[ix - d.sz]));
+	const int i = p.i, j = p.j;
+	constexpr int KZ = kLightKZ;
+	// neighbour offsets, valid for every cell (interior cells use the true ones)
+	const long long ox = i > 0 ? 1 : 0, oy = j > 0 ? d.nx : 0;
+	uint8_t f[KZ], gx[KZ], gy[KZ], gz[KZ];
+	float dc[KZ], dxm[KZ], dym[KZ], dzm[KZ], uc[KZ], vc[KZ], wc[KZ];
+#pragma unroll
+	for (int q = 0; q < KZ; q++) {
+		const int k = min(p.k + q, d.nz - 1);
+		const long long ix = d.idx(i, j, k);
+		const long long oz = k > 0 ? d.sz : 0;
+		f[q] = f8[ix];
+		gx[q] = f8[ix - ox];
+		gy[q] = f8[ix - oy];
+		gz[q] = f8[ix - oz];
+		dc[q] = den[ix];
+		dxm[q] = den[ix - ox];
+		dym[q] = den[ix - oy];
+		dzm[q] = den[ix - oz];
+		uc[q] = u[ix];
+		vc[q] = v[ix];
+		wc[q] = w[ix];
+	}
+#pragma unroll
+	for (int q = 0; q < KZ; q++) {
+		const int k = p.k + q;
+		if (k >= d.nz) break;
+		if (!interior(d, i, j, k) || !(f[q] & kFluid)) continue;
+		const long long ix = d.idx(i, j, k);
+		if (gx[q] & kFluid) u[ix] = (float)((double)uc[q] + (0.5 * (double)fx) * (double)(dc[q] + dxm[q]));
+		if (gy[q] & kFluid) v[ix] = (float)((double)vc[q] + (0.5 * (double)fy) * (double)(dc[q] + dym[q]));
+		if (d.is3d && (gz[q] & kFluid))
+			w[ix] = (float)((double)wc[q] + (0.5 * (double)fz) * (double)(dc[q] + dzm[q]));
+	}
 }
 
 // ---------------------------------------------------------------------------
@@ -563,14 +619,14 @@ __global__ void __launch_bounds__(kBlock) k_add_buoyancy(Dims d, Tiles t, const 
 static inline Vel vel_of(Grid* g) { return Vel{g->comp(0), g->comp(1), g->comp(2)}; }
 
 void op_set_wall_bcs(Scene* s, Grid* flags, Grid* vel, Grid* obvel) {
-	Tiles t = tiles_for(s->d);
+	Tiles t = tiles_for(s->d, kLightKZ);
 	hipLaunchKernelGGL(k_set_wall_bcs, dim3((unsigned)t.count()), dim3(kBlock), 0, s->stream, s->d, t, flags->f8,
 	                   vel->comp(0), vel->comp(1), vel->comp(2), obvel ? obvel->comp(0) : nullptr,
 	                   obvel ? obvel->comp(1) : nullptr, obvel ? obvel->comp(2) : nullptr);
 }
 
 void op_add_buoyancy(Scene* s, Grid* flags, Grid* density, Grid* vel, const float f[3]) {
-	Tiles t = tiles_for(s->d);
+	Tiles t = tiles_for(s->d, kLightKZ);
 	hipLaunchKernelGGL(k_add_buoyancy, dim3((unsigned)t.count()), dim3(kBlock), 0, s->stream, s->d, t, flags->f8,
 	                   (const float*)density->real(), vel->comp(0), vel->comp(1), vel->comp(2), f[0], f[1], f[2]);
 }

@@ -145,15 +145,17 @@ __host__ __device__ inline unsigned fastdiv(unsigned n, const FastDiv& f) {
 }
 
 struct Tiles {
-	int tx, ty, tz;  // tiles per axis (tz = nz)
+	int tx, ty, tz;  // tiles per axis (tz = ceil(nz / kz))
+	int kz = 1;      // planes per tile
 	FastDiv dx, dy;  // division by tx and ty
 	long long count() const { return (long long)tx * ty * tz; }
 };
-inline Tiles tiles_for(const Dims& d) {
+inline Tiles tiles_for(const Dims& d, int kz = 1) {
 	Tiles t;
 	t.tx = (d.nx + kTX - 1) / kTX;
 	t.ty = (d.ny + kTY - 1) / kTY;
-	t.tz = d.nz;
+	t.kz = kz;
+	t.tz = (d.nz + kz - 1) / kz;
 	if (t.count() >= (1ll << 31)) throw std::runtime_error("grid has too many tiles for one launch");
 	t.dx = fastdiv_make((unsigned)t.tx);
 	t.dy = fastdiv_make((unsigned)t.ty);
@@ -169,7 +171,8 @@ __device__ inline unsigned xcd_tile(unsigned wg, unsigned nwg) {
 	return x < r ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
 }
 
-// Decode the calling thread's cell (i,j,k) from a 1-D tile launch.
+// Decode the calling thread's cell (i,j,k) from a 1-D tile launch; k is the
+// first of the tile's kz planes.
 struct CellPos {
 	int i, j, k;
 };
@@ -182,9 +185,14 @@ __device__ inline CellPos tile_cell(const Tiles& t) {
 	CellPos p;
 	p.i = (int)bx * kTX + (int)(threadIdx.x & (kTX - 1));
 	p.j = (int)by * kTY + (int)(threadIdx.x >> 6);
-	p.k = (int)k;
+	p.k = (int)k * t.kz;
 	return p;
 }
+
+// Planes per thread of the light per-cell kernels: a thread walks kLightKZ
+// planes of its 64x4 tile and issues the loads of all of them before the
+// first use, so their memory latencies overlap.
+constexpr int kLightKZ = 4;
 
 // flat launch over n elements with a grid-stride loop
 inline unsigned flat_blocks(long long n, int per = 256, unsigned cap = 65536u) {

@@ -56,37 +56,68 @@ __device__ inline int off_coef(const Dims& d, const uint8_t* f8, long long c, in
 	return 1;
 }
 
+// kLightKZ planes per thread; the seven flags of every plane are loaded
+// first (offsets 0 where the neighbour is outside the grid: those flags are
+// then never used).
 __global__ void __launch_bounds__(kBlock) k_build_code(Dims d, Tiles t, const uint8_t* __restrict__ f8,
                                                        uint8_t* __restrict__ code, long long fix) {
-	CellPos p = tile_cell(t);
+	const CellPos p = tile_cell(t);
 	if (p.i >= d.nx || p.j >= d.ny) return;
-	const int i = p.i, j = p.j, k = p.k;
-	const long long c = d.idx(i, j, k);
-	const bool fl = f8[c] & kFluid;
-	int A0 = 0, bx = 0, by = 0, bz = 0;
-	if (fl && interior_c(d, i, j, k)) {
-		if (c == fix) {
-			A0 = 1;
-		} else {
-			A0 += !(f8[c - 1] & kObstacle);
-			A0 += !(f8[c + 1] & kObstacle);
-			A0 += !(f8[c - d.nx] & kObstacle);
-			A0 += !(f8[c + d.nx] & kObstacle);
-			if (d.is3d) {
-				A0 += !(f8[c - d.sz] & kObstacle);
-				A0 += !(f8[c + d.sz] & kObstacle);
-			}
-			bx = off_coef(d, f8, c, i, j, k, c + 1, fix);
-			by = off_coef(d, f8, c, i, j, k, c + d.nx, fix);
-			bz = d.is3d ? off_coef(d, f8, c, i, j, k, c + d.sz, fix) : 0;
-		}
+	const int i = p.i, j = p.j;
+	constexpr int KZ = kLightKZ;
+	const long long oxm = i > 0 ? 1 : 0, oxp = i < d.nx - 1 ? 1 : 0;
+	const long long oym = j > 0 ? d.nx : 0, oyp = j < d.ny - 1 ? d.nx : 0;
+	uint8_t f[KZ], fxm[KZ], fxp[KZ], fym[KZ], fyp[KZ], fzm[KZ], fzp[KZ];
+#pragma unroll
+	for (int q = 0; q < KZ; q++) {
+		const int k = min(p.k + q, d.nz - 1);
+		const long long c = d.idx(i, j, k);
+		const long long ozm = (d.is3d && k > 0) ? d.sz : 0, ozp = (d.is3d && k < d.nz - 1) ? d.sz : 0;
+		f[q] = f8[c];
+		fxm[q] = f8[c - oxm];
+		fxp[q] = f8[c + oxp];
+		fym[q] = f8[c - oym];
+		fyp[q] = f8[c + oyp];
+		fzm[q] = f8[c - ozm];
+		fzp[q] = f8[c + ozp];
 	}
-	int lo = 0;  // any lower off-diagonal into c
-	if (i > 0 && off_coef(d, f8, c - 1, i - 1, j, k, c, fix)) lo = 1;
-	if (j > 0 && off_coef(d, f8, c - d.nx, i, j - 1, k, c, fix)) lo = 1;
-	if (d.is3d && k > 0 && off_coef(d, f8, c - d.sz, i, j, k - 1, c, fix)) lo = 1;
-	const int trivial = (A0 == 1 && !bx && !by && !bz && !lo) ? 1 : 0;
-	code[c] = (uint8_t)(A0 | (bx << 3) | (by << 4) | (bz << 5) | ((fl ? 1 : 0) << 6) | (trivial << 7));
+#pragma unroll
+	for (int q = 0; q < KZ; q++) {
+		const int k = p.k + q;
+		if (k >= d.nz) break;
+		const long long c = d.idx(i, j, k);
+		const bool fl = f[q] & kFluid;
+		const bool inner = interior_c(d, i, j, k);
+		int A0 = 0, bx = 0, by = 0, bz = 0;
+		if (fl && inner) {
+			if (c == fix) {
+				A0 = 1;
+			} else {
+				A0 += !(fxm[q] & kObstacle);
+				A0 += !(fxp[q] & kObstacle);
+				A0 += !(fym[q] & kObstacle);
+				A0 += !(fyp[q] & kObstacle);
+				if (d.is3d) {
+					A0 += !(fzm[q] & kObstacle);
+					A0 += !(fzp[q] & kObstacle);
+				}
+				// off_coef(c -> upper neighbour): c interior fluid (true here),
+				// neighbour fluid, neither pinned
+				bx = (fxp[q] & kFluid) && c + 1 != fix;
+				by = (fyp[q] & kFluid) && c + d.nx != fix;
+				bz = d.is3d ? ((fzp[q] & kFluid) && c + d.sz != fix) : 0;
+			}
+		}
+		// any lower off-diagonal into c: off_coef(lower neighbour -> c)
+		int lo = 0;
+		if (fl && c != fix) {
+			if (i > 0 && interior_c(d, i - 1, j, k) && (fxm[q] & kFluid) && c - 1 != fix) lo = 1;
+			if (j > 0 && interior_c(d, i, j - 1, k) && (fym[q] & kFluid) && c - d.nx != fix) lo = 1;
+			if (d.is3d && k > 0 && interior_c(d, i, j, k - 1) && (fzm[q] & kFluid) && c - d.sz != fix) lo = 1;
+		}
+		const int trivial = (A0 == 1 && !bx && !by && !bz && !lo) ? 1 : 0;
+		code[c] = (uint8_t)(A0 | (bx << 3) | (by << 4) | (bz << 5) | ((fl ? 1 : 0) << 6) | (trivial << 7));
+	}
 }
 
 __device__ inline float cA0(uint8_t c) { return (float)(c & 7); }
@@ -117,23 +148,47 @@ __device__ inline float apply_row(const Dims& d, const uint8_t* __restrict__ cod
 // ---------------------------------------------------------------------------
 // RHS: MakeRhs (pressure.cpp:32-84) without fractions/obvel/surface tension
 // ---------------------------------------------------------------------------
+// kLightKZ planes per thread, all loads issued first (upper-neighbour offsets
+// are 0 on the last layer, where the value is not used).
 __global__ void __launch_bounds__(kBlock) k_make_rhs(Dims d, Tiles t, const uint8_t* __restrict__ f8,
                                                      const float* __restrict__ u, const float* __restrict__ v,
                                                      const float* __restrict__ w, float* __restrict__ rhs,
                                                      double* __restrict__ part) {
 	__shared__ double sh[8];
-	CellPos p = tile_cell(t);
+	const CellPos p = tile_cell(t);
 	double sum = 0, cnt = 0;
 	if (p.i < d.nx && p.j < d.ny) {
-		const long long c = d.idx(p.i, p.j, p.k);
-		float set = 0.f;
-		if (interior_c(d, p.i, p.j, p.k) && (f8[c] & kFluid)) {
-			set = u[c] - u[c + 1] + v[c] - v[c + d.nx];
-			if (d.is3d) set += w[c] - w[c + d.sz];
-			sum = (double)set;
-			cnt = 1;
+		constexpr int KZ = kLightKZ;
+		const int i = p.i, j = p.j;
+		const long long ox = i < d.nx - 1 ? 1 : 0, oy = j < d.ny - 1 ? d.nx : 0;
+		uint8_t f[KZ];
+		float u0[KZ], u1[KZ], v0[KZ], v1[KZ], w0[KZ], w1[KZ];
+#pragma unroll
+		for (int q = 0; q < KZ; q++) {
+			const int k = min(p.k + q, d.nz - 1);
+			const long long c = d.idx(i, j, k);
+			const long long oz = (d.is3d && k < d.nz - 1) ? d.sz : 0;
+			f[q] = f8[c];
+			u0[q] = u[c];
+			u1[q] = u[c + ox];
+			v0[q] = v[c];
+			v1[q] = v[c + oy];
+			w0[q] = w[c];
+			w1[q] = w[c + oz];
 		}
-		rhs[c] = set;
+#pragma unroll
+		for (int q = 0; q < KZ; q++) {
+			const int k = p.k + q;
+			if (k >= d.nz) break;
+			float set = 0.f;
+			if (interior_c(d, i, j, k) && (f[q] & kFluid)) {
+				set = u0[q] - u1[q] + v0[q] - v1[q];
+				if (d.is3d) set += w0[q] - w1[q];
+				sum += (double)set;
+				cnt += 1;
+			}
+			rhs[d.idx(i, j, k)] = set;
+		}
 	}
 	const double s = block_sum(sum, sh);
 	const double n = block_sum(cnt, sh);
@@ -148,33 +203,68 @@ __global__ void k_add_const(float* a, long long n, float v) {
 		a[i] += v;
 }
 
-// knCorrectVelocity (pressure.cpp:87-109), bnd=1
+// knCorrectVelocity (pressure.cpp:87-109), bnd=1.  kLightKZ planes per
+// thread with all loads issued first; each face is updated in a register in
+// the reference's order and written once.
 __global__ void __launch_bounds__(kBlock) k_correct_velocity(Dims d, Tiles t, const uint8_t* __restrict__ f8,
                                                              const float* __restrict__ pr, float* __restrict__ u,
                                                              float* __restrict__ v, float* __restrict__ w) {
-	CellPos p = tile_cell(t);
+	const CellPos p = tile_cell(t);
 	if (p.i >= d.nx || p.j >= d.ny) return;
-	if (!interior_c(d, p.i, p.j, p.k)) return;
-	const long long c = d.idx(p.i, p.j, p.k);
-	const long long xm = c - 1, ym = c - d.nx, zm = c - d.sz;
-	const uint8_t f = f8[c];
-	if (f & kFluid) {
-		const float pc = pr[c];
-		if (f8[xm] & kFluid) u[c] -= (pc - pr[xm]);
-		if (f8[ym] & kFluid) v[c] -= (pc - pr[ym]);
-		if (d.is3d && (f8[zm] & kFluid)) w[c] -= (pc - pr[zm]);
-		if (f8[xm] & kEmpty) u[c] -= pc;
-		if (f8[ym] & kEmpty) v[c] -= pc;
-		if (d.is3d && (f8[zm] & kEmpty)) w[c] -= pc;
-	} else if ((f & kEmpty) && !(f & kOutflow)) {
-		if (f8[xm] & kFluid) u[c] += pr[xm];
-		else u[c] = 0.f;
-		if (f8[ym] & kFluid) v[c] += pr[ym];
-		else v[c] = 0.f;
-		if (d.is3d) {
-			if (f8[zm] & kFluid) w[c] += pr[zm];
-			else w[c] = 0.f;
+	const int i = p.i, j = p.j;
+	constexpr int KZ = kLightKZ;
+	// neighbour offsets, valid for every cell (interior cells use the true ones)
+	const long long ox = i > 0 ? 1 : 0, oy = j > 0 ? d.nx : 0;
+	uint8_t f[KZ], gx[KZ], gy[KZ], gz[KZ];
+	float pc[KZ], px[KZ], py[KZ], pz[KZ], uc[KZ], vc[KZ], wc[KZ];
+#pragma unroll
+	for (int q = 0; q < KZ; q++) {
+		const int k = min(p.k + q, d.nz - 1);
+		const long long c = d.idx(i, j, k);
+		const long long oz = k > 0 ? d.sz : 0;
+		f[q] = f8[c];
+		gx[q] = f8[c - ox];
+		gy[q] = f8[c - oy];
+		gz[q] = f8[c - oz];
+		pc[q] = pr[c];
+		px[q] = pr[c - ox];
+		py[q] = pr[c - oy];
+		pz[q] = pr[c - oz];
+		uc[q] = u[c];
+		vc[q] = v[c];
+		wc[q] = w[c];
+	}
+#pragma unroll
+	for (int q = 0; q < KZ; q++) {
+		const int k = p.k + q;
+		if (k >= d.nz) break;
+		if (!interior_c(d, i, j, k)) continue;
+		const long long c = d.idx(i, j, k);
+		const uint8_t fc = f[q];
+		float nu = uc[q], nv = vc[q], nw = wc[q];
+		if (fc & kFluid) {
+			const float pcc = pc[q];
+			if (gx[q] & kFluid) nu -= (pcc - px[q]);
+			if (gy[q] & kFluid) nv -= (pcc - py[q]);
+			if (d.is3d && (gz[q] & kFluid)) nw -= (pcc - pz[q]);
+			if (gx[q] & kEmpty) nu -= pcc;
+			if (gy[q] & kEmpty) nv -= pcc;
+			if (d.is3d && (gz[q] & kEmpty)) nw -= pcc;
+		} else if ((fc & kEmpty) && !(fc & kOutflow)) {
+			if (gx[q] & kFluid) nu += px[q];
+			else nu = 0.f;
+			if (gy[q] & kFluid) nv += py[q];
+			else nv = 0.f;
+			if (d.is3d) {
+				if (gz[q] & kFluid) nw += pz[q];
+				else nw = 0.f;
+			}
+		} else {
+			continue;
 		}
+		u[c] = nu;
+		v[c] = nv;
+		if (d.is3d) w[c] = nw;
 	}
 }
 
@@ -534,32 +624,60 @@ __device__ __forceinline__ void body_finalize(const FinP& P, long long, long lon
 	if (st->iter >= st->maxIter) st->done = 1;
 }
 
-// true residual ||b - A x||_2^2 with FP64 accumulation of the row sum
+// true residual ||b - A x||_2^2 with FP64 accumulation of the row sum;
+// kLightKZ planes per thread, all loads issued first (linear neighbours
+// outside [0, n) are replaced by the cell itself and never used)
 __global__ void __launch_bounds__(kBlock) k_true_residual(Dims d, Tiles t, const uint8_t* __restrict__ code,
                                                           const float* __restrict__ b, const float* __restrict__ x,
                                                           double* __restrict__ part) {
 	__shared__ double sh[8];
-	CellPos p = tile_cell(t);
+	const CellPos p = tile_cell(t);
 	double rr = 0;
 	if (p.i < d.nx && p.j < d.ny) {
-		const long long c = d.idx(p.i, p.j, p.k);
-		const uint8_t cc = code[c];
-		double ax;
-		if (!(cc & 64)) ax = x[c];
-		else {
-			const long long X = 1, Y = d.nx, Z = d.sz, n = d.n;
-			ax = (double)x[c] * (double)(cc & 7);
-			if (c - X >= 0 && (code[c - X] & 8)) ax -= x[c - X];
-			if (c + X < n && (cc & 8)) ax -= x[c + X];
-			if (c - Y >= 0 && (code[c - Y] & 16)) ax -= x[c - Y];
-			if (c + Y < n && (cc & 16)) ax -= x[c + Y];
-			if (d.is3d) {
-				if (c - Z >= 0 && (code[c - Z] & 32)) ax -= x[c - Z];
-				if (c + Z < n && (cc & 32)) ax -= x[c + Z];
-			}
+		constexpr int KZ = kLightKZ;
+		const long long X = 1, Y = d.nx, Z = d.sz, n = d.n;
+		uint8_t cc[KZ], cxm[KZ], cym[KZ], czm[KZ];
+		float bc[KZ], xc[KZ], xxm[KZ], xxp[KZ], xym[KZ], xyp[KZ], xzm[KZ], xzp[KZ];
+#pragma unroll
+		for (int q = 0; q < KZ; q++) {
+			const int k = min(p.k + q, d.nz - 1);
+			const long long c = d.idx(p.i, p.j, k);
+			const long long mx = c - X >= 0 ? c - X : c, px = c + X < n ? c + X : c;
+			const long long my = c - Y >= 0 ? c - Y : c, py = c + Y < n ? c + Y : c;
+			const long long mz = c - Z >= 0 ? c - Z : c, pz = c + Z < n ? c + Z : c;
+			cc[q] = code[c];
+			cxm[q] = code[mx];
+			cym[q] = code[my];
+			czm[q] = code[mz];
+			bc[q] = b[c];
+			xc[q] = x[c];
+			xxm[q] = x[mx];
+			xxp[q] = x[px];
+			xym[q] = x[my];
+			xyp[q] = x[py];
+			xzm[q] = x[mz];
+			xzp[q] = x[pz];
 		}
-		const double r = (double)b[c] - ax;
-		rr = r * r;
+#pragma unroll
+		for (int q = 0; q < KZ; q++) {
+			if (p.k + q >= d.nz) break;
+			const long long c = d.idx(p.i, p.j, p.k + q);
+			double ax;
+			if (!(cc[q] & 64)) ax = xc[q];
+			else {
+				ax = (double)xc[q] * (double)(cc[q] & 7);
+				if (c - X >= 0 && (cxm[q] & 8)) ax -= xxm[q];
+				if (c + X < n && (cc[q] & 8)) ax -= xxp[q];
+				if (c - Y >= 0 && (cym[q] & 16)) ax -= xym[q];
+				if (c + Y < n && (cc[q] & 16)) ax -= xyp[q];
+				if (d.is3d) {
+					if (c - Z >= 0 && (czm[q] & 32)) ax -= xzm[q];
+					if (c + Z < n && (cc[q] & 32)) ax -= xzp[q];
+				}
+			}
+			const double r = (double)bc[q] - ax;
+			rr += r * r;
+		}
 	}
 	const double s = block_sum(rr, sh);
 	if (threadIdx.x == 0) part[3 * blockIdx.x] = s;
@@ -703,7 +821,7 @@ static long long find_fix_cell(Scene* s, Grid* flags) {
 }
 
 static void compute_rhs(Scene* s, Grid* flags, Grid* vel, float* rhs, int enforceCompatibility) {
-	const Tiles t = tiles_for(s->d);
+	const Tiles t = tiles_for(s->d, kLightKZ);
 	const unsigned nb = (unsigned)t.count();
 	s->ensurePartials(3 * (size_t)nb + 16, 2 * (size_t)nb + 16);
 	hipLaunchKernelGGL(k_make_rhs, dim3(nb), dim3(kBlock), 0, s->stream, s->d, t, flags->f8,
@@ -794,7 +912,11 @@ static void cg_setup(CgRun& R, Scene* s, Grid* flags, float* rhs, float* x, cons
 	// matrix
 	if (!s->code) MFX_HIP(hipMalloc((void**)&s->code, (size_t)d.n));
 	R.code = s->code;
-	hipLaunchKernelGGL(k_build_code, dim3(R.nbT), dim3(kBlock), 0, s->stream, d, R.t, flags->f8, R.code, R.fix);
+	{
+		const Tiles tc = tiles_for(d, kLightKZ);
+		hipLaunchKernelGGL(k_build_code, dim3((unsigned)tc.count()), dim3(kBlock), 0, s->stream, d, tc, flags->f8,
+		                   R.code, R.fix);
+	}
 
 	// preconditioner choice (pressure.cpp:404-433, conjugategrad.cpp:302-336)
 	int pc = o.preconditioner;
@@ -889,10 +1011,11 @@ static void cg_iteration(CgRun& R) {
 // mirror, buffers back to the pool (queued; nothing waits)
 static void cg_finish_enqueue(CgRun& R) {
 	Scene* s = R.s;
-	hipLaunchKernelGGL(k_true_residual, dim3(R.nbT), dim3(kBlock), 0, s->stream, s->d, R.t, R.code,
+	const Tiles tr = tiles_for(s->d, kLightKZ);
+	hipLaunchKernelGGL(k_true_residual, dim3((unsigned)tr.count()), dim3(kBlock), 0, s->stream, s->d, tr, R.code,
 	                   (const float*)R.rhs, (const float*)R.x, s->dpart);
 	hipLaunchKernelGGL(k_reduce_true_rr, dim3(1), dim3(1024), 0, s->stream, R.st, (const double*)s->dpart,
-	                   (int)R.nbT);
+	                   (int)tr.count());
 	MFX_HIP(hipMemcpyAsync(s->hscal, R.st, sizeof(CgState), hipMemcpyDeviceToHost, s->stream));
 	s->putReal(R.r);
 	s->putReal(R.pa);
@@ -960,7 +1083,7 @@ static void solve_system(Scene* s, Grid* flags, float* rhs, float* x, const mfx_
 
 
 static void correct_velocity(Scene* s, Grid* flags, Grid* vel, const float* pressure) {
-	const Tiles t = tiles_for(s->d);
+	const Tiles t = tiles_for(s->d, kLightKZ);
 	hipLaunchKernelGGL(k_correct_velocity, dim3((unsigned)t.count()), dim3(kBlock), 0, s->stream, s->d, t, flags->f8,
 	                   pressure, vel->comp(0), vel->comp(1), vel->comp(2));
 }
