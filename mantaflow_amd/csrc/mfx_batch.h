@@ -1,11 +1,12 @@
-// Batched launches: one kernel launch covers work items of many independent
+// Batched launches: one kernel launch covers the tiles of many independent
 // problems (pressure systems of different sizes).  Every loop kernel of the
 // CG and the multigrid V-cycle is written as a __device__ body taking a
 // parameter struct, a logical tile index and a partial-sum slot.  Two
 // wrappers instantiate it:
 //   k_one   - one problem; tile = slot = XCD-aware remap of blockIdx.x
-//   k_many  - a work list; block b runs item[b] = (problem, tile, slot) with
-//             that problem's parameter struct
+//   k_many  - a work list: the problems' tiles concatenated in problem order
+//             (problem i owns logical tiles [start[i], start[i+1])); block b
+//             runs logical tile XCD-remap(b) with its problem's parameters
 // so the batched path runs exactly the single-problem arithmetic.
 #pragma once
 
@@ -16,11 +17,10 @@
 
 namespace mfx {
 
-struct WorkItem {
-	int prob;
-	int tile;
-	int slot;
-	int pad;
+// A work list on the device: start[0..np] (start[np] = total tiles).
+struct WorkList {
+	const int* start;
+	int np;
 };
 
 // The partial-sum slot is the logical tile in both wrappers, so a problem's
@@ -32,10 +32,28 @@ __global__ void __launch_bounds__(NT) k_one(P p) {
 	BODY(p, t, t);
 }
 
+// Problem of logical tile lt: the number of problems whose start is <= lt,
+// minus one (starts are nondecreasing; empty problems share their start with
+// the next one and are skipped).  One load per lane and a ballot per 64
+// problems, identical in every wavefront of the block.
+__device__ inline int worklist_prob(const WorkList& wl, unsigned lt) {
+	int cnt = 0;
+	for (int base = 0; base < wl.np; base += 64) {
+		const int idx = base + (int)(threadIdx.x & 63);
+		const bool le = idx < wl.np && (unsigned)wl.start[idx] <= lt;
+		const int c = (int)__popcll(__ballot(le));
+		cnt += c;
+		if (c < 64) break;
+	}
+	return cnt - 1;
+}
+
 template <typename P, void (*BODY)(const P&, long long, long long), int NT>
-__global__ void __launch_bounds__(NT) k_many(const WorkItem* __restrict__ items, const P* __restrict__ params) {
-	const WorkItem w = items[blockIdx.x];
-	BODY(params[w.prob], w.tile, w.slot);
+__global__ void __launch_bounds__(NT) k_many(WorkList wl, const P* __restrict__ params) {
+	const unsigned lt = xcd_tile(blockIdx.x, gridDim.x);
+	const int prob = worklist_prob(wl, lt);
+	const long long t = (long long)lt - wl.start[prob];
+	BODY(params[prob], t, t);
 }
 
 // launch helpers (instantiated in the file that defines BODY)
@@ -44,7 +62,7 @@ inline void launch_one(hipStream_t s, long long nb, const P& p) {
 	if (nb > 0) hipLaunchKernelGGL((k_one<P, BODY, NT>), dim3((unsigned)nb), dim3(NT), 0, s, p);
 }
 template <typename P, void (*BODY)(const P&, long long, long long), int NT>
-inline void launch_many(hipStream_t s, long long nb, const WorkItem* items, const P* p) {
+inline void launch_many(hipStream_t s, long long nb, WorkList items, const P* p) {
 	if (nb > 0) hipLaunchKernelGGL((k_many<P, BODY, NT>), dim3((unsigned)nb), dim3(NT), 0, s, items, p);
 }
 
@@ -106,13 +124,13 @@ inline long long dot_tiles(const Dims& d) { return (d.n + kBlock - 1) / kBlock; 
 
 // launches: *_one with a host parameter struct, *_many over a device work list
 void run_matvec_one(hipStream_t s, unsigned nb, const MatvecP& p);
-void run_matvec_many(hipStream_t s, unsigned nb, const WorkItem* items, const MatvecP* p);
+void run_matvec_many(hipStream_t s, unsigned nb, WorkList items, const MatvecP* p);
 void run_update_one(hipStream_t s, unsigned nb, const UpdateP& p);
-void run_update_many(hipStream_t s, unsigned nb, const WorkItem* items, const UpdateP* p);
+void run_update_many(hipStream_t s, unsigned nb, WorkList items, const UpdateP* p);
 void run_finalize_one(hipStream_t s, const FinP& p);
-void run_finalize_many(hipStream_t s, unsigned nprob, const WorkItem* items, const FinP* p);
+void run_finalize_many(hipStream_t s, unsigned nprob, WorkList items, const FinP* p);
 void run_dot_one(hipStream_t s, unsigned nb, const DotP& p);
-void run_dot_many(hipStream_t s, unsigned nb, const WorkItem* items, const DotP* p);
+void run_dot_many(hipStream_t s, unsigned nb, WorkList items, const DotP* p);
 
 // ---------------------------------------------------------------------------
 // V-cycle kernels (mfx_mg.hip)
@@ -207,14 +225,14 @@ long long lc_tiles(const LvDev& c);               // 64x4 vertex tiles per plane
 long long lcblk_tiles(const LvDev& c, bool is3d); // smoother bricks
 inline long long coarsest_tiles(int nc) { return (nc + 3) / 4; }
 
-void run_l0_down_many(hipStream_t s, bool is3d, unsigned nb, const WorkItem* items, const L0DownP* p);
-void run_l0_up_many(hipStream_t s, bool is3d, unsigned nb, const WorkItem* items, const L0UpP* p);
-void run_restrict_many(hipStream_t s, bool is3d, unsigned nb, const WorkItem* items, const RestrictP* p);
+void run_l0_down_many(hipStream_t s, bool is3d, unsigned nb, WorkList items, const L0DownP* p);
+void run_l0_up_many(hipStream_t s, bool is3d, unsigned nb, WorkList items, const L0UpP* p);
+void run_restrict_many(hipStream_t s, bool is3d, unsigned nb, WorkList items, const RestrictP* p);
 // post: post-smoothing (prolongation on load, reversed colours), else pre-smoothing
-void run_smooth_many(hipStream_t s, bool is3d, bool bf16, bool post, unsigned nb, const WorkItem* items,
+void run_smooth_many(hipStream_t s, bool is3d, bool bf16, bool post, unsigned nb, WorkList items,
                      const SmoothP* p);
-void run_resid_many(hipStream_t s, bool is3d, bool bf16, unsigned nb, const WorkItem* items, const ResidP* p);
-void run_coarsest_many(hipStream_t s, unsigned nb, const WorkItem* items, const CoarsestP* p);
-void run_fill_many(hipStream_t s, unsigned nb, const WorkItem* items, const FillP* p);
+void run_resid_many(hipStream_t s, bool is3d, bool bf16, unsigned nb, WorkList items, const ResidP* p);
+void run_coarsest_many(hipStream_t s, unsigned nb, WorkList items, const CoarsestP* p);
+void run_fill_many(hipStream_t s, unsigned nb, WorkList items, const FillP* p);
 
 }  // namespace mfx

@@ -39,6 +39,7 @@ Scene::~Scene() {
 	if (hscal) hipHostFree(hscal);
 	if (ev) hipEventDestroy(ev);
 	if (arena) hipFree(arena);
+	if (hstage) hipHostFree(hstage);
 	if (stream) hipStreamDestroy(stream);
 }
 

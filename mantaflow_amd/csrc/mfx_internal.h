@@ -88,6 +88,8 @@ struct Scene {
 	hipEvent_t ev = nullptr;    // cross-stream ordering (batched solves)
 	void* arena = nullptr;      // work lists of batched solves led by this scene
 	size_t arenaCap = 0;
+	void* hstage = nullptr;     // pinned staging of the arena upload
+	size_t hstageCap = 0;
 	std::unique_ptr<MgHierarchy> mgStatic;
 	uint8_t* code = nullptr;    // per-cell Laplace matrix code of the current solve
 	std::vector<Grid*> grids;

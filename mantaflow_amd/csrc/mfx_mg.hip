@@ -1436,41 +1436,41 @@ long long lcblk_tiles(const LvDev& c, bool is3d) {
 	const long long bx = is3d ? kLcBX3 : kLcBX2, by = is3d ? kLcBY3 : kLcBY2, bz = is3d ? kLcBZ3 : 1;
 	return ((c.nx + bx - 1) / bx) * ((c.ny + by - 1) / by) * ((c.nz + bz - 1) / bz);
 }
-void run_l0_down_many(hipStream_t s, bool is3d, unsigned nb, const WorkItem* items, const L0DownP* p) {
+void run_l0_down_many(hipStream_t s, bool is3d, unsigned nb, WorkList items, const L0DownP* p) {
 	if (is3d) launch_many<L0DownP, body_l0_down_zm<kZmTY>, 64 * kZmTY>(s, nb, items, p);
 	else launch_many<L0DownP, body_l0_down_brick<64, 16, 1, false>, 1024>(s, nb, items, p);
 }
-void run_l0_up_many(hipStream_t s, bool is3d, unsigned nb, const WorkItem* items, const L0UpP* p) {
+void run_l0_up_many(hipStream_t s, bool is3d, unsigned nb, WorkList items, const L0UpP* p) {
 	if (is3d) launch_many<L0UpP, body_l0_up_zm<kZmTY>, 64 * kZmTY>(s, nb, items, p);
 	else launch_many<L0UpP, body_l0_up_brick<64, 16, 1, false>, 1024>(s, nb, items, p);
 }
-void run_restrict_many(hipStream_t s, bool is3d, unsigned nb, const WorkItem* items, const RestrictP* p) {
+void run_restrict_many(hipStream_t s, bool is3d, unsigned nb, WorkList items, const RestrictP* p) {
 	if (is3d) launch_many<RestrictP, body_restrict<true>, 256>(s, nb, items, p);
 	else launch_many<RestrictP, body_restrict<false>, 256>(s, nb, items, p);
 }
 template <bool POST>
-static void run_smooth_mode(hipStream_t s, bool is3d, bool bf16, unsigned nb, const WorkItem* items,
+static void run_smooth_mode(hipStream_t s, bool is3d, bool bf16, unsigned nb, WorkList items,
                             const SmoothP* p) {
 	if (is3d && bf16) launch_many<SmoothP, body_smooth<true, uint16_t, POST>, kSmoothNT>(s, nb, items, p);
 	else if (is3d) launch_many<SmoothP, body_smooth<true, float, POST>, kSmoothNT>(s, nb, items, p);
 	else if (bf16) launch_many<SmoothP, body_smooth<false, uint16_t, POST>, kSmoothNT>(s, nb, items, p);
 	else launch_many<SmoothP, body_smooth<false, float, POST>, kSmoothNT>(s, nb, items, p);
 }
-void run_smooth_many(hipStream_t s, bool is3d, bool bf16, bool post, unsigned nb, const WorkItem* items,
+void run_smooth_many(hipStream_t s, bool is3d, bool bf16, bool post, unsigned nb, WorkList items,
                      const SmoothP* p) {
 	if (post) run_smooth_mode<true>(s, is3d, bf16, nb, items, p);
 	else run_smooth_mode<false>(s, is3d, bf16, nb, items, p);
 }
-void run_resid_many(hipStream_t s, bool is3d, bool bf16, unsigned nb, const WorkItem* items, const ResidP* p) {
+void run_resid_many(hipStream_t s, bool is3d, bool bf16, unsigned nb, WorkList items, const ResidP* p) {
 	if (is3d && bf16) launch_many<ResidP, body_resid<true, uint16_t>, 256>(s, nb, items, p);
 	else if (is3d) launch_many<ResidP, body_resid<true, float>, 256>(s, nb, items, p);
 	else if (bf16) launch_many<ResidP, body_resid<false, uint16_t>, 256>(s, nb, items, p);
 	else launch_many<ResidP, body_resid<false, float>, 256>(s, nb, items, p);
 }
-void run_coarsest_many(hipStream_t s, unsigned nb, const WorkItem* items, const CoarsestP* p) {
+void run_coarsest_many(hipStream_t s, unsigned nb, WorkList items, const CoarsestP* p) {
 	launch_many<CoarsestP, body_coarsest, 256>(s, nb, items, p);
 }
-void run_fill_many(hipStream_t s, unsigned nb, const WorkItem* items, const FillP* p) {
+void run_fill_many(hipStream_t s, unsigned nb, WorkList items, const FillP* p) {
 	launch_many<FillP, body_fill, 256>(s, nb, items, p);
 }
 

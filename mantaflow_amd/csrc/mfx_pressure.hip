@@ -27,7 +27,9 @@
 // order of the double dot products.  Kernels read a device "done" flag and
 // return at once after convergence, so the host can queue iterations in
 // batches and only synchronise between batches.
+#include <algorithm>
 #include <chrono>
+#include <deque>
 #include <cmath>
 #include <cstring>
 
@@ -767,25 +769,25 @@ long long update_tiles(const Dims& d, long long* per) {
 void run_matvec_one(hipStream_t s, unsigned nb, const MatvecP& p) {
 	hipLaunchKernelGGL((k_one<MatvecP, body_matvec<kMvTY>, 64 * kMvTY>), dim3(nb), dim3(64 * kMvTY), 0, s, p);
 }
-void run_matvec_many(hipStream_t s, unsigned nb, const WorkItem* items, const MatvecP* p) {
+void run_matvec_many(hipStream_t s, unsigned nb, WorkList items, const MatvecP* p) {
 	hipLaunchKernelGGL((k_many<MatvecP, body_matvec<kMvTY>, 64 * kMvTY>), dim3(nb), dim3(64 * kMvTY), 0, s, items, p);
 }
 void run_update_one(hipStream_t s, unsigned nb, const UpdateP& p) {
 	hipLaunchKernelGGL((k_one<UpdateP, body_update, kBlock>), dim3(nb), dim3(kBlock), 0, s, p);
 }
-void run_update_many(hipStream_t s, unsigned nb, const WorkItem* items, const UpdateP* p) {
+void run_update_many(hipStream_t s, unsigned nb, WorkList items, const UpdateP* p) {
 	hipLaunchKernelGGL((k_many<UpdateP, body_update, kBlock>), dim3(nb), dim3(kBlock), 0, s, items, p);
 }
 void run_finalize_one(hipStream_t s, const FinP& p) {
 	hipLaunchKernelGGL((k_one<FinP, body_finalize, 1024>), dim3(1), dim3(1024), 0, s, p);
 }
-void run_finalize_many(hipStream_t s, unsigned nprob, const WorkItem* items, const FinP* p) {
+void run_finalize_many(hipStream_t s, unsigned nprob, WorkList items, const FinP* p) {
 	hipLaunchKernelGGL((k_many<FinP, body_finalize, 1024>), dim3(nprob), dim3(1024), 0, s, items, p);
 }
 void run_dot_one(hipStream_t s, unsigned nb, const DotP& p) {
 	hipLaunchKernelGGL((k_one<DotP, body_dot, kBlock>), dim3(nb), dim3(kBlock), 0, s, p);
 }
-void run_dot_many(hipStream_t s, unsigned nb, const WorkItem* items, const DotP* p) {
+void run_dot_many(hipStream_t s, unsigned nb, WorkList items, const DotP* p) {
 	hipLaunchKernelGGL((k_many<DotP, body_dot, kBlock>), dim3(nb), dim3(kBlock), 0, s, items, p);
 }
 
@@ -1100,20 +1102,6 @@ static void correct_velocity(Scene* s, Grid* flags, Grid* vel, const float* pres
 // ---------------------------------------------------------------------------
 namespace {
 
-// work list in hardware order: logical items are grouped per XCD so that the
-// tiles of one problem stay on one L2
-std::vector<WorkItem> xcd_order(const std::vector<WorkItem>& logical) {
-	const long long n = (long long)logical.size();
-	std::vector<WorkItem> hw((size_t)n);
-	const long long q = n >> 3, r = n & 7;
-	for (long long wg = 0; wg < n; wg++) {
-		const long long x = wg & 7, slot = wg >> 3;
-		const long long lt = x < r ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
-		hw[(size_t)wg] = logical[(size_t)lt];
-	}
-	return hw;
-}
-
 struct DevArena {
 	std::vector<uint8_t> host;
 	size_t add(const void* p, size_t bytes) {
@@ -1124,27 +1112,32 @@ struct DevArena {
 	}
 };
 
+// One batched launch: per-problem tile counts and parameters; on the device
+// the tile starts (prefix sums) and the parameter array.
 template <typename P>
 struct Step {
-	std::vector<WorkItem> items;
+	std::vector<int> tiles;
 	std::vector<P> params;
-	size_t itemOff = 0, paramOff = 0;
-	const WorkItem* dItems = nullptr;
+	long long total = 0;
+	size_t startOff = 0, paramOff = 0;
+	WorkList dItems{nullptr, 0};
 	const P* dParams = nullptr;
-	explicit Step(size_t nprob = 0) : params(nprob) {}
-	void add_tiles(int prob, long long tiles) {
-		for (long long t = 0; t < tiles; t++) items.push_back(WorkItem{prob, (int)t, (int)t, 0});
+	explicit Step(size_t nprob = 0) : tiles(nprob, 0), params(nprob) {}
+	void add_tiles(int prob, long long t) {
+		tiles[(size_t)prob] += (int)t;
+		total += t;
 	}
 	void place(DevArena& A) {
-		items = xcd_order(items);
-		itemOff = A.add(items.data(), sizeof(WorkItem) * items.size());
+		std::vector<int> st(tiles.size() + 1, 0);
+		for (size_t i = 0; i < tiles.size(); i++) st[i + 1] = st[i] + tiles[i];
+		startOff = A.add(st.data(), sizeof(int) * st.size());
 		paramOff = A.add(params.data(), sizeof(P) * params.size());
 	}
 	void bind(uint8_t* base) {
-		dItems = (const WorkItem*)(base + itemOff);
+		dItems = WorkList{(const int*)(base + startOff), (int)tiles.size()};
 		dParams = (const P*)(base + paramOff);
 	}
-	unsigned n() const { return (unsigned)items.size(); }
+	unsigned n() const { return (unsigned)total; }
 };
 
 __global__ void k_gather_done(int np, CgState* const* __restrict__ st, int* __restrict__ done) {
@@ -1254,9 +1247,9 @@ static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>&
 				B.up[par].add_tiles(i, r.nbU);
 			}
 			B.fa.params[(size_t)i] = r.fin((int)r.nbM, FIN_ALPHA);
-			B.fa.items.push_back(WorkItem{i, 0, 0, 0});
+			B.fa.add_tiles(i, 1);
 			B.fr.params[(size_t)i] = r.fin((int)r.nbU, FIN_RES);
-			B.fr.items.push_back(WorkItem{i, 0, 0, 0});
+			B.fr.add_tiles(i, 1);
 			if (r.pcNone) continue;
 			B.anyMg = true;
 			const MgHierarchy::Steps S = r.mg->steps(d, r.code, r.r, r.z, r.st, r.s->dpart);
@@ -1298,9 +1291,9 @@ static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>&
 				np = (int)S.l0Tiles;
 			}
 			B.fb.params[(size_t)i] = r.fin(np, FIN_BETA);
-			B.fb.items.push_back(WorkItem{i, 0, 0, 0});
+			B.fb.add_tiles(i, 1);
 			B.fb0.params[(size_t)i] = r.fin(np, FIN_BETA0);
-			B.fb0.items.push_back(WorkItem{i, 0, 0, 0});
+			B.fb0.add_tiles(i, 1);
 		}
 		// one device block for every list and parameter array
 		DevArena A;
@@ -1342,7 +1335,17 @@ static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>&
 			s0->arenaCap = A.host.size() * 2;
 		}
 		B.dmem = (uint8_t*)s0->arena;
-		MFX_HIP(hipMemcpyAsync(B.dmem, A.host.data(), A.host.size(), hipMemcpyHostToDevice, q));
+		// staged through pinned memory, so the upload does not hold the host
+		// (the previous batch led by this scene has finished: its solve synced)
+		if (s0->hstageCap < A.host.size()) {
+			if (s0->hstage) MFX_HIP(hipHostFree(s0->hstage));
+			s0->hstage = nullptr;
+			s0->hstageCap = 0;
+			MFX_HIP(hipHostMalloc(&s0->hstage, A.host.size() * 2, hipHostMallocDefault));
+			s0->hstageCap = A.host.size() * 2;
+		}
+		memcpy(s0->hstage, A.host.data(), A.host.size());
+		MFX_HIP(hipMemcpyAsync(B.dmem, s0->hstage, A.host.size(), hipMemcpyHostToDevice, q));
 		placeAll([&](auto& st) { st.bind(B.dmem); });
 		B.dStates = (CgState**)(B.dmem + stOff);
 		B.dDone = (int*)(B.dmem + doneOff);
@@ -1359,33 +1362,71 @@ static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>&
 			run_finalize_many(q, B.fb0.n(), B.fb0.dItems, B.fb0.dParams);
 		}
 
-		// capture two iterations and replay
-		hipGraph_t graph = nullptr;
-		hipGraphExec_t exec = nullptr;
-		MFX_HIP(hipStreamBeginCapture(q, hipStreamCaptureModeThreadLocal));
-		B.enqueue(q, 0);
-		B.enqueue(q, 1);
-		MFX_HIP(hipStreamEndCapture(q, &graph));
-		MFX_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+		// Iterations are queued directly on the batch stream, a few ahead of
+		// the host: after every second iteration the done flags of all
+		// problems are copied to a pinned slot and an event recorded; the host
+		// waits only for the oldest outstanding check while later iterations
+		// are already queued.  Iterations queued after every problem has
+		// converged return at once (their kernels test the done flag).
+		constexpr int kAhead = 4;  // iterations queued beyond the last completed check
+		int* hdone = (int*)s0->hscal;  // pinned, 4 KiB: one slot of P flags per outstanding check
+		const int nslot = std::max(1, (int)(4096 / (sizeof(int) * (size_t)P)));
+		const bool pinned = nslot >= 2;
 		std::vector<int> done((size_t)P, 0);
-		int its = 0;
-		int* hdone = (int*)s0->hscal;  // pinned, 4 KiB
-		const bool pinned = (size_t)P * sizeof(int) <= 4096;
-		while (its < maxIt) {
-			MFX_HIP(hipGraphLaunch(exec, q));
-			its += 2;
-			hipLaunchKernelGGL(k_gather_done, dim3((P + 255) / 256), dim3(256), 0, q, P, (CgState* const*)B.dStates,
-			                   B.dDone);
-			MFX_HIP(hipMemcpyAsync(pinned ? hdone : done.data(), B.dDone, sizeof(int) * (size_t)P,
-			                       hipMemcpyDeviceToHost, q));
-			sync(s0);
-			const int* dn = pinned ? hdone : done.data();
-			bool all = true;
-			for (int i = 0; i < P && all; i++) all = dn[i] != 0;
-			if (all) break;
+		struct Check {
+			int its;
+			int slot;
+			hipEvent_t ev;
+		};
+		std::vector<hipEvent_t> evPool;
+		std::deque<Check> pending;
+		int its = 0, checked = 0, nextSlot = 0;
+		auto take_event = [&]() {
+			hipEvent_t e;
+			if (!evPool.empty()) {
+				e = evPool.back();
+				evPool.pop_back();
+			} else {
+				MFX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+			}
+			return e;
+		};
+		bool converged = false;
+		try {
+			while (!converged) {
+				while (its < maxIt && its - checked < kAhead && (pinned ? (int)pending.size() < nslot : pending.empty())) {
+					B.enqueue(q, its & 1);
+					its++;
+					if ((its & 1) == 0 || its == maxIt) {
+						const int slot = pinned ? nextSlot : 0;
+						nextSlot = (nextSlot + 1) % nslot;
+						hipLaunchKernelGGL(k_gather_done, dim3((P + 255) / 256), dim3(256), 0, q, P,
+						                   (CgState* const*)B.dStates, B.dDone);
+						MFX_HIP(hipMemcpyAsync(pinned ? hdone + (size_t)slot * P : done.data(), B.dDone,
+						                       sizeof(int) * (size_t)P, hipMemcpyDeviceToHost, q));
+						hipEvent_t e = take_event();
+						MFX_HIP(hipEventRecord(e, q));
+						pending.push_back(Check{its, slot, e});
+					}
+				}
+				if (pending.empty()) break;
+				Check c = pending.front();
+				pending.pop_front();
+				MFX_HIP(hipEventSynchronize(c.ev));
+				evPool.push_back(c.ev);
+				checked = c.its;
+				const int* dn = pinned ? hdone + (size_t)c.slot * P : done.data();
+				bool all = true;
+				for (int i = 0; i < P && all; i++) all = dn[i] != 0;
+				converged = all || (its >= maxIt && pending.empty());
+			}
+		} catch (...) {
+			for (auto& c : pending) evPool.push_back(c.ev);
+			for (hipEvent_t e : evPool) (void)hipEventDestroy(e);
+			throw;
 		}
-		MFX_HIP(hipGraphExecDestroy(exec));
-		MFX_HIP(hipGraphDestroy(graph));
+		for (auto& c : pending) evPool.push_back(c.ev);
+		for (hipEvent_t e : evPool) (void)hipEventDestroy(e);
 		for (int i = 0; i < P; i++) R[(size_t)i].launched = its;
 		// finish and correct velocities, each problem on its own stream
 		if (!s0->ev) MFX_HIP(hipEventCreateWithFlags(&s0->ev, hipEventDisableTiming));
