@@ -40,6 +40,7 @@ Scene::~Scene() {
 	if (ev) hipEventDestroy(ev);
 	if (arena) hipFree(arena);
 	if (hstage) hipHostFree(hstage);
+	if (bexec) hipGraphExecDestroy(bexec);
 	if (stream) hipStreamDestroy(stream);
 }
 

@@ -90,6 +90,10 @@ struct Scene {
 	size_t arenaCap = 0;
 	void* hstage = nullptr;     // pinned staging of the arena upload
 	size_t hstageCap = 0;
+	// two-iteration graph of the last batch composition led by this scene
+	// that was seen twice in a row (key: launch layout)
+	hipGraphExec_t bexec = nullptr;
+	std::vector<long long> bkey, blastKey;
 	std::unique_ptr<MgHierarchy> mgStatic;
 	uint8_t* code = nullptr;    // per-cell Laplace matrix code of the current solve
 	std::vector<Grid*> grids;

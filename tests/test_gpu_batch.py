@@ -42,6 +42,31 @@ def test_batch_equals_single(manta, oracle, pc):
         np.testing.assert_array_equal(v1.toNumpy(), v2.toNumpy())
 
 
+def test_batch_repeated_direct_capture_replay(manta, oracle):
+    """The same batch solved three times in a row: queued launches, then
+    capture of the two-iteration graph, then its replay.  Every pass starts
+    from the same velocity and must give results bit-identical to single
+    solves."""
+    kw = dict(preconditioner=manta.PcMGStatic, cgAccuracy=1e-4)
+    single, batch, vel0 = [], [], []
+    for k, (shape, frac) in enumerate(PROBLEMS):
+        single.append(make_problem(manta, oracle, shape, frac, 200 + k))
+        g, f, v, p = make_problem(manta, oracle, shape, frac, 200 + k)
+        batch.append((g, f, v, p))
+        vel0.append(v.toNumpy().copy())
+    for g, f, v, p in single:
+        manta.solvePressure(flags=f, vel=v, pressure=p, **kw)
+    for rep in range(3):
+        for (g, f, v, p), v0 in zip(batch, vel0):
+            v.fromNumpy(v0)
+        manta.solvePressureBatch([(v, p, f) for g, f, v, p in batch], **kw)
+        for (g1, f1, v1, p1), (g2, f2, v2, p2) in zip(single, batch):
+            s1, s2 = manta.lastSolveStats(g1.s), manta.lastSolveStats(g2.s)
+            assert s2["converged"] and s1["iterations"] == s2["iterations"], rep
+            np.testing.assert_array_equal(p1.toNumpy(), p2.toNumpy())
+            np.testing.assert_array_equal(v1.toNumpy(), v2.toNumpy())
+
+
 def test_batch_rejects_shared_solver(manta, oracle):
     g, f, v, p = make_problem(manta, oracle, (16, 16, 16), 0.0, 1)
     with pytest.raises(RuntimeError, match="own solver"):
