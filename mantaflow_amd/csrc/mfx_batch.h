@@ -115,7 +115,7 @@ struct DotP {
 // FIN_BETA0: FIN_BETA of the initial V-cycle, which also marks the next
 // iteration as the first (p = z)
 enum { FIN_INIT = 0, FIN_ALPHA = 1, FIN_RES = 2, FIN_BETA = 3, FIN_BETA0 = 4 };
-constexpr int kMvTY = 8;
+constexpr int kMvTY = 16;
 
 // tiles of each CG kernel for one problem (and the plane chunk / range size)
 long long matvec_tiles(const Dims& d, int* kchunk);

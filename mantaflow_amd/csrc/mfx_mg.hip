@@ -1421,14 +1421,15 @@ void MgHierarchy::build_coarsest(Scene* s, const uint8_t* code0) {
 // ---------------------------------------------------------------------------
 // launches (single problem and work lists)
 // ---------------------------------------------------------------------------
-constexpr int kZmTY = 8;
+constexpr int kZmTY = 16;
 long long l0_tiles(const Dims& d, int* kchunk) {
-	// 3D: z-marching 64x8 column tiles over chunks of an even number of planes
-	// (about 1024 workgroups); 2D: bricks of 64x16 cells
+	// 3D: z-marching 64 x kZmTY column tiles over chunks of an even number of
+	// planes (about 8192 wavefronts, one per SIMD slot of the chip); 2D:
+	// bricks of 64x16 cells
 	const int BY = d.is3d ? kZmTY : 16;
 	const long long cols = (long long)((d.nx + 63) / 64) * ((d.ny + BY - 1) / BY);
 	int kc = (d.nz + 1) & ~1;
-	while (d.is3d && kc > 16 && cols * ((d.nz + kc - 1) / kc) < 1024) kc = ((kc / 2) + 1) & ~1;
+	while (d.is3d && kc > 16 && cols * ((d.nz + kc - 1) / kc) < 8192 / kZmTY) kc = ((kc / 2) + 1) & ~1;
 	*kchunk = kc;
 	return d.is3d ? cols * ((d.nz + kc - 1) / kc) : cols;
 }

@@ -751,11 +751,11 @@ __global__ void __launch_bounds__(kBlock) k_cg_replace_residual(Dims d, Tiles t,
 // launches of the CG loop kernels (single problem and work lists)
 // ---------------------------------------------------------------------------
 long long matvec_tiles(const Dims& d, int* kchunk) {
-	// 64 x kMvTY column tiles, chunks of planes sized so that about 1024
-	// workgroups cover the grid
+	// 64 x kMvTY column tiles, chunks of planes sized so that about 8192
+	// wavefronts cover the grid
 	const long long cols = (long long)((d.nx + 63) / 64) * ((d.ny + kMvTY - 1) / kMvTY);
 	int kc = d.nz;
-	while (kc > 8 && cols * ((d.nz + kc - 1) / kc) < 1024) kc = (kc + 1) / 2;
+	while (kc > 8 && cols * ((d.nz + kc - 1) / kc) < 8192 / kMvTY) kc = (kc + 1) / 2;
 	*kchunk = kc;
 	return cols * ((d.nz + kc - 1) / kc);
 }
