@@ -234,6 +234,210 @@ def bench_projection(manta, n=256, frac=0.15):
     return out
 
 
+def _rel_err(a, b):
+    return float(np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64)).max() /
+                 max(1e-30, float(np.abs(np.asarray(b, np.float64)).max())))
+
+
+def config1_plume2d(manta, steps=200, threads=16):
+    """64^2 plume (walls, box source, first-order SL, buoyancy (0,-4e-3),
+    plain CG at 1e-3): GPU, CPU oracle (FP32) and FP64 oracle, 200 steps;
+    error of the GPU fields against the FP64 oracle after the last step."""
+    from oracle import Oracle, PC_NONE, TYPE_MAC, TYPE_REAL
+    shape, n = (64, 64, 1), 64 * 64
+    s = manta.Solver(name="plume2d", gridSize=manta.vec3(64, 64, 1), dim=2)
+    s.timestep = 1.0
+    f, v, d, pr = s.create(manta.FlagGrid), s.create(manta.MACGrid), s.create(manta.RealGrid), s.create(manta.RealGrid)
+    f.initDomain()
+    f.fillGrid()
+    src = manta.Box(s, p0=manta.vec3(25.6, 3.2, 0), p1=manta.vec3(38.4, 6.4, 1))
+    its = []
+    s.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        src.applyToGrid(grid=d, value=1.0)
+        manta.advectSemiLagrange(flags=f, vel=v, grid=d, order=1)
+        manta.advectSemiLagrange(flags=f, vel=v, grid=v, order=1)
+        manta.setWallBcs(flags=f, vel=v)
+        manta.addBuoyancy(density=d, vel=v, gravity=manta.vec3(0, -4e-3, 0), flags=f)
+        manta.solvePressure(flags=f, vel=v, pressure=pr, cgAccuracy=1e-3, preconditioner=manta.PcNone)
+        its.append(manta.lastSolveStats(s)["iterations"])
+        s.step()
+    s.sync()
+    tg = time.perf_counter() - t0
+    gd, gv = d.toNumpy(), v.toNumpy()
+    out = {"gpu_ms_per_step": round(tg / steps * 1e3, 3), "cg_iterations_mean": float(np.mean(its))}
+    for f64 in (False, True):
+        o = Oracle(f64=f64)
+        o.set_threads(threads)
+        flags = o.init_domain(shape)
+        o.fill_grid(shape, flags)
+        vel, dens, p = np.zeros((n, 3), o.dtype), np.zeros(n, o.dtype), np.zeros(n, o.dtype)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            o.apply_shape(shape, 1, [25.6, 3.2, 0], [38.4, 6.4, 1], 0, dens, TYPE_REAL, 1.0)
+            o.advect(shape, 1.0, flags, vel, dens, TYPE_REAL, 1)
+            o.advect(shape, 1.0, flags, vel, vel, TYPE_MAC, 1)
+            o.set_wall_bcs(shape, flags, vel)
+            o.add_buoyancy(shape, 1.0, flags, dens, vel, (0, -4e-3, 0))
+            o.solve_pressure(shape, vel, p, flags, cgAccuracy=1e-3, preconditioner=PC_NONE)
+        tc = time.perf_counter() - t0
+        key = "fp64_oracle" if f64 else "cpu_oracle_fp32"
+        out[key] = {"ms_per_step": round(tc / steps * 1e3, 3), "threads": threads,
+                    "gpu_density_err": _rel_err(gd, dens), "gpu_velocity_err": _rel_err(gv, vel)}
+    return out
+
+
+def config2_plume3d(manta, n=128, steps=100, threads=16):
+    """128^3 open-top plume, MacCormack, MG-static PCG at 1e-4: GPU time for
+    100 steps, CPU oracle (FP32) time per step, FP64 oracle over the same 100
+    steps and the GPU's error against it after 1 and after 100 steps."""
+    from oracle import Oracle, PC_MGSTATIC, TYPE_MAC, TYPE_REAL
+    st = plume_setup(manta, n)
+    s = st["s"]
+    its = []
+    first = None
+    s.sync()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        its.append(plume_step(manta, st)["iterations"])
+        if k == 0:
+            s.sync()
+            first = (st["density"].toNumpy().copy(), st["vel"].toNumpy().copy())
+            t0 = time.perf_counter()  # timed: steps 2..100
+    s.sync()
+    tg = time.perf_counter() - t0
+    gd, gv = st["density"].toNumpy(), st["vel"].toNumpy()
+    out = {"gpu_ms_per_step": round(tg / (steps - 1) * 1e3, 3), "gpu_mcell_steps_per_s": round(n ** 3 * (steps - 1) / tg / 1e6, 1),
+           "cg_iterations_mean": float(np.mean(its))}
+    shape, N = (n, n, n), n ** 3
+    for f64 in (False, True):
+        o = Oracle(f64=f64)
+        o.set_threads(threads)
+        flags = o.init_domain(shape, boundaryWidth=1)
+        o.fill_grid(shape, flags)
+        o.set_open_bound(shape, flags, 1, "Y")
+        mask = np.zeros(N, o.dtype)
+        o.apply_shape(shape, 3, [0.5 * n, 0.1 * n, 0.5 * n], [0, 0.02 * n, 0], 0.14 * n, mask, TYPE_REAL, 1.0)
+        val = (np.random.default_rng(1).uniform(0.8, 1.0, N).astype(np.float32) * mask.astype(np.float32)).astype(o.dtype)
+        keep = (1.0 - mask).astype(o.dtype)
+        vel, dens, p = np.zeros((N, 3), o.dtype), np.zeros(N, o.dtype), np.zeros(N, o.dtype)
+        nsteps = steps if f64 else 5
+        err1 = None
+        t0 = time.perf_counter()
+        for k in range(nsteps):
+            dens = (dens * keep + val).astype(o.dtype)
+            o.advect(shape, 1.0, flags, vel, dens, TYPE_REAL, 2, 1.0, 2)
+            o.advect(shape, 1.0, flags, vel, vel, TYPE_MAC, 2, 1.0, 2)
+            o.reset_outflow(shape, flags, None, dens)
+            o.set_wall_bcs(shape, flags, vel)
+            o.add_buoyancy(shape, 1.0, flags, dens, vel, (0, -1e-3, 0))
+            o.solve_pressure(shape, vel, p, flags, cgAccuracy=1e-4, preconditioner=PC_MGSTATIC, relTol=1e-4)
+            o.set_wall_bcs(shape, flags, vel)
+            if k == 0 and f64:
+                err1 = (_rel_err(first[0], dens), _rel_err(first[1], vel))
+        tc = time.perf_counter() - t0
+        if f64:
+            out["fp64_oracle"] = {"steps": nsteps, "s": round(tc, 1), "threads": threads,
+                                  "gpu_err_after_1_step": {"density": err1[0], "velocity": err1[1]},
+                                  "gpu_err_after_%d_steps" % steps: {"density": _rel_err(gd, dens),
+                                                                      "velocity": _rel_err(gv, vel)}}
+        else:
+            out["cpu_oracle_fp32"] = {"ms_per_step": round(tc / nsteps * 1e3, 1), "threads": threads,
+                                      "mcell_steps_per_s": round(N * nsteps / tc / 1e6, 2),
+                                      "sample": "%d steps from the initial state" % nsteps}
+    return out
+
+
+def config3_fp64(manta, n=256, frac=0.15, threads=16):
+    """The 256^3 projection problem of the projection leg against the FP64
+    oracle (MG-static PCG to relative residual 1e-10): GPU error at 1e-4 and
+    1e-6, FP32 and mixed precision; the CPU oracle's FP32 solve time."""
+    from oracle import Oracle, PC_MGSTATIC
+    from mantaflow_amd import workloads
+    shape = (n, n, n)
+    rng = np.random.default_rng(3)
+    flags = workloads.obstacle_flags_np(shape, frac, rng)
+    vel0 = workloads.smooth_field(rng, shape)
+    o = Oracle(f64=True)
+    o.set_threads(threads)
+    vref = vel0.astype(np.float64)
+    o.set_wall_bcs(shape, flags, vref)
+    pref = np.zeros(n ** 3)
+    t0 = time.perf_counter()
+    sref = o.solve_pressure(shape, vref, pref, flags, cgAccuracy=1e-10, preconditioner=PC_MGSTATIC, relTol=1e-10)
+    tref = time.perf_counter() - t0
+    o32 = Oracle()
+    o32.set_threads(threads)
+    v32 = vel0.copy()
+    o32.set_wall_bcs(shape, flags, v32)
+    p32 = np.zeros(n ** 3, np.float32)
+    t0 = time.perf_counter()
+    s32 = o32.solve_pressure(shape, v32, p32, flags, cgAccuracy=1e-4, preconditioner=PC_MGSTATIC, relTol=1e-4)
+    t32 = time.perf_counter() - t0
+    p = workloads.Problem(shape, frac, seed=3)
+    out = {"fp64_oracle": {"s": round(tref, 1), "iterations": sref["iterations"], "rel_residual": sref["relResidual"]},
+           "cpu_oracle_fp32_1e-4": {"s": round(t32, 2), "iterations": s32["iterations"], "threads": threads}, "gpu": {}}
+    for prec, name in ((0, "fp32"), (1, "mixed")):
+        for tol in (1e-4, 1e-6):
+            p.s.mfxPrecision, p.s.mfxRelTol, p.s.mfxTrueResidualEvery = prec, tol, 50
+            p.reset()
+            manta.solvePressure(flags=p.flags, vel=p.vel, pressure=p.pressure, cgAccuracy=1e-4,
+                                preconditioner=manta.PcMGStatic)
+            out["gpu"]["%s_%g" % (name, tol)] = {"velocity_err": _rel_err(p.vel.toNumpy(), vref),
+                                                 "iterations": manta.lastSolveStats(p.s)["iterations"]}
+    return out
+
+
+def config4_cpu(manta, threads=16):
+    """The 64 batched-solve problems solved one after another by the CPU
+    oracle (FP32, MG-static PCG at 1e-4).  Every problem is a closed domain
+    (walls on all sides), a singular system: the reference's GridMg warns
+    that it may not converge without a fixed pressure value
+    (multigrid.cpp:401-402), so the oracle solves with zeroPressureFixing."""
+    from oracle import Oracle, PC_MGSTATIC
+    from mantaflow_amd import workloads
+    shapes, fracs = workloads.batched_shapes()
+    o = Oracle()
+    o.set_threads(threads)
+    tot, tconv, its, worst, failed = 0.0, 0.0, [], 0.0, []
+    for i, (sh, fr) in enumerate(zip(shapes, fracs)):
+        rng = np.random.default_rng(1000 + i)
+        flags = workloads.obstacle_flags_np(sh, fr, rng)
+        vel = workloads.smooth_field(rng, sh)
+        o.set_wall_bcs(sh, flags, vel)
+        pr = np.zeros(sh[0] * sh[1] * sh[2], np.float32)
+        t0 = time.perf_counter()
+        st = o.solve_pressure(sh, vel, pr, flags, cgAccuracy=1e-4, preconditioner=PC_MGSTATIC, relTol=1e-4,
+                              zeroPressureFixing=True)
+        dt = time.perf_counter() - t0
+        tot += dt
+        if st["relResidual"] <= 1e-4:
+            its.append(st["iterations"])
+            worst = max(worst, st["relResidual"])
+            tconv += dt
+        else:
+            # a single fixed cell leaves fluid pockets enclosed by obstacles
+            # singular: the reference's multigrid stalls there too
+            failed.append({"problem": i, "shape": list(sh), "s": round(dt, 2), "rel_residual": st["relResidual"]})
+    return {"problems": len(shapes), "cpu_oracle_serial_ms": round(tot * 1e3, 1), "threads": threads,
+            "converged": len(its), "converged_serial_ms": round(tconv * 1e3, 1),
+            "iterations_max_converged": int(max(its)), "rel_residual_max_converged": worst,
+            "not_converged": failed, "zero_pressure_fixing": True}
+
+
+def bench_configs(manta):
+    out = {}
+    for name, fn in (("config1_plume2d_64", config1_plume2d), ("config2_plume3d_128", config2_plume3d),
+                     ("config3_projection_256_fp64", config3_fp64), ("config4_batched_cpu", config4_cpu)):
+        print("bench --configs: %s ..." % name, file=sys.stderr, flush=True)
+        try:
+            out[name] = fn(manta)
+        except Exception as e:
+            out[name] = {"error": "%s: %s" % (type(e).__name__, e)}
+    return out
+
+
 def bench_serving(n_sessions=24, n_req=240, rate=200.0):
     """Poisson stream of plume-step / pressure-solve requests on resident
     sessions (64^2 .. 192^3 cells, log-uniform): served (continuous batching)
@@ -274,6 +478,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-problem batched-solve leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the served request-stream leg")
+    ap.add_argument("--configs", action="store_true",
+                    help="instead of the bench line: GPU, CPU-oracle and FP64-oracle figures of the 2D plume "
+                         "(64^2, 200 steps), the 3D plume (128^3, 100 steps), the 256^3 projection and the "
+                         "64 batched problems, as one JSON object (a few minutes)")
     ap.add_argument("--no-projection", action="store_true",
                     help="skip the obstacle-rich projection leg (FP32 and mixed precision, rel. res. 1e-4/1e-6)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -287,6 +495,10 @@ def main():
     from mantaflow_amd import manta
     from mantaflow_amd._abi import lib
     import ctypes as C
+
+    if a.configs:
+        print(json.dumps(bench_configs(manta)), flush=True)
+        return 0
 
     n = a.res
     st = plume_setup(manta, n)

@@ -102,17 +102,20 @@ __device__ inline float interp(const float* __restrict__ data, const Geo<IS3D, I
                                float posz) {
 	float px = posx - 0.5f, py = posy - 0.5f, pz = posz - 0.5f;
 	int xi = (int)px, yi = (int)py, zi = (int)pz;
-	float s1 = px - (float)xi, s0 = 1.0f - s1;
-	float t1 = py - (float)yi, t0 = 1.0f - t1;
-	float f1 = pz - (float)zi, f0 = 1.0f - f1;
-	if (px < 0.f) { xi = 0; s0 = 1.0f; s1 = 0.0f; }
-	if (py < 0.f) { yi = 0; t0 = 1.0f; t1 = 0.0f; }
-	if (pz < 0.f) { zi = 0; f0 = 1.0f; f1 = 0.0f; }
-	if (xi >= g.nx - 1) { xi = g.nx - 2; s0 = 0.0f; s1 = 1.0f; }
-	if (yi >= g.ny - 1) { yi = g.ny - 2; t0 = 0.0f; t1 = 1.0f; }
+	float s1 = px - (float)xi;
+	float t1 = py - (float)yi;
+	float f1 = pz - (float)zi;
+	if (px < 0.f) { xi = 0; s1 = 0.0f; }
+	if (py < 0.f) { yi = 0; t1 = 0.0f; }
+	if (pz < 0.f) { zi = 0; f1 = 0.0f; }
+	if (xi >= g.nx - 1) { xi = g.nx - 2; s1 = 1.0f; }
+	if (yi >= g.ny - 1) { yi = g.ny - 2; t1 = 1.0f; }
 	if (g.nz > 1) {
-		if (zi >= g.nz - 1) { zi = g.nz - 2; f0 = 0.0f; f1 = 1.0f; }
+		if (zi >= g.nz - 1) { zi = g.nz - 2; f1 = 1.0f; }
 	}
+	// the reference sets (s0, s1) to (1, 0) / (0, 1) in the clamps and to
+	// (1 - s1, s1) otherwise: 1 - s1 is exact in all three cases
+	const float s0 = 1.0f - s1, t0 = 1.0f - t1, f0 = 1.0f - f1;
 	const I Y = g.Y, Z = g.Z;
 	const float* __restrict__ p = data + g.at(xi, yi, IS3D ? zi : 0);
 	return ((p[0] * t0 + p[Y] * t1) * s0 + (p[1] * t0 + p[1 + Y] * t1) * s1) * f0 +

@@ -533,3 +533,25 @@ def test_double_build_agrees(oracle, oracle64, rng):
     oracle.advect(shape, np.float32(1.0), flags, v32, d32, TYPE_REAL, 2, 1.0, 2)
     oracle64.advect(shape, 1.0, flags, v64, d64, TYPE_REAL, 2, 1.0, 2)
     assert np.abs(d32 - d64).max() < 1e-5
+
+
+def test_mg_singular_closed_domain_needs_fixed_pressure(oracle):
+    """A closed domain without obstacles is a pure-Neumann (singular) system.
+    The reference's GridMg warns that it may then not converge unless a
+    pressure value is fixed (multigrid.cpp:401-402); the oracle, which
+    restates GridMg, stalls the same way, and converges with
+    zeroPressureFixing (pressure.cpp:349-388 pins one cell)."""
+    from mantaflow_amd import workloads
+    shape = (40, 40, 1)
+    rng = np.random.default_rng(5)
+    flags = workloads.obstacle_flags_np(shape, 0.0, rng)
+    vel = workloads.smooth_field(rng, shape)
+    oracle.set_wall_bcs(shape, flags, vel)
+    v, p = vel.copy(), np.zeros(40 * 40, np.float32)
+    st = oracle.solve_pressure(shape, v, p, flags, cgAccuracy=1e-4, preconditioner=PC_MGSTATIC, relTol=1e-4,
+                               maxIterOverride=10)
+    assert st["relResidual"] > 0.5  # no progress in 10 iterations: the V-cycle sees the constant mode
+    v, p = vel.copy(), np.zeros(40 * 40, np.float32)
+    st = oracle.solve_pressure(shape, v, p, flags, cgAccuracy=1e-4, preconditioner=PC_MGSTATIC, relTol=1e-4,
+                               zeroPressureFixing=True)
+    assert st["fixPidx"] >= 0 and st["iterations"] < 20 and st["relResidual"] <= 1e-4
