@@ -442,26 +442,47 @@ def bench_serving(n_sessions=24, n_req=240, rate=200.0):
     """Poisson stream of plume-step / pressure-solve requests on resident
     sessions (64^2 .. 192^3 cells, log-uniform): served (continuous batching)
     vs the plain path one request at a time; in real time at `rate` req/s
-    (latency) and with every request queued at t=0 (throughput)."""
+    (latency) and with every request queued at t=0 (throughput).  In real
+    time each request's slowdown is its latency over the isolated latency of
+    the same request (same session and kind, plain path, nothing else
+    running); p50/p99 of the slowdown are reported beside the latencies."""
     from mantaflow_amd import serve
     sessions = [serve.PlumeScene(sh, seed=20 + i) for i, sh in enumerate(serve.make_sessions(n_sessions))]
     for sess in sessions:  # warm: multigrid hierarchies, pools
         sess.run_single(serve.REQ_STEP)
     trace = serve.make_trace(n_req, n_sessions, rate=rate)
-    single = []
-    for sess in sessions[:8]:
-        t0 = time.perf_counter()
-        sess.run_single(serve.REQ_SOLVE)
-        single.append(time.perf_counter() - t0)
+    # isolated latency of every (session, kind): median of three plain runs
+    iso = {}
+    for i, sess in enumerate(sessions):
+        for kind in (serve.REQ_STEP, serve.REQ_SOLVE):
+            ts = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                sess.run_single(kind)
+                ts.append(time.perf_counter() - t0)
+            iso[(i, kind)] = float(np.median(ts))
+    single = [iso[(i, serve.REQ_SOLVE)] for i in range(min(8, n_sessions))]
     out = {"sessions": n_sessions, "requests": n_req, "rate_req_per_s": rate,
            "isolated_solve_ms_median": round(float(np.median(single)) * 1e3, 3)}
+    by_rid = {r.rid: r for r in trace}
+
+    def slowdown(lat):
+        # latency of each request over the isolated latency of the same request
+        sl = np.sort([lat[rid] / iso[(by_rid[rid].session, by_rid[rid].kind)] for rid in lat])
+        return {"slowdown_p50": round(float(np.percentile(sl, 50)), 3),
+                "slowdown_p99": round(float(np.percentile(sl, 99)), 3)}
+
     for name, realtime in (("realtime", True), ("saturated", False)):
         srv = serve.Server(sessions)
         lat, span = srv.run(trace, realtime=realtime)
         out["served_" + name] = {k: round(v, 3) for k, v in serve.summarize(lat, span).items()}
         out["served_" + name]["mean_batch"] = round(float(np.mean(srv.batch_sizes)), 2)
+        if realtime:
+            out["served_" + name].update(slowdown(lat))
         lat, span = serve.serve_serial(sessions, trace, realtime=realtime)
         out["serial_" + name] = {k: round(v, 3) for k, v in serve.summarize(lat, span).items()}
+        if realtime:
+            out["serial_" + name].update(slowdown(lat))
     out["throughput_speedup_saturated"] = round(out["served_saturated"]["solves_per_s"] /
                                                 out["serial_saturated"]["solves_per_s"], 2)
     return out
