@@ -270,13 +270,24 @@ __global__ void __launch_bounds__(kBlock) k_maccormack_centered(Vel V, Dims d, T
 			for (int a = 0; a < 3; a++) P[a] = (int)(l == 0 ? pos[a] - vdt[a] : pos[a] + vdt[a]);
 			const int i0 = clampi(P[0], 0, gux - 1), j0 = clampi(P[1], 0, guy - 1);
 			const I c0 = g.at(i0, j0, IS3D ? clampi(P[2], 0, guz - 1) : 0);
-			// corners (i0|i1, j0|j1, k0|k1) in the reference's order
+			// corners (i0|i1, j0|j1, k0|k1) in the reference's order; every
+			// corner address is inside the grid, so flags and values are all
+			// loaded before the flag tests
+			constexpr int NQ = IS3D ? 8 : 4;
 			const I off[8] = {0, 1, Y, 1 + Y, Z, 1 + Z, Y + Z, 1 + Y + Z};
+			bool cf[NQ];
+			float cv[NQ][NC];
 #pragma unroll
-			for (int q = 0; q < (IS3D ? 8 : 4); q++) {
+			for (int q = 0; q < NQ; q++) {
 				const I cx = c0 + off[q];
-				if (CHECKFLAG(cx)) {
-					for (int c = 0; c < NC; c++) min_max(orig[c][cx], mn[c], mx[c]);
+				cf[q] = CHECKFLAG(cx);
+#pragma unroll
+				for (int c = 0; c < NC; c++) cv[q][c] = orig[c][cx];
+			}
+#pragma unroll
+			for (int q = 0; q < NQ; q++) {
+				if (cf[q]) {
+					for (int c = 0; c < NC; c++) min_max(cv[q][c], mn[c], mx[c]);
 					haveFl = true;
 				}
 			}
@@ -634,11 +645,15 @@ void op_add_buoyancy(Scene* s, Grid* flags, Grid* density, Grid* vel, const floa
 	                   (const float*)density->real(), vel->comp(0), vel->comp(1), vel->comp(2), f[0], f[1], f[2]);
 }
 
+// Testing hook: route every grid through the 64-bit index instantiations
+// (otherwise only grids of 2^31 cells and more reach them).
+static int g_force_index64 = 0;
+
 // Calls f(std::integral_constant<bool, IS3D>, I{}) with the grid's dimension
 // and the narrowest index type that addresses it.
 template <typename F>
 static void with_geo(const Dims& d, F&& f) {
-	const bool i32 = index32_ok(d);
+	const bool i32 = index32_ok(d) && !g_force_index64;
 	if (d.is3d) {
 		if (i32) f(std::true_type{}, int{});
 		else f(std::true_type{}, (long long){});
@@ -740,6 +755,16 @@ void op_advect(Scene* s, Grid* flags, Grid* vel, Grid* grid, int order, float st
 }  // namespace mfx
 
 using namespace mfx;
+
+// Not part of the public interface (include/mfx.h): tests use it to run the
+// advection kernels' 64-bit index path on small grids.  Returns the previous
+// setting.
+extern "C" int mfx_testing_force_index64(int on) {
+	const int prev = g_force_index64;
+	g_force_index64 = on ? 1 : 0;
+	return prev;
+}
+
 struct mfx_grid : public Grid {};
 static inline Grid* G(const mfx_grid* g) { return const_cast<Grid*>(static_cast<const Grid*>(g)); }
 

@@ -120,6 +120,39 @@ def test_advect_mac_self(manta, oracle, rng, shape, order, clamp, open_bound):
     assert_bitexact(v.toNumpy(), ref, "advect mac")
 
 
+@pytest.mark.parametrize("shape", [(24, 20, 1), (17, 13, 11)])
+@pytest.mark.parametrize("clamp", [1, 2])
+def test_advect_index64_path(manta, oracle, rng, shape, clamp):
+    """The advection kernels' 64-bit index instantiations (used for grids of
+    2^31 cells and more) forced onto small grids: MacCormack of a real grid,
+    a Vec3 grid and the MAC velocity itself, bit-exact against the oracle."""
+    import ctypes as C
+    from mantaflow_amd._abi import lib
+    force = lib().mfx_testing_force_index64
+    force.restype, force.argtypes = C.c_int, [C.c_int]
+    n = int(np.prod(shape))
+    flags, vel = _adv_inputs(oracle, rng, shape, open_bound="Y")
+    dens = rng.uniform(0, 1, n).astype(np.float32)
+    g = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+    dt = 1.2
+    prev = force(1)
+    try:
+        sc = GpuScene(manta, shape, dt)
+        f, v, d, gg = sc.flags(flags), sc.mac(vel), sc.real(dens), sc.vec(g)
+        ref_d, ref_g, ref_v = dens.copy(), g.copy(), vel.copy()
+        oracle.advect(shape, np.float32(dt), flags, vel, ref_d, TYPE_REAL, 2, 1.0, clamp)
+        oracle.advect(shape, np.float32(dt), flags, vel, ref_g, TYPE_VEC3, 2, 1.0, clamp)
+        oracle.advect(shape, np.float32(dt), flags, ref_v, ref_v, TYPE_MAC, 2, 1.0, clamp)
+        manta.advectSemiLagrange(flags=f, vel=v, grid=d, order=2, clampMode=clamp)
+        manta.advectSemiLagrange(flags=f, vel=v, grid=gg, order=2, clampMode=clamp)
+        manta.advectSemiLagrange(flags=f, vel=v, grid=v, order=2, strength=1.0, clampMode=clamp)
+        assert_bitexact(d.toNumpy(), ref_d, "advect real (64-bit indices)")
+        assert_bitexact(gg.toNumpy(), ref_g, "advect vec3 (64-bit indices)")
+        assert_bitexact(v.toNumpy(), ref_v, "advect mac (64-bit indices)")
+    finally:
+        force(prev)
+
+
 @pytest.mark.parametrize("shape", SHAPES[:2])
 @pytest.mark.parametrize("order,clamp", [(1, 2), (2, 1), (2, 2)])
 def test_advect_vec3(manta, oracle, rng, shape, order, clamp):
