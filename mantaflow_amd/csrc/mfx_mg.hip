@@ -607,17 +607,23 @@ __device__ __forceinline__ void body_l0_up_brick(const L0UpP& P, long long blk, 
 // ---------------------------------------------------------------------------
 // z-marching level-0 kernels (3D).  A workgroup owns a 64 x TY column tile
 // and a chunk of planes [k0, k1).  It walks the planes once, staging each
-// plane of the tile plus a 2-cell ring in a 5-slot LDS ring buffer, and runs
-// the three dependent stages of red-black Gauss-Seidel as a pipeline:
+// plane of the tile plus a 2-cell ring in a 6-slot LDS ring buffer, and runs
+// the three dependent stages of red-black Gauss-Seidel as a pipeline whose
+// stages are two planes apart:
 //   A(p)   load plane p (code, rhs, x), first colour pointwise
-//   B(p-1) second colour on plane p-1 (tile + 1-cell ring)
-//   C(p-2) final output on the tile's cells of plane p-2
-// so every input element is read from HBM once plus 4 halo planes per
-// chunk.  Arithmetic and operation order equal the brick kernels above.
+//   B(p-2) second colour on plane p-2 (tile + 1-cell ring): needs the first
+//          colour of planes p-3 .. p-1
+//   C(p-4) final output on the tile's cells of plane p-4: needs both colours
+//          of planes p-5 .. p-3
+// Each stage reads only what the previous iterations finished, and the three
+// touch different ring slots, so one barrier per plane suffices.  Every input
+// element is read from HBM once plus 4 halo planes per chunk.  Arithmetic
+// and operation order equal the brick kernels above.
 // ---------------------------------------------------------------------------
+constexpr int ZRING = 6;  // LDS ring slots: planes p-5 .. p
 template <int TY>
 struct ZTile {
-	static constexpr int TX = 64, H = 2, RX = TX + 2 * H, RY = TY + 2 * H, RP = RX * RY, NS = 5, NT = TX * TY;
+	static constexpr int TX = 64, H = 2, RX = TX + 2 * H, RY = TY + 2 * H, RP = RX * RY, NS = ZRING, NT = TX * TY;
 };
 
 __device__ inline void zm_origin(long long blk, const Dims& d, int TY, int kchunk, int& ox, int& oy, int& k0,
@@ -644,7 +650,6 @@ __device__ inline float zm_nb_sum(const uint8_t* cP, const float* xP, const uint
 	return sum;
 }
 
-__device__ inline int zslot(int p) { return (p + 10) % 5; }  // p >= -2
 
 // The (at most two) region cells of one thread: q = tid and tid + NT.
 template <typename T>
@@ -705,8 +710,8 @@ struct ZmPos {
 	}
 };
 
-__device__ inline int ring_next(int s) { return s == 4 ? 0 : s + 1; }
-__device__ inline int ring_back(int s, int k) { return (s + 5 - k) % 5; }
+__device__ inline int ring_next(int s) { return s == ZRING - 1 ? 0 : s + 1; }
+__device__ inline int ring_back(int s, int k) { return (s + ZRING - k) % ZRING; }
 
 // Down leg (same sweeps as body_l0_down_brick): red from x = 0, black, residual.
 template <int TY>
@@ -746,35 +751,35 @@ __device__ __forceinline__ void body_l0_down_zm(const L0DownP& P, long long blk,
 	};
 	fetch(k0 - 2);
 	int sp = 0;  // ring slot of plane p
-	for (int p = k0 - 2; p <= k1 + 1; p++, sp = ring_next(sp)) {
+	for (int p = k0 - 2; p <= k1 + 3; p++, sp = ring_next(sp)) {
+		if (p <= k1 + 1) {
 #pragma unroll
-		for (int u = 0; u < 2; u++) {
-			const int q = pos.qa[u];
-			if (q >= T::RP) continue;
-			const uint8_t cc = pc[u];
-			const float b = pr[u] * l0_scale(cc);
-			cs[sp][q] = cc;
-			bs[sp][q] = b;
-			xs[sp][q] = (l0_active(cc) && ((cells.par[u] + p) & 1) == 0) ? l0_div(b, cc) : 0.f;
+			for (int u = 0; u < 2; u++) {
+				const int q = pos.qa[u];
+				if (q >= T::RP) continue;
+				const uint8_t cc = pc[u];
+				const float b = pr[u] * l0_scale(cc);
+				cs[sp][q] = cc;
+				bs[sp][q] = b;
+				xs[sp][q] = (l0_active(cc) && ((cells.par[u] + p) & 1) == 0) ? l0_div(b, cc) : 0.f;
+			}
+			if (p <= k1) fetch(p + 1);
 		}
-		if (p <= k1) fetch(p + 1);
-		__syncthreads();
-		const int pb = p - 1;
+		const int pb = p - 2;
 		if (pb >= k0 - 1 && pb <= k1 && pb >= 0 && pb < d.nz) {
-			const int s0 = ring_back(sp, 1), sm = ring_back(sp, 2);
+			const int s0 = ring_back(sp, 2), sm = ring_back(sp, 3), su = ring_back(sp, 1);
 #pragma unroll
 			for (int u = 0; u < 2; u++) {
 				if (!pos.inb[u]) continue;
 				const int q = pos.qb[u];
 				const uint8_t cc = cs[s0][q];
 				if (((pos.parb[u] + pb) & 1) == 1 && l0_active(cc))
-					xs[s0][q] = l0_div(bs[s0][q] + zm_nb_sum(cs[s0], xs[s0], cs[sm], xs[sm], xs[sp], q, T::RX), cc);
+					xs[s0][q] = l0_div(bs[s0][q] + zm_nb_sum(cs[s0], xs[s0], cs[sm], xs[sm], xs[su], q, T::RX), cc);
 			}
 		}
-		__syncthreads();
-		const int pz = p - 2;
+		const int pz = p - 4;
 		if (pz >= k0 && pz < k1 && pos.inc) {
-			const int s0 = ring_back(sp, 2), sm = ring_back(sp, 3), su = ring_back(sp, 1);
+			const int s0 = ring_back(sp, 4), sm = ring_back(sp, 5), su = ring_back(sp, 3);
 			const int q = pos.qc;
 			const long long g = pos.gc + d.sz * pz;
 			const uint8_t cc = cs[s0][q];
@@ -784,6 +789,7 @@ __device__ __forceinline__ void body_l0_down_zm(const L0DownP& P, long long blk,
 			            ? (bs[s0][q] + zm_nb_sum(cs[s0], xs[s0], cs[sm], xs[sm], xs[su], q, T::RX)) - l0_diag(cc) * x
 			            : 0.f;
 		}
+		__syncthreads();
 	}
 }
 
@@ -862,63 +868,63 @@ __device__ __forceinline__ void body_l0_up_zm(const L0UpP& P, long long blk, lon
 	__syncthreads();
 	double acc = 0.0;
 	int sp = 0;  // ring slot of plane p
-	for (int p = k0 - 2; p <= k1 + 1; p++, sp = ring_next(sp)) {
-		const float* c0 = xcs[(p >> 1) & 1];
-		const float* c1p = xcs[((p >> 1) + 1) & 1];
-		const bool zo = p & 1;
+	for (int p = k0 - 2; p <= k1 + 3; p++, sp = ring_next(sp)) {
+		if (p <= k1 + 1) {
+			const float* c0 = xcs[(p >> 1) & 1];
+			const float* c1p = xcs[((p >> 1) + 1) & 1];
+			const bool zo = p & 1;
 #pragma unroll
-		for (int u = 0; u < 2; u++) {
-			const int q = pos.qa[u];
-			if (q >= T::RP) continue;
-			const uint8_t cc = pc[u];
-			float x = px[u];
-			if (l0_active(cc)) {
-				// parents in the reference's loop order: z, then y, then x
-				const int o = po[u];
-				float sum = c0[o];
-				if (pxo[u]) sum += c0[o + 1];
-				if (pyo[u]) {
-					sum += c0[o + CRX];
-					if (pxo[u]) sum += c0[o + CRX + 1];
-				}
-				if (zo) {
-					sum += c1p[o];
-					if (pxo[u]) sum += c1p[o + 1];
+			for (int u = 0; u < 2; u++) {
+				const int q = pos.qa[u];
+				if (q >= T::RP) continue;
+				const uint8_t cc = pc[u];
+				float x = px[u];
+				if (l0_active(cc)) {
+					// parents in the reference's loop order: z, then y, then x
+					const int o = po[u];
+					float sum = c0[o];
+					if (pxo[u]) sum += c0[o + 1];
 					if (pyo[u]) {
-						sum += c1p[o + CRX];
-						if (pxo[u]) sum += c1p[o + CRX + 1];
+						sum += c0[o + CRX];
+						if (pxo[u]) sum += c0[o + CRX + 1];
 					}
+					if (zo) {
+						sum += c1p[o];
+						if (pxo[u]) sum += c1p[o + 1];
+						if (pyo[u]) {
+							sum += c1p[o + CRX];
+							if (pxo[u]) sum += c1p[o + CRX + 1];
+						}
+					}
+					x += (zo ? 0.5f * pw[u] : pw[u]) * sum;
 				}
-				x += (zo ? 0.5f * pw[u] : pw[u]) * sum;
+				cs[sp][q] = cc;
+				rs[sp][q] = pr[u];
+				xs[sp][q] = x;
 			}
-			cs[sp][q] = cc;
-			rs[sp][q] = pr[u];
-			xs[sp][q] = x;
+			if (!(p & 1)) {
+				// coarse plane p/2+1 is first needed by plane p+1
+				if (cq < CRP) xcs[((p >> 1) + 1) & 1][cq] = creg;
+				creg = cfetch((p >> 1) + 2);
+			}
+			if (p <= k1) fetch(p + 1);
 		}
-		if (!(p & 1)) {
-			// coarse plane p/2+1 is first needed by plane p+1
-			if (cq < CRP) xcs[((p >> 1) + 1) & 1][cq] = creg;
-			creg = cfetch((p >> 1) + 2);
-		}
-		if (p <= k1) fetch(p + 1);
-		__syncthreads();
-		const int pb = p - 1;
+		const int pb = p - 2;
 		if (pb >= k0 - 1 && pb <= k1 && pb >= 0 && pb < d.nz) {
-			const int s0 = ring_back(sp, 1), sm = ring_back(sp, 2);
+			const int s0 = ring_back(sp, 2), sm = ring_back(sp, 3), su = ring_back(sp, 1);
 #pragma unroll
 			for (int u = 0; u < 2; u++) {
 				if (!pos.inb[u]) continue;
 				const int q = pos.qb[u];
 				const uint8_t cc = cs[s0][q];
 				if (((pos.parb[u] + pb) & 1) == 1 && l0_active(cc))
-					xs[s0][q] = l0_div(rs[s0][q] * l0_scale(cc) + zm_nb_sum(cs[s0], xs[s0], cs[sm], xs[sm], xs[sp], q, T::RX),
+					xs[s0][q] = l0_div(rs[s0][q] * l0_scale(cc) + zm_nb_sum(cs[s0], xs[s0], cs[sm], xs[sm], xs[su], q, T::RX),
 					                   cc);
 			}
 		}
-		__syncthreads();
-		const int pz = p - 2;
+		const int pz = p - 4;
 		if (pz >= k0 && pz < k1 && pos.inc) {
-			const int s0 = ring_back(sp, 2), sm = ring_back(sp, 3), su = ring_back(sp, 1);
+			const int s0 = ring_back(sp, 4), sm = ring_back(sp, 5), su = ring_back(sp, 3);
 			const int q = pos.qc;
 			const long long g = pos.gc + d.sz * pz;
 			const uint8_t cc = cs[s0][q];
@@ -928,6 +934,7 @@ __device__ __forceinline__ void body_l0_up_zm(const L0UpP& P, long long blk, lon
 			z[g] = x;
 			acc += (double)(x * rs[s0][q]);
 		}
+		__syncthreads();
 	}
 	const double s = block_sum(acc, sh);
 	if (threadIdx.x == 0) part[3 * slot] = s;
