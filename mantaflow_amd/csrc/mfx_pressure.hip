@@ -341,8 +341,8 @@ __device__ __forceinline__ void body_dot(const DotP& P, long long blk, long long
 template <int TY>
 __device__ __forceinline__ void body_matvec(const MatvecP& P, long long blk, long long slot) {
 	constexpr int TX = 64, RX = TX + 2, RY = TY + 2, NH = 2 * RX + 2 * (RY - 2);
-	__shared__ float ps[RX * RY];
-	__shared__ uint8_t cs[RX * RY];
+	__shared__ float ps[2][RX * RY];
+	__shared__ uint8_t cs[2][RX * RY];
 	__shared__ double sh[16];
 	const Dims& d = P.d;
 	const int kchunk = P.kchunk;
@@ -395,33 +395,49 @@ __device__ __forceinline__ void body_matvec(const MatvecP& P, long long blk, lon
 			c = code[m];
 		}
 	};
-	float pPrev, pCur, pNext, pN2, hv;
-	uint8_t cPrev, cCur, cNext, cN2, hc;
+	// Planes alternate between two LDS buffers: plane k+1 is written while
+	// plane k is read, so one barrier per plane orders both.  Own column:
+	// planes k-1, k, k+1 in registers, k+2 in flight; ring cell: plane k+1 in
+	// a register, k+2 in flight.
+	float pPrev, pCur, pNext, pN2, hv, hv1;
+	uint8_t cPrev, cCur, cNext, cN2, hc, hc1;
 	fcol(k0 - 1, pPrev, cPrev);
 	fcol(k0, pCur, cCur);
 	fcol(k0 + 1, pNext, cNext);
 	fhalo(k0, hv, hc);
+	fhalo(k0 + 1, hv1, hc1);
 	const int L = (tx + 1) + RX * (ty + 1);
+	ps[0][L] = pCur;
+	cs[0][L] = cCur;
+	if (h < NH) {
+		ps[0][hx + RX * hy] = hv;
+		cs[0][hx + RX * hy] = hc;
+	}
+	__syncthreads();
 	double acc = 0.0;
-	for (int k = k0; k < k1; k++) {
-		ps[L] = pCur;
-		cs[L] = cCur;
-		if (h < NH) {
-			ps[hx + RX * hy] = hv;
-			cs[hx + RX * hy] = hc;
-		}
+	int b = 0;  // buffer of plane k
+	for (int k = k0; k < k1; k++, b ^= 1) {
 		fcol(k + 2, pN2, cN2);
-		fhalo(k + 1, hv, hc);
-		__syncthreads();
+		if (k + 1 < k1) {
+			ps[b ^ 1][L] = pNext;
+			cs[b ^ 1][L] = cNext;
+			if (h < NH) {
+				ps[b ^ 1][hx + RX * hy] = hv1;
+				cs[b ^ 1][hx + RX * hy] = hc1;
+			}
+		}
+		fhalo(k + 2, hv1, hc1);
 		if (in) {
+			const float* __restrict__ pp = ps[b];
+			const uint8_t* __restrict__ cp = cs[b];
 			float qv;
 			if (!(cCur & 64)) qv = pCur;
 			else if (d.is3d)
-				qv = pCur * cA0(cCur) + ps[L - 1] * cAi(cs[L - 1]) + ps[L + 1] * cAi(cCur) + ps[L - RX] * cAj(cs[L - RX]) +
-				     ps[L + RX] * cAj(cCur) + pPrev * cAk(cPrev) + pNext * cAk(cCur);
+				qv = pCur * cA0(cCur) + pp[L - 1] * cAi(cp[L - 1]) + pp[L + 1] * cAi(cCur) + pp[L - RX] * cAj(cp[L - RX]) +
+				     pp[L + RX] * cAj(cCur) + pPrev * cAk(cPrev) + pNext * cAk(cCur);
 			else
-				qv = pCur * cA0(cCur) + ps[L - 1] * cAi(cs[L - 1]) + ps[L + 1] * cAi(cCur) + ps[L - RX] * cAj(cs[L - RX]) +
-				     ps[L + RX] * cAj(cCur);
+				qv = pCur * cA0(cCur) + pp[L - 1] * cAi(cp[L - 1]) + pp[L + 1] * cAi(cCur) + pp[L - RX] * cAj(cp[L - RX]) +
+				     pp[L + RX] * cAj(cCur);
 			const long long m = d.idx(gx, gy, k);
 			pnew[m] = pCur;
 			q[m] = qv;
