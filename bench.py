@@ -600,6 +600,10 @@ def main():
         except Exception as e:  # the baseline must not hide the GPU number
             cpu = {"error": str(e)}
 
+    mvc, l0c = C.c_int(), C.c_int()
+    tq = lib().mfx_testing_zmarch_chunks
+    tq.restype, tq.argtypes = None, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    tq(n, n, n, C.byref(mvc), C.byref(l0c))
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mcell-steps/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(wall / a.steps * 1e3, 3),
@@ -609,7 +613,8 @@ def main():
                                "open top, wall BCs, buoyancy, MG-preconditioned CG projection to rel. "
                                "residual 1e-4" % n,
                    "grid": [n, n, n], "cg_iterations_mean": float(np.mean(iters)),
-                   "rel_residual_max": float(np.max(rel)), "parallelism": "single GPU"},
+                   "rel_residual_max": float(np.max(rel)), "parallelism": "single GPU",
+                   "zmarch_plane_chunks_autotuned": {"cg_matvec": mvc.value, "mg_level0": l0c.value}},
         "roofline": roof,
         "kernels": kstats,
         "projection": proj,

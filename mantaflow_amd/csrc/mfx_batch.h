@@ -117,8 +117,66 @@ struct DotP {
 enum { FIN_INIT = 0, FIN_ALPHA = 1, FIN_RES = 2, FIN_BETA = 3, FIN_BETA0 = 4 };
 constexpr int kMvTY = 16;
 
-// tiles of each CG kernel for one problem (and the plane chunk / range size)
-long long matvec_tiles(const Dims& d, int* kchunk);
+// Plane chunks of the z-marching kernels (CG matvec, level-0 multigrid) for
+// one grid shape, picked by autotune_zmarch (mfx_pressure.hip) the first
+// time a 3D shape is solved alone; 0 = not tuned (a heuristic chunk is
+// used).  3D chunks are multiples of kDotGroup planes, at most kZmMaxChunk.
+struct ZmTune {
+	int mv = 0, l0 = 0;
+};
+ZmTune zm_tuned(const Dims& d);
+void zm_set_tuned(const Dims& d, ZmTune t);
+
+// Dot products of the z-marching kernels (fl(q p) of the matvec, fl(z r) of
+// the level-0 up-sweep) leave one partial per tile column and group of
+// kDotGroup planes: a thread sums its cells of the group in plane order, the
+// wavefront sums its lanes (fixed DPP shift-add order), the tile sums its
+// wavefronts in order.  With chunks that start at multiples of kDotGroup this partition
+// does not depend on the chunk, so every chunk gives the same bits.
+constexpr int kDotGroup = 4;
+constexpr int kZmMaxChunk = 64;
+constexpr int kZmMaxGroups = kZmMaxChunk / kDotGroup;
+inline long long zm_dot_parts(const Dims& d, int ty) {
+	return (long long)((d.nx + 63) / 64) * ((d.ny + ty - 1) / ty) * ((d.nz + kDotGroup - 1) / kDotGroup);
+}
+// a 3D chunk of at least kc planes that the dot partition allows
+inline int zm_chunk(int kc) {
+	kc = (kc + kDotGroup - 1) / kDotGroup * kDotGroup;
+	return kc < kDotGroup ? kDotGroup : (kc > kZmMaxChunk ? kZmMaxChunk : kc);
+}
+
+// the calling block's tile column (its tile index modulo the columns)
+__device__ inline long long zm_column(long long blk, const Dims& d, int ty) {
+	const long long cols = (long long)((d.nx + 63) / 64) * ((d.ny + ty - 1) / ty);
+	return blk % cols;
+}
+// end of a dot group at plane k: every thread of the block calls this; the
+// wavefront sum goes to tab[group - first group of the tile][wavefront]
+template <int TY>
+__device__ inline void zm_dot_flush(double& acc, double (*tab)[TY], int k, int k0) {
+	const double s = wave_sum_to_last(acc);
+	if ((threadIdx.x & 63) == 63) tab[k / kDotGroup - k0 / kDotGroup][threadIdx.x >> 6] = s;
+	acc = 0.0;
+}
+// after a barrier that follows the last flush: one thread per group of the
+// tile writes the group's partial
+template <int TY>
+__device__ inline void zm_dot_store(const double (*tab)[TY], const Dims& d, long long col, int k0, int k1,
+                                    double* part) {
+	const int g0 = k0 / kDotGroup, ng = (k1 - 1) / kDotGroup - g0 + 1;
+	const int t = threadIdx.x;
+	if (t < ng) {
+		double s = 0.0;
+		for (int w = 0; w < TY; w++) s += tab[t][w];
+		const long long groups = (d.nz + kDotGroup - 1) / kDotGroup;
+		part[3 * (col * groups + g0 + t)] = s;
+	}
+}
+
+// tiles of each CG kernel for one problem (and the plane chunk / range
+// size); tuned = the autotuned chunk of the shape where there is one (single
+// solves), else the heuristic (batched solves)
+long long matvec_tiles(const Dims& d, int* kchunk, bool tuned);
 long long update_tiles(const Dims& d, long long* per);
 inline long long dot_tiles(const Dims& d) { return (d.n + kBlock - 1) / kBlock; }
 
@@ -206,6 +264,7 @@ struct MgHierarchy::Steps {
 	int nl = 0;
 	int kchunk = 1;
 	long long l0Tiles = 0;
+	long long nPart = 0;  // partials of fl(z r) left by the up-sweep
 	L0DownP down{};
 	L0UpP up{};
 	struct Level {  // levels 1..nl-1
@@ -220,12 +279,14 @@ struct MgHierarchy::Steps {
 };
 
 // tiles of the V-cycle kernels for one problem/level
-long long l0_tiles(const Dims& d, int* kchunk);  // z-marching (3D) or brick (2D)
+long long l0_tiles(const Dims& d, int* kchunk, bool tuned);  // z-marching (3D) or brick (2D)
 long long lc_tiles(const LvDev& c);               // 64x4 vertex tiles per plane
 long long lcblk_tiles(const LvDev& c, bool is3d); // smoother bricks
 inline long long coarsest_tiles(int nc) { return (nc + 3) / 4; }
 
 void run_l0_down_many(hipStream_t s, bool is3d, unsigned nb, WorkList items, const L0DownP* p);
+void run_l0_down_one(hipStream_t s, bool is3d, unsigned nb, const L0DownP& p);
+constexpr int kZmTYHost = 16;  // rows per z-marching level-0 tile (wavefronts per workgroup)
 void run_l0_up_many(hipStream_t s, bool is3d, unsigned nb, WorkList items, const L0UpP* p);
 void run_restrict_many(hipStream_t s, bool is3d, unsigned nb, WorkList items, const RestrictP* p);
 // post: post-smoothing (prolongation on load, reversed colours), else pre-smoothing

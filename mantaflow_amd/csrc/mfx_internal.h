@@ -215,6 +215,27 @@ __device__ inline T wave_sum(T v) {
 	for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
 	return v;
 }
+// Sum over the wavefront of a double, valid in lane 63 only: shift-adds
+// within rows of 16 lanes (DPP row_shr 1, 2, 4, 8), then rows into rows
+// (row_bcast15 into rows 1 and 3, row_bcast31 into rows 2 and 3).  Lanes
+// that a DPP move does not write read 0.0.  Data-path moves instead of the
+// LDS crossbar of __shfl_xor; the summation order is fixed.
+template <int CTRL, int ROWS>
+__device__ inline double dpp_move_d(double v) {
+	const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+	const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, ROWS, 0xf, false);
+	const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, ROWS, 0xf, false);
+	return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ inline double wave_sum_to_last(double v) {
+	v += dpp_move_d<0x111, 0xf>(v);  // row_shr:1
+	v += dpp_move_d<0x112, 0xf>(v);  // row_shr:2
+	v += dpp_move_d<0x114, 0xf>(v);  // row_shr:4
+	v += dpp_move_d<0x118, 0xf>(v);  // row_shr:8
+	v += dpp_move_d<0x142, 0xa>(v);  // row_bcast:15
+	v += dpp_move_d<0x143, 0xc>(v);  // row_bcast:31
+	return v;
+}
 __device__ inline float wave_min(float v) {
 	for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
 	return v;

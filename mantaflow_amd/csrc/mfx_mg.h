@@ -55,10 +55,11 @@ public:
 	// Returns the number of partial sums of fl(z*r) the last kernel wrote into
 	// s->dpart (stride 3), or 0 if the caller has to compute dot(z, r) itself.
 	int vcycle(Scene* s, const uint8_t* code, const float* r, float* z, const CgState* st);
-	// parameters of every V-cycle step (for batched work lists)
+	// parameters of every V-cycle step (for batched work lists); tuned = the
+	// level-0 plane chunk of single solves, else the batched heuristic
 	struct Steps;
-	Steps steps(const Dims& d, const uint8_t* code, const float* r, float* z, const CgState* st,
-	            double* part) const;
+	Steps steps(const Dims& d, const uint8_t* code, const float* r, float* z, const CgState* st, double* part,
+	            bool tuned) const;
 	bool dim3d() const { return is3d; }
 	int levels() const { return (int)L.size(); }
 	int precision() const { return prec; }

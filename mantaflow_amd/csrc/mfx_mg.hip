@@ -816,7 +816,7 @@ __device__ __forceinline__ void body_l0_up_zm(const L0UpP& P, long long blk, lon
 	__shared__ float rs[T::NS][T::RP];
 	__shared__ uint8_t cs[T::NS][T::RP];
 	__shared__ float xcs[2][CRP];
-	__shared__ double sh[16];
+	__shared__ double tab[kZmMaxGroups][TY];
 	if (st->done) return;
 	int ox, oy, k0, k1;
 	zm_origin(blk, d, TY, kchunk, ox, oy, k0, k1);
@@ -923,21 +923,25 @@ __device__ __forceinline__ void body_l0_up_zm(const L0UpP& P, long long blk, lon
 			}
 		}
 		const int pz = p - 4;
-		if (pz >= k0 && pz < k1 && pos.inc) {
-			const int s0 = ring_back(sp, 4), sm = ring_back(sp, 5), su = ring_back(sp, 3);
-			const int q = pos.qc;
-			const long long g = pos.gc + d.sz * pz;
-			const uint8_t cc = cs[s0][q];
-			float x = xs[s0][q];
-			if (((pos.parc + pz) & 1) == 0 && l0_active(cc))
-				x = l0_div(rs[s0][q] * l0_scale(cc) + zm_nb_sum(cs[s0], xs[s0], cs[sm], xs[sm], xs[su], q, T::RX), cc);
-			z[g] = x;
-			acc += (double)(x * rs[s0][q]);
+		if (pz >= k0 && pz < k1) {
+			if (pos.inc) {
+				const int s0 = ring_back(sp, 4), sm = ring_back(sp, 5), su = ring_back(sp, 3);
+				const int q = pos.qc;
+				const long long g = pos.gc + d.sz * pz;
+				const uint8_t cc = cs[s0][q];
+				float x = xs[s0][q];
+				if (((pos.parc + pz) & 1) == 0 && l0_active(cc))
+					x = l0_div(rs[s0][q] * l0_scale(cc) + zm_nb_sum(cs[s0], xs[s0], cs[sm], xs[sm], xs[su], q, T::RX),
+					           cc);
+				z[g] = x;
+				acc += (double)(x * rs[s0][q]);
+			}
+			if ((pz + 1) % kDotGroup == 0 || pz + 1 == k1) zm_dot_flush<TY>(acc, tab, pz, k0);
 		}
 		__syncthreads();
 	}
-	const double s = block_sum(acc, sh);
-	if (threadIdx.x == 0) part[3 * slot] = s;
+	(void)slot;
+	zm_dot_store<TY>(tab, d, zm_column(blk, d, TY), k0, k1, part);
 }
 
 // coarse-level brick: 32x4x4 vertices in 3D, 64x8 in 2D, 512 threads (rows
@@ -1428,21 +1432,31 @@ void MgHierarchy::build_coarsest(Scene* s, const uint8_t* code0) {
 // ---------------------------------------------------------------------------
 // launches (single problem and work lists)
 // ---------------------------------------------------------------------------
-constexpr int kZmTY = 16;
-long long l0_tiles(const Dims& d, int* kchunk) {
-	// 3D: z-marching 64 x kZmTY column tiles over chunks of an even number of
-	// planes (about 8192 wavefronts, one per SIMD slot of the chip); 2D:
-	// bricks of 64x16 cells
+constexpr int kZmTY = kZmTYHost;
+long long l0_tiles(const Dims& d, int* kchunk, bool tuned) {
+	// 3D: z-marching 64 x kZmTY column tiles over chunks of a multiple of
+	// kDotGroup planes (even, as the restriction pairs planes), halved while
+	// fewer than about 8192 wavefronts (one per SIMD slot of the chip) cover
+	// the grid; 2D: bricks of 64x16 cells
 	const int BY = d.is3d ? kZmTY : 16;
 	const long long cols = (long long)((d.nx + 63) / 64) * ((d.ny + BY - 1) / BY);
-	int kc = (d.nz + 1) & ~1;
-	while (d.is3d && kc > 16 && cols * ((d.nz + kc - 1) / kc) < 8192 / kZmTY) kc = ((kc / 2) + 1) & ~1;
+	if (!d.is3d) {
+		*kchunk = 2;
+		return cols;
+	}
+	int kc = zm_chunk(d.nz);
+	while (kc > 16 && cols * ((d.nz + kc - 1) / kc) < 8192 / kZmTY) kc = zm_chunk(kc / 2);
+	if (tuned && zm_tuned(d).l0 > 0) kc = zm_tuned(d).l0;
 	*kchunk = kc;
-	return d.is3d ? cols * ((d.nz + kc - 1) / kc) : cols;
+	return cols * ((d.nz + kc - 1) / kc);
 }
 long long lcblk_tiles(const LvDev& c, bool is3d) {
 	const long long bx = is3d ? kLcBX3 : kLcBX2, by = is3d ? kLcBY3 : kLcBY2, bz = is3d ? kLcBZ3 : 1;
 	return ((c.nx + bx - 1) / bx) * ((c.ny + by - 1) / by) * ((c.nz + bz - 1) / bz);
+}
+void run_l0_down_one(hipStream_t s, bool is3d, unsigned nb, const L0DownP& p) {
+	if (is3d) launch_one<L0DownP, body_l0_down_zm<kZmTY>, 64 * kZmTY>(s, nb, p);
+	else launch_one<L0DownP, body_l0_down_brick<64, 16, 1, false>, 1024>(s, nb, p);
 }
 void run_l0_down_many(hipStream_t s, bool is3d, unsigned nb, WorkList items, const L0DownP* p) {
 	if (is3d) launch_many<L0DownP, body_l0_down_zm<kZmTY>, 64 * kZmTY>(s, nb, items, p);
@@ -1485,11 +1499,12 @@ void run_fill_many(hipStream_t s, unsigned nb, WorkList items, const FillP* p) {
 // Parameters of every V-cycle step of this hierarchy (shared by the single
 // launch sequence below and by batched work lists).
 MgHierarchy::Steps MgHierarchy::steps(const Dims& d, const uint8_t* code, const float* r, float* z, const CgState* st,
-                                      double* part) const {
+                                      double* part, bool tuned) const {
 	Steps S_;
 	const int nl = (int)L.size();
 	S_.nl = nl;
-	S_.l0Tiles = l0_tiles(d, &S_.kchunk);
+	S_.l0Tiles = l0_tiles(d, &S_.kchunk, tuned);
+	S_.nPart = is3d ? zm_dot_parts(d, kZmTY) : S_.l0Tiles;
 	const void* noA = nullptr;
 	if (nl == 1) {
 		S_.fill = FillP{st, z, d.n};
@@ -1523,7 +1538,7 @@ MgHierarchy::Steps MgHierarchy::steps(const Dims& d, const uint8_t* code, const 
 
 int MgHierarchy::vcycle(Scene* s, const uint8_t* code, const float* r, float* z, const CgState* st) {
 	const Dims& d = s->d;
-	const Steps S_ = steps(d, code, r, z, st, s->dpart);
+	const Steps S_ = steps(d, code, r, z, st, s->dpart, true);
 	hipStream_t q = s->stream;
 	const bool bf = prec == 1;
 	if (S_.nl == 1) {
@@ -1569,7 +1584,7 @@ int MgHierarchy::vcycle(Scene* s, const uint8_t* code, const float* r, float* z,
 		if (is3d) launch_one<L0UpP, body_l0_up_zm<kZmTY>, 64 * kZmTY>(q, S_.l0Tiles, S_.up);
 		else launch_one<L0UpP, body_l0_up_brick<64, 16, 1, false>, 1024>(q, S_.l0Tiles, S_.up);
 	}
-	return (int)S_.l0Tiles;
+	return (int)S_.nPart;
 }
 
 }  // namespace mfx

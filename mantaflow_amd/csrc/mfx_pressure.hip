@@ -28,6 +28,10 @@
 // return at once after convergence, so the host can queue iterations in
 // batches and only synchronise between batches.
 #include <algorithm>
+#include <type_traits>
+#include <array>
+#include <map>
+#include <mutex>
 #include <chrono>
 #include <deque>
 #include <cmath>
@@ -343,7 +347,7 @@ __device__ __forceinline__ void body_matvec(const MatvecP& P, long long blk, lon
 	constexpr int TX = 64, RX = TX + 2, RY = TY + 2, NH = 2 * RX + 2 * (RY - 2);
 	__shared__ float ps[2][RX * RY];
 	__shared__ uint8_t cs[2][RX * RY];
-	__shared__ double sh[16];
+	__shared__ double tab[kZmMaxGroups][TY];
 	const Dims& d = P.d;
 	const int kchunk = P.kchunk;
 	const CgState* __restrict__ st = P.st;
@@ -443,6 +447,7 @@ __device__ __forceinline__ void body_matvec(const MatvecP& P, long long blk, lon
 			q[m] = qv;
 			acc += (double)(qv * pCur);
 		}
+		if ((k + 1) % kDotGroup == 0 || k + 1 == k1) zm_dot_flush<TY>(acc, tab, k, k0);
 		__syncthreads();
 		pPrev = pCur;
 		cPrev = cCur;
@@ -451,8 +456,8 @@ __device__ __forceinline__ void body_matvec(const MatvecP& P, long long blk, lon
 		pNext = pN2;
 		cNext = cN2;
 	}
-	const double s = block_sum(acc, sh);
-	if (threadIdx.x == 0) P.part[3 * slot] = s;
+	(void)slot;
+	zm_dot_store<TY>(tab, d, zm_column(blk, d, TY), k0, k1, P.part);
 }
 
 // B: x += alpha p ; r += (-alpha) q ; min/max(r), sum fl(r*r), sum r^2.
@@ -766,12 +771,29 @@ __global__ void __launch_bounds__(kBlock) k_cg_replace_residual(Dims d, Tiles t,
 // ---------------------------------------------------------------------------
 // launches of the CG loop kernels (single problem and work lists)
 // ---------------------------------------------------------------------------
-long long matvec_tiles(const Dims& d, int* kchunk) {
-	// 64 x kMvTY column tiles, chunks of planes sized so that about 8192
-	// wavefronts cover the grid
+static std::mutex g_tune_mu;
+static std::map<std::array<int, 3>, ZmTune> g_tune;
+ZmTune zm_tuned(const Dims& d) {
+	std::lock_guard<std::mutex> g(g_tune_mu);
+	auto it = g_tune.find({d.nx, d.ny, d.nz});
+	return it == g_tune.end() ? ZmTune{} : it->second;
+}
+void zm_set_tuned(const Dims& d, ZmTune t) {
+	std::lock_guard<std::mutex> g(g_tune_mu);
+	g_tune[{d.nx, d.ny, d.nz}] = t;
+}
+
+long long matvec_tiles(const Dims& d, int* kchunk, bool tuned) {
+	// 64 x kMvTY column tiles; 3D: chunks of planes halved while fewer than
+	// about 8192 wavefronts cover the grid, or the autotuned chunk
 	const long long cols = (long long)((d.nx + 63) / 64) * ((d.ny + kMvTY - 1) / kMvTY);
-	int kc = d.nz;
-	while (kc > 8 && cols * ((d.nz + kc - 1) / kc) < 8192 / kMvTY) kc = (kc + 1) / 2;
+	if (!d.is3d) {
+		*kchunk = 1;
+		return cols;
+	}
+	int kc = zm_chunk(d.nz);
+	while (kc > 8 && cols * ((d.nz + kc - 1) / kc) < 8192 / kMvTY) kc = zm_chunk(kc / 2);
+	if (tuned && zm_tuned(d).mv > 0) kc = zm_tuned(d).mv;
 	*kchunk = kc;
 	return cols * ((d.nz + kc - 1) / kc);
 }
@@ -873,6 +895,7 @@ struct CgRun {
 	std::chrono::steady_clock::time_point t0;
 	Tiles t{};
 	unsigned nbT = 0, nbF = 0, nbM = 0, nbU = 0;
+	int npM = 0;  // partials of fl(q p) left by the matvec
 	int kchunk = 1;
 	long long per = 0;
 	long long fix = -1;
@@ -902,6 +925,81 @@ struct CgRun {
 // (conjugategrad.cpp:210-235, pressure.cpp:349-433).  With batchInit the
 // initial V-cycle and sigma = dot(z, r) are left to the caller's batched
 // launches (FIN_BETA0 afterwards).
+// Autotuner of the z-marching plane chunk.  The first time a 3D grid shape
+// is solved alone, the CG matvec and the level-0 multigrid down-sweep are
+// timed (HIP events, three launches after one warm-up) for chunks of 4..64
+// planes on the problem's own matrix code and pooled scratch grids under a
+// scratch CgState; the fastest chunk of each is kept for the shape (the
+// up-sweep shares the down-sweep's).  Single solves are latency-bound, so
+// the winners are short chunks for small grids; batched solves keep the
+// heuristic chunk (fewer halo planes per cell, throughput-bound).  The dot
+// partition of the z-marching kernels makes the result independent of the
+// chunk, so single and batched solves still agree bit for bit.
+static void autotune_zmarch(Scene* s, const uint8_t* code) {
+	const Dims& d = s->d;
+	if (!d.is3d) return;
+	if (zm_tuned(d).mv > 0) return;
+	float* a = s->getReal();
+	float* b = s->getReal();
+	float* c = s->getReal();
+	float* e = s->getReal();
+	CgState* st = nullptr;
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	ZmTune best;
+	auto cleanup = [&]() {
+		if (st) (void)hipFree(st);
+		if (e0) (void)hipEventDestroy(e0);
+		if (e1) (void)hipEventDestroy(e1);
+		s->putReal(a);
+		s->putReal(b);
+		s->putReal(c);
+		s->putReal(e);
+	};
+	try {
+		MFX_HIP(hipMalloc(&st, sizeof(CgState)));
+		MFX_HIP(hipEventCreate(&e0));
+		MFX_HIP(hipEventCreate(&e1));
+		hipLaunchKernelGGL(k_cg_state_init, dim3(1), dim3(64), 0, s->stream, st, 1e-3f, 1, 0, 0.0);
+		for (float* g : {a, b}) MFX_HIP(hipMemsetAsync(g, 0, sizeof(float) * (size_t)d.n, s->stream));
+		const int maxc = zm_chunk(d.nz);
+		auto timed = [&](auto launch) -> float {
+			float ms = 0.f;
+			launch();
+			MFX_HIP(hipEventRecord(e0, s->stream));
+			for (int r = 0; r < 3; r++) launch();
+			MFX_HIP(hipEventRecord(e1, s->stream));
+			MFX_HIP(hipEventSynchronize(e1));
+			MFX_HIP(hipEventElapsedTime(&ms, e0, e1));
+			return ms;
+		};
+		float tmv = 1e30f, tl0 = 1e30f;
+		for (int kc = kDotGroup; kc <= maxc; kc *= 2) {
+			zm_set_tuned(d, ZmTune{kc, kc});
+			int k1 = 0, k2 = 0;
+			const unsigned nbm = (unsigned)matvec_tiles(d, &k1, true);
+			const unsigned nbl = (unsigned)l0_tiles(d, &k2, true);
+			const MatvecP mp{d, k1, st, code, a, b, c, e, s->dpart};
+			const L0DownP lp{d, k2, st, code, a, c, e};
+			const float tm = timed([&]() { run_matvec_one(s->stream, nbm, mp); });
+			if (tm < tmv) {
+				tmv = tm;
+				best.mv = kc;
+			}
+			const float tl = timed([&]() { run_l0_down_one(s->stream, true, nbl, lp); });
+			if (tl < tl0) {
+				tl0 = tl;
+				best.l0 = kc;
+			}
+		}
+		zm_set_tuned(d, best);
+	} catch (...) {
+		zm_set_tuned(d, ZmTune{});
+		cleanup();
+		throw;
+	}
+	cleanup();
+}
+
 static void cg_setup(CgRun& R, Scene* s, Grid* flags, float* rhs, float* x, const mfx_pressure_opts& o,
                      bool batchInit = false) {
 	R.t0 = std::chrono::steady_clock::now();
@@ -914,7 +1012,6 @@ static void cg_setup(CgRun& R, Scene* s, Grid* flags, float* rhs, float* x, cons
 	R.t = tiles_for(d);
 	R.nbT = (unsigned)R.t.count();
 	R.nbF = flat_count(d.n);
-	R.nbM = (unsigned)matvec_tiles(d, &R.kchunk);
 	R.nbU = (unsigned)update_tiles(d, &R.per);
 	const unsigned nbMax = std::max(R.nbT, R.nbF);
 	s->ensurePartials(3 * (size_t)nbMax + 16, 2 * (size_t)nbMax + 16);
@@ -935,6 +1032,9 @@ static void cg_setup(CgRun& R, Scene* s, Grid* flags, float* rhs, float* x, cons
 		hipLaunchKernelGGL(k_build_code, dim3((unsigned)tc.count()), dim3(kBlock), 0, s->stream, d, tc, flags->f8,
 		                   R.code, R.fix);
 	}
+	autotune_zmarch(s, R.code);
+	R.nbM = (unsigned)matvec_tiles(d, &R.kchunk, true);
+	R.npM = (int)zm_dot_parts(d, kMvTY);
 
 	// preconditioner choice (pressure.cpp:404-433, conjugategrad.cpp:302-336)
 	int pc = o.preconditioner;
@@ -1000,7 +1100,7 @@ static void cg_iteration(CgRun& R) {
 		KtScope kt("cg_matvec", s->stream);
 		run_matvec_one(s->stream, R.nbM, R.mvp(odd));
 	}
-	run_finalize_one(s->stream, R.fin((int)R.nbM, FIN_ALPHA));
+	run_finalize_one(s->stream, R.fin(R.npM, FIN_ALPHA));
 	{
 		KtScope kt("cg_update", s->stream);
 		run_update_one(s->stream, R.nbU, R.upd(odd));
@@ -1258,19 +1358,22 @@ static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>&
 			CgRun& r = R[(size_t)i];
 			const Dims& d = r.s->d;
 			maxIt = std::max(maxIt, r.maxIter);
+			int kb = 1;
+			const long long nbMv = matvec_tiles(d, &kb, false);
 			for (int par = 0; par < 2; par++) {
 				B.mv[par].params[(size_t)i] = r.mvp(par == 1);
-				B.mv[par].add_tiles(i, r.nbM);
+				B.mv[par].params[(size_t)i].kchunk = kb;
+				B.mv[par].add_tiles(i, nbMv);
 				B.up[par].params[(size_t)i] = r.upd(par == 1);
 				B.up[par].add_tiles(i, r.nbU);
 			}
-			B.fa.params[(size_t)i] = r.fin((int)r.nbM, FIN_ALPHA);
+			B.fa.params[(size_t)i] = r.fin(r.npM, FIN_ALPHA);
 			B.fa.add_tiles(i, 1);
 			B.fr.params[(size_t)i] = r.fin((int)r.nbU, FIN_RES);
 			B.fr.add_tiles(i, 1);
 			if (r.pcNone) continue;
 			B.anyMg = true;
-			const MgHierarchy::Steps S = r.mg->steps(d, r.code, r.r, r.z, r.st, r.s->dpart);
+			const MgHierarchy::Steps S = r.mg->steps(d, r.code, r.r, r.z, r.st, r.s->dpart, false);
 			const int d3 = d.is3d ? 1 : 0;
 			int np;
 			if (S.nl == 1) {
@@ -1306,7 +1409,7 @@ static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>&
 				B.crs.add_tiles(i, coarsest_tiles(S.coarsest.nc));
 				B.l0up[d3].params[(size_t)i] = S.up;
 				B.l0up[d3].add_tiles(i, S.l0Tiles);
-				np = (int)S.l0Tiles;
+				np = (int)S.nPart;
 			}
 			B.fb.params[(size_t)i] = r.fin(np, FIN_BETA);
 			B.fb.add_tiles(i, 1);
@@ -1513,6 +1616,33 @@ static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>&
 }  // namespace mfx
 
 using namespace mfx;
+
+// Not part of the public interface (include/mfx.h): the plane chunks the
+// autotuner picked for a grid shape (0 = not tuned).  Used by tests and the
+// bench report.
+extern "C" void mfx_testing_zmarch_chunks(int nx, int ny, int nz, int* mv, int* l0) {
+	Dims d{};
+	d.nx = nx;
+	d.ny = ny;
+	d.nz = nz;
+	const ZmTune t = zm_tuned(d);
+	if (mv) *mv = t.mv;
+	if (l0) *l0 = t.l0;
+}
+// Testing hook: set the plane chunks that single solves of a shape use
+// (multiples of 4 up to 64; 0,0 = autotune again).  Returns 0, or -1 for a
+// chunk the z-marching kernels do not accept.
+extern "C" int mfx_testing_set_zmarch_chunks(int nx, int ny, int nz, int mv, int l0) {
+	auto ok = [](int c) { return c > 0 && c <= kZmMaxChunk && c % kDotGroup == 0; };
+	if (!(mv == 0 && l0 == 0) && !(ok(mv) && ok(l0))) return -1;
+	Dims d{};
+	d.nx = nx;
+	d.ny = ny;
+	d.nz = nz;
+	zm_set_tuned(d, ZmTune{mv, l0});
+	return 0;
+}
+
 struct mfx_grid : public Grid {};
 struct mfx_scene : public Scene {};
 static inline Grid* G(const mfx_grid* g) { return const_cast<Grid*>(static_cast<const Grid*>(g)); }

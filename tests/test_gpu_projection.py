@@ -95,3 +95,71 @@ def test_multigrid_converges_on_obstacle_rich_problems(manta, index):
                         preconditioner=manta.PcMGStatic)
     st = manta.lastSolveStats(p.s)
     assert st["converged"] and st["relResidual"] <= 1e-4 and st["iterations"] <= 12, st
+
+
+def _zm_hooks():
+    import ctypes as C
+    from mantaflow_amd._abi import lib
+    q = lib().mfx_testing_zmarch_chunks
+    q.restype, q.argtypes = None, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    st = lib().mfx_testing_set_zmarch_chunks
+    st.restype, st.argtypes = C.c_int, [C.c_int] * 5
+
+    def query(shape):
+        mv, l0 = C.c_int(), C.c_int()
+        q(*shape, C.byref(mv), C.byref(l0))
+        return mv.value, l0.value
+    return query, st
+
+
+def _solve_3d(manta, shape, flags, vel):
+    g = GpuScene(manta, shape)
+    f, v, p = g.flags(flags), g.mac(vel), g.real()
+    manta.solvePressure(flags=f, vel=v, pressure=p, cgAccuracy=1e-4, preconditioner=manta.PcMGStatic)
+    assert manta.lastSolveStats(g.s)["converged"]
+    return p.toNumpy().copy(), v.toNumpy().copy()
+
+
+def test_zmarch_autotuner_picks_chunks(manta, oracle):
+    """The first single solve of a 3D shape times the z-marching kernels for
+    plane chunks 4, 8, ... (up to the grid depth, at most 64) and keeps the
+    fastest; a second solve of the same shape (on another solver) reuses the
+    table and gives the same result."""
+    query, _ = _zm_hooks()
+    shape = (40, 36, 44)
+    rng = np.random.default_rng(11)
+    flags = obstacle_flags(oracle, shape, 0.1, rng)
+    vel = smooth_vel(rng, shape, passes=1)
+    oracle.set_wall_bcs(shape, flags, vel)
+    a = _solve_3d(manta, shape, flags, vel)
+    mv, l0 = query(shape)
+    assert mv in (4, 8, 16, 32) and l0 in (4, 8, 16, 32), (mv, l0)
+    b = _solve_3d(manta, shape, flags, vel)
+    assert query(shape) == (mv, l0)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+
+
+def test_zmarch_result_independent_of_chunk(manta, oracle):
+    """The z-marching kernels' dot products are summed per tile column and
+    group of 4 planes, so every plane chunk (matvec and level-0 multigrid
+    chosen independently) gives the same pressure and velocity bit for bit;
+    chunks the kernels do not accept are refused."""
+    query, setc = _zm_hooks()
+    shape = (36, 44, 52)
+    rng = np.random.default_rng(12)
+    flags = obstacle_flags(oracle, shape, 0.1, rng)
+    vel = smooth_vel(rng, shape, passes=1)
+    oracle.set_wall_bcs(shape, flags, vel)
+    try:
+        res = []
+        for mv, l0 in [(4, 4), (8, 16), (16, 8), (32, 64), (64, 32), (12, 20)]:
+            assert setc(*shape, mv, l0) == 0
+            res.append(((mv, l0), _solve_3d(manta, shape, flags, vel)))
+            assert query(shape) == (mv, l0)
+        for key, (p, v) in res[1:]:
+            np.testing.assert_array_equal(res[0][1][0], p, err_msg=str(key))
+            np.testing.assert_array_equal(res[0][1][1], v, err_msg=str(key))
+        assert setc(*shape, 6, 8) == -1 and setc(*shape, 8, 68) == -1 and setc(*shape, 0, 8) == -1
+    finally:
+        setc(*shape, 0, 0)
