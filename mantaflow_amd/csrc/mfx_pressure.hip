@@ -342,11 +342,17 @@ __device__ __forceinline__ void body_dot(const DotP& P, long long blk, long long
 // LDS, the planes below and above stay in registers, so every value is read
 // from HBM once.  Row arithmetic and term order follow apply_row; neighbours
 // outside the grid contribute 0 (their coefficient is 0 there).
+//
+// Loads run one plane ahead as raw values (z, p_old, code): p_new = z +
+// beta p_old is formed only when a plane enters LDS, so the loads of plane
+// k+2 are in flight during the whole step of plane k.
 template <int TY>
 __device__ __forceinline__ void body_matvec(const MatvecP& P, long long blk, long long slot) {
 	constexpr int TX = 64, RX = TX + 2, RY = TY + 2, NH = 2 * RX + 2 * (RY - 2);
-	__shared__ float ps[2][RX * RY];
-	__shared__ uint8_t cs[2][RX * RY];
+	// plane buffers; the 64 slots past the ring take the writes of lanes
+	// that have no ring cell
+	__shared__ float ps[2][RX * RY + 64];
+	__shared__ uint8_t cs[2][RX * RY + 64];
 	__shared__ double tab[kZmMaxGroups][TY];
 	const Dims& d = P.d;
 	const int kchunk = P.kchunk;
@@ -367,19 +373,7 @@ __device__ __forceinline__ void body_matvec(const MatvecP& P, long long blk, lon
 	const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
 	const int gx = ox + tx, gy = oy + ty;
 	const bool in = gx < d.nx && gy < d.ny;
-	auto pn = [&](long long m) -> float { return first ? z[m] : z[m] + beta * pold[m]; };
-	// own column: planes k-1, k, k+1 in registers, plane k+2 prefetched
-	const long long gcol = in ? (long long)gx + (long long)d.nx * gy : 0;
-	auto fcol = [&](int kk, float& v, uint8_t& c) {
-		v = 0.f;
-		c = 0;
-		if (in && kk >= 0 && kk < d.nz) {
-			const long long m = gcol + d.sz * kk;
-			v = pn(m);
-			c = code[m];
-		}
-	};
-	// one ring cell per thread (threads < NH), prefetched one plane ahead
+	// one ring cell per thread (threads < NH)
 	static_assert(NH <= 64 * TY, "one ring cell per thread");
 	const int h = threadIdx.x;
 	int hx = 0, hy = 0;
@@ -389,48 +383,70 @@ __device__ __forceinline__ void body_matvec(const MatvecP& P, long long blk, lon
 	else { hx = RX - 1; hy = 1 + h - 2 * RX - (RY - 2); }
 	const int hgx = ox - 1 + hx, hgy = oy - 1 + hy;
 	const bool hin = h < NH && hgx >= 0 && hgy >= 0 && hgx < d.nx && hgy < d.ny;
+	const long long gcol = in ? (long long)gx + (long long)d.nx * gy : 0;
 	const long long hcol = hin ? (long long)hgx + (long long)d.nx * hgy : 0;
-	auto fhalo = [&](int kk, float& v, uint8_t& c) {
-		v = 0.f;
-		c = 0;
-		if (hin && kk < d.nz) {
-			const long long m = hcol + d.sz * kk;
-			v = pn(m);
-			c = code[m];
-		}
+	// Raw values of one plane: own column (z, p_old, code) and ring cell
+	// (same).  The six loads are issued without branches (a plane outside
+	// the tile's planes k0-1..k1 or the grid reads a clamped plane, a lane
+	// outside the grid reads column 0; p_old is read also in the first
+	// iteration, where it is ignored), so the compiler can wait for one
+	// plane's loads while the next plane's stay in flight; validity is
+	// applied where the values are used.
+	struct Raw {
+		float z, po, hz, hpo;
+		uint8_t c, hc;
+		bool ok;
 	};
-	// Planes alternate between two LDS buffers: plane k+1 is written while
-	// plane k is read, so one barrier per plane orders both.  Own column:
-	// planes k-1, k, k+1 in registers, k+2 in flight; ring cell: plane k+1 in
-	// a register, k+2 in flight.
-	float pPrev, pCur, pNext, pN2, hv, hv1;
-	uint8_t cPrev, cCur, cNext, cN2, hc, hc1;
-	fcol(k0 - 1, pPrev, cPrev);
-	fcol(k0, pCur, cCur);
-	fcol(k0 + 1, pNext, cNext);
-	fhalo(k0, hv, hc);
-	fhalo(k0 + 1, hv1, hc1);
+	const int kLo = max(k0 - 1, 0), kHi = min(k1, d.nz - 1);
+	auto load = [&](int kk, Raw& r) {
+		const long long pl = d.sz * min(max(kk, kLo), kHi);
+		r.z = z[gcol + pl];
+		r.po = pold[gcol + pl];
+		r.c = code[gcol + pl];
+		r.hz = z[hcol + pl];
+		r.hpo = pold[hcol + pl];
+		r.hc = code[hcol + pl];
+		r.ok = kk >= 0 && kk < d.nz;
+	};
+	auto pn = [&](float zv, float pv) -> float { return first ? zv : zv + beta * pv; };
+	auto own = [&](const Raw& r) -> float { return in && r.ok ? pn(r.z, r.po) : 0.f; };
+	auto ownc = [&](const Raw& r) -> uint8_t { return in && r.ok ? r.c : (uint8_t)0; };
+	auto ring = [&](const Raw& r) -> float { return hin && r.ok ? pn(r.hz, r.hpo) : 0.f; };
+	auto ringc = [&](const Raw& r) -> uint8_t { return hin && r.ok ? r.hc : (uint8_t)0; };
 	const int L = (tx + 1) + RX * (ty + 1);
-	ps[0][L] = pCur;
-	cs[0][L] = cCur;
-	if (h < NH) {
-		ps[0][hx + RX * hy] = hv;
-		cs[0][hx + RX * hy] = hc;
+	const int HS = h < NH ? hx + RX * hy : RX * RY + tx;  // ring slot or spare
+	float pPrev, pCur, pNext;
+	uint8_t cPrev, cCur, cNext;
+	Raw R;
+	{
+		load(k0 - 1, R);
+		pPrev = own(R);
+		cPrev = ownc(R);
+		load(k0, R);
+		pCur = own(R);
+		cCur = ownc(R);
+		ps[0][L] = pCur;
+		cs[0][L] = cCur;
+		ps[0][HS] = ring(R);
+		cs[0][HS] = ringc(R);
 	}
+	load(k0 + 1, R);
 	__syncthreads();
 	double acc = 0.0;
-	int b = 0;  // buffer of plane k
+	int b = 0;  // LDS buffer of plane k
 	for (int k = k0; k < k1; k++, b ^= 1) {
-		fcol(k + 2, pN2, cN2);
-		if (k + 1 < k1) {
-			ps[b ^ 1][L] = pNext;
-			cs[b ^ 1][L] = cNext;
-			if (h < NH) {
-				ps[b ^ 1][hx + RX * hy] = hv1;
-				cs[b ^ 1][hx + RX * hy] = hc1;
-			}
-		}
-		fhalo(k + 2, hv1, hc1);
+		// R holds plane k+1: it goes to LDS buffer b^1 (whose plane k-1 was
+		// last read before the previous barrier; after the last plane the
+		// write is unused) without a branch, and only then is R reloaded
+		// with plane k+2, so R keeps its registers across iterations
+		pNext = own(R);
+		cNext = ownc(R);
+		ps[b ^ 1][L] = pNext;
+		cs[b ^ 1][L] = cNext;
+		ps[b ^ 1][HS] = ring(R);
+		cs[b ^ 1][HS] = ringc(R);
+		__builtin_amdgcn_sched_barrier(0);
+		load(k + 2, R);
 		if (in) {
 			const float* __restrict__ pp = ps[b];
 			const uint8_t* __restrict__ cp = cs[b];
@@ -442,7 +458,7 @@ __device__ __forceinline__ void body_matvec(const MatvecP& P, long long blk, lon
 			else
 				qv = pCur * cA0(cCur) + pp[L - 1] * cAi(cp[L - 1]) + pp[L + 1] * cAi(cCur) + pp[L - RX] * cAj(cp[L - RX]) +
 				     pp[L + RX] * cAj(cCur);
-			const long long m = d.idx(gx, gy, k);
+			const long long m = gcol + d.sz * k;
 			pnew[m] = pCur;
 			q[m] = qv;
 			acc += (double)(qv * pCur);
@@ -453,8 +469,6 @@ __device__ __forceinline__ void body_matvec(const MatvecP& P, long long blk, lon
 		cPrev = cCur;
 		pCur = pNext;
 		cCur = cNext;
-		pNext = pN2;
-		cNext = cN2;
 	}
 	(void)slot;
 	zm_dot_store<TY>(tab, d, zm_column(blk, d, TY), k0, k1, P.part);
