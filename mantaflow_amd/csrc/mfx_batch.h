@@ -26,8 +26,10 @@ struct WorkList {
 // The partial-sum slot is the logical tile in both wrappers, so a problem's
 // partials are reduced in the same order alone and in a batch (bit-identical
 // results).
-template <typename P, void (*BODY)(const P&, long long, long long), int NT>
-__global__ void __launch_bounds__(NT) k_one(P p) {
+// WPE: waves per SIMD the kernel must allow (limits its registers; 1 =
+// no limit beyond NT)
+template <typename P, void (*BODY)(const P&, long long, long long), int NT, int WPE = 1>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) k_one(P p) {
 	const long long t = xcd_tile(blockIdx.x, gridDim.x);
 	BODY(p, t, t);
 }
@@ -48,8 +50,9 @@ __device__ inline int worklist_prob(const WorkList& wl, unsigned lt) {
 	return cnt - 1;
 }
 
-template <typename P, void (*BODY)(const P&, long long, long long), int NT>
-__global__ void __launch_bounds__(NT) k_many(WorkList wl, const P* __restrict__ params) {
+template <typename P, void (*BODY)(const P&, long long, long long), int NT, int WPE = 1>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) k_many(WorkList wl,
+                                                                                       const P* __restrict__ params) {
 	const unsigned lt = xcd_tile(blockIdx.x, gridDim.x);
 	const int prob = worklist_prob(wl, lt);
 	const long long t = (long long)lt - wl.start[prob];
@@ -57,13 +60,13 @@ __global__ void __launch_bounds__(NT) k_many(WorkList wl, const P* __restrict__ 
 }
 
 // launch helpers (instantiated in the file that defines BODY)
-template <typename P, void (*BODY)(const P&, long long, long long), int NT>
+template <typename P, void (*BODY)(const P&, long long, long long), int NT, int WPE = 1>
 inline void launch_one(hipStream_t s, long long nb, const P& p) {
-	if (nb > 0) hipLaunchKernelGGL((k_one<P, BODY, NT>), dim3((unsigned)nb), dim3(NT), 0, s, p);
+	if (nb > 0) hipLaunchKernelGGL((k_one<P, BODY, NT, WPE>), dim3((unsigned)nb), dim3(NT), 0, s, p);
 }
-template <typename P, void (*BODY)(const P&, long long, long long), int NT>
+template <typename P, void (*BODY)(const P&, long long, long long), int NT, int WPE = 1>
 inline void launch_many(hipStream_t s, long long nb, WorkList items, const P* p) {
-	if (nb > 0) hipLaunchKernelGGL((k_many<P, BODY, NT>), dim3((unsigned)nb), dim3(NT), 0, s, items, p);
+	if (nb > 0) hipLaunchKernelGGL((k_many<P, BODY, NT, WPE>), dim3((unsigned)nb), dim3(NT), 0, s, items, p);
 }
 
 // ---------------------------------------------------------------------------

@@ -23,6 +23,7 @@
 //  * level 0 is matrix-free: its stencil comes from the per-cell matrix code.
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "mfx_batch.h"
 #include "mfx_mg.h"
@@ -636,18 +637,55 @@ __device__ inline void zm_origin(long long blk, const Dims& d, int TY, int kchun
 	k1 = min(d.nz, k0 + kchunk);
 }
 
-// l0_nb_sum with the plane below (cM, xM) and above (xU) in separate slots
-__device__ inline float zm_nb_sum(const uint8_t* cP, const float* xP, const uint8_t* cM, const float* xM,
+// x if bit B of c is set, else +0.0: the bit pattern AND the bit's sign
+// extension (v_bfe_i32 + v_and_b32 instead of a compare and a select; the
+// extract is written as an instruction because the compiler turns a
+// sign-extended bit of a byte back into a compare and a select)
+template <int B>
+__device__ inline float zm_keep(uint8_t c, float x) {
+	int m;
+	asm("v_bfe_i32 %0, %1, %2, 1" : "=v"(m) : "v"((int)c), "i"(B));
+	return __int_as_float(__float_as_int(x) & m);
+}
+// l0_nb_sum with the plane below (cM, xM) and above (xU) in separate slots;
+// cc = cP[q].  Every neighbour is read and added, masked to +0.0 where its
+// coupling flag is clear.  The running sum starts at +0.0 and a round-to-
+// nearest sum is -0.0 only when both addends are, so it never is; adding
+// +0.0 to it is therefore exact and this equals the conditional adds of
+// l0_nb_sum, in the same order, without a branch per neighbour.
+__device__ inline float zm_nb_sum(uint8_t cc, const uint8_t* cP, const float* xP, const uint8_t* cM, const float* xM,
                                   const float* xU, int q, int RX) {
-	const uint8_t cc = cP[q];
+	const uint8_t cw = cP[q - 1], cs = cP[q - RX], cm = cM[q];
+	const float w = xP[q - 1], e = xP[q + 1], so = xP[q - RX], no = xP[q + RX], dn = xM[q], up = xU[q];
 	float sum = 0.f;
-	if (cP[q - 1] & 8) sum += xP[q - 1];
-	if (cc & 8) sum += xP[q + 1];
-	if (cP[q - RX] & 16) sum += xP[q - RX];
-	if (cc & 16) sum += xP[q + RX];
-	if (cM[q] & 32) sum += xM[q];
-	if (cc & 32) sum += xU[q];
+	sum += zm_keep<3>(cw, w);
+	sum += zm_keep<3>(cc, e);
+	sum += zm_keep<4>(cs, so);
+	sum += zm_keep<4>(cc, no);
+	sum += zm_keep<5>(cm, dn);
+	sum += zm_keep<5>(cc, up);
 	return sum;
+}
+
+// (diag, 1/diag) of the level-0 row codes in LDS, index (c & 7) + 8 for
+// the 1e-6-scaled trivial rows (bit 7); filled by threads 0..15 before the
+// kernel's first barrier
+__device__ inline int l0_tab_index(uint8_t c) { return (c & 7) | ((c >> 4) & 8); }
+__device__ inline void l0_tab_fill(float2* t) {
+	if (threadIdx.x < 16) {
+		const int a = threadIdx.x & 7;
+		const float dg = (threadIdx.x & 8) ? (float)a * 1e-6f : (float)a;
+		t[threadIdx.x] = make_float2(dg, a ? 1.0f / dg : 0.f);
+	}
+}
+// v / diag(c) as q0 = v (1/d) plus one residual correction (Markstein):
+// checked exhaustively over all binary32 v, it equals the IEEE quotient for
+// every finite v when d = 1..5 and for every normal quotient when d = 6; on
+// the 1e-6-scaled trivial rows it can differ from it in the last bit
+__device__ inline float l0_div_t(float v, uint8_t c, const float2* t) {
+	const float2 e = t[l0_tab_index(c)];
+	const float q0 = v * e.y;
+	return fmaf(fmaf(-q0, e.x, v), e.y, q0);
 }
 
 
@@ -655,17 +693,17 @@ __device__ inline float zm_nb_sum(const uint8_t* cP, const float* xP, const uint
 template <typename T>
 struct ZmCells {
 	static_assert(T::RP <= 2 * T::NT, "two region cells per thread");
-	long long g[2];  // in-plane offset i + nx*j
+	unsigned g[2];   // in-plane offset i + nx*j (nx*ny < 2^31 is checked on the host)
 	bool ok[2];      // inside the region and the grid
-	int par[2];      // (i + j) & 1
+	bool par[2];     // (i + j) & 1
 	__device__ ZmCells(const Dims& d, int rx0, int ry0) {
 #pragma unroll
 		for (int u = 0; u < 2; u++) {
 			const int q = threadIdx.x + u * T::NT;
 			const int gx = rx0 + q % T::RX, gy = ry0 + q / T::RX;
 			ok[u] = q < T::RP && gx >= 0 && gy >= 0 && gx < d.nx && gy < d.ny;
-			g[u] = ok[u] ? (long long)gx + (long long)d.nx * gy : 0;
-			par[u] = (gx + gy) & 1;
+			g[u] = ok[u] ? (unsigned)gx + (unsigned)d.nx * (unsigned)gy : 0u;
+			par[u] = ((gx + gy) & 1) != 0;
 		}
 	}
 };
@@ -681,12 +719,12 @@ struct ZmPos {
 	int qa[2];
 	// second-colour stage: region without its outer ring (at most two cells)
 	int qb[2];
-	int parb[2];
+	bool parb[2];
 	bool inb[2];
 	// output stage: the thread's tile cell
 	int qc;
-	long long gc;
-	int parc;
+	unsigned gc;
+	bool parc;
 	bool inc;
 	__device__ ZmPos(const Dims& d, int ox, int oy) {
 		const int rx0 = ox - T::H, ry0 = oy - T::H;
@@ -699,14 +737,14 @@ struct ZmPos {
 			const int lx = 1 + t % IX, ly = 1 + t / IX;
 			inb[u] = t < IN;
 			qb[u] = lx + T::RX * ly;
-			parb[u] = (rx0 + lx + ry0 + ly) & 1;
+			parb[u] = ((rx0 + lx + ry0 + ly) & 1) != 0;
 		}
 		const int lx = T::H + (threadIdx.x & 63), ly = T::H + (threadIdx.x >> 6);
 		const int gx = rx0 + lx, gy = ry0 + ly;
 		qc = lx + T::RX * ly;
 		inc = gx < d.nx && gy < d.ny;
-		gc = inc ? (long long)gx + (long long)d.nx * gy : 0;
-		parc = (gx + gy) & 1;
+		gc = inc ? (unsigned)gx + (unsigned)d.nx * (unsigned)gy : 0u;
+		parc = ((gx + gy) & 1) != 0;
 	}
 };
 
@@ -714,6 +752,10 @@ __device__ inline int ring_next(int s) { return s == ZRING - 1 ? 0 : s + 1; }
 __device__ inline int ring_back(int s, int k) { return (s + ZRING - k) % ZRING; }
 
 // Down leg (same sweeps as body_l0_down_brick): red from x = 0, black, residual.
+// The stages are branch-free where lanes would diverge: plane loads read a
+// clamped plane and are zeroed where used, colour and activity select
+// between the new and the old value, and the division goes through the
+// (diag, 1/diag) table.
 template <int TY>
 __device__ __forceinline__ void body_l0_down_zm(const L0DownP& P, long long blk, long long) {
 	const Dims& d = P.d;
@@ -727,69 +769,97 @@ __device__ __forceinline__ void body_l0_down_zm(const L0DownP& P, long long blk,
 	__shared__ float xs[T::NS][T::RP];
 	__shared__ float bs[T::NS][T::RP];
 	__shared__ uint8_t cs[T::NS][T::RP];
+	__shared__ float2 dt[16];
 	if (st->done) return;
+	l0_tab_fill(dt);
 	int ox, oy, k0, k1;
 	zm_origin(blk, d, TY, kchunk, ox, oy, k0, k1);
 	const int rx0 = ox - T::H, ry0 = oy - T::H;
 	ZmCells<T> cells(d, rx0, ry0);
 	const ZmPos<T> pos(d, ox, oy);
+	// plane p's inputs are fetched into registers one plane ahead, so the
+	// loads are in flight during the barriers and the sweeps of plane p-1;
+	// a plane outside k0-2..k1+1 or the grid reads a clamped plane and pv
+	// marks whether the values count
+	const int pLo = max(k0 - 2, 0), pHi = min(k1 + 1, d.nz - 1);
 	uint8_t pc[2];
 	float pr[2];
-	// plane p's inputs are fetched into registers one plane ahead, so the
-	// loads are in flight during the barriers and the sweeps of plane p-1
+	bool pv = false;
 	auto fetch = [&](int p) {
+		const long long pl = d.sz * min(max(p, pLo), pHi);
 #pragma unroll
 		for (int u = 0; u < 2; u++) {
-			pc[u] = 0;
-			pr[u] = 0.f;
-			if (cells.ok[u] && p >= 0 && p < d.nz) {
-				const long long g = cells.g[u] + d.sz * p;
-				pc[u] = code[g];
-				pr[u] = rin[g];
-			}
+			pc[u] = code[cells.g[u] + pl];
+			pr[u] = rin[cells.g[u] + pl];
 		}
+		pv = p >= 0 && p < d.nz;
 	};
 	fetch(k0 - 2);
-	int sp = 0;  // ring slot of plane p
-	for (int p = k0 - 2; p <= k1 + 3; p++, sp = ring_next(sp)) {
-		if (p <= k1 + 1) {
+	__syncthreads();
+	// One plane step; J = ring slot of plane p = (p - k0 + 2) mod ZRING, a
+	// compile-time constant (the loop is unrolled by ZRING), so every slot
+	// offset is an immediate of the LDS instructions.  k0 is even, so the
+	// parity of p is that of J.
+	auto step = [&](auto jc, int p) {
+		constexpr int J = decltype(jc)::value;
+		constexpr int sA = J, sB0 = (J + 4) % ZRING, sBm = (J + 3) % ZRING, sBu = (J + 5) % ZRING;
+		constexpr int sC0 = (J + 2) % ZRING, sCm = (J + 1) % ZRING, sCu = (J + 3) % ZRING;
+		constexpr int par = J & 1;
+		// plane p into slot sA (after p = k1+1 the slot is not read again)
+		{
+			const bool pvp = pv;
 #pragma unroll
 			for (int u = 0; u < 2; u++) {
 				const int q = pos.qa[u];
 				if (q >= T::RP) continue;
-				const uint8_t cc = pc[u];
-				const float b = pr[u] * l0_scale(cc);
-				cs[sp][q] = cc;
-				bs[sp][q] = b;
-				xs[sp][q] = (l0_active(cc) && ((cells.par[u] + p) & 1) == 0) ? l0_div(b, cc) : 0.f;
+				const bool ok = cells.ok[u] && pvp;
+				const uint8_t cc = ok ? pc[u] : (uint8_t)0;
+				const float b = ok ? pr[u] * l0_scale(cc) : 0.f;
+				cs[sA][q] = cc;
+				bs[sA][q] = b;
+				const float x = l0_div_t(b, cc, dt);
+				xs[sA][q] = (l0_active(cc) && cells.par[u] == (par != 0)) ? x : 0.f;
 			}
-			if (p <= k1) fetch(p + 1);
+			fetch(p + 1);
 		}
 		const int pb = p - 2;
 		if (pb >= k0 - 1 && pb <= k1 && pb >= 0 && pb < d.nz) {
-			const int s0 = ring_back(sp, 2), sm = ring_back(sp, 3), su = ring_back(sp, 1);
 #pragma unroll
 			for (int u = 0; u < 2; u++) {
 				if (!pos.inb[u]) continue;
 				const int q = pos.qb[u];
-				const uint8_t cc = cs[s0][q];
-				if (((pos.parb[u] + pb) & 1) == 1 && l0_active(cc))
-					xs[s0][q] = l0_div(bs[s0][q] + zm_nb_sum(cs[s0], xs[s0], cs[sm], xs[sm], xs[su], q, T::RX), cc);
+				const uint8_t cc = cs[sB0][q];
+				const float xo = xs[sB0][q];
+				const float v =
+				    l0_div_t(bs[sB0][q] + zm_nb_sum(cc, cs[sB0], xs[sB0], cs[sBm], xs[sBm], xs[sBu], q, T::RX), cc, dt);
+				// the other colour is written back unchanged (its readers
+				// see the same value either way)
+				xs[sB0][q] = (pos.parb[u] != (par != 0) && l0_active(cc)) ? v : xo;
 			}
 		}
 		const int pz = p - 4;
-		if (pz >= k0 && pz < k1 && pos.inc) {
-			const int s0 = ring_back(sp, 4), sm = ring_back(sp, 5), su = ring_back(sp, 3);
+		if (pz >= k0 && pz < k1) {
 			const int q = pos.qc;
-			const long long g = pos.gc + d.sz * pz;
-			const uint8_t cc = cs[s0][q];
-			const float x = xs[s0][q];
-			xout[g] = x;
-			r0[g] = l0_active(cc)
-			            ? (bs[s0][q] + zm_nb_sum(cs[s0], xs[s0], cs[sm], xs[sm], xs[su], q, T::RX)) - l0_diag(cc) * x
-			            : 0.f;
+			const uint8_t cc = cs[sC0][q];
+			const float x = xs[sC0][q];
+			const float rr = (bs[sC0][q] + zm_nb_sum(cc, cs[sC0], xs[sC0], cs[sCm], xs[sCm], xs[sCu], q, T::RX)) -
+			                 dt[l0_tab_index(cc)].x * x;
+			if (pos.inc) {
+				const long long g = pos.gc + d.sz * pz;
+				xout[g] = x;
+				r0[g] = l0_active(cc) ? rr : 0.f;
+			}
 		}
 		__syncthreads();
+	};
+	static_assert(ZRING == 6, "the plane loop is unrolled by the ring size");
+	for (int p = k0 - 2; p <= k1 + 3; p += ZRING) {
+		step(std::integral_constant<int, 0>{}, p);
+		if (p + 1 <= k1 + 3) step(std::integral_constant<int, 1>{}, p + 1);
+		if (p + 2 <= k1 + 3) step(std::integral_constant<int, 2>{}, p + 2);
+		if (p + 3 <= k1 + 3) step(std::integral_constant<int, 3>{}, p + 3);
+		if (p + 4 <= k1 + 3) step(std::integral_constant<int, 4>{}, p + 4);
+		if (p + 5 <= k1 + 3) step(std::integral_constant<int, 5>{}, p + 5);
 	}
 }
 
@@ -797,7 +867,9 @@ __device__ __forceinline__ void body_l0_down_zm(const L0DownP& P, long long blk,
 // write z and the partial sums of fl(z*r).  Coarse planes are staged in a
 // 2-slot ring: fine plane p needs coarse planes p>>1 and (p+1)>>1.  The x/y
 // parents of the thread's region cells and their weights are fixed for the
-// whole column, only the z parents change with the plane.
+// whole column, only the z parents change with the plane.  Branch-free in
+// the same way as the down leg; the prolongation reads all four x/y parents
+// of a coarse plane and adds each with a select.
 template <int TY>
 __device__ __forceinline__ void body_l0_up_zm(const L0UpP& P, long long blk, long long slot) {
 	const Dims& d = P.d;
@@ -817,18 +889,22 @@ __device__ __forceinline__ void body_l0_up_zm(const L0UpP& P, long long blk, lon
 	__shared__ uint8_t cs[T::NS][T::RP];
 	__shared__ float xcs[2][CRP];
 	__shared__ double tab[kZmMaxGroups][TY];
+	__shared__ float2 dt[16];
 	if (st->done) return;
+	l0_tab_fill(dt);
 	int ox, oy, k0, k1;
 	zm_origin(blk, d, TY, kchunk, ox, oy, k0, k1);
 	const int rx0 = ox - T::H, ry0 = oy - T::H;
 	const int cx0 = rx0 >> 1, cy0 = ry0 >> 1;  // floor division
 	static_assert(CRP <= T::NT, "one coarse value per thread");
-	// this thread's vertex of the coarse-plane region
+	// this thread's vertex of the coarse-plane region; coarse planes outside
+	// the level read a clamped plane and are zeroed where stored
 	const int cq = threadIdx.x;
 	const int cpx = cx0 + cq % CRX, cpy = cy0 + cq / CRX;
 	const bool cok = cq < CRP && cpx >= 0 && cpy >= 0 && cpx < c1.nx && cpy < c1.ny;
-	const long long cg = cok ? (long long)cpx + c1.sy * cpy : 0;
-	auto cfetch = [&](int Pz) -> float { return (cok && Pz >= 0 && Pz < c1.nz) ? xc[cg + c1.sz * Pz] : 0.f; };
+	const unsigned cg = cok ? (unsigned)cpx + (unsigned)c1.sy * (unsigned)cpy : 0u;  // coarse n < 2^31
+	auto cfetch = [&](int Pz) -> float { return xc[cg + (unsigned)c1.sz * (unsigned)min(max(Pz, 0), c1.nz - 1)]; };
+	auto cvalid = [&](int Pz) -> bool { return cok && Pz >= 0 && Pz < c1.nz; };
 	ZmCells<T> cells(d, rx0, ry0);
 	const ZmPos<T> pos(d, ox, oy);
 	// prolongation offsets (coarse-plane region) and x/y weights per region cell
@@ -844,101 +920,116 @@ __device__ __forceinline__ void body_l0_up_zm(const L0UpP& P, long long blk, lon
 		po[u] = ((gx >> 1) - cx0) + CRX * ((gy >> 1) - cy0);
 		pw[u] = (pxo[u] ? 0.5f : 1.0f) * (pyo[u] ? 0.5f : 1.0f);
 	}
+	const int pLo = max(k0 - 2, 0), pHi = min(k1 + 1, d.nz - 1);
 	uint8_t pc[2];
 	float pr[2], px[2];
+	bool pv = false;
 	auto fetch = [&](int p) {
+		const long long pl = d.sz * min(max(p, pLo), pHi);
 #pragma unroll
 		for (int u = 0; u < 2; u++) {
-			pc[u] = 0;
-			pr[u] = 0.f;
-			px[u] = 0.f;
-			if (cells.ok[u] && p >= 0 && p < d.nz) {
-				const long long g = cells.g[u] + d.sz * p;
-				pc[u] = code[g];
-				pr[u] = rin[g];
-				px[u] = xpre[g];
-			}
+			pc[u] = code[cells.g[u] + pl];
+			pr[u] = rin[cells.g[u] + pl];
+			px[u] = xpre[cells.g[u] + pl];
 		}
+		pv = p >= 0 && p < d.nz;
 	};
 	// coarse plane (k0-2)/2 goes to LDS now; later planes are fetched into a
 	// register two fine planes before they are needed (k0 is even)
-	if (cq < CRP) xcs[((k0 - 2) >> 1) & 1][cq] = cfetch((k0 - 2) >> 1);
+	if (cq < CRP) xcs[((k0 - 2) >> 1) & 1][cq] = cvalid((k0 - 2) >> 1) ? cfetch((k0 - 2) >> 1) : 0.f;
 	float creg = cfetch(k0 >> 1);
+	bool cregv = cvalid(k0 >> 1);
 	fetch(k0 - 2);
 	__syncthreads();
 	double acc = 0.0;
-	int sp = 0;  // ring slot of plane p
-	for (int p = k0 - 2; p <= k1 + 3; p++, sp = ring_next(sp)) {
-		if (p <= k1 + 1) {
+	// One plane step, J = ring slot of plane p (compile-time, as in the down
+	// leg; the parity of p is that of J)
+	auto step = [&](auto jc, int p) {
+		constexpr int J = decltype(jc)::value;
+		constexpr int sA = J, sB0 = (J + 4) % ZRING, sBm = (J + 3) % ZRING, sBu = (J + 5) % ZRING;
+		constexpr int sC0 = (J + 2) % ZRING, sCm = (J + 1) % ZRING, sCu = (J + 3) % ZRING;
+		constexpr int par = J & 1;
+		constexpr bool zo = (J & 1) != 0;
+		// plane p into slot sA (after p = k1+1 the slot is not read again)
+		{
 			const float* c0 = xcs[(p >> 1) & 1];
 			const float* c1p = xcs[((p >> 1) + 1) & 1];
-			const bool zo = p & 1;
+			const bool pvp = pv;
 #pragma unroll
 			for (int u = 0; u < 2; u++) {
 				const int q = pos.qa[u];
 				if (q >= T::RP) continue;
-				const uint8_t cc = pc[u];
-				float x = px[u];
-				if (l0_active(cc)) {
-					// parents in the reference's loop order: z, then y, then x
-					const int o = po[u];
-					float sum = c0[o];
-					if (pxo[u]) sum += c0[o + 1];
-					if (pyo[u]) {
-						sum += c0[o + CRX];
-						if (pxo[u]) sum += c0[o + CRX + 1];
-					}
-					if (zo) {
-						sum += c1p[o];
-						if (pxo[u]) sum += c1p[o + 1];
-						if (pyo[u]) {
-							sum += c1p[o + CRX];
-							if (pxo[u]) sum += c1p[o + CRX + 1];
-						}
-					}
-					x += (zo ? 0.5f * pw[u] : pw[u]) * sum;
+				const bool ok = cells.ok[u] && pvp;
+				const uint8_t cc = ok ? pc[u] : (uint8_t)0;
+				const float rr = ok ? pr[u] : 0.f;
+				float x = ok ? px[u] : 0.f;
+				// parents in the reference's loop order: z, then y, then x
+				const int o = po[u];
+				const bool bx = pxo[u], by = pyo[u], bxy = pxo[u] && pyo[u];
+				float sum = c0[o];
+				const float a01 = c0[o + 1], a10 = c0[o + CRX], a11 = c0[o + CRX + 1];
+				sum = bx ? sum + a01 : sum;
+				sum = by ? sum + a10 : sum;
+				sum = bxy ? sum + a11 : sum;
+				if (zo) {
+					const float b00 = c1p[o], b01 = c1p[o + 1], b10 = c1p[o + CRX], b11 = c1p[o + CRX + 1];
+					sum += b00;
+					sum = bx ? sum + b01 : sum;
+					sum = by ? sum + b10 : sum;
+					sum = bxy ? sum + b11 : sum;
 				}
-				cs[sp][q] = cc;
-				rs[sp][q] = pr[u];
-				xs[sp][q] = x;
+				const float xn = x + (zo ? 0.5f * pw[u] : pw[u]) * sum;
+				x = l0_active(cc) ? xn : x;
+				cs[sA][q] = cc;
+				rs[sA][q] = rr;
+				xs[sA][q] = x;
 			}
-			if (!(p & 1)) {
+			if (!zo) {
 				// coarse plane p/2+1 is first needed by plane p+1
-				if (cq < CRP) xcs[((p >> 1) + 1) & 1][cq] = creg;
+				if (cq < CRP) xcs[((p >> 1) + 1) & 1][cq] = cregv ? creg : 0.f;
 				creg = cfetch((p >> 1) + 2);
+				cregv = cvalid((p >> 1) + 2);
 			}
-			if (p <= k1) fetch(p + 1);
+			fetch(p + 1);
 		}
 		const int pb = p - 2;
 		if (pb >= k0 - 1 && pb <= k1 && pb >= 0 && pb < d.nz) {
-			const int s0 = ring_back(sp, 2), sm = ring_back(sp, 3), su = ring_back(sp, 1);
 #pragma unroll
 			for (int u = 0; u < 2; u++) {
 				if (!pos.inb[u]) continue;
 				const int q = pos.qb[u];
-				const uint8_t cc = cs[s0][q];
-				if (((pos.parb[u] + pb) & 1) == 1 && l0_active(cc))
-					xs[s0][q] = l0_div(rs[s0][q] * l0_scale(cc) + zm_nb_sum(cs[s0], xs[s0], cs[sm], xs[sm], xs[su], q, T::RX),
-					                   cc);
+				const uint8_t cc = cs[sB0][q];
+				const float xo = xs[sB0][q];
+				const float v = l0_div_t(
+				    rs[sB0][q] * l0_scale(cc) + zm_nb_sum(cc, cs[sB0], xs[sB0], cs[sBm], xs[sBm], xs[sBu], q, T::RX), cc,
+				    dt);
+				// the other colour is written back unchanged
+				xs[sB0][q] = (pos.parb[u] != (par != 0) && l0_active(cc)) ? v : xo;
 			}
 		}
 		const int pz = p - 4;
 		if (pz >= k0 && pz < k1) {
-			if (pos.inc) {
-				const int s0 = ring_back(sp, 4), sm = ring_back(sp, 5), su = ring_back(sp, 3);
-				const int q = pos.qc;
-				const long long g = pos.gc + d.sz * pz;
-				const uint8_t cc = cs[s0][q];
-				float x = xs[s0][q];
-				if (((pos.parc + pz) & 1) == 0 && l0_active(cc))
-					x = l0_div(rs[s0][q] * l0_scale(cc) + zm_nb_sum(cs[s0], xs[s0], cs[sm], xs[sm], xs[su], q, T::RX),
-					           cc);
-				z[g] = x;
-				acc += (double)(x * rs[s0][q]);
-			}
+			const int q = pos.qc;
+			const uint8_t cc = cs[sC0][q];
+			const float xo = xs[sC0][q];
+			const float rq = rs[sC0][q];
+			const float v =
+			    l0_div_t(rq * l0_scale(cc) + zm_nb_sum(cc, cs[sC0], xs[sC0], cs[sCm], xs[sCm], xs[sCu], q, T::RX), cc, dt);
+			const float x = (pos.parc == (par != 0) && l0_active(cc)) ? v : xo;
+			if (pos.inc) z[pos.gc + d.sz * pz] = x;
+			const double prod = (double)(x * rq);
+			acc = pos.inc ? acc + prod : acc;
 			if ((pz + 1) % kDotGroup == 0 || pz + 1 == k1) zm_dot_flush<TY>(acc, tab, pz, k0);
 		}
 		__syncthreads();
+	};
+	for (int p = k0 - 2; p <= k1 + 3; p += ZRING) {
+		step(std::integral_constant<int, 0>{}, p);
+		if (p + 1 <= k1 + 3) step(std::integral_constant<int, 1>{}, p + 1);
+		if (p + 2 <= k1 + 3) step(std::integral_constant<int, 2>{}, p + 2);
+		if (p + 3 <= k1 + 3) step(std::integral_constant<int, 3>{}, p + 3);
+		if (p + 4 <= k1 + 3) step(std::integral_constant<int, 4>{}, p + 4);
+		if (p + 5 <= k1 + 3) step(std::integral_constant<int, 5>{}, p + 5);
 	}
 	(void)slot;
 	zm_dot_store<TY>(tab, d, zm_column(blk, d, TY), k0, k1, part);
@@ -1156,6 +1247,8 @@ void MgHierarchy::build(Scene* s, const uint8_t* code, int precision) {
 	prec = precision;
 	const Dims& d = s->d;
 	is3d = d.is3d != 0;
+	// the z-marching level-0 kernels keep in-plane offsets in 32 bits
+	MFX_CHECK((long long)d.nx * d.ny < (1ll << 31), "multigrid: a grid plane has too many cells");
 	S = is3d ? 14 : 5;
 	L.clear();
 	// level sizes (multigrid.cpp:242-262)
@@ -1455,15 +1548,15 @@ long long lcblk_tiles(const LvDev& c, bool is3d) {
 	return ((c.nx + bx - 1) / bx) * ((c.ny + by - 1) / by) * ((c.nz + bz - 1) / bz);
 }
 void run_l0_down_one(hipStream_t s, bool is3d, unsigned nb, const L0DownP& p) {
-	if (is3d) launch_one<L0DownP, body_l0_down_zm<kZmTY>, 64 * kZmTY>(s, nb, p);
+	if (is3d) launch_one<L0DownP, body_l0_down_zm<kZmTY>, 64 * kZmTY, 8>(s, nb, p);
 	else launch_one<L0DownP, body_l0_down_brick<64, 16, 1, false>, 1024>(s, nb, p);
 }
 void run_l0_down_many(hipStream_t s, bool is3d, unsigned nb, WorkList items, const L0DownP* p) {
-	if (is3d) launch_many<L0DownP, body_l0_down_zm<kZmTY>, 64 * kZmTY>(s, nb, items, p);
+	if (is3d) launch_many<L0DownP, body_l0_down_zm<kZmTY>, 64 * kZmTY, 8>(s, nb, items, p);
 	else launch_many<L0DownP, body_l0_down_brick<64, 16, 1, false>, 1024>(s, nb, items, p);
 }
 void run_l0_up_many(hipStream_t s, bool is3d, unsigned nb, WorkList items, const L0UpP* p) {
-	if (is3d) launch_many<L0UpP, body_l0_up_zm<kZmTY>, 64 * kZmTY>(s, nb, items, p);
+	if (is3d) launch_many<L0UpP, body_l0_up_zm<kZmTY>, 64 * kZmTY, 8>(s, nb, items, p);
 	else launch_many<L0UpP, body_l0_up_brick<64, 16, 1, false>, 1024>(s, nb, items, p);
 }
 void run_restrict_many(hipStream_t s, bool is3d, unsigned nb, WorkList items, const RestrictP* p) {
@@ -1548,7 +1641,7 @@ int MgHierarchy::vcycle(Scene* s, const uint8_t* code, const float* r, float* z,
 	}
 	{
 		KtScope kt("mg_l0_down", q);
-		if (is3d) launch_one<L0DownP, body_l0_down_zm<kZmTY>, 64 * kZmTY>(q, S_.l0Tiles, S_.down);
+		if (is3d) launch_one<L0DownP, body_l0_down_zm<kZmTY>, 64 * kZmTY, 8>(q, S_.l0Tiles, S_.down);
 		else launch_one<L0DownP, body_l0_down_brick<64, 16, 1, false>, 1024>(q, S_.l0Tiles, S_.down);
 	}
 	const int nlv = (int)S_.lv.size();
@@ -1581,7 +1674,7 @@ int MgHierarchy::vcycle(Scene* s, const uint8_t* code, const float* r, float* z,
 	}
 	{
 		KtScope kt("mg_l0_up", q);
-		if (is3d) launch_one<L0UpP, body_l0_up_zm<kZmTY>, 64 * kZmTY>(q, S_.l0Tiles, S_.up);
+		if (is3d) launch_one<L0UpP, body_l0_up_zm<kZmTY>, 64 * kZmTY, 8>(q, S_.l0Tiles, S_.up);
 		else launch_one<L0UpP, body_l0_up_brick<64, 16, 1, false>, 1024>(q, S_.l0Tiles, S_.up);
 	}
 	return (int)S_.nPart;
