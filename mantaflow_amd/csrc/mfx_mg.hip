@@ -766,10 +766,21 @@ __device__ __forceinline__ void body_l0_down_zm(const L0DownP& P, long long blk,
 	float* __restrict__ xout = P.xout;
 	float* __restrict__ r0 = P.r0;
 	using T = ZTile<TY>;
-	__shared__ float xs[T::NS][T::RP];
-	__shared__ float bs[T::NS][T::RP];
-	__shared__ uint8_t cs[T::NS][T::RP];
-	__shared__ float2 dt[16];
+	// one LDS block in a fixed order: codes first (byte offsets from the
+	// cell index), then the float rings contiguous, so every ring access is
+	// an immediate offset (< 64 KiB) from one of two per-cell bases
+	struct Lds {
+		uint8_t cs[T::NS][T::RP];
+		float xs[T::NS][T::RP];
+		float bs[T::NS][T::RP];
+		float2 dt[16];
+	};
+	static_assert((T::NS * T::RP) % 16 == 0 && 2 * sizeof(float) * T::NS * T::RP < 65536, "ring layout");
+	__shared__ Lds lds;
+	auto& cs = lds.cs;
+	auto& xs = lds.xs;
+	auto& bs = lds.bs;
+	float2* dt = lds.dt;
 	if (st->done) return;
 	l0_tab_fill(dt);
 	int ox, oy, k0, k1;
@@ -884,12 +895,23 @@ __device__ __forceinline__ void body_l0_up_zm(const L0UpP& P, long long blk, lon
 	double* __restrict__ part = P.part;
 	using T = ZTile<TY>;
 	constexpr int CRX = T::RX / 2 + 2, CRY = T::RY / 2 + 2, CRP = CRX * CRY;
-	__shared__ float xs[T::NS][T::RP];
-	__shared__ float rs[T::NS][T::RP];
-	__shared__ uint8_t cs[T::NS][T::RP];
-	__shared__ float xcs[2][CRP];
-	__shared__ double tab[kZmMaxGroups][TY];
-	__shared__ float2 dt[16];
+	// LDS in a fixed order as in the down leg
+	struct Lds {
+		uint8_t cs[T::NS][T::RP];
+		float xs[T::NS][T::RP];
+		float rs[T::NS][T::RP];
+		float xcs[2][CRP];
+		double tab[kZmMaxGroups][TY];
+		float2 dt[16];
+	};
+	static_assert((T::NS * T::RP) % 16 == 0 && 2 * sizeof(float) * T::NS * T::RP < 65536, "ring layout");
+	__shared__ Lds lds;
+	auto& cs = lds.cs;
+	auto& xs = lds.xs;
+	auto& rs = lds.rs;
+	auto& xcs = lds.xcs;
+	auto& tab = lds.tab;
+	float2* dt = lds.dt;
 	if (st->done) return;
 	l0_tab_fill(dt);
 	int ox, oy, k0, k1;
