@@ -394,24 +394,32 @@ __device__ __forceinline__ void body_resid(const ResidP& P, long long blk, long 
 		return;
 	}
 	// general stencil, branch-free: neighbours outside the level or inactive
-	// contribute an exact +0 (every load is issued before the sum)
-	constexpr int NS = IS3D ? 27 : 9;
-	float av[NS], xv[NS];
-	bool ok[NS];
+	// are skipped; one z layer (9 neighbours) at a time, all of its loads
+	// issued before its terms are summed (27 terms in stencil order)
 #pragma unroll
-	for (int s = 0; s < NS; s++) {
-		const int dx = s % 3 - 1, dy = (s / 3) % 3 - 1, dz = IS3D ? s / 9 - 1 : 0;
-		const int ni = i + dx, nj = j + dy, nk = k + dz;
-		const bool in = ni >= 0 && nj >= 0 && nk >= 0 && ni < c.nx && nj < c.ny && nk < c.nz;
-		const int nn = in ? ni + sy * nj + sz * nk : v;
-		const uint8_t tn = type[nn];  // unconditional: nn is always a valid vertex
-		ok[s] = in & (tn != 0);
-		av[s] = s >= C ? ldA(A, (long long)(s - C) * c.n + v) : ldA(A, (long long)(C - s) * c.n + nn);
-		xv[s] = x[nn];
+	for (int lz = 0; lz < (IS3D ? 3 : 1); lz++) {
+		float av[9], xv[9];
+		bool ok[9];
+#pragma unroll
+		for (int t = 0; t < 9; t++) {
+			const int s = 9 * lz + t;
+			const int dx = s % 3 - 1, dy = (s / 3) % 3 - 1, dz = IS3D ? s / 9 - 1 : 0;
+			const int ni = i + dx, nj = j + dy, nk = k + dz;
+			const bool in = ni >= 0 && nj >= 0 && nk >= 0 && ni < c.nx && nj < c.ny && nk < c.nz;
+			const int nn = in ? ni + sy * nj + sz * nk : v;
+			const uint8_t tn = type[nn];  // unconditional: nn is always a valid vertex
+			ok[t] = in & (tn != 0);
+			av[t] = s >= C ? ldA(A, (long long)(s - C) * c.n + v) : ldA(A, (long long)(C - s) * c.n + nn);
+			xv[t] = x[nn];
+		}
+#pragma unroll
+		for (int t = 0; t < 9; t++)
+			if (ok[t]) sum -= av[t] * xv[t];
+		// keep the next layer's loads below this sum (bounds the registers
+		// held: otherwise the 54 loads are all hoisted and the kernel drops
+		// to 5 waves per SIMD)
+		__builtin_amdgcn_sched_barrier(0);
 	}
-#pragma unroll
-	for (int s = 0; s < NS; s++)
-		if (ok[s]) sum -= av[s] * xv[s];
 	r[v] = sum;
 }
 
@@ -667,23 +675,24 @@ __device__ inline float zm_nb_sum(uint8_t cc, const uint8_t* cP, const float* xP
 	return sum;
 }
 
-// (diag, 1/diag) of the level-0 row codes in LDS, index (c & 7) + 8 for
-// the 1e-6-scaled trivial rows (bit 7); filled by threads 0..15 before the
-// kernel's first barrier
+// (diag, 1/diag, scale) of the level-0 row codes in LDS, index (c & 7) + 8
+// for the 1e-6-scaled trivial rows (bit 7); filled by threads 0..15 before
+// the kernel's first barrier (scale = l0_scale, diag = l0_diag)
 __device__ inline int l0_tab_index(uint8_t c) { return (c & 7) | ((c >> 4) & 8); }
-__device__ inline void l0_tab_fill(float2* t) {
+__device__ inline void l0_tab_fill(float4* t) {
 	if (threadIdx.x < 16) {
 		const int a = threadIdx.x & 7;
-		const float dg = (threadIdx.x & 8) ? (float)a * 1e-6f : (float)a;
-		t[threadIdx.x] = make_float2(dg, a ? 1.0f / dg : 0.f);
+		const bool tr = (threadIdx.x & 8) != 0;
+		const float dg = tr ? (float)a * 1e-6f : (float)a;
+		t[threadIdx.x] = make_float4(dg, a ? 1.0f / dg : 0.f, tr ? 1e-6f : 1.0f, 0.f);
 	}
 }
-// v / diag(c) as q0 = v (1/d) plus one residual correction (Markstein):
+__device__ inline float4 l0_row(uint8_t c, const float4* t) { return t[l0_tab_index(c)]; }
+// v / diag as q0 = v (1/d) plus one residual correction (Markstein):
 // checked exhaustively over all binary32 v, it equals the IEEE quotient for
 // every finite v when d = 1..5 and for every normal quotient when d = 6; on
 // the 1e-6-scaled trivial rows it can differ from it in the last bit
-__device__ inline float l0_div_t(float v, uint8_t c, const float2* t) {
-	const float2 e = t[l0_tab_index(c)];
+__device__ inline float l0_div_r(float v, const float4& e) {
 	const float q0 = v * e.y;
 	return fmaf(fmaf(-q0, e.x, v), e.y, q0);
 }
@@ -748,6 +757,26 @@ struct ZmPos {
 	}
 };
 
+// Colour-packed positions of the z-marching stages.  Tile origins are even,
+// so the colour of a cell is the parity of its tile-local x + y plus the
+// plane.  Stage B updates the black cells of the region without its outer
+// ring: IX/2 per row, one per thread for threads < BN (the other wavefronts
+// skip the stage).
+template <typename T>
+struct ZmPack {
+	static constexpr int IX = T::RX - 2, IY = T::RY - 2, BN = (IX / 2) * IY;
+	static_assert(IX % 2 == 0 && BN <= T::NT, "colour packing");
+	int bly, bm;  // stage B: region row (1-based) and index within the row
+	__device__ ZmPack() {
+		const int t = threadIdx.x;
+		bly = 1 + t / (IX / 2);
+		bm = t % (IX / 2);
+	}
+	// LDS index of the thread's stage-B black cell at plane parity par:
+	// black means region-local x + y + plane odd
+	__device__ int qb(int par) const { return 2 * bm + 1 + ((bly + par) & 1) + T::RX * bly; }
+};
+
 __device__ inline int ring_next(int s) { return s == ZRING - 1 ? 0 : s + 1; }
 __device__ inline int ring_back(int s, int k) { return (s + ZRING - k) % ZRING; }
 
@@ -773,14 +802,14 @@ __device__ __forceinline__ void body_l0_down_zm(const L0DownP& P, long long blk,
 		uint8_t cs[T::NS][T::RP];
 		float xs[T::NS][T::RP];
 		float bs[T::NS][T::RP];
-		float2 dt[16];
+		float4 dt[16];
 	};
 	static_assert((T::NS * T::RP) % 16 == 0 && 2 * sizeof(float) * T::NS * T::RP < 65536, "ring layout");
 	__shared__ Lds lds;
 	auto& cs = lds.cs;
 	auto& xs = lds.xs;
 	auto& bs = lds.bs;
-	float2* dt = lds.dt;
+	float4* dt = lds.dt;
 	if (st->done) return;
 	l0_tab_fill(dt);
 	int ox, oy, k0, k1;
@@ -788,6 +817,7 @@ __device__ __forceinline__ void body_l0_down_zm(const L0DownP& P, long long blk,
 	const int rx0 = ox - T::H, ry0 = oy - T::H;
 	ZmCells<T> cells(d, rx0, ry0);
 	const ZmPos<T> pos(d, ox, oy);
+	const ZmPack<T> pk;
 	// plane p's inputs are fetched into registers one plane ahead, so the
 	// loads are in flight during the barriers and the sweeps of plane p-1;
 	// a plane outside k0-2..k1+1 or the grid reads a clamped plane and pv
@@ -825,28 +855,23 @@ __device__ __forceinline__ void body_l0_down_zm(const L0DownP& P, long long blk,
 				if (q >= T::RP) continue;
 				const bool ok = cells.ok[u] && pvp;
 				const uint8_t cc = ok ? pc[u] : (uint8_t)0;
-				const float b = ok ? pr[u] * l0_scale(cc) : 0.f;
+				const float4 e = l0_row(cc, dt);
+				const float b = ok ? pr[u] * e.z : 0.f;
 				cs[sA][q] = cc;
 				bs[sA][q] = b;
-				const float x = l0_div_t(b, cc, dt);
+				const float x = l0_div_r(b, e);
 				xs[sA][q] = (l0_active(cc) && cells.par[u] == (par != 0)) ? x : 0.f;
 			}
 			fetch(p + 1);
 		}
 		const int pb = p - 2;
-		if (pb >= k0 - 1 && pb <= k1 && pb >= 0 && pb < d.nz) {
-#pragma unroll
-			for (int u = 0; u < 2; u++) {
-				if (!pos.inb[u]) continue;
-				const int q = pos.qb[u];
-				const uint8_t cc = cs[sB0][q];
-				const float xo = xs[sB0][q];
-				const float v =
-				    l0_div_t(bs[sB0][q] + zm_nb_sum(cc, cs[sB0], xs[sB0], cs[sBm], xs[sBm], xs[sBu], q, T::RX), cc, dt);
-				// the other colour is written back unchanged (its readers
-				// see the same value either way)
-				xs[sB0][q] = (pos.parb[u] != (par != 0) && l0_active(cc)) ? v : xo;
-			}
+		if (pb >= k0 - 1 && pb <= k1 && pb >= 0 && pb < d.nz && threadIdx.x < ZmPack<T>::BN) {
+			const int q = pk.qb(par);
+			const uint8_t cc = cs[sB0][q];
+			const float xo = xs[sB0][q];
+			const float v =
+			    l0_div_r(bs[sB0][q] + zm_nb_sum(cc, cs[sB0], xs[sB0], cs[sBm], xs[sBm], xs[sBu], q, T::RX), l0_row(cc, dt));
+			xs[sB0][q] = l0_active(cc) ? v : xo;
 		}
 		const int pz = p - 4;
 		if (pz >= k0 && pz < k1) {
@@ -854,7 +879,7 @@ __device__ __forceinline__ void body_l0_down_zm(const L0DownP& P, long long blk,
 			const uint8_t cc = cs[sC0][q];
 			const float x = xs[sC0][q];
 			const float rr = (bs[sC0][q] + zm_nb_sum(cc, cs[sC0], xs[sC0], cs[sCm], xs[sCm], xs[sCu], q, T::RX)) -
-			                 dt[l0_tab_index(cc)].x * x;
+			                 l0_row(cc, dt).x * x;
 			if (pos.inc) {
 				const long long g = pos.gc + d.sz * pz;
 				xout[g] = x;
@@ -877,8 +902,8 @@ __device__ __forceinline__ void body_l0_down_zm(const L0DownP& P, long long blk,
 // Up leg (same sweeps as body_l0_up_brick): x = xpre + I x_1, black, red,
 // write z and the partial sums of fl(z*r).  Coarse planes are staged in a
 // 2-slot ring: fine plane p needs coarse planes p>>1 and (p+1)>>1.  The x/y
-// parents of the thread's region cells and their weights are fixed for the
-// whole column, only the z parents change with the plane.  Branch-free in
+// parents of the thread's region cells are fixed for the whole column, only
+// the z parents change with the plane.  Branch-free in
 // the same way as the down leg; the prolongation reads all four x/y parents
 // of a coarse plane and adds each with a select.
 template <int TY>
@@ -902,7 +927,7 @@ __device__ __forceinline__ void body_l0_up_zm(const L0UpP& P, long long blk, lon
 		float rs[T::NS][T::RP];
 		float xcs[2][CRP];
 		double tab[kZmMaxGroups][TY];
-		float2 dt[16];
+		float4 dt[16];
 	};
 	static_assert((T::NS * T::RP) % 16 == 0 && 2 * sizeof(float) * T::NS * T::RP < 65536, "ring layout");
 	__shared__ Lds lds;
@@ -911,7 +936,7 @@ __device__ __forceinline__ void body_l0_up_zm(const L0UpP& P, long long blk, lon
 	auto& rs = lds.rs;
 	auto& xcs = lds.xcs;
 	auto& tab = lds.tab;
-	float2* dt = lds.dt;
+	float4* dt = lds.dt;
 	if (st->done) return;
 	l0_tab_fill(dt);
 	int ox, oy, k0, k1;
@@ -929,10 +954,10 @@ __device__ __forceinline__ void body_l0_up_zm(const L0UpP& P, long long blk, lon
 	auto cvalid = [&](int Pz) -> bool { return cok && Pz >= 0 && Pz < c1.nz; };
 	ZmCells<T> cells(d, rx0, ry0);
 	const ZmPos<T> pos(d, ox, oy);
-	// prolongation offsets (coarse-plane region) and x/y weights per region cell
+	const ZmPack<T> pk;
+	// prolongation offsets (coarse-plane region) and x/y parities per region cell
 	int po[2];
 	bool pxo[2], pyo[2];
-	float pw[2];
 #pragma unroll
 	for (int u = 0; u < 2; u++) {
 		const int q = pos.qa[u];
@@ -940,7 +965,6 @@ __device__ __forceinline__ void body_l0_up_zm(const L0UpP& P, long long blk, lon
 		pxo[u] = gx & 1;
 		pyo[u] = gy & 1;
 		po[u] = ((gx >> 1) - cx0) + CRX * ((gy >> 1) - cy0);
-		pw[u] = (pxo[u] ? 0.5f : 1.0f) * (pyo[u] ? 0.5f : 1.0f);
 	}
 	const int pLo = max(k0 - 2, 0), pHi = min(k1 + 1, d.nz - 1);
 	uint8_t pc[2];
@@ -1000,7 +1024,9 @@ __device__ __forceinline__ void body_l0_up_zm(const L0UpP& P, long long blk, lon
 					sum = by ? sum + b10 : sum;
 					sum = bxy ? sum + b11 : sum;
 				}
-				const float xn = x + (zo ? 0.5f * pw[u] : pw[u]) * sum;
+				// x/y weight 1/2 per odd coordinate (exact powers of two)
+				const float pw = (bx ? 0.5f : 1.0f) * (by ? 0.5f : 1.0f);
+				const float xn = x + (zo ? 0.5f * pw : pw) * sum;
 				x = l0_active(cc) ? xn : x;
 				cs[sA][q] = cc;
 				rs[sA][q] = rr;
@@ -1015,19 +1041,18 @@ __device__ __forceinline__ void body_l0_up_zm(const L0UpP& P, long long blk, lon
 			fetch(p + 1);
 		}
 		const int pb = p - 2;
-		if (pb >= k0 - 1 && pb <= k1 && pb >= 0 && pb < d.nz) {
-#pragma unroll
-			for (int u = 0; u < 2; u++) {
-				if (!pos.inb[u]) continue;
-				const int q = pos.qb[u];
-				const uint8_t cc = cs[sB0][q];
-				const float xo = xs[sB0][q];
-				const float v = l0_div_t(
-				    rs[sB0][q] * l0_scale(cc) + zm_nb_sum(cc, cs[sB0], xs[sB0], cs[sBm], xs[sBm], xs[sBu], q, T::RX), cc,
-				    dt);
-				// the other colour is written back unchanged
-				xs[sB0][q] = (pos.parb[u] != (par != 0) && l0_active(cc)) ? v : xo;
-			}
+		// packed positions from the plane's run-time parity (equal to par;
+		// recomputed per plane instead of holding both parities' positions
+		// in registers across the loop)
+		const int rpar = p & 1;
+		if (pb >= k0 - 1 && pb <= k1 && pb >= 0 && pb < d.nz && threadIdx.x < ZmPack<T>::BN) {
+			const int q = pk.qb(rpar);
+			const uint8_t cc = cs[sB0][q];
+			const float xo = xs[sB0][q];
+			const float4 e = l0_row(cc, dt);
+			const float v =
+			    l0_div_r(rs[sB0][q] * e.z + zm_nb_sum(cc, cs[sB0], xs[sB0], cs[sBm], xs[sBm], xs[sBu], q, T::RX), e);
+			xs[sB0][q] = l0_active(cc) ? v : xo;
 		}
 		const int pz = p - 4;
 		if (pz >= k0 && pz < k1) {
@@ -1035,8 +1060,9 @@ __device__ __forceinline__ void body_l0_up_zm(const L0UpP& P, long long blk, lon
 			const uint8_t cc = cs[sC0][q];
 			const float xo = xs[sC0][q];
 			const float rq = rs[sC0][q];
+			const float4 e = l0_row(cc, dt);
 			const float v =
-			    l0_div_t(rq * l0_scale(cc) + zm_nb_sum(cc, cs[sC0], xs[sC0], cs[sCm], xs[sCm], xs[sCu], q, T::RX), cc, dt);
+			    l0_div_r(rq * e.z + zm_nb_sum(cc, cs[sC0], xs[sC0], cs[sCm], xs[sCm], xs[sCu], q, T::RX), e);
 			const float x = (pos.parc == (par != 0) && l0_active(cc)) ? v : xo;
 			if (pos.inc) z[pos.gc + d.sz * pz] = x;
 			const double prod = (double)(x * rq);

@@ -358,13 +358,15 @@ __device__ __forceinline__ void body_matvec(const MatvecP& P, long long blk, lon
 	const int kchunk = P.kchunk;
 	const CgState* __restrict__ st = P.st;
 	const uint8_t* __restrict__ code = P.code;
-	const float* __restrict__ z = P.z;
-	const float* __restrict__ pold = P.pold;
-	float* __restrict__ pnew = P.pnew;
-	float* __restrict__ q = P.q;
 	if (st->done) return;
 	const float beta = st->beta;
 	const bool first = st->first;
+	const float* __restrict__ z = P.z;
+	// in the first iteration p_old is not used: its loads read z's lines
+	// (cache hits) instead of a second stream
+	const float* __restrict__ pold = first ? P.z : P.pold;
+	float* __restrict__ pnew = P.pnew;
+	float* __restrict__ q = P.q;
 	const unsigned ntx = (unsigned)(d.nx + TX - 1) / TX, nty = (unsigned)(d.ny + TY - 1) / TY;
 	const unsigned b32 = (unsigned)blk, rest = b32 / ntx;
 	const int ox = (int)(b32 - rest * ntx) * TX;
@@ -388,8 +390,8 @@ __device__ __forceinline__ void body_matvec(const MatvecP& P, long long blk, lon
 	// Raw values of one plane: own column (z, p_old, code) and ring cell
 	// (same).  The six loads are issued without branches (a plane outside
 	// the tile's planes k0-1..k1 or the grid reads a clamped plane, a lane
-	// outside the grid reads column 0; p_old is read also in the first
-	// iteration, where it is ignored), so the compiler can wait for one
+	// outside the grid reads column 0; in the first iteration the p_old
+	// loads read z and are ignored), so the compiler can wait for one
 	// plane's loads while the next plane's stay in flight; validity is
 	// applied where the values are used.
 	struct Raw {
