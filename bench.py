@@ -440,8 +440,9 @@ def bench_configs(manta):
 
 def bench_serving(n_sessions=24, n_req=240, rate=200.0):
     """Poisson stream of plume-step / pressure-solve requests on resident
-    sessions (64^2 .. 192^3 cells, log-uniform): served (continuous batching)
-    vs the plain path one request at a time; in real time at `rate` req/s
+    sessions (64^2 .. 192^3 cells, log-uniform): served (continuous batching,
+    up to four ticks in flight on worker threads) vs the plain path one request at
+    a time; in real time at `rate` req/s
     (latency) and with every request queued at t=0 (throughput).  In real
     time each request's slowdown is its latency over the isolated latency of
     the same request (same session and kind, plain path, nothing else
@@ -473,7 +474,7 @@ def bench_serving(n_sessions=24, n_req=240, rate=200.0):
                 "slowdown_p99": round(float(np.percentile(sl, 99)), 3)}
 
     for name, realtime in (("realtime", True), ("saturated", False)):
-        srv = serve.Server(sessions)
+        srv = serve.Server(sessions, workers=4)
         lat, span = srv.run(trace, realtime=realtime)
         out["served_" + name] = {k: round(v, 3) for k, v in serve.summarize(lat, span).items()}
         out["served_" + name]["mean_batch"] = round(float(np.mean(srv.batch_sizes)), 2)

@@ -17,12 +17,20 @@ admitted problem), then the post phases.  Requests arriving during a tick
 are admitted by the next one.  A session's results do not depend on what it
 was batched with (the batched solve is bit-identical to a single solve).
 
+With workers > 1 the ticks run on worker threads (one per hardware queue of
+the GPU; the library call releases the interpreter lock), so a small request
+no longer waits for a large one to finish: the dispatcher hands the ready
+requests of idle sessions to free workers, one request per worker while
+there are no more than free workers, else all of them as one batched tick.
+
 serve_serial replays the same trace on the plain path: one request at a
 time, each with its own solvePressure.
 """
 import collections
 import ctypes as C
 import math
+import queue
+import threading
 import time
 
 import numpy as np
@@ -151,14 +159,17 @@ def _percentiles(lat):
 
 
 class Server:
-    """Continuous-batching engine over resident sessions."""
+    """Continuous-batching engine over resident sessions (workers: ticks in
+    flight at once, each on its own thread)."""
 
-    def __init__(self, sessions, max_batch=64, hbm_reserve=2 << 30):
+    def __init__(self, sessions, max_batch=64, hbm_reserve=2 << 30, workers=1):
         self.sessions = sessions
         self.max_batch = max_batch
         self.hbm_reserve = hbm_reserve
+        self.workers = max(1, int(workers))
         self.ticks = 0
         self.batch_sizes = []
+        self._lock = threading.Lock()
 
     def _admit(self, ready, free_bytes):
         chosen, used, budget = [], set(), free_bytes - self.hbm_reserve
@@ -175,12 +186,16 @@ class Server:
             chosen.append(r)
         return chosen
 
+    def _count(self, n):
+        with self._lock:
+            self.ticks += 1
+            self.batch_sizes.append(n)
+
     def tick(self, batch):
         if len(batch) == 1:
             # nothing to share: the plain path has the lowest latency
             self.sessions[batch[0].session].run_single(batch[0].kind)
-            self.ticks += 1
-            self.batch_sizes.append(1)
+            self._count(1)
             return
         for r in batch:
             if r.kind == REQ_STEP:
@@ -191,11 +206,12 @@ class Server:
             self.sessions[r.session].post(r.kind)
         for r in batch:
             self.sessions[r.session].sync()
-        self.ticks += 1
-        self.batch_sizes.append(len(batch))
+        self._count(len(batch))
 
     def run(self, trace, realtime=True):
         """Replay `trace`; returns per-request latency (s) and the makespan."""
+        if self.workers > 1:
+            return self._run_workers(trace, realtime)
         pending = collections.deque(sorted(trace, key=lambda r: r.arrival))
         ready = []
         lat = {}
@@ -217,6 +233,84 @@ class Server:
             for r in batch:
                 lat[r.rid] = done - (r.arrival if realtime else 0.0)
             ready = [r for r in ready if r.rid not in ids]
+        return lat, time.perf_counter() - t0
+
+    def _split(self, batch, free_workers):
+        """Few ready requests (no more than the free workers): one tick each,
+        in flight at once (latency); more: one batched tick (a batch shares
+        every launch, which is what raises throughput)."""
+        if len(batch) <= free_workers:
+            return [[r] for r in batch]
+        return [batch]
+
+    def _run_workers(self, trace, realtime):
+        pending = collections.deque(sorted(trace, key=lambda r: r.arrival))
+        ready = []
+        busy = set()
+        lat = {}
+        work_q, done_q = queue.Queue(), queue.Queue()
+        t0 = time.perf_counter()
+
+        def worker():
+            while True:
+                batch = work_q.get()
+                if batch is None:
+                    return
+                err = None
+                try:
+                    self.tick(batch)
+                except BaseException as e:  # reported by the dispatcher
+                    err = e
+                done_q.put((batch, time.perf_counter() - t0, err))
+
+        threads = [threading.Thread(target=worker, daemon=True) for _ in range(self.workers)]
+        for th in threads:
+            th.start()
+        in_flight = 0
+        try:
+            while pending or ready or in_flight:
+                now = time.perf_counter() - t0
+                while pending and (not realtime or pending[0].arrival <= now):
+                    ready.append(pending.popleft())
+                free_workers = self.workers - in_flight
+                if free_workers > 0:
+                    cand = [r for r in ready if r.session not in busy]
+                    batch = []
+                    if cand:
+                        free, _ = device_memory()
+                        batch = self._admit(cand, free)
+                        if not batch and in_flight == 0:
+                            raise RuntimeError("serve: no request fits in free device memory")
+                    if batch:
+                        for group in self._split(batch, free_workers):
+                            busy.update(r.session for r in group)
+                            ids = {r.rid for r in group}
+                            ready = [r for r in ready if r.rid not in ids]
+                            work_q.put(group)
+                            in_flight += 1
+                        continue
+                # wait for a completion, or for the next arrival
+                wait = None
+                if realtime and pending:
+                    wait = max(0.0, pending[0].arrival - (time.perf_counter() - t0))
+                if in_flight == 0:
+                    time.sleep(wait or 0.0)
+                    continue
+                try:
+                    group, done, err = done_q.get(timeout=wait)
+                except queue.Empty:
+                    continue
+                in_flight -= 1
+                if err is not None:
+                    raise err
+                for r in group:
+                    lat[r.rid] = done - (r.arrival if realtime else 0.0)
+                    busy.discard(r.session)
+        finally:
+            for _ in threads:
+                work_q.put(None)
+            for th in threads:
+                th.join()
         return lat, time.perf_counter() - t0
 
 

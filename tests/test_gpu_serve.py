@@ -40,3 +40,22 @@ def test_admission_respects_memory():
     # one request per session per tick, and nothing beyond the memory budget
     assert [r.rid for r in srv._admit(ready, 1 << 40)] == [0, 1]
     assert [r.rid for r in srv._admit(ready, 6 * 4 * 16 ** 3)] == [0]
+
+
+def test_threaded_served_equals_serial():
+    """Four worker threads, ticks in flight at once on their sessions'
+    streams (real-time arrivals and all at once): the same per-session
+    states as the serial replay, every request answered once."""
+    for realtime, rate in ((False, 1e9), (True, 400.0)):
+        a = [serve.PlumeScene(sh, seed=30 + i) for i, sh in enumerate(SHAPES)]
+        b = [serve.PlumeScene(sh, seed=30 + i) for i, sh in enumerate(SHAPES)]
+        trace = serve.make_trace(40, len(SHAPES), rate=rate, seed=5)
+        serve.serve_serial(a, trace, realtime=False)
+        srv = serve.Server(b, max_batch=8, workers=4)
+        lat, _ = srv.run(trace, realtime=realtime)
+        assert sorted(lat) == sorted(r.rid for r in trace)
+        assert sum(srv.batch_sizes) == len(trace)
+        for i, (x, y) in enumerate(zip(a, b)):
+            assert x.steps == y.steps
+            for u, v, name in zip(state(x), state(y), ("density", "vel", "pressure")):
+                np.testing.assert_array_equal(u, v, err_msg="realtime=%s session %d %s" % (realtime, i, name))
