@@ -394,32 +394,24 @@ __device__ __forceinline__ void body_resid(const ResidP& P, long long blk, long 
 		return;
 	}
 	// general stencil, branch-free: neighbours outside the level or inactive
-	// are skipped; one z layer (9 neighbours) at a time, all of its loads
-	// issued before its terms are summed (27 terms in stencil order)
+	// contribute an exact +0 (every load is issued before the sum)
+	constexpr int NS = IS3D ? 27 : 9;
+	float av[NS], xv[NS];
+	bool ok[NS];
 #pragma unroll
-	for (int lz = 0; lz < (IS3D ? 3 : 1); lz++) {
-		float av[9], xv[9];
-		bool ok[9];
-#pragma unroll
-		for (int t = 0; t < 9; t++) {
-			const int s = 9 * lz + t;
-			const int dx = s % 3 - 1, dy = (s / 3) % 3 - 1, dz = IS3D ? s / 9 - 1 : 0;
-			const int ni = i + dx, nj = j + dy, nk = k + dz;
-			const bool in = ni >= 0 && nj >= 0 && nk >= 0 && ni < c.nx && nj < c.ny && nk < c.nz;
-			const int nn = in ? ni + sy * nj + sz * nk : v;
-			const uint8_t tn = type[nn];  // unconditional: nn is always a valid vertex
-			ok[t] = in & (tn != 0);
-			av[t] = s >= C ? ldA(A, (long long)(s - C) * c.n + v) : ldA(A, (long long)(C - s) * c.n + nn);
-			xv[t] = x[nn];
-		}
-#pragma unroll
-		for (int t = 0; t < 9; t++)
-			if (ok[t]) sum -= av[t] * xv[t];
-		// keep the next layer's loads below this sum (bounds the registers
-		// held: otherwise the 54 loads are all hoisted and the kernel drops
-		// to 5 waves per SIMD)
-		__builtin_amdgcn_sched_barrier(0);
+	for (int s = 0; s < NS; s++) {
+		const int dx = s % 3 - 1, dy = (s / 3) % 3 - 1, dz = IS3D ? s / 9 - 1 : 0;
+		const int ni = i + dx, nj = j + dy, nk = k + dz;
+		const bool in = ni >= 0 && nj >= 0 && nk >= 0 && ni < c.nx && nj < c.ny && nk < c.nz;
+		const int nn = in ? ni + sy * nj + sz * nk : v;
+		const uint8_t tn = type[nn];  // unconditional: nn is always a valid vertex
+		ok[s] = in & (tn != 0);
+		av[s] = s >= C ? ldA(A, (long long)(s - C) * c.n + v) : ldA(A, (long long)(C - s) * c.n + nn);
+		xv[s] = x[nn];
 	}
+#pragma unroll
+	for (int s = 0; s < NS; s++)
+		if (ok[s]) sum -= av[s] * xv[s];
 	r[v] = sum;
 }
 
