@@ -277,17 +277,34 @@ __global__ void __launch_bounds__(kBlock) k_correct_velocity(Dims d, Tiles t, co
 // ---------------------------------------------------------------------------
 // CG kernels
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_cg_init(Dims d, const float* __restrict__ rhs, float* __restrict__ x,
-                                                    float* __restrict__ r, int withDot, double* __restrict__ part) {
+// doInit (conjugategrad.cpp:210-235): x = 0, r = b, partial sums of
+// fl(b*b) (GridDotProduct of the unpreconditioned tmp = r) and of b^2 in
+// FP64; float4 groups with a grid stride of per*256 (as body_update), the
+// n % 4 trailing cells spread the same way; one partial per block
+__global__ void __launch_bounds__(kBlock) k_cg_init(Dims d, long long per, const float* __restrict__ rhs,
+                                                    float* __restrict__ x, float* __restrict__ r, int withDot,
+                                                    double* __restrict__ part) {
 	__shared__ double sh[8];
-	const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
 	double zr = 0, bb = 0;
-	if (c < d.n) {
+	const long long n4 = d.n >> 2;
+	const long long stride = per * kBlock;
+	const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+	for (long long v = blockIdx.x * (long long)kBlock + threadIdx.x; v < n4; v += stride) {
+		const float4 bv = reinterpret_cast<const float4*>(rhs)[v];
+		reinterpret_cast<float4*>(x)[v] = zero4;
+		reinterpret_cast<float4*>(r)[v] = bv;
+		const float e[4] = {bv.x, bv.y, bv.z, bv.w};
+		for (int u = 0; u < 4; u++) {
+			zr += (double)(e[u] * e[u]);
+			bb += (double)e[u] * (double)e[u];
+		}
+	}
+	for (long long c = (n4 << 2) + blockIdx.x * (long long)kBlock + threadIdx.x; c < d.n; c += stride) {
 		const float b = rhs[c];
 		x[c] = 0.f;
 		r[c] = b;
-		zr = (double)(b * b);  // GridDotProduct(tmp=r, r): float product
-		bb = (double)b * (double)b;
+		zr += (double)(b * b);
+		bb += (double)b * (double)b;
 	}
 	const double s0 = block_sum(zr, sh);
 	const double s1 = block_sum(bb, sh);
@@ -1094,9 +1111,9 @@ static void cg_setup(CgRun& R, Scene* s, Grid* flags, float* rhs, float* x, cons
 	                   o.relTol);
 
 	// doInit: x = 0, r = b, bnorm; preconditioned: z = M r, sigma = dot(z, r)
-	hipLaunchKernelGGL(k_cg_init, dim3(R.nbF), dim3(kBlock), 0, s->stream, d, (const float*)rhs, x, R.r,
+	hipLaunchKernelGGL(k_cg_init, dim3(R.nbU), dim3(kBlock), 0, s->stream, d, R.per, (const float*)rhs, x, R.r,
 	                   R.pcNone ? 1 : 0, s->dpart);
-	run_finalize_one(s->stream, FinP{R.st, s->dpart, s->fpart, (int)R.nbF, FIN_INIT, 0});
+	run_finalize_one(s->stream, FinP{R.st, s->dpart, s->fpart, (int)R.nbU, FIN_INIT, 0});
 	if (!R.pcNone && !batchInit) {
 		int np = R.mg->vcycle(s, R.code, R.r, R.z, R.st);
 		if (np == 0) {
