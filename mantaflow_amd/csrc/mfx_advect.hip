@@ -95,8 +95,16 @@ __device__ inline void getAtMACZ(const Vel& V, const Geo<IS3D, I>& g, I ix, floa
 	o[2] = V.w[ix];
 }
 
+// Two x-adjacent floats in one 8-byte load (addresses are 4-byte aligned:
+// a dword-aligned multi-dword global load is a single global_load_dwordx2)
+struct __attribute__((packed, aligned(4))) FPair {
+	float a, b;
+};
+__device__ inline FPair ld_pair(const float* __restrict__ p) { return *reinterpret_cast<const FPair*>(p); }
+
 // interpol<Real> / interpolComponent (interpol.h:50-90).  The eight corners
-// are read at one base index plus constant strides.
+// are read at one base index plus constant strides, x-adjacent pairs in one
+// load each.
 template <bool IS3D, typename I>
 __device__ inline float interp(const float* __restrict__ data, const Geo<IS3D, I>& g, float posx, float posy,
                                float posz) {
@@ -118,8 +126,9 @@ __device__ inline float interp(const float* __restrict__ data, const Geo<IS3D, I
 	const float s0 = 1.0f - s1, t0 = 1.0f - t1, f0 = 1.0f - f1;
 	const I Y = g.Y, Z = g.Z;
 	const float* __restrict__ p = data + g.at(xi, yi, IS3D ? zi : 0);
-	return ((p[0] * t0 + p[Y] * t1) * s0 + (p[1] * t0 + p[1 + Y] * t1) * s1) * f0 +
-	       ((p[Z] * t0 + p[Y + Z] * t1) * s0 + (p[1 + Z] * t0 + p[1 + Y + Z] * t1) * s1) * f1;
+	const FPair c00 = ld_pair(p), c10 = ld_pair(p + Y), c01 = ld_pair(p + Z), c11 = ld_pair(p + Y + Z);
+	return ((c00.a * t0 + c10.a * t1) * s0 + (c00.b * t0 + c10.b * t1) * s1) * f0 +
+	       ((c01.a * t0 + c11.a * t1) * s0 + (c01.b * t0 + c11.b * t1) * s1) * f1;
 }
 
 __device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -278,11 +287,15 @@ __global__ void __launch_bounds__(kBlock) k_maccormack_centered(Vel V, Dims d, T
 			bool cf[NQ];
 			float cv[NQ][NC];
 #pragma unroll
-			for (int q = 0; q < NQ; q++) {
-				const I cx = c0 + off[q];
-				cf[q] = CHECKFLAG(cx);
+			for (int q = 0; q < NQ; q++) cf[q] = CHECKFLAG(c0 + off[q]);
 #pragma unroll
-				for (int c = 0; c < NC; c++) cv[q][c] = orig[c][cx];
+			for (int q = 0; q < NQ; q += 2) {  // x-adjacent corner pairs
+#pragma unroll
+				for (int c = 0; c < NC; c++) {
+					const FPair e = ld_pair(orig[c] + c0 + off[q]);
+					cv[q][c] = e.a;
+					cv[q + 1][c] = e.b;
+				}
 			}
 #pragma unroll
 			for (int q = 0; q < NQ; q++) {
@@ -340,15 +353,17 @@ __device__ inline float clamp_comp_mac(const Geo<IS3D, I>& g, const uint8_t* __r
 		for (int a = 0; a < 3; a++) P[a] = (int)(l == 0 ? pos[a] - vel[a] : pos[a] + vel[a]);
 		const int i0 = clampi(P[0], 0, gux - 1), j0 = clampi(P[1], 0, guy - 1);
 		const float* __restrict__ o = origc + g.at(i0, j0, IS3D ? clampi(P[2], 0, guz - 1) : 0);
-		min_max(o[0], mn, mx);
-		min_max(o[1], mn, mx);
-		min_max(o[Y], mn, mx);
-		min_max(o[1 + Y], mn, mx);
+		const FPair e00 = ld_pair(o), e10 = ld_pair(o + Y);
+		min_max(e00.a, mn, mx);
+		min_max(e00.b, mn, mx);
+		min_max(e10.a, mn, mx);
+		min_max(e10.b, mn, mx);
 		if (IS3D) {
-			min_max(o[Z], mn, mx);
-			min_max(o[1 + Z], mn, mx);
-			min_max(o[Y + Z], mn, mx);
-			min_max(o[1 + Y + Z], mn, mx);
+			const FPair e01 = ld_pair(o + Z), e11 = ld_pair(o + Y + Z);
+			min_max(e01.a, mn, mx);
+			min_max(e01.b, mn, mx);
+			min_max(e11.a, mn, mx);
+			min_max(e11.b, mn, mx);
 		}
 	}
 	if (clampMode == 1) return clampf_ref(dst, mn, mx);
