@@ -23,7 +23,7 @@ import time
 
 import numpy as np
 
-from . import _abi
+from . import _abi, gridio
 from ._abi import check, lib
 
 lib()  # the GPU path is the only path: fail at import if libmfx.so is missing
@@ -520,24 +520,44 @@ class _ScalarOps:
             _err("array size %d does not match grid size %d" % (a.size, self._n * self._ncomp))
         check(lib().mfx_grid_upload(self._h, a.ctypes.data_as(C.c_void_p), a.nbytes))
 
+    def _dims(self):
+        return (self.getSizeX(), self.getSizeY(), self.getSizeZ())
+
     def save(self, name):
-        """Grid<T>::save for .npz / .npy (numpy container, x fastest)."""
-        if name.endswith(".npz"):
-            np.savez_compressed(name, arr_0=self.toNumpy())
-        elif name.endswith(".npy"):
-            np.save(name, self.toNumpy())
-        else:
-            _err("Grid::save: only .npz and .npy are supported by this build")
+        """Grid<T>::save (grid.cpp:110-130): .uni and .raw in the reference's
+        formats (gridio), .npz / .npy as numpy containers (x fastest)."""
+        try:
+            if name.endswith(".uni"):
+                gridio.write_uni(name, self.toNumpy(), self._dims(), self.getGridType())
+            elif name.endswith(".raw"):
+                gridio.write_raw(name, self.toNumpy(), self._np == np.int32)
+            elif name.endswith(".npz"):
+                np.savez_compressed(name, arr_0=self.toNumpy())
+            elif name.endswith(".npy"):
+                np.save(name, self.toNumpy())
+            else:
+                _err("Grid::save: only .uni, .raw, .npz and .npy are supported by this build")
+        except gridio.GridIOError as e:
+            _err(str(e))
         return 1
 
     def load(self, name):
-        if name.endswith(".npz"):
-            with np.load(name, allow_pickle=False) as z:
-                self.fromNumpy(z[z.files[0]])
-        elif name.endswith(".npy"):
-            self.fromNumpy(np.load(name, allow_pickle=False))
-        else:
-            _err("Grid::load: only .npz and .npy are supported by this build")
+        """Grid<T>::load (grid.cpp:132-152), the formats of save."""
+        try:
+            if name.endswith(".uni"):
+                _, a = gridio.read_uni(name, self._dims(), self.getGridType())
+                self.fromNumpy(a)
+            elif name.endswith(".raw"):
+                self.fromNumpy(gridio.read_raw(name, self._n * self._ncomp, self._np == np.int32))
+            elif name.endswith(".npz"):
+                with np.load(name, allow_pickle=False) as z:
+                    self.fromNumpy(z[z.files[0]])
+            elif name.endswith(".npy"):
+                self.fromNumpy(np.load(name, allow_pickle=False))
+            else:
+                _err("Grid::load: only .uni, .raw, .npz and .npy are supported by this build")
+        except gridio.GridIOError as e:
+            _err(str(e))
         return 1
 
 

@@ -110,6 +110,33 @@ class PlumeScene:
         self.post(kind)
         self.s.sync()
 
+    # checkpoint / resume: every grid of the session in the reference's .uni
+    # format (one file each) plus the step counter; a restored session
+    # continues exactly as the saved one would have
+    _GRIDS = ("flags", "vel", "density", "pressure", "srcval", "keep")
+
+    def save(self, directory):
+        import json
+        import os
+        os.makedirs(directory, exist_ok=True)
+        for name in self._GRIDS:
+            getattr(self, name).save(os.path.join(directory, name + ".uni"))
+        with open(os.path.join(directory, "session.json"), "w") as f:
+            json.dump({"shape": list(self.shape), "steps": self.steps,
+                       "gravity": [self.gravity.x, self.gravity.y, self.gravity.z]}, f)
+
+    def load(self, directory):
+        import json
+        import os
+        with open(os.path.join(directory, "session.json")) as f:
+            meta = json.load(f)
+        if tuple(meta["shape"]) != self.shape:
+            raise RuntimeError("session checkpoint of shape %s, session %s" % (tuple(meta["shape"]), self.shape))
+        for name in self._GRIDS:
+            getattr(self, name).load(os.path.join(directory, name + ".uni"))
+        self.steps = int(meta["steps"])
+        self.gravity = m.vec3(*meta["gravity"])
+
     def memory(self):
         b = C.c_size_t()
         check(lib().mfx_scene_memory(self.s._h, C.byref(b)))

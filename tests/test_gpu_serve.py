@@ -59,3 +59,22 @@ def test_threaded_served_equals_serial():
             assert x.steps == y.steps
             for u, v, name in zip(state(x), state(y), ("density", "vel", "pressure")):
                 np.testing.assert_array_equal(u, v, err_msg="realtime=%s session %d %s" % (realtime, i, name))
+
+
+def test_session_checkpoint_resume(tmp_path):
+    """A session saved after some requests and restored into a fresh session
+    of the same shape continues bit-identically."""
+    a = serve.PlumeScene((20, 20, 20), seed=4)
+    for kind in (serve.REQ_STEP, serve.REQ_SOLVE, serve.REQ_STEP):
+        a.run_single(kind)
+    a.save(str(tmp_path / "ck"))
+    b = serve.PlumeScene((20, 20, 20), seed=99)  # different source values until restored
+    b.load(str(tmp_path / "ck"))
+    assert b.steps == a.steps
+    for kind in (serve.REQ_STEP, serve.REQ_STEP, serve.REQ_SOLVE):
+        a.run_single(kind)
+        b.run_single(kind)
+    for u, v, name in zip(state(a), state(b), ("density", "vel", "pressure")):
+        np.testing.assert_array_equal(u, v, err_msg=name)
+    with pytest.raises(RuntimeError, match="shape"):
+        serve.PlumeScene((16, 16, 16), seed=1).load(str(tmp_path / "ck"))
