@@ -554,44 +554,33 @@ __global__ void __launch_bounds__(kBlock) k_set_wall_bcs(Dims d, Tiles t, const 
 			if (d.is3d) ob[q][2] = ow[ix];
 		}
 	}
+	// The per-cell rules below, written as boolean algebra and selects (no
+	// branch per condition): a face next to an obstacle (or an obstacle face
+	// next to fluid) takes the obstacle velocity; a stick wall across x, y or
+	// z zeroes the two other components of a fluid cell, overriding the
+	// obstacle value (the stick tests come last in the reference).
 #pragma unroll
 	for (int q = 0; q < KZ; q++) {
 		const int k = p.k + q;
 		if (k >= d.nz) break;
 		const long long ix = d.idx(i, j, k);
-		const bool curFluid = f[q] & kFluid, curObs = f[q] & kObstacle;
-		if (!curFluid && !curObs) continue;
-		bool wu = false, wv = false, ww = false;
-		float nu = 0.f, nv = 0.f, nw = 0.f;
-		if (i > 0 && ((fxm[q] & kObstacle) || (curObs && (fxm[q] & kFluid)))) {
-			nu = ob[q][0];
-			wu = true;
-		}
-		if (j > 0 && ((fym[q] & kObstacle) || (curObs && (fym[q] & kFluid)))) {
-			nv = ob[q][1];
-			wv = true;
-		}
-		if (!d.is3d) {
-			nw = 0.f;
-			ww = true;
-		} else if (k > 0 && ((fzm[q] & kObstacle) || (curObs && (fzm[q] & kFluid)))) {
-			nw = ob[q][2];
-			ww = true;
-		}
-		if (curFluid) {
-			if ((i > 0 && (fxm[q] & kStick)) || (i < d.nx - 1 && (fxp[q] & kStick))) {
-				nv = nw = 0.f;
-				wv = ww = true;
-			}
-			if ((j > 0 && (fym[q] & kStick)) || (j < d.ny - 1 && (fyp[q] & kStick))) {
-				nu = nw = 0.f;
-				wu = ww = true;
-			}
-			if (d.is3d && ((k > 0 && (fzm[q] & kStick)) || (k < d.nz - 1 && (fzp[q] & kStick)))) {
-				nu = nv = 0.f;
-				wu = wv = true;
-			}
-		}
+		// predicates as 0/1 integers (vector ALU) rather than lane masks
+		const int fl = f[q], xm = fxm[q], xp = fxp[q], ym = fym[q], yp = fyp[q], zm = fzm[q], zp = fzp[q];
+		const int curF = fl & 1, curO = (fl >> 1) & 1;  // kFluid = 1, kObstacle = 2, kStick = 64
+		const int act = curF | curO;
+		const int gi = min(i, 1), li = min(d.nx - 1 - i, 1), gj = min(j, 1), lj = min(d.ny - 1 - j, 1);
+		const int gk = min(k, 1), lk = min(d.nz - 1 - k, 1), is3 = d.is3d ? 1 : 0;
+		const int ou_ = act & gi & (((xm >> 1) & 1) | (curO & xm & 1));
+		const int ov_ = act & gj & (((ym >> 1) & 1) | (curO & ym & 1));
+		const int ow_ = act & ((is3 ^ 1) | (gk & (((zm >> 1) & 1) | (curO & zm & 1))));
+		const int sx = curF & ((gi & (xm >> 6)) | (li & (xp >> 6))) & 1;
+		const int sy = curF & ((gj & (ym >> 6)) | (lj & (yp >> 6))) & 1;
+		const int sz = curF & is3 & ((gk & (zm >> 6)) | (lk & (zp >> 6))) & 1;
+		const int wu = ou_ | sy | sz, wv = ov_ | sx | sz, ww = ow_ | sx | sy;
+		// value masks: all ones keeps the obstacle velocity, zero gives 0.0
+		const float nu = __int_as_float(__float_as_int(ob[q][0]) & ((sy | sz) - 1));
+		const float nv = __int_as_float(__float_as_int(ob[q][1]) & ((sx | sz) - 1));
+		const float nw = __int_as_float(__float_as_int(ob[q][2]) & ((sx | sy | (is3 ^ 1)) - 1));
 		if (wu) u[ix] = nu;
 		if (wv) v[ix] = nv;
 		if (ww) w[ix] = nw;
