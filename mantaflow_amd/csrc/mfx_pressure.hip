@@ -87,42 +87,36 @@ __global__ void __launch_bounds__(kBlock) k_build_code(Dims d, Tiles t, const ui
 		fzm[q] = f8[c - ozm];
 		fzp[q] = f8[c + ozp];
 	}
+	// The rules below as 0/1 integer predicates (vector ALU, no branch per
+	// condition); the same values as MakeLaplaceMatrix + fixPressure give.
+	const int is3 = d.is3d ? 1 : 0;
+	const int gi = min(i, 1), li = min(d.nx - 1 - i, 1), g2i = min(max(i - 1, 0), 1);  // i>=1, i<nx-1, i>=2
+	const int gj = min(j, 1), lj = min(d.ny - 1 - j, 1), g2j = min(max(j - 1, 0), 1);
 #pragma unroll
 	for (int q = 0; q < KZ; q++) {
 		const int k = p.k + q;
 		if (k >= d.nz) break;
 		const long long c = d.idx(i, j, k);
-		const bool fl = f[q] & kFluid;
-		const bool inner = interior_c(d, i, j, k);
-		int A0 = 0, bx = 0, by = 0, bz = 0;
-		if (fl && inner) {
-			if (c == fix) {
-				A0 = 1;
-			} else {
-				A0 += !(fxm[q] & kObstacle);
-				A0 += !(fxp[q] & kObstacle);
-				A0 += !(fym[q] & kObstacle);
-				A0 += !(fyp[q] & kObstacle);
-				if (d.is3d) {
-					A0 += !(fzm[q] & kObstacle);
-					A0 += !(fzp[q] & kObstacle);
-				}
-				// off_coef(c -> upper neighbour): c interior fluid (true here),
-				// neighbour fluid, neither pinned
-				bx = (fxp[q] & kFluid) && c + 1 != fix;
-				by = (fyp[q] & kFluid) && c + d.nx != fix;
-				bz = d.is3d ? ((fzp[q] & kFluid) && c + d.sz != fix) : 0;
-			}
-		}
+		const int gk = min(k, 1), lk = min(d.nz - 1 - k, 1), g2k = min(max(k - 1, 0), 1);
+		const int kin = is3 ? (gk & lk) : 1;  // k inside the interior (always in 2D)
+		const int fl = f[q] & 1;               // kFluid = 1, kObstacle = 2
+		const int inner = gi & li & gj & lj & kin;
+		const int isfix = c == fix ? 1 : 0, nfix = isfix ^ 1;
+		const int fi = fl & inner;
+		const int nobs = (((fxm[q] >> 1) & 1) ^ 1) + (((fxp[q] >> 1) & 1) ^ 1) + (((fym[q] >> 1) & 1) ^ 1) +
+		                 (((fyp[q] >> 1) & 1) ^ 1) + is3 * ((((fzm[q] >> 1) & 1) ^ 1) + (((fzp[q] >> 1) & 1) ^ 1));
+		const int A0 = fi * (isfix + nfix * nobs);
+		// off_coef(c -> upper neighbour): c interior fluid, neighbour fluid, neither pinned
+		const int bx = fi & nfix & fxp[q] & (c + 1 != fix ? 1 : 0);
+		const int by = fi & nfix & fyp[q] & (c + d.nx != fix ? 1 : 0);
+		const int bz = fi & nfix & is3 & fzp[q] & (c + d.sz != fix ? 1 : 0);
 		// any lower off-diagonal into c: off_coef(lower neighbour -> c)
-		int lo = 0;
-		if (fl && c != fix) {
-			if (i > 0 && interior_c(d, i - 1, j, k) && (fxm[q] & kFluid) && c - 1 != fix) lo = 1;
-			if (j > 0 && interior_c(d, i, j - 1, k) && (fym[q] & kFluid) && c - d.nx != fix) lo = 1;
-			if (d.is3d && k > 0 && interior_c(d, i, j, k - 1) && (fzm[q] & kFluid) && c - d.sz != fix) lo = 1;
-		}
-		const int trivial = (A0 == 1 && !bx && !by && !bz && !lo) ? 1 : 0;
-		code[c] = (uint8_t)(A0 | (bx << 3) | (by << 4) | (bz << 5) | ((fl ? 1 : 0) << 6) | (trivial << 7));
+		const int lox = gi & g2i & gj & lj & kin & fxm[q] & (c - 1 != fix ? 1 : 0);
+		const int loy = gj & g2j & gi & li & kin & fym[q] & (c - d.nx != fix ? 1 : 0);
+		const int loz = is3 & gk & g2k & gi & li & gj & lj & fzm[q] & (c - d.sz != fix ? 1 : 0);
+		const int lo = fl & nfix & (lox | loy | loz) & 1;
+		const int trivial = (A0 == 1 ? 1 : 0) & ((bx | by | bz | lo) & 1 ^ 1);
+		code[c] = (uint8_t)(A0 | ((bx & 1) << 3) | ((by & 1) << 4) | ((bz & 1) << 5) | (fl << 6) | (trivial << 7));
 	}
 }
 
