@@ -301,9 +301,18 @@ __global__ void k_to_bf16(const float* __restrict__ in, uint16_t* __restrict__ o
 		out[i] = to_bf16(in[i]);
 }
 
+// Three consecutive floats at a 4-byte aligned address, read by one load.
+struct __attribute__((packed, aligned(4))) FTriple {
+	float a, b, c;
+};
+
 // b_c[V] = sum_{U in [2V-1,2V+1]} w(U) r_f[U]   (r_f is 0 on inactive vertices).
 // Fixed 3x3x3 (3x3) trip counts with predicates: the loops unroll and all
-// fine loads are in flight together; skipped terms are exact zeros.
+// fine loads are in flight together; skipped terms are exact zeros.  The
+// three fine values of a row (x = 2i-1, 2i, 2i+1) come from one 12-byte load
+// at x0 = clamp(2i-1, 0, nx_f-3); only the first and last coarse vertex of a
+// row have 2i-1 != x0 and pick their values at a shifted slot.  Fine levels
+// narrower than 3 vertices in x take one load per value.
 template <bool IS3D>
 __device__ __forceinline__ void body_restrict(const RestrictP& P, long long blk, long long) {
 	const CgState* __restrict__ st = P.st;
@@ -317,6 +326,38 @@ __device__ __forceinline__ void body_restrict(const RestrictP& P, long long blk,
 	if (!p.in) return;
 	const int i = p.i, j = p.j, k = p.k, v = p.v;
 	float sum = 0.f;
+	if (f.nx >= 3) {
+		constexpr int NZ = IS3D ? 3 : 1;
+		const int x0 = min(max(2 * i - 1, 0), f.nx - 3);
+		const int d = 2 * i - 1 - x0;  // -1 (i = 0), 0, 1 or 2 (last vertex of a row)
+		long long ay[3], az[3];
+		bool kx[3], ky[3], kz[3];
+#pragma unroll
+		for (int e = 0; e < 3; e++) {
+			const int ux = 2 * i + e - 1, uy = 2 * j + e - 1, uz = 2 * k + (IS3D ? e - 1 : 0);
+			kx[e] = ux >= 0 && ux < f.nx;
+			ky[e] = uy >= 0 && uy < f.ny;
+			kz[e] = uz >= 0 && uz < f.nz;
+			ay[e] = ky[e] ? f.sy * uy : 0;
+			az[e] = kz[e] ? f.sz * uz : 0;
+		}
+		const uint8_t ct = ctype[v];
+		FTriple row[NZ * 3];
+#pragma unroll
+		for (int r = 0; r < NZ * 3; r++) row[r] = *reinterpret_cast<const FTriple*>(rf + x0 + ay[r % 3] + az[IS3D ? r / 3 : 1]);
+#pragma unroll
+		for (int e = 0; e < NZ * 9; e++) {
+			const int ex = e % 3, ey = (e / 3) % 3, ez = IS3D ? e / 9 : 1;
+			const FTriple& t = row[ey + 3 * (IS3D ? ez : 0)];
+			// slot of x = 2i - 1 + ex in the triple: d + ex (0..2 whenever kx[ex])
+			const int s = d + ex;
+			const float val = s <= 0 ? t.a : (s == 1 ? t.b : t.c);
+			const int ux = 2 * i + ex - 1, uy = 2 * j + ey - 1, uz = 2 * k + (IS3D ? ez - 1 : 0);
+			if (kx[ex] && ky[ey] && kz[ez]) sum += wgt(ux, uy, uz) * val;
+		}
+		bc[v] = ct ? sum : 0.f;
+		return;
+	}
 	if (ctype[v]) {
 		// branch-free: per-axis offsets of the fine coordinates 2V-1..2V+1
 		// (0 where outside the level, so every address is valid), every load
