@@ -418,17 +418,24 @@ __device__ __forceinline__ void body_resid(const ResidP& P, long long blk, long 
 	float sum = b[v];
 	if (tv == 2) {
 		// canonical stencil, every neighbour inside and active: one fused
-		// multiply-add chain per z layer
-		const float* __restrict__ xv = x + v;
+		// multiply-add chain per z layer; the three x-neighbours of a row
+		// (v-1, v, v+1 shifted by the row offset) come from one 12-byte load
+		const float* __restrict__ xv = x + v - 1;
+		FTriple xr[IS3D ? 9 : 3];
+#pragma unroll
+		for (int r = 0; r < (IS3D ? 9 : 3); r++)
+			xr[r] = *reinterpret_cast<const FTriple*>(xv + sy * (r % 3 - 1) + sz * (IS3D ? r / 3 - 1 : 0));
 		float acc[IS3D ? 3 : 1];
 #pragma unroll
 		for (int lz = 0; lz < (IS3D ? 3 : 1); lz++) {
 			float sacc = 0.f;
 #pragma unroll
-			for (int dy = -1; dy <= 1; dy++)
-#pragma unroll
-				for (int dx = -1; dx <= 1; dx++)
-					sacc = fmaf(cn.a[(dx + 1) + 3 * (dy + 1) + 9 * lz], xv[dx + sy * dy + sz * (IS3D ? lz - 1 : 0)], sacc);
+			for (int dy = -1; dy <= 1; dy++) {
+				const FTriple& t = xr[(dy + 1) + 3 * lz];
+				sacc = fmaf(cn.a[3 * (dy + 1) + 9 * lz], t.a, sacc);
+				sacc = fmaf(cn.a[1 + 3 * (dy + 1) + 9 * lz], t.b, sacc);
+				sacc = fmaf(cn.a[2 + 3 * (dy + 1) + 9 * lz], t.c, sacc);
+			}
 			acc[lz] = sacc;
 		}
 		r[v] = sum - (IS3D ? (acc[0] + acc[1]) + acc[IS3D ? 2 : 0] : acc[0]);
