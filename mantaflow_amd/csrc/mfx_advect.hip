@@ -427,6 +427,115 @@ __global__ void __launch_bounds__(kBlock) k_maccormack_mac(Vel V, Dims d, Tiles 
 	nw[ix] = val[2];
 }
 
+// Clamp bounds of doClampComponentMAC (advection.cpp:191-235), clampMode 2:
+// min and max of the component over the 8 (4) corners at toVec3i(pos - vel),
+// the same corners as clamp_comp_mac.  In clampMode 2 the bounds are only
+// compared with, so fminf / fmaxf (v_min / v_max, which skip NaN operands as
+// getMinMax's strict comparisons do) give getMinMax's values; only the sign
+// of a zero bound could differ, which no comparison sees.
+template <bool IS3D, typename I>
+__device__ inline void clamp_bounds_mac(const Geo<IS3D, I>& g, const float* __restrict__ origc, int i, int j, int k,
+                                        const float* vel, float& mn, float& mx) {
+	const I Y = g.Y, Z = g.Z;
+	const int P0 = (int)((float)i - vel[0]), P1 = (int)((float)j - vel[1]), P2 = (int)((float)k - vel[2]);
+	const int i0 = min(max(P0, 0), g.nx - 2), j0 = min(max(P1, 0), g.ny - 2);
+	const float* __restrict__ o = origc + g.at(i0, j0, IS3D ? min(max(P2, 0), g.nz - 2) : 0);
+	const FPair e00 = ld_pair(o), e10 = ld_pair(o + Y);
+	// seeded with +-FLT_MAX as getMinMax is (so infinite and NaN corners give
+	// its values too)
+	mn = fminf(fminf(3.402823466e+38f, fminf(e00.a, e00.b)), fminf(e10.a, e10.b));
+	mx = fmaxf(fmaxf(-3.402823466e+38f, fmaxf(e00.a, e00.b)), fmaxf(e10.a, e10.b));
+	if (IS3D) {
+		const FPair e01 = ld_pair(o + Z), e11 = ld_pair(o + Y + Z);
+		mn = fminf(mn, fminf(fminf(e01.a, e01.b), fminf(e11.a, e11.b)));
+		mx = fmaxf(mx, fmaxf(fmaxf(e01.a, e01.b), fmaxf(e11.a, e11.b)));
+	}
+}
+
+// k_maccormack_mac for clampMode 2 on grids of at least 3 cells per axis, the
+// same arithmetic in an order with two dependent memory rounds instead of one
+// per conditional: every lane evaluates the trace at its cell clamped into
+// the interior (equal to its own cell where the reference evaluates it; the
+// other lanes' results are discarded), so there is no divergent branch; the
+// cell's flags, forward and original values load with the face velocities,
+// and the clamp bounds (which depend on the face velocities and the original
+// field only) load with the backward trace.
+template <bool IS3D, typename I>
+__global__ void __launch_bounds__(kBlock) k_maccormack_mac2(Vel V, Dims d, Tiles t, float dt, float strength,
+                                                            const uint8_t* __restrict__ f8, const float* ou,
+                                                            const float* ov, const float* ow,
+                                                            const float* __restrict__ fu, const float* __restrict__ fv,
+                                                            const float* __restrict__ fw, float* __restrict__ nu,
+                                                            float* __restrict__ nv, float* __restrict__ nw) {
+	const CellPos p = tile_cell(t);
+	if (p.i >= d.nx || p.j >= d.ny) return;
+	const int i = p.i, j = p.j, k = p.k;
+	const Geo<IS3D, I> g(d);
+	const I ix = g.at(i, j, k);
+	const bool inner = g.interior(i, j, k);
+	const int ic = clampi(i, 1, g.nx - 2), jc = clampi(j, 1, g.ny - 2), kc = IS3D ? clampi(k, 1, g.nz - 2) : k;
+	const I ixc = g.at(ic, jc, kc);
+	// the cell's flags (lower neighbours clamped into the grid; used only
+	// where they exist), forward and original values
+	constexpr int FM = kFluid | kEmpty;
+	const uint8_t f0 = f8[ix];
+	const uint8_t fxm = f8[i > 0 ? ix - 1 : ix];
+	const uint8_t fym = f8[j > 0 ? ix - g.Y : ix];
+	const uint8_t fzm = IS3D ? f8[k > 0 ? ix - g.Z : ix] : (uint8_t)0;
+	const float* orig[3] = {ou, ov, ow};
+	const float* fwd[3] = {fu, fv, fw};
+	float fvv[3], org[3];
+#pragma unroll
+	for (int c = 0; c < 3; c++) {
+		fvv[c] = fwd[c][ix];
+		org[c] = orig[c][ix];
+	}
+	float vx[3], vy[3], vz[3];
+	getAtMACX(V, g, ixc, vx);
+	getAtMACY(V, g, ixc, vy);
+	getAtMACZ(V, g, ixc, vz);
+	float mn[3], mx[3];
+	{
+		float vdt[3];
+		for (int a = 0; a < 3; a++) vdt[a] = vx[a] * dt;
+		clamp_bounds_mac(g, ou, ic, jc, kc, vdt, mn[0], mx[0]);
+		for (int a = 0; a < 3; a++) vdt[a] = vy[a] * dt;
+		clamp_bounds_mac(g, ov, ic, jc, kc, vdt, mn[1], mx[1]);
+		if (IS3D) {
+			for (int a = 0; a < 3; a++) vdt[a] = vz[a] * dt;
+			clamp_bounds_mac(g, ow, ic, jc, kc, vdt, mn[2], mx[2]);
+		}
+	}
+	float bwd[3];
+	sl_mac_cell(g, ic, jc, kc, -dt, vx, vy, vz, fu, fv, fw, bwd);
+	if (!inner) bwd[0] = bwd[1] = bwd[2] = 0.f;
+	// MacCormackCorrectMAC skip logic (isMAC = true)
+	const bool sk = !(f0 & kFluid);
+	const bool skip[3] = {sk || ((i > 0) && !(fxm & kFluid)), sk || ((j > 0) && !(fym & kFluid)),
+	                      sk || (IS3D && (k > 0) && !(fzm & kFluid))};
+	// clampMode 2: no clamp unless the cell and its lower face neighbour are
+	// fluid or empty
+	const bool cf[3] = {(f0 & FM) && (fxm & FM), (f0 & FM) && (fym & FM), (f0 & FM) && (fzm & FM)};
+	float val[3];
+#pragma unroll
+	for (int c = 0; c < 3; c++) {
+		// evaluated for every lane and selected; the empty asm keeps the
+		// compiler from turning the select back into a branch around the
+		// load of org
+		float corr = (float)((double)fvv[c] + (double)strength * 0.5 * (double)(org[c] - bwd[c]));
+		asm("" : "+v"(corr));
+		float x = skip[c] ? fvv[c] : corr;
+		if (inner && (c < 2 || IS3D)) {
+			if (!cf[c]) x = fvv[c];
+			else if (x < mn[c] || x > mx[c]) x = fvv[c];
+		}
+		val[c] = x;
+	}
+	nu[ix] = val[0];
+	nv[ix] = val[1];
+	nw[ix] = val[2];
+}
+
 // ---------------------------------------------------------------------------
 // outflow extrapolation (advection.cpp:326-391): only outflow cells do work;
 // velDst is kept in a scratch Vec grid and copied back in a second pass.
@@ -689,11 +798,18 @@ void op_advect(Scene* s, Grid* flags, Grid* vel, Grid* grid, int order, float st
 			float* ng = s->getVec();
 			float* nw[3] = {ng, ng + s->cstride, ng + 2 * s->cstride};
 			KtScope kt("advect_maccormack_mac", s->stream);
+			const bool fast = clampMode == 2 && d.nx >= 3 && d.ny >= 3 && (!d.is3d || d.nz >= 3);
 			with_geo(d, [&](auto D3, auto ix) {
-				hipLaunchKernelGGL((k_maccormack_mac<decltype(D3)::value, decltype(ix)>), gb, tb, 0, s->stream, V, d,
-				                   t, dt, strength, clampMode, flags->f8, (const float*)grid->comp(0),
-				                   (const float*)grid->comp(1), (const float*)grid->comp(2), (const float*)fw[0],
-				                   (const float*)fw[1], (const float*)fw[2], nw[0], nw[1], nw[2]);
+				if (fast)
+					hipLaunchKernelGGL((k_maccormack_mac2<decltype(D3)::value, decltype(ix)>), gb, tb, 0, s->stream, V,
+					                   d, t, dt, strength, flags->f8, (const float*)grid->comp(0),
+					                   (const float*)grid->comp(1), (const float*)grid->comp(2), (const float*)fw[0],
+					                   (const float*)fw[1], (const float*)fw[2], nw[0], nw[1], nw[2]);
+				else
+					hipLaunchKernelGGL((k_maccormack_mac<decltype(D3)::value, decltype(ix)>), gb, tb, 0, s->stream, V,
+					                   d, t, dt, strength, clampMode, flags->f8, (const float*)grid->comp(0),
+					                   (const float*)grid->comp(1), (const float*)grid->comp(2), (const float*)fw[0],
+					                   (const float*)fw[1], (const float*)fw[2], nw[0], nw[1], nw[2]);
 			});
 			s->putVec(fwd);
 			result = ng;
