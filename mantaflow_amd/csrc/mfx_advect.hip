@@ -333,6 +333,114 @@ __global__ void __launch_bounds__(kBlock) k_maccormack_centered(Vel V, Dims d, T
 	for (int c = 0; c < NC; c++) dst[c][ix] = val[c];
 }
 
+// k_maccormack_centered for clampMode 2 on grids of at least 3 cells per
+// axis: the same arithmetic without divergent branches, as k_maccormack_mac2
+// (the trace runs at the interior-clamped cell and is discarded outside the
+// interior; the clamp corners load with the backward trace).  The bounds
+// take only fluid / empty corners, as getMinMax behind CHECKFLAG does: a
+// corner that fails the flag test contributes +-FLT_MAX, the seeds, which
+// leave fminf / fmaxf unchanged.  In clampMode 2 the bounds are only
+// compared with, so the sign of a zero bound (the one thing fminf / fmaxf may
+// choose differently from the strict comparisons) is never seen.
+template <int NC, bool IS3D, typename I>
+__global__ void __launch_bounds__(kBlock) k_maccormack_centered2(Vel V, Dims d, Tiles t, float dt, float strength,
+                                                                 const uint8_t* __restrict__ f8, const float* o0,
+                                                                 const float* o1, const float* o2,
+                                                                 const float* __restrict__ fw0,
+                                                                 const float* __restrict__ fw1,
+                                                                 const float* __restrict__ fw2, float* __restrict__ n0,
+                                                                 float* __restrict__ n1, float* __restrict__ n2) {
+	const CellPos p = tile_cell(t);
+	if (p.i >= d.nx || p.j >= d.ny) return;
+	const Geo<IS3D, I> g(d);
+	const I ix = g.at(p.i, p.j, p.k);
+	const bool inner = g.interior(p.i, p.j, p.k);
+	const int ic = clampi(p.i, 1, g.nx - 2), jc = clampi(p.j, 1, g.ny - 2), kc = IS3D ? clampi(p.k, 1, g.nz - 2) : p.k;
+	const I ixc = g.at(ic, jc, kc);
+	const float* orig[3] = {o0, o1, o2};
+	const float* fwd[3] = {fw0, fw1, fw2};
+	float* dst[3] = {n0, n1, n2};
+	const bool fl = f8[ix] & kFluid;
+	float fv[NC], org[NC];
+#pragma unroll
+	for (int c = 0; c < NC; c++) {
+		fv[c] = fwd[c][ix];
+		org[c] = orig[c][ix];
+	}
+	float vc[3];
+	getCentered(V, g, ixc, vc);
+	// bwd position uses -dt: pos - v*(-dt)
+	const float mdt = -dt;
+	const float px = ((float)ic + 0.5f) - vc[0] * mdt;
+	const float py = ((float)jc + 0.5f) - vc[1] * mdt;
+	const float pz = ((float)kc + 0.5f) - vc[2] * mdt;
+	// doClampComponent (advection.cpp:144-186) corners at toVec3i(pos - v dt)
+	const float vdt[3] = {vc[0] * dt, vc[1] * dt, vc[2] * dt};
+	const int P0 = (int)((float)ic - vdt[0]), P1 = (int)((float)jc - vdt[1]), P2 = (int)((float)kc - vdt[2]);
+	const int i0 = min(max(P0, 0), g.nx - 2), j0 = min(max(P1, 0), g.ny - 2);
+	const I c0 = g.at(i0, j0, IS3D ? min(max(P2, 0), g.nz - 2) : 0);
+	constexpr int NQ = IS3D ? 8 : 4;
+	const I Y = g.Y, Z = g.Z;
+	const I off[8] = {0, 1, Y, 1 + Y, Z, 1 + Z, Y + Z, 1 + Y + Z};
+	bool cf[NQ];
+	float cv[NQ][NC];
+#pragma unroll
+	for (int q = 0; q < NQ; q++) cf[q] = CHECKFLAG(c0 + off[q]);
+#pragma unroll
+	for (int q = 0; q < NQ; q += 2) {
+#pragma unroll
+		for (int c = 0; c < NC; c++) {
+			const FPair e = ld_pair(orig[c] + c0 + off[q]);
+			cv[q][c] = e.a;
+			cv[q + 1][c] = e.b;
+		}
+	}
+	float bwd[NC];
+#pragma unroll
+	for (int c = 0; c < NC; c++) bwd[c] = interp(fwd[c], g, px, py, pz);
+	bool haveFl = false;
+	float mn[NC], mx[NC];
+#pragma unroll
+	for (int c = 0; c < NC; c++) {
+		mn[c] = 3.402823466e+38f;
+		mx[c] = -3.402823466e+38f;
+	}
+#pragma unroll
+	for (int q = 0; q < NQ; q++) {
+		haveFl = haveFl || cf[q];
+#pragma unroll
+		for (int c = 0; c < NC; c++) {
+			mn[c] = fminf(mn[c], cf[q] ? cv[q][c] : 3.402823466e+38f);
+			mx[c] = fmaxf(mx[c], cf[q] ? cv[q][c] : -3.402823466e+38f);
+		}
+	}
+	float val[NC];
+	bool out = false;
+#pragma unroll
+	for (int c = 0; c < NC; c++) {
+		const float b = inner ? bwd[c] : 0.f;
+		float x;
+		if (NC == 1) {
+			// float "+=" double expression (advection.cpp:89)
+			x = (float)((double)fv[c] + (double)strength * 0.5 * (double)(org[c] - b));
+		} else {
+			// Vec3: double product narrowed per component, float add
+			x = fv[c] + (float)((double)strength * 0.5 * (double)(org[c] - b));
+		}
+		// evaluated for every lane and selected; the empty asm keeps the
+		// select from becoming a branch around the load of org
+		asm("" : "+v"(x));
+		x = fl ? x : fv[c];
+		val[c] = x;
+		out = out || (x < mn[c]) || (x > mx[c]);
+	}
+	// clamp (interior only): no fluid corner, or any component outside its
+	// bounds, reverts to the forward values
+	const bool revert = inner && (!haveFl || out);
+#pragma unroll
+	for (int c = 0; c < NC; c++) dst[c][ix] = revert ? fv[c] : val[c];
+}
+
 // doClampComponentMAC<C> (advection.cpp:191-235); vel = face velocity * dt
 template <int C, bool IS3D, typename I>
 __device__ inline float clamp_comp_mac(const Geo<IS3D, I>& g, const uint8_t* __restrict__ f8, I ix, float dst,
@@ -851,10 +959,19 @@ void op_advect(Scene* s, Grid* flags, Grid* vel, Grid* grid, int order, float st
 	if (order == 2) {
 		float* ng = nc == 3 ? s->getVec() : s->getReal();
 		float* nw[3] = {ng, nc == 3 ? ng + s->cstride : nullptr, nc == 3 ? ng + 2 * s->cstride : nullptr};
+		const bool fast = clampMode == 2 && d.nx >= 3 && d.ny >= 3 && (!d.is3d || d.nz >= 3);
 		with_geo(d, [&](auto D3, auto ix) {
 			constexpr bool D = decltype(D3)::value;
 			using I = decltype(ix);
-			if (nc == 1)
+			if (fast && nc == 1)
+				hipLaunchKernelGGL((k_maccormack_centered2<1, D, I>), gb, tb, 0, s->stream, V, d, t, dt, strength,
+				                   flags->f8, src[0], src[1], src[2], (const float*)fw[0], (const float*)fw[1],
+				                   (const float*)fw[2], nw[0], nw[1], nw[2]);
+			else if (fast)
+				hipLaunchKernelGGL((k_maccormack_centered2<3, D, I>), gb, tb, 0, s->stream, V, d, t, dt, strength,
+				                   flags->f8, src[0], src[1], src[2], (const float*)fw[0], (const float*)fw[1],
+				                   (const float*)fw[2], nw[0], nw[1], nw[2]);
+			else if (nc == 1)
 				hipLaunchKernelGGL((k_maccormack_centered<1, D, I>), gb, tb, 0, s->stream, V, d, t, dt, strength,
 				                   clampMode, flags->f8, src[0], src[1], src[2], (const float*)fw[0],
 				                   (const float*)fw[1], (const float*)fw[2], nw[0], nw[1], nw[2]);
