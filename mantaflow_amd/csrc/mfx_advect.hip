@@ -720,17 +720,24 @@ __global__ void __launch_bounds__(kBlock) k_outflow_extrap(Dims d, Tiles t, cons
 	dw[ix] = vd[2];
 }
 
+// kLightKZ planes per thread (tiles_for(d, kLightKZ)): the flags of all
+// planes load first and only outflow cells copy.
 __global__ void __launch_bounds__(kBlock) k_outflow_copy(Dims d, Tiles t, const uint8_t* __restrict__ f8,
                                                          const float* __restrict__ du, const float* __restrict__ dv,
                                                          const float* __restrict__ dw, float* __restrict__ vu,
                                                          float* __restrict__ vv, float* __restrict__ vw) {
-	CellPos p = tile_cell(t);
+	const CellPos p = tile_cell(t);
 	if (p.i >= d.nx || p.j >= d.ny) return;
-	const long long ix = d.idx(p.i, p.j, p.k);
-	if (!(f8[ix] & kOutflow)) return;
-	vu[ix] = du[ix];
-	vv[ix] = dv[ix];
-	vw[ix] = dw[ix];
+	uint8_t fl[kLightKZ];
+#pragma unroll
+	for (int kk = 0; kk < kLightKZ; kk++) fl[kk] = p.k + kk < d.nz ? f8[d.idx(p.i, p.j, p.k + kk)] : (uint8_t)0;
+	for (int kk = 0; kk < kLightKZ; kk++) {
+		if (!(fl[kk] & kOutflow)) continue;
+		const long long ix = d.idx(p.i, p.j, p.k + kk);
+		vu[ix] = du[ix];
+		vv[ix] = dv[ix];
+		vw[ix] = dw[ix];
+	}
 }
 
 // ---------------------------------------------------------------------------
@@ -929,10 +936,12 @@ void op_advect(Scene* s, Grid* flags, Grid* vel, Grid* grid, int order, float st
 			float* vdst = s->getVec();
 			float* r3[3] = {result, result + s->cstride, result + 2 * s->cstride};
 			float* d3[3] = {vdst, vdst + s->cstride, vdst + 2 * s->cstride};
+			const Tiles tl = tiles_for(d, kLightKZ);
+			const dim3 gl((unsigned)tl.count());
 			hipLaunchKernelGGL(k_outflow_extrap, gb, tb, 0, s->stream, d, t, flags->f8, (const float*)r3[0],
 			                   (const float*)r3[1], (const float*)r3[2], (const float*)grid->comp(0),
 			                   (const float*)grid->comp(1), (const float*)grid->comp(2), ts, d3[0], d3[1], d3[2]);
-			hipLaunchKernelGGL(k_outflow_copy, gb, tb, 0, s->stream, d, t, flags->f8, (const float*)d3[0],
+			hipLaunchKernelGGL(k_outflow_copy, gl, tb, 0, s->stream, d, tl, flags->f8, (const float*)d3[0],
 			                   (const float*)d3[1], (const float*)d3[2], r3[0], r3[1], r3[2]);
 			s->putVec(vdst);
 		}

@@ -540,17 +540,17 @@ __global__ void k_set_open_bound(int* f, Dims d, int bw, int lox, int loy, int l
 	}
 }
 
+// The uint8 mirror f8 holds the outflow bit, so the scan reads 1 B per cell
+// and touches the int32 flags only at outflow cells.
 __global__ void k_reset_outflow(int* f, uint8_t* f8, float* phi, float* real, long long n) {
 	for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
 	     i += (long long)gridDim.x * blockDim.x) {
-		int v = f[i];
-		if (v & kOutflow) {
-			v = (v | kEmpty) & ~kFluid;
-			f[i] = v;
-			f8[i] = (uint8_t)(v & 0x7f);
-			if (phi) phi[i] = 0.5f;
-			if (real) real[i] = 0.f;
-		}
+		if (!(f8[i] & kOutflow)) continue;
+		const int v = (f[i] | kEmpty) & ~kFluid;
+		f[i] = v;
+		f8[i] = (uint8_t)(v & 0x7f);
+		if (phi) phi[i] = 0.5f;
+		if (real) real[i] = 0.f;
 	}
 }
 
