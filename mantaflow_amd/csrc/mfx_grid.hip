@@ -161,11 +161,23 @@ enum { EW_SET = 0, EW_ADDC, EW_MULC, EW_ADD, EW_SUB, EW_MUL, EW_ADDSCALED, EW_CL
 template <typename T>
 __global__ void k_elementwise(T* __restrict__ a, const T* __restrict__ b, long long n, int op, T v,
                               T v2) {
+	// set and copy do not read the old values (b is null for a set)
+	if (op == EW_SET) {
+		for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+		     i += (long long)gridDim.x * blockDim.x)
+			a[i] = v;
+		return;
+	}
+	if (op == EW_COPY) {
+		for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+		     i += (long long)gridDim.x * blockDim.x)
+			a[i] = b[i];
+		return;
+	}
 	for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
 	     i += (long long)gridDim.x * blockDim.x) {
 		T x = a[i];
 		switch (op) {
-		case EW_SET: x = v; break;
 		case EW_ADDC: x += v; break;
 		case EW_MULC: x *= v; break;
 		case EW_ADD: x += b[i]; break;
@@ -173,10 +185,48 @@ __global__ void k_elementwise(T* __restrict__ a, const T* __restrict__ b, long l
 		case EW_MUL: x *= b[i]; break;
 		case EW_ADDSCALED: x += v * b[i]; break;  // gridScaledAdd: me += factor * other
 		case EW_CLAMP: x = x < v ? v : (x > v2 ? v2 : x); break;
-		case EW_COPY: x = b[i]; break;
 		}
 		a[i] = x;
 	}
+}
+
+// Four floats per element: the same per-value operation as k_elementwise,
+// for float grids whose size is a multiple of 4 and whose planes are 16-byte
+// aligned (every pooled grid is).
+__device__ inline float ew_op(float x, float bv, int op, float v, float v2) {
+	switch (op) {
+	case EW_ADDC: return x + v;
+	case EW_MULC: return x * v;
+	case EW_ADD: return x + bv;
+	case EW_SUB: return x - bv;
+	case EW_MUL: return x * bv;
+	case EW_ADDSCALED: return x + v * bv;
+	case EW_CLAMP: return x < v ? v : (x > v2 ? v2 : x);
+	}
+	return x;
+}
+__global__ void k_elementwise4(float4* __restrict__ a, const float4* __restrict__ b, long long n4, int op, float v,
+                               float v2) {
+	for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n4;
+	     i += (long long)gridDim.x * blockDim.x) {
+		if (op == EW_SET) {
+			a[i] = make_float4(v, v, v, v);
+		} else if (op == EW_COPY) {
+			a[i] = b[i];
+		} else {
+			const float4 x = a[i];
+			const float4 y = b ? b[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+			a[i] = make_float4(ew_op(x.x, y.x, op, v, v2), ew_op(x.y, y.y, op, v, v2), ew_op(x.z, y.z, op, v, v2),
+			                   ew_op(x.w, y.w, op, v, v2));
+		}
+	}
+}
+static void launch_elementwise_f(Scene* s, float* a, const float* b, long long n, int op, float v, float v2) {
+	if (n % 4 == 0 && ((uintptr_t)a | (uintptr_t)b) % 16 == 0)
+		hipLaunchKernelGGL(k_elementwise4, dim3(flat_blocks(n / 4)), dim3(256), 0, s->stream, (float4*)a,
+		                   (const float4*)b, n / 4, op, v, v2);
+	else
+		hipLaunchKernelGGL(k_elementwise<float>, dim3(flat_blocks(n)), dim3(256), 0, s->stream, a, b, n, op, v, v2);
 }
 
 static void elementwise(Grid* g, const Grid* other, int op, const float* v, const float* v2) {
@@ -192,8 +242,7 @@ static void elementwise(Grid* g, const Grid* other, int op, const float* v, cons
 	if (g->isVec()) {
 		for (int c = 0; c < 3; c++) {
 			float a = v ? v[c] : 0.f, b = v2 ? v2[c] : 0.f;
-			hipLaunchKernelGGL(k_elementwise<float>, dim3(nb), dim3(256), 0, s->stream, g->comp(c),
-			                   other ? (const float*)other->comp(c) : nullptr, n, op, a, b);
+			launch_elementwise_f(s, g->comp(c), other ? (const float*)other->comp(c) : nullptr, n, op, a, b);
 		}
 	} else if (g->isInt()) {
 		int a = v ? (int)v[0] : 0, b = v2 ? (int)v2[0] : 0;
@@ -202,8 +251,7 @@ static void elementwise(Grid* g, const Grid* other, int op, const float* v, cons
 		if (g->isFlags()) flags_repack(g);
 	} else {
 		float a = v ? v[0] : 0.f, b = v2 ? v2[0] : 0.f;
-		hipLaunchKernelGGL(k_elementwise<float>, dim3(nb), dim3(256), 0, s->stream, g->real(),
-		                   other ? (const float*)other->real() : nullptr, n, op, a, b);
+		launch_elementwise_f(s, g->real(), other ? (const float*)other->real() : nullptr, n, op, a, b);
 	}
 }
 

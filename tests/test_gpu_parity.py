@@ -391,3 +391,39 @@ def test_plume3d_open_top_mg(manta, oracle):
     print("3D plume 48^3, 8 steps: (oracle, GPU) MG-PCG iterations", its,
           "max |density diff| %.3g, max |velocity diff| %.3g (max |vel| %.3g)" %
           (np.abs(gd - dens).max(), np.abs(gv - vel).max(), np.abs(vel).max()))
+
+
+@pytest.mark.parametrize("shape", [(16, 12, 10), (7, 5, 3)])
+@pytest.mark.parametrize("kind", ["real", "vec"])
+def test_grid_elementwise_ops(manta, rng, shape, kind):
+    """Grid<T> arithmetic (grid.h:96-213: setConst, addConst, multConst, add,
+    sub, mult, addScaled, clamp) against numpy in float32.  16*12*10 cells
+    takes the four-floats-per-element kernel, 7*5*3 (not a multiple of 4) the
+    one-float kernel; every operation is one IEEE float operation per value,
+    so results are bit-exact."""
+    n = int(np.prod(shape))
+    sc = GpuScene(manta, shape)
+    cols = 3 if kind == "vec" else 1
+    a0 = rng.uniform(-2, 2, (n, cols) if cols == 3 else n).astype(np.float32)
+    b0 = rng.uniform(-2, 2, a0.shape).astype(np.float32)
+    mk = sc.vec if kind == "vec" else sc.real
+    c = manta.vec3(0.75, -1.5, 0.25) if kind == "vec" else 0.75
+    cv = np.array([0.75, -1.5, 0.25], np.float32) if kind == "vec" else np.float32(0.75)
+    fac = manta.vec3(0.5, 2.0, -1.0) if kind == "vec" else 0.5
+    fv = np.array([0.5, 2.0, -1.0], np.float32) if kind == "vec" else np.float32(0.5)
+    cases = [
+        ("setConst", lambda g, h: g.setConst(c), lambda a, b: np.broadcast_to(cv, a.shape)),
+        ("addConst", lambda g, h: g.addConst(c), lambda a, b: a + cv),
+        ("multConst", lambda g, h: g.multConst(c), lambda a, b: a * cv),
+        ("add", lambda g, h: g.add(h), lambda a, b: a + b),
+        ("sub", lambda g, h: g.sub(h), lambda a, b: a - b),
+        ("mult", lambda g, h: g.mult(h), lambda a, b: a * b),
+        ("addScaled", lambda g, h: g.addScaled(h, fac), lambda a, b: a + (fv * b).astype(np.float32)),
+        ("copyFrom", lambda g, h: g.copyFrom(h), lambda a, b: b),
+    ]
+    if kind == "real":
+        cases.append(("clamp", lambda g, h: g.clamp(-0.5, 1.25), lambda a, b: np.where(a < np.float32(-0.5), np.float32(-0.5), np.where(a > np.float32(1.25), np.float32(1.25), a))))
+    for name, op, ref in cases:
+        g, h = mk(a0), mk(b0)
+        op(g, h)
+        assert_bitexact(g.toNumpy(), np.asarray(ref(a0, b0), np.float32).reshape(a0.shape), name)
