@@ -674,9 +674,19 @@ __device__ __forceinline__ void body_finalize(const FinP& P, long long, long lon
 	if (st->iter >= st->maxIter) st->done = 1;
 }
 
+// Three consecutive floats at a 4-byte aligned address, read by one load.
+struct __attribute__((packed, aligned(4))) FTriple3 {
+	float a, b, c;
+};
+
 // true residual ||b - A x||_2^2 with FP64 accumulation of the row sum;
 // kLightKZ planes per thread, all loads issued first (linear neighbours
-// outside [0, n) are replaced by the cell itself and never used)
+// outside [0, n) are replaced by the cell itself and never used).  The
+// x-neighbours come from one 12-byte load at clamp(c - 1, 0, n - 3) (TRI:
+// grids of at least 3 cells); the
+// z-neighbours of inner planes are the thread's own adjacent planes, so only
+// the first plane's lower and the last plane's upper neighbour are loaded.
+template <bool TRI>
 __global__ void __launch_bounds__(kBlock) k_true_residual(Dims d, Tiles t, const uint8_t* __restrict__ code,
                                                           const float* __restrict__ b, const float* __restrict__ x,
                                                           double* __restrict__ part) {
@@ -688,25 +698,45 @@ __global__ void __launch_bounds__(kBlock) k_true_residual(Dims d, Tiles t, const
 		const long long X = 1, Y = d.nx, Z = d.sz, n = d.n;
 		uint8_t cc[KZ], cxm[KZ], cym[KZ], czm[KZ];
 		float bc[KZ], xc[KZ], xxm[KZ], xxp[KZ], xym[KZ], xyp[KZ], xzm[KZ], xzp[KZ];
+		long long cl[KZ];
 #pragma unroll
 		for (int q = 0; q < KZ; q++) {
 			const int k = min(p.k + q, d.nz - 1);
 			const long long c = d.idx(p.i, p.j, k);
-			const long long mx = c - X >= 0 ? c - X : c, px = c + X < n ? c + X : c;
+			cl[q] = c;
+			const long long mx = c - X >= 0 ? c - X : c;
 			const long long my = c - Y >= 0 ? c - Y : c, py = c + Y < n ? c + Y : c;
-			const long long mz = c - Z >= 0 ? c - Z : c, pz = c + Z < n ? c + Z : c;
+			if (TRI) {
+				const long long s0 = min(max(c - 1, 0ll), n - 3);
+				const FTriple3 tr = *reinterpret_cast<const FTriple3*>(x + s0);
+				const long long o = c - s0;  // slot of c: 0 (c = 0), 1, or 2 (c = n - 1)
+				xxm[q] = o == 2 ? tr.b : tr.a;
+				xc[q] = o == 0 ? tr.a : (o == 1 ? tr.b : tr.c);
+				xxp[q] = o == 0 ? tr.b : tr.c;
+			} else {
+				xxm[q] = x[mx];
+				xc[q] = x[c];
+				xxp[q] = x[c + X < n ? c + X : c];
+			}
 			cc[q] = code[c];
 			cxm[q] = code[mx];
 			cym[q] = code[my];
-			czm[q] = code[mz];
 			bc[q] = b[c];
-			xc[q] = x[c];
-			xxm[q] = x[mx];
-			xxp[q] = x[px];
 			xym[q] = x[my];
 			xyp[q] = x[py];
-			xzm[q] = x[mz];
-			xzp[q] = x[pz];
+		}
+		{
+			const long long mz = cl[0] - Z >= 0 ? cl[0] - Z : cl[0];
+			const long long pz = cl[KZ - 1] + Z < n ? cl[KZ - 1] + Z : cl[KZ - 1];
+			czm[0] = code[mz];
+			xzm[0] = x[mz];
+			xzp[KZ - 1] = x[pz];
+		}
+#pragma unroll
+		for (int q = 1; q < KZ; q++) {
+			czm[q] = cc[q - 1];
+			xzm[q] = xc[q - 1];
+			xzp[q - 1] = xc[q];
 		}
 #pragma unroll
 		for (int q = 0; q < KZ; q++) {
@@ -1157,8 +1187,12 @@ static void cg_iteration(CgRun& R) {
 static void cg_finish_enqueue(CgRun& R) {
 	Scene* s = R.s;
 	const Tiles tr = tiles_for(s->d, kLightKZ);
-	hipLaunchKernelGGL(k_true_residual, dim3((unsigned)tr.count()), dim3(kBlock), 0, s->stream, s->d, tr, R.code,
-	                   (const float*)R.rhs, (const float*)R.x, s->dpart);
+	if (s->d.n >= 3)
+		hipLaunchKernelGGL(k_true_residual<true>, dim3((unsigned)tr.count()), dim3(kBlock), 0, s->stream, s->d, tr,
+		                   R.code, (const float*)R.rhs, (const float*)R.x, s->dpart);
+	else
+		hipLaunchKernelGGL(k_true_residual<false>, dim3((unsigned)tr.count()), dim3(kBlock), 0, s->stream, s->d, tr,
+		                   R.code, (const float*)R.rhs, (const float*)R.x, s->dpart);
 	hipLaunchKernelGGL(k_reduce_true_rr, dim3(1), dim3(1024), 0, s->stream, R.st, (const double*)s->dpart,
 	                   (int)tr.count());
 	MFX_HIP(hipMemcpyAsync(s->hscal, R.st, sizeof(CgState), hipMemcpyDeviceToHost, s->stream));
