@@ -829,18 +829,28 @@ __global__ void __launch_bounds__(kBlock) k_add_buoyancy(Dims d, Tiles t, const 
 	for (int q = 0; q < KZ; q++) {
 		const int k = min(p.k + q, d.nz - 1);
 		const long long ix = d.idx(i, j, k);
-		const long long oz = k > 0 ? d.sz : 0;
 		f[q] = f8[ix];
 		gx[q] = f8[ix - ox];
 		gy[q] = f8[ix - oy];
-		gz[q] = f8[ix - oz];
 		dc[q] = den[ix];
 		dxm[q] = den[ix - ox];
 		dym[q] = den[ix - oy];
-		dzm[q] = den[ix - oz];
 		uc[q] = u[ix];
 		vc[q] = v[ix];
 		wc[q] = w[ix];
+	}
+	// z-neighbours: the plane below the first is loaded; for the others it is
+	// the thread's previous plane (planes past nz are clamped and never used)
+	{
+		const int k0 = min(p.k, d.nz - 1);
+		const long long c0 = d.idx(i, j, k0) - (k0 > 0 ? d.sz : 0);
+		gz[0] = f8[c0];
+		dzm[0] = den[c0];
+	}
+#pragma unroll
+	for (int q = 1; q < KZ; q++) {
+		gz[q] = f[q - 1];
+		dzm[q] = dc[q - 1];
 	}
 #pragma unroll
 	for (int q = 0; q < KZ; q++) {

@@ -221,18 +221,28 @@ __global__ void __launch_bounds__(kBlock) k_correct_velocity(Dims d, Tiles t, co
 	for (int q = 0; q < KZ; q++) {
 		const int k = min(p.k + q, d.nz - 1);
 		const long long c = d.idx(i, j, k);
-		const long long oz = k > 0 ? d.sz : 0;
 		f[q] = f8[c];
 		gx[q] = f8[c - ox];
 		gy[q] = f8[c - oy];
-		gz[q] = f8[c - oz];
 		pc[q] = pr[c];
 		px[q] = pr[c - ox];
 		py[q] = pr[c - oy];
-		pz[q] = pr[c - oz];
 		uc[q] = u[c];
 		vc[q] = v[c];
 		wc[q] = w[c];
+	}
+	// z-neighbours: the plane below the first is loaded; for the others it is
+	// the thread's previous plane (planes past nz are clamped and never used)
+	{
+		const int k0 = min(p.k, d.nz - 1);
+		const long long c0 = d.idx(i, j, k0) - (k0 > 0 ? d.sz : 0);
+		gz[0] = f8[c0];
+		pz[0] = pr[c0];
+	}
+#pragma unroll
+	for (int q = 1; q < KZ; q++) {
+		gz[q] = f[q - 1];
+		pz[q] = pc[q - 1];
 	}
 #pragma unroll
 	for (int q = 0; q < KZ; q++) {
