@@ -42,9 +42,11 @@ def _stats_struct(real):
 
 
 class Oracle:
-    def __init__(self, f64=False):
+    def __init__(self, f64=False, path=None):
+        """``path`` loads another build of the float oracle (the planted-mistake
+        builds of the oracle's own tests)."""
         name = "liboracle_f64.so" if f64 else "liboracle.so"
-        path = os.path.join(_HERE, name)
+        path = path or os.path.join(_HERE, name)
         if not os.path.exists(path):
             raise RuntimeError("oracle library %s is not built (run make in %s)" % (name, _HERE))
         self.lib = C.CDLL(path)

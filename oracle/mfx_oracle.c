@@ -35,6 +35,26 @@
 #define OG_VEC_EPS 1e-6f
 #endif
 
+/* Planted mistakes for the oracle's own tests (tests/test_oracle_mutants.py):
+ * built with -DOG_MUTANT=k, the library carries mistake k of the list below,
+ * and the pinning tests must reject it.  The normal builds have OG_MUTANT 0.
+ *   1  getAtMACX: y neighbour taken at i+1 instead of i-1
+ *   2  getAtMACZ: x neighbour taken at k+1 instead of k-1
+ *   3  MacCormackCorrectMAC: no skip for a non-fluid lower x neighbour
+ *   4  doClampComponentMAC, clampMode 1: min and max bounds swapped
+ *   5  doClampComponent (centered), clampMode 1: min and max bounds swapped
+ *   6  SemiLagrangeMAC: component lookup shifted back by half a cell
+ *   7  extrapolateVelConvectiveBC: adds vel at the fluid cell, not at low/up
+ *   8  extrapolateVelConvectiveBC: bulk x velocity used for every axis
+ *   9  KnSetWallBcs: obstacle cell above a fluid cell not set
+ *  10  KnSetWallBcs: a stick x neighbour zeroes x instead of y and z
+ *  11  getCentered: y average with the lower neighbour
+ *  12  doClampComponentMAC, clampMode 2: flag check on the upper neighbour */
+#ifndef OG_MUTANT
+#define OG_MUTANT 0
+#endif
+#define OG_MUT(k) (OG_MUTANT == (k))
+
 static int g_threads = 1;
 void og_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
 int og_get_threads(void) { return g_threads; }
@@ -244,15 +264,16 @@ og_real og_interpol_component(og_dims d, const og_real* v, int c, og_real px, og
 static inline void getCentered(og_dims d, const og_real* v, int i, int j, int k, og_real* o) {
 	long ix = IX(d, i, j, k);
 	o[0] = (og_real)(0.5 * (v[3 * ix] + v[3 * (ix + 1)]));
-	o[1] = (og_real)(0.5 * (v[3 * ix + 1] + v[3 * (ix + d.nx) + 1]));
+	o[1] = (og_real)(0.5 * (v[3 * ix + 1] + v[3 * (ix + (OG_MUT(11) ? -d.nx : d.nx)) + 1]));
 	o[2] = 0.;
 	if (d.is3d) o[2] = (og_real)(0.5 * (v[3 * ix + 2] + v[3 * (ix + SZ(d)) + 2]));
 }
 static inline void getAtMACX(og_dims d, const og_real* v, int i, int j, int k, og_real* o) {
 	long ix = IX(d, i, j, k), Y = d.nx, Z = SZ(d);
+	const long m1 = OG_MUT(1) ? 1 : -1;
 	o[0] = v[3 * ix];
-	o[1] = (og_real)(0.25 * (v[3 * ix + 1] + v[3 * (ix - 1) + 1] + v[3 * (ix + Y) + 1] +
-	                         v[3 * (ix + Y - 1) + 1]));
+	o[1] = (og_real)(0.25 * (v[3 * ix + 1] + v[3 * (ix + m1) + 1] + v[3 * (ix + Y) + 1] +
+	                         v[3 * (ix + Y + m1) + 1]));
 	o[2] = 0.;
 	if (d.is3d)
 		o[2] = (og_real)(0.25 * (v[3 * ix + 2] + v[3 * (ix - 1) + 2] + v[3 * (ix + Z) + 2] +
@@ -270,7 +291,8 @@ static inline void getAtMACY(og_dims d, const og_real* v, int i, int j, int k, o
 /* note: not guarded by is3D in the reference, Z = 0 in 2D */
 static inline void getAtMACZ(og_dims d, const og_real* v, int i, int j, int k, og_real* o) {
 	long ix = IX(d, i, j, k), Y = d.nx, Z = SZ(d);
-	o[0] = (og_real)(0.25 * (v[3 * ix] + v[3 * (ix - Z)] + v[3 * (ix + 1)] + v[3 * (ix + 1 - Z)]));
+	const long mz = OG_MUT(2) ? Z : -Z;
+	o[0] = (og_real)(0.25 * (v[3 * ix] + v[3 * (ix + mz)] + v[3 * (ix + 1)] + v[3 * (ix + 1 + mz)]));
 	o[1] = (og_real)(0.25 * (v[3 * ix + 1] + v[3 * (ix - Z) + 1] + v[3 * (ix + Y) + 1] +
 	                         v[3 * (ix + Y - Z) + 1]));
 	o[2] = v[3 * ix + 2];
@@ -315,7 +337,9 @@ void og_sl_mac(og_dims d, const og_real* vel, og_real* dst, const og_real* src, 
 				        p0z = (og_real)((float)k + 0.5f);
 				og_real v[3], out[3];
 				getAtMACX(d, vel, i, j, k, v);
+				if (OG_MUT(6)) p0x -= 0.5f;
 				out[0] = interp_core(d, src + 0, 3, p0x - v[0] * dt, p0y - v[1] * dt, p0z - v[2] * dt);
+				if (OG_MUT(6)) p0x += 0.5f;
 				getAtMACY(d, vel, i, j, k, v);
 				out[1] = interp_core(d, src + 1, 3, p0x - v[0] * dt, p0y - v[1] * dt, p0z - v[2] * dt);
 				getAtMACZ(d, vel, i, j, k, v);
@@ -356,7 +380,7 @@ void og_mac_cormack_correct(og_dims d, const int* flags, og_real* dst, const og_
 					long ix = IX(d, i, j, k);
 					int skip[3] = {0, 0, 0};
 					if (!isFluid(flags, ix)) skip[0] = skip[1] = skip[2] = 1;
-					if ((i > 0) && (!isFluid(flags, IX(d, i - 1, j, k)))) skip[0] = 1;
+					if (!OG_MUT(3) && (i > 0) && (!isFluid(flags, IX(d, i - 1, j, k)))) skip[0] = 1;
 					if ((j > 0) && (!isFluid(flags, IX(d, i, j - 1, k)))) skip[1] = 1;
 					if ((k > 0) && (!isFluid(flags, IX(d, i, j, k - 1)))) skip[2] = 1;
 					for (int c = 0; c < 3; ++c) {
@@ -427,7 +451,8 @@ void og_mac_cormack_clamp(og_dims d, const int* flags, const og_real* vel, og_re
 				if (!haveFl) {
 					for (int c = 0; c < ncomp; c++) res[c] = fv[c];
 				} else if (clampMode == 1) {
-					for (int c = 0; c < ncomp; c++) res[c] = clampr(dval[c], minv[c], maxv[c]);
+					for (int c = 0; c < ncomp; c++)
+						res[c] = OG_MUT(5) ? clampr(dval[c], maxv[c], minv[c]) : clampr(dval[c], minv[c], maxv[c]);
 				} else {
 					int out = 0;
 					for (int c = 0; c < ncomp; c++) out |= (dval[c] < minv[c]) | (dval[c] > maxv[c]);
@@ -466,7 +491,7 @@ static og_real clamp_comp_mac(og_dims d, const int* flags, int c, og_real dst, c
 	}
 	int o[3] = {(int)pos[0], (int)pos[1], (int)pos[2]};
 	int nb[3] = {o[0], o[1], o[2]};
-	nb[c] -= 1;
+	nb[c] += OG_MUT(12) ? 1 : -1;
 	if (clampMode == 2 &&
 	    (!(CHECKFLAG(flags, IX(d, o[0], o[1], o[2])) && CHECKFLAG(flags, IX(d, nb[0], nb[1], nb[2])))))
 		return fwd;
@@ -485,7 +510,7 @@ static og_real clamp_comp_mac(og_dims d, const int* flags, int c, og_real dst, c
 		}
 	}
 	if (clampMode == 1) {
-		dst = clampr(dst, minv, maxv);
+		dst = OG_MUT(4) ? clampr(dst, maxv, minv) : clampr(dst, minv, maxv);
 	} else {
 		if (dst < minv || dst > maxv) dst = fwd;
 	}
@@ -562,7 +587,8 @@ void og_apply_outflow_bc(og_dims d, const int* flags, og_real* vel, const og_rea
 				for (int c = 0; c < dim; c++) {
 					int low[3], up[3], flLow[3], flUp[3];
 					for (int a = 0; a < 3; a++) low[a] = up[a] = flLow[a] = flUp[a] = cur[a];
-					og_real mb = bulk[c] > (og_real)1.0 ? bulk[c] : (og_real)1.0; /* max((Real)1.0, bulk) */
+					const int bc = OG_MUT(8) ? 0 : c;
+					og_real mb = bulk[bc] > (og_real)1.0 ? bulk[bc] : (og_real)1.0; /* max((Real)1.0, bulk) */
 					og_real factor = timeStep * mb;
 					low[c] = flLow[c] = cur[c] - 1;
 					up[c] = flUp[c] = cur[c] + 1;
@@ -573,13 +599,13 @@ void og_apply_outflow_bc(og_dims d, const int* flags, og_real* vel, const og_rea
 						                isFluid(flags, IX(d, flUp[0], flUp[1], flUp[2]));
 						if (fromLower || fromUpper) {
 							if (fromLower) {
-								long q = IX(d, low[0], low[1], low[2]);
+								long q = OG_MUT(7) ? IX(d, flLow[0], flLow[1], flLow[2]) : IX(d, low[0], low[1], low[2]);
 								for (int a = 0; a < 3; a++)
 									vd[a] += ((vel[3 * ix + a] - velPrev[3 * ix + a]) / factor) + vel[3 * q + a];
 								cnt++;
 							}
 							if (fromUpper) {
-								long q = IX(d, up[0], up[1], up[2]);
+								long q = OG_MUT(7) ? IX(d, flUp[0], flUp[1], flUp[2]) : IX(d, up[0], up[1], up[2]);
 								for (int a = 0; a < 3; a++)
 									vd[a] += ((vel[3 * ix + a] - velPrev[3 * ix + a]) / factor) + vel[3 * q + a];
 								cnt++;
@@ -659,7 +685,7 @@ void og_set_wall_bcs(og_dims d, const int* flags, og_real* vel, const og_real* o
 				}
 				og_real* v = vel + 3 * ix;
 				if (i > 0 && isObs(flags, IX(d, i - 1, j, k))) v[0] = bc[0];
-				if (i > 0 && curObs && isFluid(flags, IX(d, i - 1, j, k))) v[0] = bc[0];
+				if (!OG_MUT(9) && i > 0 && curObs && isFluid(flags, IX(d, i - 1, j, k))) v[0] = bc[0];
 				if (j > 0 && isObs(flags, IX(d, i, j - 1, k))) v[1] = bc[1];
 				if (j > 0 && curObs && isFluid(flags, IX(d, i, j - 1, k))) v[1] = bc[1];
 				if (!d.is3d) {
@@ -670,8 +696,10 @@ void og_set_wall_bcs(og_dims d, const int* flags, og_real* vel, const og_real* o
 				}
 				if (curFluid) {
 					if ((i > 0 && isStick(flags, IX(d, i - 1, j, k))) ||
-					    (i < d.nx - 1 && isStick(flags, IX(d, i + 1, j, k))))
-						v[1] = v[2] = 0;
+					    (i < d.nx - 1 && isStick(flags, IX(d, i + 1, j, k)))) {
+						if (OG_MUT(10)) v[0] = 0;
+						else v[1] = v[2] = 0;
+					}
 					if ((j > 0 && isStick(flags, IX(d, i, j - 1, k))) ||
 					    (j < d.ny - 1 && isStick(flags, IX(d, i, j + 1, k))))
 						v[0] = v[2] = 0;
