@@ -129,6 +129,10 @@ struct ZmTune {
 };
 ZmTune zm_tuned(const Dims& d);
 void zm_set_tuned(const Dims& d, ZmTune t);
+// claims the tuning of a shape for the calling thread: false if the shape is
+// tuned already or another thread is tuning it (that thread publishes the
+// winners with zm_set_tuned; the others use the heuristic chunk meanwhile)
+bool zm_claim_tuning(const Dims& d);
 
 // Dot products of the z-marching kernels (fl(q p) of the matvec, fl(z r) of
 // the level-0 up-sweep) leave one partial per tile column and group of
@@ -179,7 +183,7 @@ __device__ inline void zm_dot_store(const double (*tab)[TY], const Dims& d, long
 // tiles of each CG kernel for one problem (and the plane chunk / range
 // size); tuned = the autotuned chunk of the shape where there is one (single
 // solves), else the heuristic (batched solves)
-long long matvec_tiles(const Dims& d, int* kchunk, bool tuned);
+long long matvec_tiles(const Dims& d, int* kchunk, bool tuned, int force = 0);
 long long update_tiles(const Dims& d, long long* per);
 inline long long dot_tiles(const Dims& d) { return (d.n + kBlock - 1) / kBlock; }
 
@@ -282,7 +286,9 @@ struct MgHierarchy::Steps {
 };
 
 // tiles of the V-cycle kernels for one problem/level
-long long l0_tiles(const Dims& d, int* kchunk, bool tuned);  // z-marching (3D) or brick (2D)
+// force > 0 uses that plane chunk (the autotuner's trials) instead of the
+// heuristic or tuned one
+long long l0_tiles(const Dims& d, int* kchunk, bool tuned, int force = 0);  // z-marching (3D) or brick (2D)
 long long lc_tiles(const LvDev& c);               // 64x4 vertex tiles per plane
 long long lcblk_tiles(const LvDev& c, bool is3d); // smoother bricks
 inline long long coarsest_tiles(int nc) { return (nc + 3) / 4; }

@@ -1,8 +1,11 @@
 // Scenes, grids, element-wise grid operations, reductions, flag-grid setup
 // and shape rasterisation of libmfx.
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <mutex>
+#include <set>
 
 #include "mfx_internal.h"
 #include "mfx_mg.h"
@@ -31,6 +34,9 @@ void set_last_error(const std::string& s) { g_err = s; }
 Scene::~Scene() {
 	if (stream) hipStreamSynchronize(stream);
 	mgStatic.reset();
+	mgMic.reset();
+	if (dhash) hipFree(dhash);
+	if (hhash) hipHostFree(hhash);
 	for (void* p : owned) hipFree(p);
 	if (dpart) hipFree(dpart);
 	if (fpart) hipFree(fpart);
@@ -111,8 +117,12 @@ __global__ void k_pack_flags(const int* __restrict__ f, uint8_t* __restrict__ f8
 		f8[i] = (uint8_t)(f[i] & 0x7f);
 }
 
+static std::atomic<uint64_t> g_flag_gen{0};
+void flags_touched(Grid* g) { g->gen = ++g_flag_gen; }
+
 void flags_repack(Grid* g) {
 	Scene* s = g->s;
+	flags_touched(g);
 	hipLaunchKernelGGL(k_pack_flags, dim3(flat_blocks(s->d.n)), dim3(256), 0, s->stream,
 	                   (const int*)g->data, g->f8, s->d.n);
 }
@@ -741,6 +751,13 @@ int mfx_device_memory(size_t* freeBytes, size_t* totalBytes) {
 	API_END
 }
 
+static std::mutex g_scenes_mu;
+static std::set<Scene*> g_scenes;
+std::vector<Scene*> live_scenes() {
+	std::lock_guard<std::mutex> g(g_scenes_mu);
+	return std::vector<Scene*>(g_scenes.begin(), g_scenes.end());
+}
+
 int mfx_scene_create(int nx, int ny, int nz, int dim, mfx_scene** out) {
 	API_BEGIN
 	MFX_CHECK(dim == 2 || dim == 3, "Only 2D and 3D solvers allowed.");
@@ -765,6 +782,10 @@ int mfx_scene_create(int nx, int ny, int nz, int dim, mfx_scene** out) {
 		delete s;
 		throw;
 	}
+	{
+		std::lock_guard<std::mutex> g(g_scenes_mu);
+		g_scenes.insert(s);
+	}
 	*out = static_cast<mfx_scene*>(s);
 	API_END
 }
@@ -773,6 +794,10 @@ int mfx_scene_destroy(mfx_scene* s) {
 	API_BEGIN
 	if (!s) return 0;
 	Scene* sc = S(s);
+	{
+		std::lock_guard<std::mutex> g(g_scenes_mu);
+		g_scenes.erase(sc);
+	}
 	while (!sc->grids.empty()) free_grid(sc->grids.back());
 	delete static_cast<mfx_scene*>(s);
 	API_END
@@ -914,6 +939,10 @@ int mfx_grid_swap(mfx_grid* x, mfx_grid* y) {
 	MFX_CHECK(a->isVec() == b->isVec() && a->isFlags() == b->isFlags(), "Grid::swap(): grid types differ.");
 	std::swap(a->data, b->data);
 	std::swap(a->f8, b->f8);
+	if (a->isFlags()) {
+		flags_touched(a);
+		flags_touched(b);
+	}
 	API_END
 }
 
@@ -1165,6 +1194,7 @@ int mfx_reset_outflow(mfx_grid* gg, mfx_grid* phi, mfx_grid* real) {
 	MFX_CHECK(!phi || G(phi)->isReal(), "resetOutflow: phi must be a Real grid");
 	MFX_CHECK(!real || G(real)->isReal(), "resetOutflow: real must be a Real grid");
 	Scene* s = g->s;
+	flags_touched(g);
 	hipLaunchKernelGGL(k_reset_outflow, dim3(flat_blocks(s->d.n)), dim3(256), 0, s->stream, g->ints(), g->f8,
 	                   phi ? G(phi)->real() : nullptr, real ? G(real)->real() : nullptr, s->d.n);
 	API_END

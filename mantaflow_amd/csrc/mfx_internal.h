@@ -95,6 +95,15 @@ struct Scene {
 	hipGraphExec_t bexec = nullptr;
 	std::vector<long long> bkey, blastKey;
 	std::unique_ptr<MgHierarchy> mgStatic;
+	// multigrid hierarchy standing in for PcMIC (the reference's mIC keeps
+	// nothing between solves): rebuilt whenever the matrix code changes,
+	// checked by the flag grid's generation and a hash of the code
+	std::unique_ptr<MgHierarchy> mgMic;
+	const Grid* micFlags = nullptr;
+	uint64_t micGen = 0, micHash = 0;
+	long long micFix = -1;
+	uint64_t* dhash = nullptr;  // device word of the code hash
+	uint64_t* hhash = nullptr;  // its pinned host mirror
 	uint8_t* code = nullptr;    // per-cell Laplace matrix code of the current solve
 	std::vector<Grid*> grids;
 	~Scene();
@@ -110,6 +119,7 @@ struct Grid {
 	int type;
 	void* data = nullptr;           // float[n] / int32[n] / float[3*cstride] (SoA)
 	uint8_t* f8 = nullptr;          // flag grids: packed bits 0..6
+	uint64_t gen = 0;               // flag grids: bumped by every write (flags_touched)
 	bool external = false;
 	float* comp(int c) const { return (float*)data + (size_t)c * s->cstride; }
 	float* real() const { return (float*)data; }
@@ -265,7 +275,10 @@ __device__ inline T block_sum(T v, T* sh) {
 // ---------------------------------------------------------------------------
 Grid* make_grid(Scene* s, int type);
 void free_grid(Grid* g);
-void flags_repack(Grid* flags);      // int32 -> uint8 mirror
+void flags_repack(Grid* flags);      // int32 -> uint8 mirror (also flags_touched)
+void flags_touched(Grid* flags);     // new generation number for a written flag grid
+// every live scene (releaseMG() without a solver releases the hierarchies of all)
+std::vector<Scene*> live_scenes();
 void sync(Scene* s);
 double reduce_sum_double(Scene* s, const float* a, int mode, long long n);
 

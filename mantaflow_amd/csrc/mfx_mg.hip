@@ -1614,7 +1614,7 @@ void MgHierarchy::build_coarsest(Scene* s, const uint8_t* code0) {
 // launches (single problem and work lists)
 // ---------------------------------------------------------------------------
 constexpr int kZmTY = kZmTYHost;
-long long l0_tiles(const Dims& d, int* kchunk, bool tuned) {
+long long l0_tiles(const Dims& d, int* kchunk, bool tuned, int force) {
 	// 3D: z-marching 64 x kZmTY column tiles over chunks of a multiple of
 	// kDotGroup planes (even, as the restriction pairs planes), halved while
 	// fewer than about 8192 wavefronts (one per SIMD slot of the chip) cover
@@ -1628,6 +1628,7 @@ long long l0_tiles(const Dims& d, int* kchunk, bool tuned) {
 	int kc = zm_chunk(d.nz);
 	while (kc > 16 && cols * ((d.nz + kc - 1) / kc) < 8192 / kZmTY) kc = zm_chunk(kc / 2);
 	if (tuned && zm_tuned(d).l0 > 0) kc = zm_tuned(d).l0;
+	if (force > 0) kc = force;
 	*kchunk = kc;
 	return cols * ((d.nz + kc - 1) / kc);
 }

@@ -31,6 +31,7 @@
 #include <type_traits>
 #include <array>
 #include <map>
+#include <set>
 #include <mutex>
 #include <chrono>
 #include <deque>
@@ -840,6 +841,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_replace_residual(Dims d, Tiles t,
 // ---------------------------------------------------------------------------
 static std::mutex g_tune_mu;
 static std::map<std::array<int, 3>, ZmTune> g_tune;
+static std::set<std::array<int, 3>> g_tuning;  // shapes being tuned right now
 ZmTune zm_tuned(const Dims& d) {
 	std::lock_guard<std::mutex> g(g_tune_mu);
 	auto it = g_tune.find({d.nx, d.ny, d.nz});
@@ -848,9 +850,17 @@ ZmTune zm_tuned(const Dims& d) {
 void zm_set_tuned(const Dims& d, ZmTune t) {
 	std::lock_guard<std::mutex> g(g_tune_mu);
 	g_tune[{d.nx, d.ny, d.nz}] = t;
+	g_tuning.erase({d.nx, d.ny, d.nz});
+}
+bool zm_claim_tuning(const Dims& d) {
+	std::lock_guard<std::mutex> g(g_tune_mu);
+	const std::array<int, 3> key{d.nx, d.ny, d.nz};
+	auto it = g_tune.find(key);
+	if (it != g_tune.end() && it->second.mv > 0) return false;
+	return g_tuning.insert(key).second;
 }
 
-long long matvec_tiles(const Dims& d, int* kchunk, bool tuned) {
+long long matvec_tiles(const Dims& d, int* kchunk, bool tuned, int force) {
 	// 64 x kMvTY column tiles; 3D: chunks of planes halved while fewer than
 	// about 8192 wavefronts cover the grid, or the autotuned chunk
 	const long long cols = (long long)((d.nx + 63) / 64) * ((d.ny + kMvTY - 1) / kMvTY);
@@ -861,6 +871,7 @@ long long matvec_tiles(const Dims& d, int* kchunk, bool tuned) {
 	int kc = zm_chunk(d.nz);
 	while (kc > 8 && cols * ((d.nz + kc - 1) / kc) < 8192 / kMvTY) kc = zm_chunk(kc / 2);
 	if (tuned && zm_tuned(d).mv > 0) kc = zm_tuned(d).mv;
+	if (force > 0) kc = force;
 	*kchunk = kc;
 	return cols * ((d.nz + kc - 1) / kc);
 }
@@ -896,10 +907,63 @@ void run_dot_many(hipStream_t s, unsigned nb, WorkList items, const DotP* p) {
 	hipLaunchKernelGGL((k_many<DotP, body_dot, kBlock>), dim3(nb), dim3(kBlock), 0, s, items, p);
 }
 
+// 64-bit hash of a matrix code array: the wrapping sum over cells of a
+// mixed (index, code) word (splitmix64 finaliser), so it does not depend on
+// the summation order.  Decides whether the hierarchy standing in for PcMIC
+// still matches the matrix.
+__global__ void k_code_hash(const uint8_t* __restrict__ code, long long n, unsigned long long* out) {
+	unsigned long long h = 0;
+	for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+		unsigned long long x = (unsigned long long)i * 0x9E3779B97F4A7C15ull + ((unsigned long long)code[i] + 1ull) * 0xC2B2AE3D27D4EB4Full;
+		x ^= x >> 30;
+		x *= 0xBF58476D1CE4E5B9ull;
+		x ^= x >> 27;
+		x *= 0x94D049BB133111EBull;
+		x ^= x >> 31;
+		h += x;
+	}
+	for (int off = 32; off > 0; off >>= 1) h += __shfl_down(h, off, 64);
+	if ((threadIdx.x & 63) == 0) atomicAdd(out, h);
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 static unsigned flat_count(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+static uint64_t code_hash(Scene* s, const uint8_t* code) {
+	if (!s->dhash) MFX_HIP(hipMalloc((void**)&s->dhash, sizeof(uint64_t)));
+	if (!s->hhash) MFX_HIP(hipHostMalloc((void**)&s->hhash, sizeof(uint64_t), hipHostMallocDefault));
+	MFX_HIP(hipMemsetAsync(s->dhash, 0, sizeof(uint64_t), s->stream));
+	const unsigned nb = std::min<unsigned>(2048u, flat_blocks(s->d.n));
+	hipLaunchKernelGGL(k_code_hash, dim3(nb), dim3(256), 0, s->stream, code, s->d.n,
+	                   (unsigned long long*)s->dhash);
+	MFX_HIP(hipMemcpyAsync(s->hhash, s->dhash, sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream));
+	sync(s);
+	return *s->hhash;
+}
+
+// The hierarchy that stands in for PcMIC: kept between solves while the
+// matrix stays the same (same flag grid generation and pinned cell, else
+// the same code hash), rebuilt otherwise.  The reference's mIC is built
+// from each solve's matrix (conjugategrad.cpp:66-97), so a stale hierarchy
+// would change the preconditioner's meaning.
+static MgHierarchy* mic_hierarchy(Scene* s, const Grid* flags, const uint8_t* code, long long fix, int precision) {
+	const bool sameKey = s->mgMic && s->mgMic->precision() == precision && s->micFlags == flags &&
+	                     s->micGen == flags->gen && s->micFix == fix;
+	if (!sameKey) {
+		const uint64_t h = code_hash(s, code);
+		if (!(s->mgMic && s->mgMic->precision() == precision && s->micHash == h)) {
+			s->mgMic.reset(new MgHierarchy());
+			s->mgMic->build(s, code, precision);
+		}
+		s->micHash = h;
+		s->micFlags = flags;
+		s->micGen = flags->gen;
+		s->micFix = fix;
+	}
+	return s->mgMic.get();
+}
 
 // zero-pressure cell selection (pressure.cpp:349-381), on the host from the
 // packed flags (setup-time work)
@@ -1005,7 +1069,7 @@ struct CgRun {
 static void autotune_zmarch(Scene* s, const uint8_t* code) {
 	const Dims& d = s->d;
 	if (!d.is3d) return;
-	if (zm_tuned(d).mv > 0) return;
+	if (!zm_claim_tuning(d)) return;
 	float* a = s->getReal();
 	float* b = s->getReal();
 	float* c = s->getReal();
@@ -1041,10 +1105,11 @@ static void autotune_zmarch(Scene* s, const uint8_t* code) {
 		};
 		float tmv = 1e30f, tl0 = 1e30f;
 		for (int kc = kDotGroup; kc <= maxc; kc *= 2) {
-			zm_set_tuned(d, ZmTune{kc, kc});
+			// trial chunks stay local to this thread: other solves of the
+			// shape use the heuristic chunk until the winners are published
 			int k1 = 0, k2 = 0;
-			const unsigned nbm = (unsigned)matvec_tiles(d, &k1, true);
-			const unsigned nbl = (unsigned)l0_tiles(d, &k2, true);
+			const unsigned nbm = (unsigned)matvec_tiles(d, &k1, false, kc);
+			const unsigned nbl = (unsigned)l0_tiles(d, &k2, false, kc);
 			const MatvecP mp{d, k1, st, code, a, b, c, e, s->dpart};
 			const L0DownP lp{d, k2, st, code, a, c, e};
 			const float tm = timed([&]() { run_matvec_one(s->stream, nbm, mp); });
@@ -1086,10 +1151,7 @@ static void cg_setup(CgRun& R, Scene* s, Grid* flags, float* rhs, float* x, cons
 	// zero-pressure fixing (pressure.cpp:349-388)
 	if (o.zeroPressureFixing || o.cgAccuracy < 1e-07f) {
 		R.fix = find_fix_cell(s, flags);
-		if (R.fix >= 0) {
-			const float zero = 0.f;
-			MFX_HIP(hipMemcpyAsync(rhs + R.fix, &zero, sizeof(float), hipMemcpyHostToDevice, s->stream));
-		}
+		if (R.fix >= 0) MFX_HIP(hipMemsetAsync(rhs + R.fix, 0, sizeof(float), s->stream));
 	}
 	// matrix
 	if (!s->code) MFX_HIP(hipMalloc((void**)&s->code, (size_t)d.n));
@@ -1125,6 +1187,8 @@ static void cg_setup(CgRun& R, Scene* s, Grid* flags, float* rhs, float* x, cons
 			R.mgDyn.reset(new MgHierarchy());
 			R.mgDyn->build(s, R.code, o.precision);
 			R.mg = R.mgDyn.get();
+		} else if (o.preconditioner == MFX_PC_MIC) {
+			R.mg = mic_hierarchy(s, flags, R.code, R.fix, o.precision);
 		} else {
 			if (!s->mgStatic || s->mgStatic->precision() != o.precision) {
 				s->mgStatic.reset(new MgHierarchy());
@@ -1833,7 +1897,13 @@ int mfx_correct_velocity(mfx_grid* vel, const mfx_grid* pressure, const mfx_grid
 
 int mfx_release_mg(mfx_scene* s) {
 	API_BEGIN
-	if (s) static_cast<Scene*>(s)->mgStatic.reset();
+	// without a scene: every live scene's hierarchies (pressure.cpp:254-258)
+	std::vector<Scene*> all = s ? std::vector<Scene*>{static_cast<Scene*>(s)} : live_scenes();
+	for (Scene* sc : all) {
+		sync(sc);
+		sc->mgStatic.reset();
+		sc->mgMic.reset();
+	}
 	API_END
 }
 

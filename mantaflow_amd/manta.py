@@ -482,9 +482,10 @@ class _ScalarOps:
         check(lib().mfx_grid_clamp(self._h, C.c_float(min), C.c_float(max)))
 
     def _reduce(self, op, bnd=0):
+        # the reference returns Real (grid.cpp:315-389): FP64 sums, rounded once
         out = C.c_double()
         check(lib().mfx_grid_reduce(self._h, op, bnd, C.byref(out)))
-        return out.value
+        return float(np.float32(out.value))
 
     def getMax(self):
         return self._reduce(_abi.RED_MAX)
@@ -774,7 +775,7 @@ def _opts(a, solver):
     o.cgAccuracy = a["cgAccuracy"]
     o.cgMaxIterFac = a["cgMaxIterFac"]
     pc = int(a["preconditioner"])
-    if a["precondition"] is False:
+    if not a["precondition"]:
         pc = PcNone  # backwards compatibility (pressure.cpp:327)
     o.preconditioner = pc
     o.enforceCompatibility = int(bool(a["enforceCompatibility"]))
@@ -870,9 +871,10 @@ def releaseMG(*args, **kwargs):
 
 
 def _maxdiff(g1, g2):
+    # FP64 maximum returned as Real (grid.cpp:400-430)
     out = C.c_double()
     check(lib().mfx_grid_max_diff(g1._h, g2._h, C.byref(out)))
-    return out.value
+    return float(np.float32(out.value))
 
 
 @_plugin

@@ -381,6 +381,28 @@ def test_cg_solution_matches_dense_solve(oracle, rng):
     assert np.abs(p[fl] - ref[fl]).max() < 1e-3 * max(1.0, np.abs(ref).max())
 
 
+@pytest.mark.parametrize("shape,blocked,exp", [
+    ((9, 8, 1), 0, 4 + 9 * 6),          # top centre (4,7) is the wall: one below
+    ((9, 8, 1), 1, 4 + 9 * 5),          # (4,6) blocked: two below
+    ((9, 8, 1), 2, 1 + 9 * 1),          # all three blocked: first fluid cell, bnd 1
+    ((8, 7, 6), 0, 4 + 8 * 5 + 56 * 3),
+    ((8, 7, 6), 2, 1 + 8 * 1 + 56 * 1),
+])
+def test_fix_cell_choice(oracle, shape, blocked, exp):
+    """Zero-pressure cell (pressure.cpp:349-381), hand-derived: the preferred
+    positions (nx/2, ny-1-o, nz/2) for o = 0, 1, 2, else the first fluid cell
+    in ijk order inside a one-cell boundary; none if any cell is empty."""
+    nx, ny, nz = shape
+    f = oracle.init_domain(shape)
+    oracle.fill_grid(shape, f)
+    g = f.reshape(nz, ny, nx)
+    for o in range(blocked):
+        g[nz // 2 if nz > 1 else 0, ny - 2 - o, nx // 2] = OBSTACLE
+    assert oracle.find_fix_cell(shape, f) == exp
+    g[nz // 2 if nz > 1 else 0, 2, 2] = EMPTY
+    assert oracle.find_fix_cell(shape, f) == -1
+
+
 def test_mic_reduces_to_jacobi_without_couplings(oracle, rng):
     shape = (7, 7, 7)
     n = 343
