@@ -718,6 +718,13 @@ __global__ void k_maxdiff(const float* a0, const float* a1, const float* a2, con
 	}
 }
 
+static std::mutex g_scenes_mu;
+static std::set<Scene*> g_scenes;
+std::vector<Scene*> live_scenes() {
+	std::lock_guard<std::mutex> g(g_scenes_mu);
+	return std::vector<Scene*>(g_scenes.begin(), g_scenes.end());
+}
+
 }  // namespace mfx
 
 using namespace mfx;
@@ -749,13 +756,6 @@ int mfx_device_memory(size_t* freeBytes, size_t* totalBytes) {
 	if (freeBytes) *freeBytes = f;
 	if (totalBytes) *totalBytes = t;
 	API_END
-}
-
-static std::mutex g_scenes_mu;
-static std::set<Scene*> g_scenes;
-std::vector<Scene*> live_scenes() {
-	std::lock_guard<std::mutex> g(g_scenes_mu);
-	return std::vector<Scene*>(g_scenes.begin(), g_scenes.end());
 }
 
 int mfx_scene_create(int nx, int ny, int nz, int dim, mfx_scene** out) {

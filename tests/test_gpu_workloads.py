@@ -7,8 +7,8 @@ with per-problem / per-request checks that do not depend on the GPU path:
   ||b - A x||_2 / ||b||_2, recomputed on the CPU with the oracle's
   double-precision build from the downloaded pressure, is <= 1e-4, the
   corrected velocity is divergence-free to the same tolerance, the batched
-  result equals the single solve bit for bit, and the two smallest problems
-  agree with the oracle's own multigrid-preconditioned CG;
+  result equals the single solve bit for bit, and the smallest 2D and 3D
+  problems agree with the oracle's own CG solve of the same system;
 - serving: the 24 sessions of serve.make_sessions(24) (64^2 to 192^3 cells,
   log-uniform, 2D and 3D) under a stream of step and solve requests: the
   served (continuously batched, four workers) replay leaves every session
@@ -18,7 +18,7 @@ with per-problem / per-request checks that do not depend on the GPU path:
 import numpy as np
 import pytest
 
-from oracle import OUTFLOW, PC_MGSTATIC
+from oracle import OUTFLOW, PC_NONE
 
 pytestmark = pytest.mark.gpu
 
@@ -57,7 +57,7 @@ def test_batched_problems_full_size(manta, oracle, oracle64):
     for p in single:
         manta.solvePressure(flags=p.flags, vel=p.vel, pressure=p.pressure, **kw)
     small = sorted(range(64), key=lambda i: batch[i].cells)
-    check_oracle = {small[0]} | {i for i in small if batch[i].shape[2] > 1][:1]
+    check_oracle = {small[0]} | set([i for i in small if batch[i].shape[2] > 1][:1])
     worst = 0.0
     for i, (pb, ps) in enumerate(zip(batch, single)):
         sb, ss = manta.lastSolveStats(pb.s), manta.lastSolveStats(ps.s)
@@ -77,8 +77,9 @@ def test_batched_problems_full_size(manta, oracle, oracle64):
         assert float(np.sqrt((d * d).sum())) <= 1.05 * TOL * nb, i
         if i in check_oracle:
             ref_v, ref_p = v0.copy(), np.zeros(pb.cells, np.float32)
-            st = oracle.solve_pressure(pb.shape, ref_v, ref_p, flags, cgAccuracy=TOL, preconditioner=PC_MGSTATIC,
-                                       relTol=TOL)
+            # plain CG: robust on obstacle-enclosed pockets (singular, compatible)
+            st = oracle.solve_pressure(pb.shape, ref_v, ref_p, flags, cgAccuracy=TOL, preconditioner=PC_NONE,
+                                       relTol=TOL, maxIterOverride=4000)
             assert st["relResidual"] <= TOL
             scale = max(float(np.abs(ref_v).max()), 1e-6)
             assert float(np.abs(gv - ref_v).max()) <= 2e-2 * scale, (i, pb.shape)
