@@ -66,6 +66,10 @@ class Oracle:
             "og_interpol": (R, [D, P, R, R, R]),
             "og_interpol_component": (R, [D, P, I, R, R, R]),
             "og_advect_semi_lagrange": (I, [D, R, P, P, P, I, I, R, I]),
+            "og_advect_semi_lagrange_hi": (I, [D, R, P, P, P, I, I, R, I, I, I]),
+            "og_interpol_cubic": (R, [D, P, R, R, R]),
+            "og_interpol_cubic_component": (R, [D, P, I, R, R, R]),
+            "og_interpol_mac": (None, [D, P, R, R, R, P]),
             "og_sl_scalar": (None, [D, P, P, P, R, I]),
             "og_sl_mac": (None, [D, P, P, P, R]),
             "og_apply_outflow_bc": (None, [D, P, P, P, R]),
@@ -143,6 +147,27 @@ class Oracle:
                                               strength, clampMode)
         if rc != 0:
             raise RuntimeError("AdvectSemiLagrange: Only order 1 (regular SL) and 2 (MacCormack) supported")
+
+    def advect_hi(self, shape, dt, flags, vel, grid, gridType, order=1, strength=1.0, clampMode=2,
+                  orderSpace=1, orderTrace=1):
+        rc = self.lib.og_advect_semi_lagrange_hi(self.dims(shape), dt, self._arr(flags, np.int32),
+                                                 self._arr(vel), self._arr(grid), gridType, order,
+                                                 strength, clampMode, orderSpace, orderTrace)
+        if rc != 0:
+            raise RuntimeError("AdvectSemiLagrange: unsupported order (%d)" % rc)
+
+    def interpol_cubic(self, shape, data, pos):
+        return self.lib.og_interpol_cubic(self.dims(shape), self._arr(data), *[self.creal(p) for p in pos])
+
+    def interpol_cubic_component(self, shape, vec, c, pos):
+        return self.lib.og_interpol_cubic_component(self.dims(shape), self._arr(vec), c,
+                                                    *[self.creal(p) for p in pos])
+
+    def interpol_mac(self, shape, vec, pos):
+        out = np.zeros(3, self.dtype)
+        self.lib.og_interpol_mac(self.dims(shape), self._arr(vec), *[self.creal(p) for p in pos],
+                                 self._arr(out))
+        return out
 
     def apply_outflow_bc(self, shape, flags, vel, velPrev, dt):
         self.lib.og_apply_outflow_bc(self.dims(shape), self._arr(flags, np.int32), self._arr(vel),

@@ -49,7 +49,13 @@
  *   9  KnSetWallBcs: obstacle cell above a fluid cell not set
  *  10  KnSetWallBcs: a stick x neighbour zeroes x instead of y and z
  *  11  getCentered: y average with the lower neighbour
- *  12  doClampComponentMAC, clampMode 2: flag check on the upper neighbour */
+ *  12  doClampComponentMAC, clampMode 2: flag check on the upper neighbour
+ *  13  cubicInterp<Real>: a2 with +d1 instead of -d1
+ *  14  interpolCubic: support starting at x1 instead of x1-1
+ *  15  midpoint trace of centred grids: full step to the midpoint
+ *  16  interpolCubicMAC: no half-cell shift of the component's lookup
+ *  17  MAC midpoint trace: velocity instead of the advected field
+ *  18  interpolMAC: x component without its shifted index */
 #ifndef OG_MUTANT
 #define OG_MUTANT 0
 #endif
@@ -259,6 +265,125 @@ og_real og_interpol_component(og_dims d, const og_real* v, int c, og_real px, og
 	return interp_core(d, v + c, 3, px, py, pz);
 }
 
+/* cubicInterp<T> (util/interpolHigh.h:22-39), the Catmull-Rom segment
+ * between points[1] and points[2].  For T = Real the coefficient
+ * expressions are evaluated in double and stored as Real (cubic_real); for
+ * T = Vec3 every operator rounds its component (Vec3 * double narrows,
+ * Vec3 +- Vec3 is a Real operation, vectorbase.h:235-279; cubic_vec).  The
+ * polynomial itself is evaluated in Real either way. */
+static inline og_real cubic_real(og_real t, const og_real* p) {
+	const og_real d0 = (og_real)((p[2] - p[0]) * 0.5);
+	const og_real d1 = (og_real)((p[3] - p[1]) * 0.5);
+	const og_real dk = p[2] - p[1];
+	const og_real a0 = p[1], a1 = d0;
+	const og_real a2 = (og_real)(3.0 * dk - 2.0 * d0 + (OG_MUT(13) ? d1 : -d1));
+	const og_real a3 = (og_real)(-2.0 * dk + d0 + d1);
+	const og_real sq = t * t, cu = sq * t;
+	return a3 * cu + a2 * sq + a1 * t + a0;
+}
+static inline og_real cubic_vec(og_real t, const og_real* p) {
+	const og_real d0 = (og_real)((og_real)(p[2] - p[0]) * 0.5);
+	const og_real d1 = (og_real)((og_real)(p[3] - p[1]) * 0.5);
+	const og_real dk = p[2] - p[1];
+	const og_real a0 = p[1], a1 = d0;
+	og_real a2 = (og_real)(3.0 * dk) - (og_real)(2.0 * d0);
+	a2 = a2 - d1;
+	og_real a3 = (og_real)(-2.0 * dk) + d0;
+	a3 = a3 + d1;
+	const og_real sq = t * t, cu = sq * t;
+	return a3 * cu + a2 * sq + a1 * t + a0;
+}
+
+/* interpolCubic<T> / interpolCubic2D<T> (util/interpolHigh.h:41-170) on one
+ * component (element stride es); positions whose 4x4(x4) support leaves the
+ * grid fall back to the trilinear interpol<T> (:58-60, :97-99). */
+static og_real cubic_core(og_dims d, const og_real* data, long es, og_real posx, og_real posy, og_real posz,
+                          int vecRules) {
+	og_real (*cub)(og_real, const og_real*) = vecRules ? cubic_vec : cubic_real;
+	const og_real px = posx - 0.5f, py = posy - 0.5f;
+	const int x1 = (int)px, y1 = (int)py;
+	const int x0 = x1 - (OG_MUT(14) ? 0 : 1), x3 = x0 + 3, y0 = y1 - 1, y3 = y1 + 2;
+	if (!d.is3d) {
+		if (x0 < 0 || y0 < 0 || x3 >= d.nx || y3 >= d.ny) return interp_core(d, data, es, posx, posy, posz);
+		const og_real xt = px - (og_real)x1, yt = py - (og_real)y1;
+		og_real rows[4];
+		for (int r = 0; r < 4; r++) {
+			og_real pts[4];
+			for (int q = 0; q < 4; q++) pts[q] = data[es * ((long)(x0 + q) + (long)d.nx * (y0 + r))];
+			rows[r] = cub(xt, pts);
+		}
+		return cub(yt, rows);
+	}
+	const og_real pz = posz - 0.5f;
+	const int z1 = (int)pz, z0 = z1 - 1, z3 = z1 + 2;
+	if (x0 < 0 || y0 < 0 || z0 < 0 || x3 >= d.nx || y3 >= d.ny || z3 >= d.nz)
+		return interp_core(d, data, es, posx, posy, posz);
+	const og_real xt = px - (og_real)x1, yt = py - (og_real)y1, zt = pz - (og_real)z1;
+	og_real slabs[4];
+	for (int c = 0; c < 4; c++) {
+		og_real rows[4];
+		for (int r = 0; r < 4; r++) {
+			og_real pts[4];
+			for (int q = 0; q < 4; q++) pts[q] = data[es * IX(d, x0 + q, y0 + r, z0 + c)];
+			rows[r] = cub(xt, pts);
+		}
+		slabs[c] = cub(yt, rows);
+	}
+	return cub(zt, slabs);
+}
+
+og_real og_interpol_cubic(og_dims d, const og_real* data, og_real px, og_real py, og_real pz) {
+	return cubic_core(d, data, 1, px, py, pz, 0);
+}
+og_real og_interpol_cubic_component(og_dims d, const og_real* v, int c, og_real px, og_real py, og_real pz) {
+	return cubic_core(d, v + c, 3, px, py, pz, 1);
+}
+
+/* interpolMAC (util/interpol.h:121-157): every component of a MAC grid at
+ * pos, each with the lookup shifted by half a cell along its own axis
+ * (BUILD_INDEX_SHIFT); out[3] */
+void og_interpol_mac(og_dims d, const og_real* v, og_real px0, og_real py0, og_real pz0, og_real* out) {
+	/* BUILD_INDEX */
+	og_real px = px0 - 0.5f, py = py0 - 0.5f, pz = pz0 - 0.5f;
+	int xi = (int)px, yi = (int)py, zi = (int)pz;
+	og_real s1 = px - (og_real)xi, s0 = 1. - s1;
+	og_real t1 = py - (og_real)yi, t0 = 1. - t1;
+	og_real f1 = pz - (og_real)zi, f0 = 1. - f1;
+	if (px < 0.) { xi = 0; s0 = 1.0; s1 = 0.0; }
+	if (py < 0.) { yi = 0; t0 = 1.0; t1 = 0.0; }
+	if (pz < 0.) { zi = 0; f0 = 1.0; f1 = 0.0; }
+	if (xi >= d.nx - 1) { xi = d.nx - 2; s0 = 0.0; s1 = 1.0; }
+	if (yi >= d.ny - 1) { yi = d.ny - 2; t0 = 0.0; t1 = 1.0; }
+	if (d.nz > 1) { if (zi >= d.nz - 1) { zi = d.nz - 2; f0 = 0.0; f1 = 1.0; } }
+	/* shifted coordinates */
+	int sxi = (int)px0, syi = (int)py0, szi = (int)pz0;
+	og_real ss1 = px0 - (og_real)sxi, ss0 = 1. - ss1;
+	og_real st1 = py0 - (og_real)syi, st0 = 1. - st1;
+	og_real sf1 = pz0 - (og_real)szi, sf0 = 1. - sf1;
+	if (px0 < 0) { sxi = 0; ss0 = 1.0; ss1 = 0.0; }
+	if (py0 < 0) { syi = 0; st0 = 1.0; st1 = 0.0; }
+	if (pz0 < 0) { szi = 0; sf0 = 1.0; sf1 = 0.0; }
+	if (sxi >= d.nx - 1) { sxi = d.nx - 2; ss0 = 0.0; ss1 = 1.0; }
+	if (syi >= d.ny - 1) { syi = d.ny - 2; st0 = 0.0; st1 = 1.0; }
+	if (d.nz > 1) { if (szi >= d.nz - 1) { szi = d.nz - 2; sf0 = 0.0; sf1 = 1.0; } }
+	const long X = 3, Y = 3L * d.nx, Z = 3 * SZ(d);
+	{
+		const og_real* r = v + 3 * (((long)zi * d.ny + yi) * d.nx + (OG_MUT(18) ? xi : sxi));
+		out[0] = f0 * ((r[0] * t0 + r[Y] * t1) * ss0 + (r[X] * t0 + r[X + Y] * t1) * ss1) +
+		         f1 * ((r[Z] * t0 + r[Z + Y] * t1) * ss0 + (r[X + Z] * t0 + r[X + Y + Z] * t1) * ss1);
+	}
+	{
+		const og_real* r = v + 3 * (((long)zi * d.ny + syi) * d.nx + xi) + 1;
+		out[1] = f0 * ((r[0] * st0 + r[Y] * st1) * s0 + (r[X] * st0 + r[X + Y] * st1) * s1) +
+		         f1 * ((r[Z] * st0 + r[Z + Y] * st1) * s0 + (r[X + Z] * st0 + r[X + Y + Z] * st1) * s1);
+	}
+	{
+		const og_real* r = v + 3 * (((long)szi * d.ny + yi) * d.nx + xi) + 2;
+		out[2] = sf0 * ((r[0] * t0 + r[Y] * t1) * s0 + (r[X] * t0 + r[X + Y] * t1) * s1) +
+		         sf1 * ((r[Z] * t0 + r[Z + Y] * t1) * s0 + (r[X + Z] * t0 + r[X + Y + Z] * t1) * s1);
+	}
+}
+
 /* MACGrid accessors (grid.h:424-470).  0.5*(float) is evaluated in double and
  * narrowed, which equals the float product (scaling by a power of two). */
 static inline void getCentered(og_dims d, const og_real* v, int i, int j, int k, og_real* o) {
@@ -302,53 +427,104 @@ static inline void getAtMACZ(og_dims d, const og_real* v, int i, int j, int k, o
  * Semi-Lagrangian advection
  * ========================================================================= */
 
-/* SemiLagrange<T>, orderTrace 1, orderSpace 1 (advection.cpp:25-31); ncomp 1
- * for Real grids, 3 for centered Vec3 grids (interpol<Vec3> is per component). */
-void og_sl_scalar(og_dims d, const og_real* vel, og_real* dst, const og_real* src, og_real dt,
-                  int ncomp) {
+/* one value of a centred grid at pos: interpol (orderSpace 1) or
+ * interpolCubic (orderSpace 2) with Real (ncomp 1) or Vec3 (ncomp 3) rules */
+static inline og_real lookup_centered(og_dims d, const og_real* src, int ncomp, int c, int orderSpace, og_real px,
+                                      og_real py, og_real pz) {
+	if (orderSpace == 2) return cubic_core(d, src + c, ncomp, px, py, pz, ncomp == 3);
+	return interp_core(d, src + c, ncomp, px, py, pz);
+}
+
+/* SemiLagrange<T> with orderTrace 1 or 2 (explicit midpoint) and orderSpace
+ * 1 or 2 (advection.cpp:25-41; grid.h:146-151 getInterpolatedHi) */
+void og_sl_scalar_hi(og_dims d, const og_real* vel, og_real* dst, const og_real* src, og_real dt, int ncomp,
+                     int orderSpace, int orderTrace) {
 	const int b = 1;
 	OMP_FOR
 	for (int k = KMIN(d, b); k < KMAX(d, b); k++)
 		for (int j = b; j < d.ny - b; j++)
 			for (int i = b; i < d.nx - b; i++) {
-				og_real v[3];
+				og_real v[3], p0[3] = {(og_real)((float)i + 0.5f), (og_real)((float)j + 0.5f),
+				                       (og_real)((float)k + 0.5f)};
+				og_real pos[3];
 				getCentered(d, vel, i, j, k, v);
-				og_real px = (og_real)((float)i + 0.5f) - v[0] * dt;
-				og_real py = (og_real)((float)j + 0.5f) - v[1] * dt;
-				og_real pz = (og_real)((float)k + 0.5f) - v[2] * dt;
-				long ix = IX(d, i, j, k);
-				if (ncomp == 1) {
-					dst[ix] = interp_core(d, src, 1, px, py, pz);
+				if (orderTrace == 2) {
+					/* p1 = p0 - getCentered*dt*0.5; p2 = p0 - getInterpolated(p1)*dt */
+					og_real p1[3], u[3];
+					for (int a = 0; a < 3; a++)
+						p1[a] = p0[a] - (og_real)((og_real)(v[a] * dt) * (OG_MUT(15) ? 1.0 : 0.5));
+					og_interpol_mac(d, vel, p1[0], p1[1], p1[2], u);
+					for (int a = 0; a < 3; a++) pos[a] = p0[a] - u[a] * dt;
 				} else {
-					for (int c = 0; c < 3; c++)
-						dst[3 * ix + c] = interp_core(d, src + c, 3, px, py, pz);
+					for (int a = 0; a < 3; a++) pos[a] = p0[a] - v[a] * dt;
 				}
+				long ix = IX(d, i, j, k);
+				for (int c = 0; c < ncomp; c++)
+					dst[ncomp * ix + c] = lookup_centered(d, src, ncomp, c, orderSpace, pos[0], pos[1], pos[2]);
 			}
 }
 
-/* SemiLagrangeMAC, orderTrace 1, orderSpace 1 (advection.cpp:44-57) */
-void og_sl_mac(og_dims d, const og_real* vel, og_real* dst, const og_real* src, og_real dt) {
+/* component c of a MAC grid at pos: interpolComponent<c> (orderSpace 1) or
+ * interpolCubicMAC(pos)[c] = interpolCubic<Vec3>(pos + 0.5 e_c)[c], which is 0
+ * for c = 2 in 2D (grid.h:268-273, interpolHigh.h:172-179) */
+static inline og_real lookup_mac(og_dims d, const og_real* src, int c, int orderSpace, const og_real* pos) {
+	if (orderSpace == 2) {
+		if (c == 2 && !d.is3d) return 0;
+		og_real q[3] = {pos[0], pos[1], pos[2]};
+		if (!OG_MUT(16)) q[c] = q[c] + 0.5f;
+		return cubic_core(d, src + c, 3, q[0], q[1], q[2], 1);
+	}
+	return interp_core(d, src + c, 3, pos[0] - ((OG_MUT(6) && c == 0) ? 0.5f : 0.f), pos[1], pos[2]);
+}
+
+/* SemiLagrangeMAC with orderTrace 1 or 2 and orderSpace 1 or 2
+ * (advection.cpp:44-77).  The midpoint trace takes the face velocity and
+ * the interpolated velocity from src, the field being advected (:59-73). */
+void og_sl_mac_hi(og_dims d, const og_real* vel, og_real* dst, const og_real* src, og_real dt, int orderSpace,
+                  int orderTrace) {
 	const int b = 1;
 	OMP_FOR
 	for (int k = KMIN(d, b); k < KMAX(d, b); k++)
 		for (int j = b; j < d.ny - b; j++)
 			for (int i = b; i < d.nx - b; i++) {
-				og_real p0x = (og_real)((float)i + 0.5f), p0y = (og_real)((float)j + 0.5f),
-				        p0z = (og_real)((float)k + 0.5f);
-				og_real v[3], out[3];
-				getAtMACX(d, vel, i, j, k, v);
-				if (OG_MUT(6)) p0x -= 0.5f;
-				out[0] = interp_core(d, src + 0, 3, p0x - v[0] * dt, p0y - v[1] * dt, p0z - v[2] * dt);
-				if (OG_MUT(6)) p0x += 0.5f;
-				getAtMACY(d, vel, i, j, k, v);
-				out[1] = interp_core(d, src + 1, 3, p0x - v[0] * dt, p0y - v[1] * dt, p0z - v[2] * dt);
-				getAtMACZ(d, vel, i, j, k, v);
-				out[2] = interp_core(d, src + 2, 3, p0x - v[0] * dt, p0y - v[1] * dt, p0z - v[2] * dt);
+				const og_real hi = (og_real)((float)i + 0.5f), hj = (og_real)((float)j + 0.5f),
+				              hk = (og_real)((float)k + 0.5f);
+				og_real out[3];
+				for (int c = 0; c < 3; c++) {
+					og_real v[3], pos[3];
+					const og_real* fld = (orderTrace == 2 && !OG_MUT(17)) ? src : vel;
+					if (c == 0) getAtMACX(d, fld, i, j, k, v);
+					else if (c == 1) getAtMACY(d, fld, i, j, k, v);
+					else getAtMACZ(d, fld, i, j, k, v);
+					if (orderTrace == 2) {
+						og_real f0[3] = {hi, hj, hk};
+						f0[c] = (og_real)(c == 0 ? i : (c == 1 ? j : k));
+						og_real p1[3], u[3];
+						for (int a = 0; a < 3; a++) p1[a] = f0[a] - (og_real)((og_real)(v[a] * dt) * 0.5);
+						og_interpol_mac(d, OG_MUT(17) ? vel : src, p1[0], p1[1], p1[2], u);
+						pos[0] = hi - u[0] * dt;
+						pos[1] = hj - u[1] * dt;
+						pos[2] = hk - u[2] * dt;
+					} else {
+						pos[0] = hi - v[0] * dt;
+						pos[1] = hj - v[1] * dt;
+						pos[2] = hk - v[2] * dt;
+					}
+					out[c] = lookup_mac(d, src, c, orderSpace, pos);
+				}
 				long ix = IX(d, i, j, k);
 				dst[3 * ix] = out[0];
 				dst[3 * ix + 1] = out[1];
 				dst[3 * ix + 2] = out[2];
 			}
+}
+
+/* SemiLagrange<T> / SemiLagrangeMAC with orderTrace 1 and orderSpace 1 */
+void og_sl_scalar(og_dims d, const og_real* vel, og_real* dst, const og_real* src, og_real dt, int ncomp) {
+	og_sl_scalar_hi(d, vel, dst, src, dt, ncomp, 1, 1);
+}
+void og_sl_mac(og_dims d, const og_real* vel, og_real* dst, const og_real* src, og_real dt) {
+	og_sl_mac_hi(d, vel, dst, src, dt, 1, 1);
 }
 
 /* MacCormackCorrect<T> (advection.cpp:81-91, all idx) and MacCormackCorrectMAC
@@ -625,19 +801,21 @@ void og_apply_outflow_bc(og_dims d, const int* flags, og_real* vel, const og_rea
 	free(velDst);
 }
 
-/* advectSemiLagrange -> fnAdvectSemiLagrange (advection.cpp:292-321,406-460),
- * orderSpace 1 / orderTrace 1.  Temporaries start at zero like the
+/* advectSemiLagrange -> fnAdvectSemiLagrange (advection.cpp:292-321,406-460)
+ * with orderSpace / orderTrace 1 or 2.  Temporaries start at zero like the
  * reference's pooled temp grids (Grid constructor clears). */
-int og_advect_semi_lagrange(og_dims d, og_real dt, const int* flags, const og_real* vel,
-                            og_real* grid, int gridType, int order, og_real strength,
-                            int clampMode) {
+int og_advect_semi_lagrange_hi(og_dims d, og_real dt, const int* flags, const og_real* vel, og_real* grid,
+                               int gridType, int order, og_real strength, int clampMode, int orderSpace,
+                               int orderTrace) {
 	if (order != 1 && order != 2) return -1;
+	if (orderSpace != 1 && orderSpace != 2) return -2;
+	if (orderTrace != 1 && orderTrace != 2) return -3;
 	long n = NCELLS(d);
 	int nc = (gridType == OG_TYPE_REAL) ? 1 : 3;
 	size_t bytes = sizeof(og_real) * (size_t)(nc * n);
 	og_real* fwd = (og_real*)calloc(1, bytes);
-	if (gridType == OG_TYPE_MAC) og_sl_mac(d, vel, fwd, grid, dt);
-	else og_sl_scalar(d, vel, fwd, grid, dt, nc);
+	if (gridType == OG_TYPE_MAC) og_sl_mac_hi(d, vel, fwd, grid, dt, orderSpace, orderTrace);
+	else og_sl_scalar_hi(d, vel, fwd, grid, dt, nc, orderSpace, orderTrace);
 	if (order == 1) {
 		if (gridType == OG_TYPE_MAC) og_apply_outflow_bc(d, flags, fwd, grid, dt);
 		memcpy(grid, fwd, bytes);
@@ -647,12 +825,12 @@ int og_advect_semi_lagrange(og_dims d, og_real dt, const int* flags, const og_re
 	og_real* bwd = (og_real*)calloc(1, bytes);
 	og_real* ng = (og_real*)calloc(1, bytes);
 	if (gridType == OG_TYPE_MAC) {
-		og_sl_mac(d, vel, bwd, fwd, -dt);
+		og_sl_mac_hi(d, vel, bwd, fwd, -dt, orderSpace, orderTrace);
 		og_mac_cormack_correct(d, flags, ng, grid, fwd, bwd, strength, OG_TYPE_MAC);
 		og_mac_cormack_clamp_mac(d, flags, vel, ng, grid, fwd, dt, clampMode);
 		og_apply_outflow_bc(d, flags, ng, grid, dt);
 	} else {
-		og_sl_scalar(d, vel, bwd, fwd, -dt, nc);
+		og_sl_scalar_hi(d, vel, bwd, fwd, -dt, nc, orderSpace, orderTrace);
 		og_mac_cormack_correct(d, flags, ng, grid, fwd, bwd, strength, gridType);
 		og_mac_cormack_clamp(d, flags, vel, ng, grid, fwd, dt, clampMode, nc);
 	}
@@ -661,6 +839,12 @@ int og_advect_semi_lagrange(og_dims d, og_real dt, const int* flags, const og_re
 	free(bwd);
 	free(ng);
 	return 0;
+}
+
+int og_advect_semi_lagrange(og_dims d, og_real dt, const int* flags, const og_real* vel,
+                            og_real* grid, int gridType, int order, og_real strength,
+                            int clampMode) {
+	return og_advect_semi_lagrange_hi(d, dt, flags, vel, grid, gridType, order, strength, clampMode, 1, 1);
 }
 
 /* ===========================================================================

@@ -87,16 +87,31 @@ void og_reset_outflow(og_dims d, int* flags, og_real* phi, og_real* real);
 og_real og_interpol(og_dims d, const og_real* data, og_real px, og_real py, og_real pz);
 og_real og_interpol_component(og_dims d, const og_real* vec3data, int c,
                               og_real px, og_real py, og_real pz);
+/* cubic lookup, util/interpolHigh.h:22-170: Real rules / Vec3 rules per component */
+og_real og_interpol_cubic(og_dims d, const og_real* data, og_real px, og_real py, og_real pz);
+og_real og_interpol_cubic_component(og_dims d, const og_real* vec3data, int c, og_real px, og_real py,
+                                    og_real pz);
+/* interpolMAC (util/interpol.h:121-157): all three components of a MAC grid, out[3] */
+void og_interpol_mac(og_dims d, const og_real* vec3data, og_real px, og_real py, og_real pz, og_real* out);
 
 /* ---- advection (advection.cpp:25-460) ---- */
 /* gridType: OG_TYPE_REAL, OG_TYPE_VEC3 or OG_TYPE_MAC; grid may alias vel */
 int og_advect_semi_lagrange(og_dims d, og_real dt, const int* flags, const og_real* vel,
                             og_real* grid, int gridType, int order, og_real strength,
                             int clampMode);
+/* the same with orderSpace (1 trilinear, 2 cubic) and orderTrace (1 Euler, 2
+ * midpoint); returns -2 / -3 for other orders */
+int og_advect_semi_lagrange_hi(og_dims d, og_real dt, const int* flags, const og_real* vel, og_real* grid,
+                               int gridType, int order, og_real strength, int clampMode, int orderSpace,
+                               int orderTrace);
 /* individual steps, exposed for unit tests */
 void og_sl_scalar(og_dims d, const og_real* vel, og_real* dst, const og_real* src,
                   og_real dt, int ncomp);
 void og_sl_mac(og_dims d, const og_real* vel, og_real* dst, const og_real* src, og_real dt);
+void og_sl_scalar_hi(og_dims d, const og_real* vel, og_real* dst, const og_real* src, og_real dt, int ncomp,
+                     int orderSpace, int orderTrace);
+void og_sl_mac_hi(og_dims d, const og_real* vel, og_real* dst, const og_real* src, og_real dt, int orderSpace,
+                  int orderTrace);
 void og_mac_cormack_correct(og_dims d, const int* flags, og_real* dst, const og_real* old,
                             const og_real* fwd, const og_real* bwd, og_real strength,
                             int gridType);

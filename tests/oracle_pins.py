@@ -342,4 +342,223 @@ def all_pins():
     for is3d in (False, True):
         pins.append(("outflow-%dd" % (3 if is3d else 2), lambda o, a=is3d: pin_outflow_column(o, a)))
     pins.append(("wall_bcs_3d", pin_wall_bcs_3d))
+    return pins + high_order_pins()
+
+
+# ---------------------------------------------------------------------------
+# higher-order lookups and traces: orderSpace 2 (Catmull-Rom cubic,
+# util/interpolHigh.h) and orderTrace 2 (explicit midpoint, advection.cpp:33-37,58-73)
+# ---------------------------------------------------------------------------
+def _cr_weights(t):
+    """Catmull-Rom weights of the points at offsets -1, 0, 1, 2 for the
+    fraction t: the cardinal-spline basis with tension 0, written as weights
+    (the reference writes the same segment as polynomial coefficients)."""
+    t2, t3 = t * t, t * t * t
+    return np.array([(-t + 2 * t2 - t3) / 2, (2 - 5 * t2 + 3 * t3) / 2, (t + 4 * t2 - 3 * t3) / 2, (-t2 + t3) / 2])
+
+
+def pin_cubic_weights(o, is3d, vec, seed=31):
+    """The cubic lookup equals the tensor-product Catmull-Rom weight sum over
+    the 4x4(x4) cells around the point (cell centres at i + 1/2); where that
+    support leaves the grid it is the trilinear lookup, exactly."""
+    rng = np.random.default_rng(seed)
+    shape = (11, 10, 9) if is3d else (13, 11, 1)
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    data = rng.uniform(-1, 1, (n, 3) if vec else n).astype(np.float32)
+    arr = (data[:, 1] if vec else data).reshape(nz, ny, nx).astype(np.float64)
+    look = (lambda p: o.interpol_cubic_component(shape, data, 1, p)) if vec else \
+        (lambda p: o.interpol_cubic(shape, data, p))
+    lin = (lambda p: o.lib.og_interpol_component(o.dims(shape), o._arr(data), 1, *[o.creal(x) for x in p])) if vec \
+        else (lambda p: o.interpol(shape, data, p))
+    for _ in range(150):
+        p = rng.uniform([2.0, 2.0, 2.0], [nx - 1.5, ny - 1.5, max(nz - 1.5, 2.01)])
+        if not is3d:
+            p[2] = 0.5
+        q = p - 0.5
+        b = np.floor(q).astype(int)
+        wx, wy = _cr_weights(q[0] - b[0]), _cr_weights(q[1] - b[1])
+        if is3d:
+            wz = _cr_weights(q[2] - b[2])
+            blk = arr[b[2] - 1:b[2] + 3, b[1] - 1:b[1] + 3, b[0] - 1:b[0] + 3]
+            ref = np.einsum("k,j,i,kji->", wz, wy, wx, blk)
+        else:
+            blk = arr[0, b[1] - 1:b[1] + 3, b[0] - 1:b[0] + 3]
+            ref = np.einsum("j,i,ji->", wy, wx, blk)
+        assert abs(look(p) - ref) < 2e-5, ("cubic", p, look(p), ref)
+    for _ in range(60):  # supports that leave the grid: trilinear fallback
+        p = rng.uniform([0.2, 0.2, 0.2], [1.4, ny - 0.2, max(nz - 0.2, 1.0)])
+        if not is3d:
+            p[2] = 0.5
+        assert look(p) == lin(p), ("fallback", p)
+
+
+def pin_cubic_quadratic(o, is3d, vec):
+    """Catmull-Rom segments reproduce quadratics, so the tensor-product cubic
+    lookup reproduces q(x) = prod over axes of a quadratic, exactly up to
+    rounding, wherever its support is inside the grid."""
+    rng = np.random.default_rng(41)
+    shape = (12, 11, 10) if is3d else (14, 12, 1)
+    nx, ny, nz = shape
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    c = rng.uniform(-0.05, 0.05, (3, 3))
+    qa = lambda x, a: 0.5 + c[a, 0] + c[a, 1] * x + c[a, 2] * x * x / 4
+    q = lambda x, y, z: qa(x, 0) * qa(y, 1) * (qa(z, 2) if is3d else 1.0)
+    vals = q(i + 0.5, j + 0.5, k + 0.5).reshape(-1)
+    data = np.repeat(vals[:, None], 3, 1).astype(np.float32) if vec else vals.astype(np.float32)
+    for _ in range(120):
+        p = rng.uniform([2.0, 2.0, 2.0], [nx - 1.5, ny - 1.5, max(nz - 1.5, 2.01)])
+        if not is3d:
+            p[2] = 0.5
+        got = o.interpol_cubic_component(shape, data, 2, p) if vec else o.interpol_cubic(shape, data, p)
+        assert abs(got - q(*p)) < 3e-6, (p, got, q(*p))
+
+
+def _linear_vel_mac(shape, a0, G):
+    return _linear_mac(shape, a0, G).astype(np.float32).reshape(-1, 3)
+
+
+def pin_midpoint_centered(o, gridType, orderSpace, is3d):
+    """orderTrace 2 on Real / centred Vec3 grids with a linear velocity u and a
+    linear source f: the cell value becomes f(p0 - u(p0 - u(p0) dt/2) dt),
+    p0 the cell centre (getCentered and interpolMAC are exact on linear
+    fields, and so are the trilinear and cubic lookups of f)."""
+    shape = (16, 15, 14) if is3d else (18, 17, 1)
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    a0 = np.array([0.15, -0.1, 0.12 if is3d else 0.0])
+    G = np.array([[0.02, -0.015, 0.01], [0.01, 0.02, -0.02], [-0.01, 0.015, 0.02]])
+    if not is3d:
+        G[2, :] = 0
+        G[:, 2] = 0
+    vel = _linear_vel_mac(shape, a0, G)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    P0 = np.stack([i + 0.5, j + 0.5, (k + 0.5) if is3d else np.full(i.shape, 0.5)], -1)
+    cf = np.array([[0.3, 0.04, -0.03, 0.02], [-0.2, 0.01, 0.03, -0.01], [0.1, -0.02, 0.01, 0.03]])
+    nc = 1 if gridType == TYPE_REAL else 3
+    f = lambda P, cc: cf[cc, 0] + P[..., 0] * cf[cc, 1] + P[..., 1] * cf[cc, 2] + (P[..., 2] * cf[cc, 3] if is3d else 0)
+    src = np.stack([f(P0, cc) for cc in range(nc)], -1).astype(np.float32).reshape(n, nc)
+    g = src.reshape(-1).copy() if nc == 1 else src.copy()
+    flags = o.init_domain(shape)
+    o.fill_grid(shape, flags)
+    dt = 0.9
+    o.advect_hi(shape, np.float32(dt), flags, vel, g, gridType, 1, 1.0, 2, orderSpace, 2)
+    u = lambda P: a0 + P @ G.T
+    P2 = P0 - u(P0 - u(P0) * dt / 2) * dt
+    exp = np.stack([f(P2, cc) for cc in range(nc)], -1)
+    m = 3
+    sl = (slice(m, nz - m) if is3d else slice(None), slice(m, ny - m), slice(m, nx - m))
+    diff = np.abs(g.reshape(nz, ny, nx, nc)[sl] - exp[sl]).max()
+    assert diff < 2e-5, ("midpoint centred", gridType, orderSpace, diff)
+
+
+def pin_midpoint_mac(o, orderSpace, is3d, self_adv):
+    """orderTrace 2 on MAC grids, velocity u and advected MAC field s both
+    linear (s = u for self-advection).  The midpoint trace of face c at f0
+    uses the advected field (advection.cpp:59-73): q = p0 - s(f0 - s(f0) dt/2) dt
+    with p0 the cell centre; the value is s_c(q - e_c/2) with the trilinear
+    lookup and s_c(q) with the cubic one (interpolCubicMAC looks up at
+    q + e_c/2 in cell-centred coordinates, interpolHigh.h:172-179); in 2D the
+    cubic z component is 0."""
+    shape = (16, 15, 14) if is3d else (18, 17, 1)
+    nx, ny, nz = shape
+    aU = np.array([0.1, -0.12, 0.08 if is3d else 0.0])
+    GU = np.array([[0.015, -0.02, 0.01], [0.02, 0.01, -0.015], [-0.01, 0.02, 0.015]])
+    aS = np.array([-0.15, 0.1, -0.05 if is3d else 0.0])
+    GS = np.array([[-0.02, 0.01, 0.015], [0.015, -0.02, 0.01], [0.01, -0.015, -0.02]])
+    if self_adv:
+        aS, GS = aU, GU
+    for Gm in (GU, GS):
+        if not is3d:
+            Gm[2, :] = 0
+            Gm[:, 2] = 0
+    vel = _linear_vel_mac(shape, aU, GU)
+    g = _linear_vel_mac(shape, aS, GS).copy()
+    flags = o.init_domain(shape)
+    o.fill_grid(shape, flags)
+    dt = 0.9
+    o.advect_hi(shape, np.float32(dt), flags, g.copy() if self_adv else vel, g, TYPE_MAC, 1, 1.0, 2, orderSpace, 2)
+    sfun = lambda P: aS + P @ GS.T
+    pos = _face_positions(shape)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    P0 = np.stack([i + 0.5, j + 0.5, (k + 0.5) if is3d else np.full(i.shape, 0.5)], -1)
+    exp = np.zeros(pos[0].shape)
+    for c in range(3):
+        f0 = pos[c]
+        q = P0 - sfun(f0 - sfun(f0) * dt / 2) * dt
+        if orderSpace == 1:
+            q = q.copy()
+            q[..., c] -= 0.5
+        exp[..., c] = aS[c] + q @ GS[c]
+    if not is3d and orderSpace == 2:
+        exp[..., 2] = 0
+    got = g.reshape(nz, ny, nx, 3)
+    m = 3
+    sl = (slice(m, nz - m) if is3d else slice(None), slice(m, ny - m), slice(m, nx - m))
+    comps = 3 if is3d else 2
+    diff = np.abs(got[sl][..., :comps] - exp[sl][..., :comps]).max()
+    assert diff < 2e-5, ("midpoint MAC", orderSpace, self_adv, diff)
+
+
+def pin_high_order_maccormack(o, gridType, orderSpace, orderTrace):
+    """MacCormack with the higher-order lookups and traces keeps its defining
+    property: a linear field under a uniform velocity is translated exactly
+    (backward trace returns the original, zero correction, forward value kept
+    by the clamp) for both clamp modes.  MAC grids with the cubic lookup are
+    translated from the half-cell-shifted point of interpolCubicMAC."""
+    shape = (16, 15, 14)
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    flags = o.init_domain(shape)
+    o.fill_grid(shape, flags)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    nc = 1 if gridType == TYPE_REAL else 3
+    u0 = np.array([0.37, -0.61, 0.29])
+    if gridType == TYPE_MAC:
+        G = np.zeros((3, 3))
+        src = _linear_vel_mac(shape, np.array([0.2, -0.1, 0.3]), np.array([[0, 0.02, -0.01], [0.01, 0, 0.02], [-0.02, 0.01, 0]]))
+        pos = _face_positions(shape)
+        Gs = np.array([[0, 0.02, -0.01], [0.01, 0, 0.02], [-0.02, 0.01, 0]])
+        a0 = np.array([0.2, -0.1, 0.3])
+        exp = np.stack([a0[c] + (pos[c] - u0) @ Gs[c] for c in range(3)], -1)
+        if orderSpace == 2:
+            exp = np.stack([a0[c] + (pos[c] + 0.5 * np.eye(3)[c] - u0) @ Gs[c] for c in range(3)], -1)
+        vel = np.tile(u0.astype(np.float32), (n, 1))
+        g = src.copy()
+    else:
+        X = np.stack([i + 0.5, j + 0.5, k + 0.5], -1)
+        cf = np.array([[0.2, 0.03, -0.02, 0.01], [-0.1, 0.01, 0.02, -0.03], [0.3, -0.02, 0.01, 0.02]])
+        f = lambda P, cc: cf[cc, 0] + P @ cf[cc, 1:]
+        src = np.stack([f(X, cc) for cc in range(nc)], -1).astype(np.float32).reshape(n, nc)
+        exp = np.stack([f(X - u0, cc) for cc in range(nc)], -1)
+        vel = np.tile(u0.astype(np.float32), (n, 1))
+        g = src.reshape(-1).copy() if nc == 1 else src.copy()
+    for cm in (1, 2):
+        gg = g.copy()
+        o.advect_hi(shape, np.float32(1.0), flags, vel, gg, gridType, 2, 1.0, cm, orderSpace, orderTrace)
+        got = gg.reshape(nz, ny, nx, -1)
+        sl = (slice(4, -4),) * 3
+        diff = np.abs(got[sl] - exp.reshape(nz, ny, nx, -1)[sl]).max()
+        assert diff < 3e-5, ("high-order MacCormack", gridType, orderSpace, orderTrace, cm, diff)
+
+
+def high_order_pins():
+    pins = []
+    for is3d in (False, True):
+        for vec in (False, True):
+            pins.append(("cubic_weights-%dd-v%d" % (3 if is3d else 2, vec), lambda o, a=(is3d, vec): pin_cubic_weights(o, *a)))
+            pins.append(("cubic_quadratic-%dd-v%d" % (3 if is3d else 2, vec), lambda o, a=(is3d, vec): pin_cubic_quadratic(o, *a)))
+        for os_ in (1, 2):
+            for gt in (TYPE_REAL, TYPE_VEC3):
+                pins.append(("midpoint_centered-%dd-t%d-s%d" % (3 if is3d else 2, gt, os_),
+                             lambda o, a=(gt, os_, is3d): pin_midpoint_centered(o, *a)))
+            for self_adv in (False, True):
+                pins.append(("midpoint_mac-%dd-s%d-self%d" % (3 if is3d else 2, os_, self_adv),
+                             lambda o, a=(os_, is3d, self_adv): pin_midpoint_mac(o, *a)))
+    for gt in (TYPE_REAL, TYPE_VEC3, TYPE_MAC):
+        for os_, ot in ((2, 1), (1, 2), (2, 2)):
+            if gt == TYPE_MAC and ot == 2:
+                continue  # the MAC midpoint trace follows the advected field, not vel (pin_midpoint_mac)
+            pins.append(("hi_maccormack-t%d-s%d-tr%d" % (gt, os_, ot),
+                         lambda o, a=(gt, os_, ot): pin_high_order_maccormack(o, *a)))
     return pins
