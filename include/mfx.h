@@ -128,7 +128,9 @@ int mfx_apply_shape(mfx_grid* grid, int shape, const float* a, const float* b, f
 
 /* ------------------------------------------------------------- advection */
 /* advectSemiLagrange (source/plugin/advection.cpp:442-460); dt is the scene's.
- * orderSpace and orderTrace other than 1 are rejected with an error. */
+ * orderSpace 1 (trilinear) or 2 (cubic, source/util/interpolHigh.h) and
+ * orderTrace 1 (Euler) or 2 (midpoint, advection.cpp:33-37,58-73); other
+ * values are rejected with the reference's assertion text. */
 int mfx_advect_semi_lagrange(const mfx_grid* flags, const mfx_grid* vel, mfx_grid* grid, int order,
                              float strength, int orderSpace, int clampMode, int orderTrace);
 
@@ -184,6 +186,32 @@ int mfx_solve_pressure(mfx_grid* vel, mfx_grid* pressure, const mfx_grid* flags,
  * reference (its callers loop over solvePressure, plugin/pressure.cpp:480). */
 int mfx_solve_pressure_batch(int count, mfx_grid* const* vel, mfx_grid* const* pressure,
                              const mfx_grid* const* flags, const mfx_pressure_opts* opts, mfx_solve_stats* stats);
+
+/* solvePressure (without retRhs) that allocates nothing: the same RHS,
+ * unpreconditioned CG iterations and velocity correction as mfx_solve_pressure
+ * with preconditioner MFX_PC_NONE (or MFX_PC_MIC in 2D), bit-identical to it,
+ * with the right-hand side, CG vectors, matrix code and partial sums in one
+ * library-held device array instead of pooled allocations.  No counterpart in
+ * the reference; it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops).  Limits: at most 2^20 cells
+ * and 65536 partial-sum slots per launch (rejected with an error beyond).
+ * Every call on a GPU shares that GPU's array, so calls that overlap on one
+ * GPU (two scenes, two threads) race; serialise them.  Other preconditioners
+ * are rejected with an error. */
+int mfx_solve_pressure_noalloc(mfx_grid* vel, mfx_grid* pressure, const mfx_grid* flags,
+                               const mfx_pressure_opts* opts, mfx_solve_stats* stats);
+
+/* cgSolveDiffusion (source/conjugategrad.cpp:347-423): grid (Real, Vec3 or
+ * MAC; every component, two in 2D) <- solution of (I + alpha L) u = grid by
+ * unpreconditioned CG (maxIter = (int)(cgMaxIterFac * max size), x4 in 2D);
+ * stats (may be NULL) gets the last component's solve */
+int mfx_cg_solve_diffusion(const mfx_grid* flags, mfx_grid* grid, float alpha, float cgMaxIterFac, float cgAccuracy,
+                           mfx_solve_stats* stats);
+/* cgSolveWE (source/plugin/waves.cpp:87-141): one implicit wave-equation
+ * step with s = dt^2 cSqr / 2 on the scene's dt; out <- solution, then
+ * utm1 <- ut (buffer swap) and ut <- out */
+int mfx_cg_solve_we(const mfx_grid* flags, mfx_grid* ut, mfx_grid* utm1, mfx_grid* out, int crankNic, float cSqr,
+                    float cgMaxIterFac, float cgAccuracy, mfx_solve_stats* stats);
 
 /* computePressureRhs (source/plugin/pressure.cpp:277-299) */
 int mfx_compute_pressure_rhs(mfx_grid* rhs, const mfx_grid* vel, const mfx_grid* flags,

@@ -78,6 +78,9 @@ SIGNATURES = {
     "mfx_add_buoyancy": (_I, [_P, _P, _P, _FP, _F, _I]),
     "mfx_pressure_opts_default": (None, [C.POINTER(PressureOpts)]),
     "mfx_solve_pressure": (_I, [_P, _P, _P, C.POINTER(PressureOpts), C.POINTER(SolveStats), _P]),
+    "mfx_solve_pressure_noalloc": (_I, [_P, _P, _P, C.POINTER(PressureOpts), C.POINTER(SolveStats)]),
+    "mfx_cg_solve_diffusion": (_I, [_P, _P, _F, _F, _F, C.POINTER(SolveStats)]),
+    "mfx_cg_solve_we": (_I, [_P, _P, _P, _P, _I, _F, _F, _F, C.POINTER(SolveStats)]),
     "mfx_compute_pressure_rhs": (_I, [_P, _P, _P, C.POINTER(PressureOpts)]),
     "mfx_solve_pressure_system": (_I, [_P, _P, _P, C.POINTER(PressureOpts), C.POINTER(SolveStats)]),
     "mfx_solve_pressure_batch": (_I, [C.c_int, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P),

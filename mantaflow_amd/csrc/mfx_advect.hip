@@ -216,17 +216,235 @@ __global__ void __launch_bounds__(kBlock) k_sl_mac(Vel V, Dims d, Tiles t, float
 }
 
 // ---------------------------------------------------------------------------
+// orderSpace 2 (Catmull-Rom cubic lookups) and orderTrace 2 (explicit
+// midpoint trace).  Rarely used options: one plain gather kernel per SL step,
+// the MacCormack correction and clamp then read the backward values from
+// memory (k_maccormack_* with PRE).
+// ---------------------------------------------------------------------------
+
+// cubicInterp<Real> (util/interpolHigh.h:22-39): the coefficient expressions
+// are double (Real * double) and stored as float; the polynomial is float
+__device__ inline float cubic_real(float t, const float* p) {
+	const float d0 = (float)((double)(p[2] - p[0]) * 0.5);
+	const float d1 = (float)((double)(p[3] - p[1]) * 0.5);
+	const float dk = p[2] - p[1];
+	const float a2 = (float)(3.0 * (double)dk - 2.0 * (double)d0 - (double)d1);
+	const float a3 = (float)(-2.0 * (double)dk + (double)d0 + (double)d1);
+	const float sq = t * t, cu = sq * t;
+	return a3 * cu + a2 * sq + d0 * t + p[1];
+}
+// cubicInterp<Vec3> on one component: every Vec3 * double product is
+// narrowed on its own and Vec3 sums are float (util/vectorbase.h:235-279)
+__device__ inline float cubic_vec(float t, const float* p) {
+	const float d0 = (float)((double)(p[2] - p[0]) * 0.5);
+	const float d1 = (float)((double)(p[3] - p[1]) * 0.5);
+	const float dk = p[2] - p[1];
+	const float a2 = ((float)(3.0 * (double)dk) - (float)(2.0 * (double)d0)) - d1;
+	const float a3 = ((float)(-2.0 * (double)dk) + d0) + d1;
+	const float sq = t * t, cu = sq * t;
+	return a3 * cu + a2 * sq + d0 * t + p[1];
+}
+
+// interpolCubic / interpolCubic2D (util/interpolHigh.h:41-170) of one
+// component; VEC selects the Vec3 rounding rules.  A 4x4(x4) support that
+// leaves the grid falls back to interpol (:58-60, :97-99).  The support test
+// is written on x1 = (int)px itself (x1 - 1 >= 0 and x1 + 2 < nx), which is
+// the reference's test without its integer overflow for huge positions.
+template <bool VEC, bool IS3D, typename I>
+__device__ inline float cubic_lookup(const float* __restrict__ data, const Geo<IS3D, I>& g, float posx, float posy,
+                                     float posz) {
+	const float px = posx - 0.5f, py = posy - 0.5f;
+	const int x1 = (int)px, y1 = (int)py;
+	const I Y = g.Y;
+	if (!IS3D) {
+		if (x1 < 1 || y1 < 1 || x1 > g.nx - 3 || y1 > g.ny - 3) return interp(data, g, posx, posy, posz);
+		const float xt = px - (float)x1, yt = py - (float)y1;
+		const float* __restrict__ b = data + g.at(x1 - 1, y1 - 1, 0);
+		float rows[4];
+#pragma unroll
+		for (int r = 0; r < 4; r++) {
+			const float pts[4] = {b[r * Y], b[r * Y + 1], b[r * Y + 2], b[r * Y + 3]};
+			rows[r] = VEC ? cubic_vec(xt, pts) : cubic_real(xt, pts);
+		}
+		return VEC ? cubic_vec(yt, rows) : cubic_real(yt, rows);
+	}
+	const float pz = posz - 0.5f;
+	const int z1 = (int)pz;
+	if (x1 < 1 || y1 < 1 || z1 < 1 || x1 > g.nx - 3 || y1 > g.ny - 3 || z1 > g.nz - 3)
+		return interp(data, g, posx, posy, posz);
+	const float xt = px - (float)x1, yt = py - (float)y1, zt = pz - (float)z1;
+	const I Z = g.Z;
+	const float* __restrict__ b = data + g.at(x1 - 1, y1 - 1, z1 - 1);
+	float slabs[4];
+#pragma unroll
+	for (int c = 0; c < 4; c++) {
+		float rows[4];
+#pragma unroll
+		for (int r = 0; r < 4; r++) {
+			const float* __restrict__ q = b + c * Z + r * Y;
+			const float pts[4] = {q[0], q[1], q[2], q[3]};
+			rows[r] = VEC ? cubic_vec(xt, pts) : cubic_real(xt, pts);
+		}
+		slabs[c] = VEC ? cubic_vec(yt, rows) : cubic_real(yt, rows);
+	}
+	return VEC ? cubic_vec(zt, slabs) : cubic_real(zt, slabs);
+}
+
+// BUILD_INDEX of one axis (util/interpol.h:50-66): cell and upper weight
+__device__ inline void build_axis(float p, int n, bool clampHi, int& xi, float& s1) {
+	xi = (int)p;
+	s1 = p - (float)xi;
+	if (p < 0.f) {
+		xi = 0;
+		s1 = 0.f;
+	}
+	if (clampHi && xi >= n - 1) {
+		xi = n - 2;
+		s1 = 1.f;
+	}
+}
+
+// interpolMAC (util/interpol.h:121-157): every component of a MAC grid at
+// pos, each with the shifted (BUILD_INDEX_SHIFT) weights along its own axis.
+// In 2D the lookups stay in plane 0 (the reference would read outside the
+// grid for a z position of 1.5 or more there).
+template <bool IS3D, typename I>
+__device__ inline void interp_mac(const Vel& S, const Geo<IS3D, I>& g, float x, float y, float z, float* out) {
+	int xi, yi, zi, sx, sy, sz;
+	float s1, t1, f1, ss1, st1, sf1;
+	build_axis(x - 0.5f, g.nx, true, xi, s1);
+	build_axis(y - 0.5f, g.ny, true, yi, t1);
+	build_axis(z - 0.5f, g.nz, g.nz > 1, zi, f1);
+	build_axis(x, g.nx, true, sx, ss1);
+	build_axis(y, g.ny, true, sy, st1);
+	build_axis(z, g.nz, g.nz > 1, sz, sf1);
+	const float s0 = 1.0f - s1, t0 = 1.0f - t1, f0 = 1.0f - f1;
+	const float ss0 = 1.0f - ss1, st0 = 1.0f - st1, sf0 = 1.0f - sf1;
+	const I Y = g.Y, Z = g.Z;
+	{
+		const float* __restrict__ r = S.u + g.at(sx, yi, IS3D ? zi : 0);
+		out[0] = f0 * ((r[0] * t0 + r[Y] * t1) * ss0 + (r[1] * t0 + r[1 + Y] * t1) * ss1) +
+		         f1 * ((r[Z] * t0 + r[Z + Y] * t1) * ss0 + (r[1 + Z] * t0 + r[1 + Y + Z] * t1) * ss1);
+	}
+	{
+		const float* __restrict__ r = S.v + g.at(xi, sy, IS3D ? zi : 0);
+		out[1] = f0 * ((r[0] * st0 + r[Y] * st1) * s0 + (r[1] * st0 + r[1 + Y] * st1) * s1) +
+		         f1 * ((r[Z] * st0 + r[Z + Y] * st1) * s0 + (r[1 + Z] * st0 + r[1 + Y + Z] * st1) * s1);
+	}
+	{
+		const float* __restrict__ r = S.w + g.at(xi, yi, IS3D ? sz : 0);
+		out[2] = sf0 * ((r[0] * t0 + r[Y] * t1) * s0 + (r[1] * t0 + r[1 + Y] * t1) * s1) +
+		         sf1 * ((r[Z] * t0 + r[Z + Y] * t1) * s0 + (r[1 + Z] * t0 + r[1 + Y + Z] * t1) * s1);
+	}
+}
+
+// SemiLagrange<T> with orderTrace 1/2 and orderSpace 1/2 (advection.cpp:25-41,
+// grid.h:146-151): the midpoint trace takes p1 = p0 - getCentered*dt*0.5 and
+// the interpolated velocity there; NC == 3 uses the Vec3 cubic rules
+template <int NC, bool IS3D, typename I>
+__global__ void __launch_bounds__(kBlock) k_sl_centered_hi(Vel V, Dims d, Tiles t, float dt, int orderSpace,
+                                                           int orderTrace, const float* __restrict__ s0,
+                                                           const float* __restrict__ s1, const float* __restrict__ s2,
+                                                           float* __restrict__ o0, float* __restrict__ o1,
+                                                           float* __restrict__ o2) {
+	const CellPos p = tile_cell(t);
+	if (p.i >= d.nx || p.j >= d.ny) return;
+	const Geo<IS3D, I> g(d);
+	const I ix = g.at(p.i, p.j, p.k);
+	float* dst[3] = {o0, o1, o2};
+	if (!g.interior(p.i, p.j, p.k)) {
+		for (int c = 0; c < NC; c++) dst[c][ix] = 0.f;
+		return;
+	}
+	float v[3];
+	getCentered(V, g, ix, v);
+	const float p0[3] = {(float)p.i + 0.5f, (float)p.j + 0.5f, (float)p.k + 0.5f};
+	float pos[3];
+	if (orderTrace == 2) {
+		float p1[3], u[3];
+		for (int a = 0; a < 3; a++) p1[a] = p0[a] - (float)((double)(v[a] * dt) * 0.5);
+		interp_mac(V, g, p1[0], p1[1], p1[2], u);
+		for (int a = 0; a < 3; a++) pos[a] = p0[a] - u[a] * dt;
+	} else {
+		for (int a = 0; a < 3; a++) pos[a] = p0[a] - v[a] * dt;
+	}
+	const float* src[3] = {s0, s1, s2};
+	for (int c = 0; c < NC; c++)
+		dst[c][ix] = orderSpace == 2 ? cubic_lookup<NC == 3>(src[c], g, pos[0], pos[1], pos[2])
+		                             : interp(src[c], g, pos[0], pos[1], pos[2]);
+}
+
+// SemiLagrangeMAC with orderTrace 1/2 and orderSpace 1/2 (advection.cpp:44-77).
+// The midpoint trace starts at the face centre and takes the face velocity
+// and the interpolated velocity from S, the field being advected (:59-73).
+// orderSpace 2 is interpolCubicMAC(pos)[c] = interpolCubic<Vec3>(pos + e_c/2)[c],
+// 0 for z in 2D (grid.h:268-273, interpolHigh.h:172-179).
+template <bool IS3D, typename I>
+__global__ void __launch_bounds__(kBlock) k_sl_mac_hi(Vel V, Dims d, Tiles t, float dt, int orderSpace,
+                                                      int orderTrace, Vel S, float* __restrict__ ou,
+                                                      float* __restrict__ ov, float* __restrict__ ow) {
+	const CellPos p = tile_cell(t);
+	if (p.i >= d.nx || p.j >= d.ny) return;
+	const Geo<IS3D, I> g(d);
+	const I ix = g.at(p.i, p.j, p.k);
+	if (!g.interior(p.i, p.j, p.k)) {
+		ou[ix] = 0.f;
+		ov[ix] = 0.f;
+		ow[ix] = 0.f;
+		return;
+	}
+	const Vel& F = orderTrace == 2 ? S : V;
+	float vf[3][3];
+	getAtMACX(F, g, ix, vf[0]);
+	getAtMACY(F, g, ix, vf[1]);
+	getAtMACZ(F, g, ix, vf[2]);
+	const float h[3] = {(float)p.i + 0.5f, (float)p.j + 0.5f, (float)p.k + 0.5f};
+	const float cell[3] = {(float)p.i, (float)p.j, (float)p.k};
+	const float* src[3] = {S.u, S.v, S.w};
+	float out[3];
+	for (int c = 0; c < 3; c++) {
+		float pos[3];
+		if (orderTrace == 2) {
+			float p1[3], u[3];
+			for (int a = 0; a < 3; a++) p1[a] = (a == c ? cell[a] : h[a]) - (float)((double)(vf[c][a] * dt) * 0.5);
+			interp_mac(S, g, p1[0], p1[1], p1[2], u);
+			for (int a = 0; a < 3; a++) pos[a] = h[a] - u[a] * dt;
+		} else {
+			for (int a = 0; a < 3; a++) pos[a] = h[a] - vf[c][a] * dt;
+		}
+		if (orderSpace == 2) {
+			if (c == 2 && !IS3D) {
+				out[c] = 0.f;
+			} else {
+				pos[c] = pos[c] + 0.5f;
+				out[c] = cubic_lookup<true>(src[c], g, pos[0], pos[1], pos[2]);
+			}
+		} else {
+			out[c] = interp(src[c], g, pos[0], pos[1], pos[2]);
+		}
+	}
+	ou[ix] = out[0];
+	ov[ix] = out[1];
+	ow[ix] = out[2];
+}
+
+// ---------------------------------------------------------------------------
 // MacCormack for centered grids: backward SL (fwd, -dt), correction
 // (MacCormackCorrect, all idx) and clamp (MacCormackClamp, bnd=1) fused.
 // ---------------------------------------------------------------------------
-template <int NC, bool IS3D, typename I>
+// With PRE the backward values come from pb0..pb2 (a backward SL step of
+// orderSpace / orderTrace 2 run as its own pass, zero on the outer layer).
+template <int NC, bool IS3D, typename I, bool PRE = false>
 __global__ void __launch_bounds__(kBlock) k_maccormack_centered(Vel V, Dims d, Tiles t, float dt, float strength,
                                                                 int clampMode, const uint8_t* __restrict__ f8,
                                                                 const float* o0, const float* o1, const float* o2,
                                                                 const float* __restrict__ fw0,
                                                                 const float* __restrict__ fw1,
                                                                 const float* __restrict__ fw2, float* __restrict__ n0,
-                                                                float* __restrict__ n1, float* __restrict__ n2) {
+                                                                float* __restrict__ n1, float* __restrict__ n2,
+                                                                const float* __restrict__ pb0 = nullptr,
+                                                                const float* __restrict__ pb1 = nullptr,
+                                                                const float* __restrict__ pb2 = nullptr) {
 	const CellPos p = tile_cell(t);
 	if (p.i >= d.nx || p.j >= d.ny) return;
 	const Geo<IS3D, I> g(d);
@@ -249,7 +467,9 @@ __global__ void __launch_bounds__(kBlock) k_maccormack_centered(Vel V, Dims d, T
 	float val[3], fv[3];
 #pragma unroll
 	for (int c = 0; c < NC; c++) {
-		const float bwd = inner ? interp(fwd[c], g, px, py, pz) : 0.f;
+		float bwd;
+		if constexpr (PRE) bwd = (c == 0 ? pb0 : (c == 1 ? pb1 : pb2))[ix];
+		else bwd = inner ? interp(fwd[c], g, px, py, pz) : 0.f;
 		fv[c] = fwd[c][ix];
 		float x = fv[c];
 		if (fl) {
@@ -483,13 +703,17 @@ __device__ inline float clamp_comp_mac(const Geo<IS3D, I>& g, const uint8_t* __r
 // (MacCormackCorrectMAC, all ijk) and clamp (MacCormackClampMAC, bnd=1) fused.
 // The face velocities of the cell are read once and serve both the backward
 // trace and the clamp.
-template <bool IS3D, typename I>
+// With PRE the backward values come from pbu/pbv/pbw (as k_maccormack_centered).
+template <bool IS3D, typename I, bool PRE = false>
 __global__ void __launch_bounds__(kBlock) k_maccormack_mac(Vel V, Dims d, Tiles t, float dt, float strength,
                                                            int clampMode, const uint8_t* __restrict__ f8,
                                                            const float* ou, const float* ov, const float* ow,
                                                            const float* __restrict__ fu, const float* __restrict__ fv,
                                                            const float* __restrict__ fw, float* __restrict__ nu,
-                                                           float* __restrict__ nv, float* __restrict__ nw) {
+                                                           float* __restrict__ nv, float* __restrict__ nw,
+                                                           const float* __restrict__ pbu = nullptr,
+                                                           const float* __restrict__ pbv = nullptr,
+                                                           const float* __restrict__ pbw = nullptr) {
 	const CellPos p = tile_cell(t);
 	if (p.i >= d.nx || p.j >= d.ny) return;
 	const int i = p.i, j = p.j, k = p.k;
@@ -502,7 +726,13 @@ __global__ void __launch_bounds__(kBlock) k_maccormack_mac(Vel V, Dims d, Tiles 
 		getAtMACX(V, g, ix, vx);
 		getAtMACY(V, g, ix, vy);
 		getAtMACZ(V, g, ix, vz);
-		sl_mac_cell(g, i, j, k, -dt, vx, vy, vz, fu, fv, fw, bwd);
+		if constexpr (PRE) {
+			bwd[0] = pbu[ix];
+			bwd[1] = pbv[ix];
+			bwd[2] = pbw[ix];
+		} else {
+			sl_mac_cell(g, i, j, k, -dt, vx, vy, vz, fu, fv, fw, bwd);
+		}
 	}
 	// MacCormackCorrectMAC skip logic (isMAC = true)
 	bool skip[3] = {false, false, false};
@@ -901,7 +1131,117 @@ static void with_geo(const Dims& d, F&& f) {
 	}
 }
 
-void op_advect(Scene* s, Grid* flags, Grid* vel, Grid* grid, int order, float strength, int clampMode) {
+// applyOutflowBC(flags, result, orig, dt) (advection.cpp:387-391):
+// timeStep = max(1.0, dt*4); the extrapolated values go to a scratch grid and
+// are copied back into result
+static void outflow_bc(Scene* s, Grid* flags, Grid* grid, float* result) {
+	const Dims d = s->d;
+	const Tiles t = tiles_for(d);
+	const double tsd = (double)s->dt * 4;
+	const float ts = (float)((1.0 < tsd) ? tsd : 1.0);
+	float* vdst = s->getVec();
+	float* r3[3] = {result, result + s->cstride, result + 2 * s->cstride};
+	float* d3[3] = {vdst, vdst + s->cstride, vdst + 2 * s->cstride};
+	const Tiles tl = tiles_for(d, kLightKZ);
+	hipLaunchKernelGGL(k_outflow_extrap, dim3((unsigned)t.count()), dim3(kBlock), 0, s->stream, d, t, flags->f8,
+	                   (const float*)r3[0], (const float*)r3[1], (const float*)r3[2], (const float*)grid->comp(0),
+	                   (const float*)grid->comp(1), (const float*)grid->comp(2), ts, d3[0], d3[1], d3[2]);
+	hipLaunchKernelGGL(k_outflow_copy, dim3((unsigned)tl.count()), dim3(kBlock), 0, s->stream, d, tl, flags->f8,
+	                   (const float*)d3[0], (const float*)d3[1], (const float*)d3[2], r3[0], r3[1], r3[2]);
+	s->putVec(vdst);
+}
+
+// fnAdvectSemiLagrange with orderSpace or orderTrace 2 (advection.cpp:292-321,
+// 406-436): forward SL, backward SL of the forward result and the fused
+// correction + clamp as three passes
+static void op_advect_hi(Scene* s, Grid* flags, Grid* vel, Grid* grid, int order, float strength, int clampMode,
+                         int orderSpace, int orderTrace) {
+	const Dims d = s->d;
+	const Tiles t = tiles_for(d);
+	const dim3 gb((unsigned)t.count()), tb(kBlock);
+	const float dt = s->dt;
+	const Vel V = vel_of(vel);
+	const bool mac = grid->isMAC();
+	const int nc = grid->isVec() ? 3 : 1;
+	auto take = [&]() { return nc == 3 ? s->getVec() : s->getReal(); };
+	auto give = [&](float* q) {
+		if (nc == 3) s->putVec(q);
+		else s->putReal(q);
+	};
+	auto comps = [&](float* q, float** c3) {
+		c3[0] = q;
+		c3[1] = nc == 3 ? q + s->cstride : nullptr;
+		c3[2] = nc == 3 ? q + 2 * s->cstride : nullptr;
+	};
+	// one SL step of src into dst with time step h
+	auto sl = [&](float* const* src, float* const* dst, float h) {
+		with_geo(d, [&](auto D3, auto ix) {
+			constexpr bool D = decltype(D3)::value;
+			using I = decltype(ix);
+			if (mac)
+				hipLaunchKernelGGL((k_sl_mac_hi<D, I>), gb, tb, 0, s->stream, V, d, t, h, orderSpace, orderTrace,
+				                   Vel{src[0], src[1], src[2]}, dst[0], dst[1], dst[2]);
+			else if (nc == 1)
+				hipLaunchKernelGGL((k_sl_centered_hi<1, D, I>), gb, tb, 0, s->stream, V, d, t, h, orderSpace,
+				                   orderTrace, (const float*)src[0], (const float*)src[1], (const float*)src[2],
+				                   dst[0], dst[1], dst[2]);
+			else
+				hipLaunchKernelGGL((k_sl_centered_hi<3, D, I>), gb, tb, 0, s->stream, V, d, t, h, orderSpace,
+				                   orderTrace, (const float*)src[0], (const float*)src[1], (const float*)src[2],
+				                   dst[0], dst[1], dst[2]);
+		});
+	};
+	float* orig[3];
+	comps(grid->real(), orig);
+	float* fwd = take();
+	float* fw[3];
+	comps(fwd, fw);
+	sl(orig, fw, dt);
+	float* result = fwd;
+	if (order == 2) {
+		float* bwd = take();
+		float* ng = take();
+		float *bw[3], *nw[3];
+		comps(bwd, bw);
+		comps(ng, nw);
+		sl(fw, bw, -dt);
+		with_geo(d, [&](auto D3, auto ix) {
+			constexpr bool D = decltype(D3)::value;
+			using I = decltype(ix);
+			if (mac)
+				hipLaunchKernelGGL((k_maccormack_mac<D, I, true>), gb, tb, 0, s->stream, V, d, t, dt, strength,
+				                   clampMode, flags->f8, (const float*)orig[0], (const float*)orig[1],
+				                   (const float*)orig[2], (const float*)fw[0], (const float*)fw[1],
+				                   (const float*)fw[2], nw[0], nw[1], nw[2], (const float*)bw[0], (const float*)bw[1],
+				                   (const float*)bw[2]);
+			else if (nc == 1)
+				hipLaunchKernelGGL((k_maccormack_centered<1, D, I, true>), gb, tb, 0, s->stream, V, d, t, dt,
+				                   strength, clampMode, flags->f8, (const float*)orig[0], (const float*)orig[1],
+				                   (const float*)orig[2], (const float*)fw[0], (const float*)fw[1],
+				                   (const float*)fw[2], nw[0], nw[1], nw[2], (const float*)bw[0], (const float*)bw[1],
+				                   (const float*)bw[2]);
+			else
+				hipLaunchKernelGGL((k_maccormack_centered<3, D, I, true>), gb, tb, 0, s->stream, V, d, t, dt,
+				                   strength, clampMode, flags->f8, (const float*)orig[0], (const float*)orig[1],
+				                   (const float*)orig[2], (const float*)fw[0], (const float*)fw[1],
+				                   (const float*)fw[2], nw[0], nw[1], nw[2], (const float*)bw[0], (const float*)bw[1],
+				                   (const float*)bw[2]);
+		});
+		give(bwd);
+		give(fwd);
+		result = ng;
+	}
+	if (mac) outflow_bc(s, flags, grid, result);
+	give(grid->real());
+	grid->data = result;
+}
+
+void op_advect(Scene* s, Grid* flags, Grid* vel, Grid* grid, int order, float strength, int clampMode, int orderSpace,
+               int orderTrace) {
+	if (orderSpace != 1 || orderTrace != 1) {
+		op_advect_hi(s, flags, vel, grid, order, strength, clampMode, orderSpace, orderTrace);
+		return;
+	}
 	const Dims d = s->d;
 	const Tiles t = tiles_for(d);
 	const dim3 gb((unsigned)t.count()), tb(kBlock);
@@ -939,22 +1279,7 @@ void op_advect(Scene* s, Grid* flags, Grid* vel, Grid* grid, int order, float st
 			s->putVec(fwd);
 			result = ng;
 		}
-		// applyOutflowBC(flags, result, orig=grid, dt): timeStep = max(1.0, dt*4)
-		{
-			const double tsd = (double)dt * 4;
-			const float ts = (float)((1.0 < tsd) ? tsd : 1.0);
-			float* vdst = s->getVec();
-			float* r3[3] = {result, result + s->cstride, result + 2 * s->cstride};
-			float* d3[3] = {vdst, vdst + s->cstride, vdst + 2 * s->cstride};
-			const Tiles tl = tiles_for(d, kLightKZ);
-			const dim3 gl((unsigned)tl.count());
-			hipLaunchKernelGGL(k_outflow_extrap, gb, tb, 0, s->stream, d, t, flags->f8, (const float*)r3[0],
-			                   (const float*)r3[1], (const float*)r3[2], (const float*)grid->comp(0),
-			                   (const float*)grid->comp(1), (const float*)grid->comp(2), ts, d3[0], d3[1], d3[2]);
-			hipLaunchKernelGGL(k_outflow_copy, gl, tb, 0, s->stream, d, tl, flags->f8, (const float*)d3[0],
-			                   (const float*)d3[1], (const float*)d3[2], r3[0], r3[1], r3[2]);
-			s->putVec(vdst);
-		}
+		outflow_bc(s, flags, grid, result);
 		// orig.swap(result)
 		s->putVec(grid->real());
 		grid->data = result;
@@ -1048,15 +1373,17 @@ int mfx_advect_semi_lagrange(const mfx_grid* flags, const mfx_grid* vel, mfx_gri
 	API_BEGIN
 	MFX_CHECK(flags && vel && grid, "advectSemiLagrange: flags, vel and grid are required");
 	MFX_CHECK(order == 1 || order == 2, "AdvectSemiLagrange: Only order 1 (regular SL) and 2 (MacCormack) supported");
-	MFX_CHECK(orderSpace == 1, "advectSemiLagrange: orderSpace != 1 (cubic interpolation) is not supported on the GPU path");
-	MFX_CHECK(orderTrace == 1, "advectSemiLagrange: orderTrace != 1 (midpoint backtrace) is not supported on the GPU path");
+	// the reference asserts "Unknown interpolation order" / "Unknown
+	// backtracing order" (grid.h:150, advection.cpp:39)
+	MFX_CHECK(orderSpace == 1 || orderSpace == 2, "Unknown interpolation order " + std::to_string(orderSpace));
+	MFX_CHECK(orderTrace == 1 || orderTrace == 2, "Unknown backtracing order " + std::to_string(orderTrace));
 	Grid *f = G(flags), *v = G(vel), *g = G(grid);
 	MFX_CHECK(f->isFlags(), "advectSemiLagrange: flags must be a FlagGrid");
 	MFX_CHECK(v->isMAC(), "advectSemiLagrange: vel must be a MACGrid");
 	same_scene(f, v);
 	same_scene(f, g);
 	MFX_CHECK(g->isReal() || g->isVec(), "AdvectSemiLagrange: Grid Type is not supported (only Real, Vec3, MAC, Levelset)");
-	op_advect(f->s, f, v, g, order, strength, clampMode);
+	op_advect(f->s, f, v, g, order, strength, clampMode, orderSpace, orderTrace);
 	API_END
 }
 

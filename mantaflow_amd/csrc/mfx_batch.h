@@ -83,6 +83,10 @@ struct MatvecP {
 	float* pnew;
 	float* q;
 	double* part;
+	// scaled Laplace matrices of the other GridCg clients (cgSolveDiffusion,
+	// cgSolveWE): diagonal fl(fl(cnt * scale) + 1), off-diagonals -scale
+	int scaled = 0;
+	float scale = 0.f;
 };
 // B: x += alpha p, r -= alpha q, tiles striding over the float4 groups
 struct UpdateP {

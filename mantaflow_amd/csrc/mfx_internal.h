@@ -285,6 +285,7 @@ double reduce_sum_double(Scene* s, const float* a, int mode, long long n);
 // kernels / ops implemented in the various .hip files
 void op_set_wall_bcs(Scene* s, Grid* flags, Grid* vel, Grid* obvel);
 void op_add_buoyancy(Scene* s, Grid* flags, Grid* density, Grid* vel, const float f[3]);
-void op_advect(Scene* s, Grid* flags, Grid* vel, Grid* grid, int order, float strength, int clampMode);
+void op_advect(Scene* s, Grid* flags, Grid* vel, Grid* grid, int order, float strength, int clampMode, int orderSpace,
+               int orderTrace);
 
 }  // namespace mfx

@@ -15,6 +15,11 @@ struct CgState {
 	int iter, maxIter, done, converged, diverged, useL2, first, pad;
 	float hist[64];  // residual norm per iteration (batch sizing)
 	double trueRR;   // ||b - A x||^2 with FP64 row sums, after the solve
+	// zero-pressure fixing (pressure.cpp:349-388), chosen on the device
+	long long fix;                 // pinned cell, -1 if none
+	unsigned long long fixFirst;   // first fluid cell of the bnd=1 interior (~0: none)
+	int fixAnyEmpty;               // some cell is empty: nothing is pinned
+	int fixPad;
 };
 
 // canonical full stencil of a coarse level (27 entries in 3D, 9 in 2D,

@@ -67,9 +67,10 @@ __device__ inline int off_coef(const Dims& d, const uint8_t* f8, long long c, in
 // first (offsets 0 where the neighbour is outside the grid: those flags are
 // then never used).
 __global__ void __launch_bounds__(kBlock) k_build_code(Dims d, Tiles t, const uint8_t* __restrict__ f8,
-                                                       uint8_t* __restrict__ code, long long fix) {
+                                                       uint8_t* __restrict__ code, const CgState* __restrict__ st) {
 	const CellPos p = tile_cell(t);
 	if (p.i >= d.nx || p.j >= d.ny) return;
+	const long long fix = st->fix;
 	const int i = p.i, j = p.j;
 	constexpr int KZ = kLightKZ;
 	const long long oxm = i > 0 ? 1 : 0, oxp = i < d.nx - 1 ? 1 : 0;
@@ -125,6 +126,14 @@ __device__ inline float cA0(uint8_t c) { return (float)(c & 7); }
 __device__ inline float cAi(uint8_t c) { return (c & 8) ? -1.0f : 0.0f; }
 __device__ inline float cAj(uint8_t c) { return (c & 16) ? -1.0f : 0.0f; }
 __device__ inline float cAk(uint8_t c) { return (c & 32) ? -1.0f : 0.0f; }
+// Scaled matrices (cgSolveDiffusion, conjugategrad.cpp:355-373; cgSolveWE,
+// waves.cpp:106-115): A0 (= cnt, a float) *= scale; A0 += 1. (a double sum
+// stored as float); off-diagonals -1 * scale
+__device__ inline float sA0(uint8_t c, float scale) {
+	const float a = (float)(c & 7) * scale;
+	return (float)((double)a + 1.0);
+}
+__device__ inline float sOff(uint8_t c, int bit, float scale) { return (c & bit) ? -scale : 0.0f; }
 
 // ApplyMatrix / ApplyMatrix2D at cell c (conjugategrad.h:118-151) with
 // src values given by a functor; out-of-range linear neighbours contribute 0
@@ -329,7 +338,136 @@ __global__ void k_cg_state_init(CgState* st, float acc, int maxIter, int useL2, 
 	z.maxIter = maxIter;
 	z.useL2 = useL2;
 	z.relTol = relTol;
+	z.fix = -1;
+	z.fixFirst = ~0ull;
 	*st = z;
+}
+
+// Zero-pressure cell selection (pressure.cpp:349-381) on the device, in two
+// kernels: k_fix_scan notes whether any cell is empty (then nothing is
+// pinned) and the first fluid cell of the bnd=1 interior in ijk order, i.e.
+// the smallest such index; k_fix_select takes the first fluid cell among the
+// top-centre positions (nx/2, ny-1-o, nz/2), o = 0, 1, 2, else that cell,
+// and zeroes its right-hand side (fixPressure, pressure.cpp:226-245).
+__global__ void __launch_bounds__(kBlock) k_fix_scan(Dims d, Tiles t, const uint8_t* __restrict__ f8,
+                                                     CgState* st) {
+	const CellPos p = tile_cell(t);
+	const bool in = p.i < d.nx && p.j < d.ny;
+	const long long c = in ? d.idx(p.i, p.j, p.k) : 0;
+	const uint8_t f = in ? f8[c] : (uint8_t)0;
+	const bool inner = p.i >= 1 && p.i < d.nx - 1 && p.j >= 1 && p.j < d.ny - 1 &&
+	                   (!d.is3d || (p.k >= 1 && p.k < d.nz - 1));
+	unsigned long long first = (in && inner && (f & kFluid)) ? (unsigned long long)c : ~0ull;
+	for (int o = 32; o > 0; o >>= 1) {
+		const unsigned long long q = __shfl_xor(first, o, 64);
+		first = q < first ? q : first;
+	}
+	const bool anyEmpty = __ballot((f & kEmpty) != 0) != 0;
+	if ((threadIdx.x & 63) == 0) {
+		if (first != ~0ull) atomicMin(&st->fixFirst, first);
+		if (anyEmpty) atomicOr(&st->fixAnyEmpty, 1);
+	}
+}
+__global__ void k_fix_select(Dims d, const uint8_t* __restrict__ f8, CgState* st, float* __restrict__ rhs) {
+	if (threadIdx.x != 0) return;
+	long long fix = -1;
+	if (!st->fixAnyEmpty) {
+		const int tcx = d.nx / 2, tcy = d.ny - 1, tcz = d.is3d ? d.nz / 2 : 0;
+		for (int o = 0; o < 3 && fix < 0; o++) {
+			const int y = tcy - o;
+			if (y < 0) continue;
+			const long long c = d.idx(tcx, y, tcz);
+			if (f8[c] & kFluid) fix = c;
+		}
+		if (fix < 0 && st->fixFirst != ~0ull) fix = (long long)st->fixFirst;
+	}
+	st->fix = fix;
+	if (fix >= 0) rhs[fix] = 0.f;
+}
+
+// enforceCompatibility (pressure.cpp:297-298): rhs += (Real)(-sum / (Real)cnt)
+// over every cell; sum and cnt are the FP64 partials of k_make_rhs (pairs
+// per block), added in a fixed order by one block
+__global__ void __launch_bounds__(1024) k_compat_corr(const double* __restrict__ part, int nb,
+                                                      float* __restrict__ corr) {
+	__shared__ double sh[2][16];
+	double sum = 0, cnt = 0;
+	for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+		sum += part[2 * b];
+		cnt += part[2 * b + 1];
+	}
+	sum = wave_sum(sum);
+	cnt = wave_sum(cnt);
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	if (lane == 0) {
+		sh[0][w] = sum;
+		sh[1][w] = cnt;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		double a = 0, c = 0;
+		for (int q = 0; q < (int)(blockDim.x >> 6); q++) {
+			a += sh[0][q];
+			c += sh[1][q];
+		}
+		*corr = (float)(-a / (double)(float)c);
+	}
+}
+__global__ void k_add_const_dev(float* a, long long n, const float* __restrict__ v) {
+	const float c = *v;
+	for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+		a[i] += c;
+}
+
+// Matrix code of cgSolveDiffusion (conjugategrad.cpp:355-373): the Laplace
+// matrix of an all-fluid grid (every bnd=1 interior cell: 4 or 6 counted
+// neighbours, off-diagonals toward every upper neighbour), obstacle rows of
+// the real flags replaced by identity rows; the fluid bit (ApplyMatrix's
+// identity test) is the real flags'.  Scaled by alpha in the matvec (sA0, sOff).
+__global__ void __launch_bounds__(kBlock) k_build_code_diff(Dims d, Tiles t, const uint8_t* __restrict__ f8,
+                                                            uint8_t* __restrict__ code) {
+	const CellPos p = tile_cell(t);
+	if (p.i >= d.nx || p.j >= d.ny) return;
+	const bool inIJ = p.i >= 1 && p.i < d.nx - 1 && p.j >= 1 && p.j < d.ny - 1;
+#pragma unroll
+	for (int q = 0; q < kLightKZ; q++) {
+		const int k = p.k + q;
+		if (k >= d.nz) break;
+		const long long c = d.idx(p.i, p.j, k);
+		const uint8_t f = f8[c];
+		const bool inner = inIJ && (!d.is3d || (k >= 1 && k < d.nz - 1));
+		const bool rows = inner && !(f & kObstacle);
+		const int cnt = rows ? (d.is3d ? 6 : 4) : 0;
+		const int off = rows ? (8 | 16 | (d.is3d ? 32 : 0)) : 0;
+		code[c] = (uint8_t)(cnt | off | ((f & kFluid) ? 64 : 0));
+	}
+}
+
+// MakeRhsWE (waves.cpp:69-79) after rhs.clear(): bnd=1 cells get
+// (Real)(2 ut - utm1) in double; with crankNic, rhs += s * (x-y 5-point
+// stencil of ut) (double, stored as float); every other cell 0
+__global__ void __launch_bounds__(kBlock) k_make_rhs_we(Dims d, Tiles t, const float* __restrict__ ut,
+                                                        const float* __restrict__ utm1, float s, int crankNic,
+                                                        float* __restrict__ rhs) {
+	const CellPos p = tile_cell(t);
+	if (p.i >= d.nx || p.j >= d.ny) return;
+	const bool inIJ = p.i >= 1 && p.i < d.nx - 1 && p.j >= 1 && p.j < d.ny - 1;
+	const long long X = 1, Y = d.nx;
+#pragma unroll
+	for (int q = 0; q < kLightKZ; q++) {
+		const int k = p.k + q;
+		if (k >= d.nz) break;
+		const long long c = d.idx(p.i, p.j, k);
+		float v = 0.f;
+		if (inIJ && (!d.is3d || (k >= 1 && k < d.nz - 1))) {
+			v = (float)(2. * (double)ut[c] - (double)utm1[c]);
+			if (crankNic)
+				v = (float)((double)v + (double)s * (-4. * (double)ut[c] + 1. * (double)ut[c - X] +
+				                                     1. * (double)ut[c + X] + 1. * (double)ut[c - Y] +
+				                                     1. * (double)ut[c + Y]));
+		}
+		rhs[c] = v;
+	}
 }
 
 // st->trueRR = sum of nb true-residual partials (fixed order)
@@ -368,7 +506,7 @@ __device__ __forceinline__ void body_dot(const DotP& P, long long blk, long long
 // Loads run one plane ahead as raw values (z, p_old, code): p_new = z +
 // beta p_old is formed only when a plane enters LDS, so the loads of plane
 // k+2 are in flight during the whole step of plane k.
-template <int TY>
+template <int TY, bool SC = false>
 __device__ __forceinline__ void body_matvec(const MatvecP& P, long long blk, long long slot) {
 	constexpr int TX = 64, RX = TX + 2, RY = TY + 2, NH = 2 * RX + 2 * (RY - 2);
 	// plane buffers; the 64 slots past the ring take the writes of lanes
@@ -476,7 +614,12 @@ __device__ __forceinline__ void body_matvec(const MatvecP& P, long long blk, lon
 			const uint8_t* __restrict__ cp = cs[b];
 			float qv;
 			if (!(cCur & 64)) qv = pCur;
-			else if (d.is3d)
+			else if (SC) {
+				const float sc = P.scale;
+				qv = pCur * sA0(cCur, sc) + pp[L - 1] * sOff(cp[L - 1], 8, sc) + pp[L + 1] * sOff(cCur, 8, sc) +
+				     pp[L - RX] * sOff(cp[L - RX], 16, sc) + pp[L + RX] * sOff(cCur, 16, sc);
+				if (d.is3d) qv = qv + pPrev * sOff(cPrev, 32, sc) + pNext * sOff(cCur, 32, sc);
+			} else if (d.is3d)
 				qv = pCur * cA0(cCur) + pp[L - 1] * cAi(cp[L - 1]) + pp[L + 1] * cAi(cCur) + pp[L - RX] * cAj(cp[L - RX]) +
 				     pp[L + RX] * cAj(cCur) + pPrev * cAk(cPrev) + pNext * cAk(cCur);
 			else
@@ -697,10 +840,10 @@ struct __attribute__((packed, aligned(4))) FTriple3 {
 // grids of at least 3 cells); the
 // z-neighbours of inner planes are the thread's own adjacent planes, so only
 // the first plane's lower and the last plane's upper neighbour are loaded.
-template <bool TRI>
+template <bool TRI, bool SC = false>
 __global__ void __launch_bounds__(kBlock) k_true_residual(Dims d, Tiles t, const uint8_t* __restrict__ code,
                                                           const float* __restrict__ b, const float* __restrict__ x,
-                                                          double* __restrict__ part) {
+                                                          double* __restrict__ part, float scale = 0.f) {
 	__shared__ double sh[8];
 	const CellPos p = tile_cell(t);
 	double rr = 0;
@@ -755,7 +898,18 @@ __global__ void __launch_bounds__(kBlock) k_true_residual(Dims d, Tiles t, const
 			const long long c = d.idx(p.i, p.j, p.k + q);
 			double ax;
 			if (!(cc[q] & 64)) ax = xc[q];
-			else {
+			else if (SC) {
+				const double o = -(double)scale;
+				ax = (double)xc[q] * (double)sA0(cc[q], scale);
+				if (c - X >= 0 && (cxm[q] & 8)) ax += o * xxm[q];
+				if (c + X < n && (cc[q] & 8)) ax += o * xxp[q];
+				if (c - Y >= 0 && (cym[q] & 16)) ax += o * xym[q];
+				if (c + Y < n && (cc[q] & 16)) ax += o * xyp[q];
+				if (d.is3d) {
+					if (c - Z >= 0 && (czm[q] & 32)) ax += o * xzm[q];
+					if (c + Z < n && (cc[q] & 32)) ax += o * xzp[q];
+				}
+			} else {
 				ax = (double)xc[q] * (double)(cc[q] & 7);
 				if (c - X >= 0 && (cxm[q] & 8)) ax -= xxm[q];
 				if (c + X < n && (cc[q] & 8)) ax -= xxp[q];
@@ -883,7 +1037,11 @@ long long update_tiles(const Dims& d, long long* per) {
 	return nt;
 }
 void run_matvec_one(hipStream_t s, unsigned nb, const MatvecP& p) {
-	hipLaunchKernelGGL((k_one<MatvecP, body_matvec<kMvTY>, 64 * kMvTY>), dim3(nb), dim3(64 * kMvTY), 0, s, p);
+	if (p.scaled)
+		hipLaunchKernelGGL((k_one<MatvecP, body_matvec<kMvTY, true>, 64 * kMvTY>), dim3(nb), dim3(64 * kMvTY), 0, s,
+		                   p);
+	else
+		hipLaunchKernelGGL((k_one<MatvecP, body_matvec<kMvTY>, 64 * kMvTY>), dim3(nb), dim3(64 * kMvTY), 0, s, p);
 }
 void run_matvec_many(hipStream_t s, unsigned nb, WorkList items, const MatvecP* p) {
 	hipLaunchKernelGGL((k_many<MatvecP, body_matvec<kMvTY>, 64 * kMvTY>), dim3(nb), dim3(64 * kMvTY), 0, s, items, p);
@@ -948,7 +1106,10 @@ static uint64_t code_hash(Scene* s, const uint8_t* code) {
 // the same code hash), rebuilt otherwise.  The reference's mIC is built
 // from each solve's matrix (conjugategrad.cpp:66-97), so a stale hierarchy
 // would change the preconditioner's meaning.
-static MgHierarchy* mic_hierarchy(Scene* s, const Grid* flags, const uint8_t* code, long long fix, int precision) {
+static MgHierarchy* mic_hierarchy(Scene* s, const Grid* flags, const uint8_t* code, bool fixing, int precision) {
+	// the pinned cell is a function of the flags, so (flags, generation,
+	// fixing on/off) determines the matrix code
+	const long long fix = fixing ? 1 : 0;
 	const bool sameKey = s->mgMic && s->mgMic->precision() == precision && s->micFlags == flags &&
 	                     s->micGen == flags->gen && s->micFix == fix;
 	if (!sameKey) {
@@ -965,51 +1126,25 @@ static MgHierarchy* mic_hierarchy(Scene* s, const Grid* flags, const uint8_t* co
 	return s->mgMic.get();
 }
 
-// zero-pressure cell selection (pressure.cpp:349-381), on the host from the
-// packed flags (setup-time work)
-static long long find_fix_cell(Scene* s, Grid* flags) {
-	const Dims& d = s->d;
-	std::vector<uint8_t> f((size_t)d.n);
-	MFX_HIP(hipMemcpyAsync(f.data(), flags->f8, (size_t)d.n, hipMemcpyDeviceToHost, s->stream));
-	sync(s);
-	long long numEmpty = 0;
-	for (long long i = 0; i < d.n; i++)
-		if (f[i] & kEmpty) numEmpty++;
-	if (numEmpty != 0) return -1;
-	const int tcx = d.nx / 2, tcy = d.ny - 1, tcz = d.is3d ? d.nz / 2 : 0;
-	for (int o = 0; o < 3; o++) {
-		const int y = tcy - o;
-		if (y < 0) continue;
-		const long long ix = d.idx(tcx, y, tcz);
-		if (f[ix] & kFluid) return ix;
-	}
-	const int k0 = d.is3d ? 1 : 0, k1 = d.is3d ? d.nz - 1 : 1;
-	for (int k = k0; k < k1; k++)
-		for (int j = 1; j < d.ny - 1; j++)
-			for (int i = 1; i < d.nx - 1; i++)
-				if (f[d.idx(i, j, k)] & kFluid) return d.idx(i, j, k);
-	return -1;
-}
-
-static void compute_rhs(Scene* s, Grid* flags, Grid* vel, float* rhs, int enforceCompatibility) {
+// RHS (MakeRhs, pressure.cpp:32-84) into rhs; with enforceCompatibility the
+// mean correction is reduced and added on the device.  part: FP64 partials
+// (2 per block of the k_make_rhs launch, plus one float slot after them).
+static void compute_rhs(Scene* s, Grid* flags, Grid* vel, float* rhs, int enforceCompatibility,
+                        double* part = nullptr) {
 	const Tiles t = tiles_for(s->d, kLightKZ);
 	const unsigned nb = (unsigned)t.count();
-	s->ensurePartials(3 * (size_t)nb + 16, 2 * (size_t)nb + 16);
+	if (!part) {
+		s->ensurePartials(3 * (size_t)nb + 16, 2 * (size_t)nb + 16);
+		part = s->dpart;
+	}
 	hipLaunchKernelGGL(k_make_rhs, dim3(nb), dim3(kBlock), 0, s->stream, s->d, t, flags->f8,
 	                   (const float*)vel->comp(0), (const float*)vel->comp(1), (const float*)vel->comp(2), rhs,
-	                   s->dpart);
+	                   part);
 	if (enforceCompatibility) {
-		std::vector<double> h(2 * (size_t)nb);
-		MFX_HIP(hipMemcpyAsync(h.data(), s->dpart, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, s->stream));
-		sync(s);
-		double sum = 0, cnt = 0;
-		for (unsigned b = 0; b < nb; b++) {
-			sum += h[2 * b];
-			cnt += h[2 * b + 1];
-		}
-		// rhs += (Real)(-sum / (Real)cnt), over all cells (pressure.cpp:297-298)
-		const float corr = (float)(-sum / (double)(float)cnt);
-		hipLaunchKernelGGL(k_add_const, dim3(flat_blocks(s->d.n)), dim3(256), 0, s->stream, rhs, s->d.n, corr);
+		float* corr = (float*)(part + 2 * (size_t)nb);
+		hipLaunchKernelGGL(k_compat_corr, dim3(1), dim3(1024), 0, s->stream, (const double*)part, (int)nb, corr);
+		hipLaunchKernelGGL(k_add_const_dev, dim3(flat_blocks(s->d.n)), dim3(256), 0, s->stream, rhs, s->d.n,
+		                   (const float*)corr);
 	}
 }
 
@@ -1029,7 +1164,13 @@ struct CgRun {
 	int npM = 0;  // partials of fl(q p) left by the matvec
 	int kchunk = 1;
 	long long per = 0;
-	long long fix = -1;
+	bool fixing = false;  // zero-pressure fixing requested (the cell is chosen on the device)
+	// pooled: buffers, matrix code and partial sums come from the scene (and
+	// the z-marching chunk may be autotuned); otherwise the caller set code,
+	// r, pa, pb, q, z, dpart and fpart before cg_setup
+	bool pooled = true;
+	double* dpart = nullptr;
+	float* fpart = nullptr;
 	uint8_t* code = nullptr;
 	int pc = 0, maxIter = 0;
 	bool pcNone = true;
@@ -1038,16 +1179,18 @@ struct CgRun {
 	float *r = nullptr, *pa = nullptr, *pb = nullptr, *q = nullptr, *z = nullptr;
 	CgState* st = nullptr;  // device
 	int launched = 0;
+	int scaled = 0;     // scaled Laplace matrix (cgSolveDiffusion, cgSolveWE)
+	float scale = 0.f;
 	MatvecP mvp(bool odd) const {
-		MatvecP p{s->d, kchunk, st, code, z, odd ? pb : pa, odd ? pa : pb, q, s->dpart};
+		MatvecP p{s->d, kchunk, st, code, z, odd ? pb : pa, odd ? pa : pb, q, dpart, scaled, scale};
 		return p;
 	}
 	UpdateP upd(bool odd) const {
-		UpdateP p{s->d, per, st, x, r, odd ? pa : pb, q, pcNone ? 1 : 0, s->dpart, s->fpart};
+		UpdateP p{s->d, per, st, x, r, odd ? pa : pb, q, pcNone ? 1 : 0, dpart, fpart};
 		return p;
 	}
 	FinP fin(int nb, int phase) const {
-		FinP p{st, s->dpart, s->fpart, nb, phase, pcNone ? 1 : 0};
+		FinP p{st, dpart, fpart, nb, phase, pcNone ? 1 : 0};
 		return p;
 	}
 };
@@ -1146,24 +1289,11 @@ static void cg_setup(CgRun& R, Scene* s, Grid* flags, float* rhs, float* x, cons
 	R.nbF = flat_count(d.n);
 	R.nbU = (unsigned)update_tiles(d, &R.per);
 	const unsigned nbMax = std::max(R.nbT, R.nbF);
-	s->ensurePartials(3 * (size_t)nbMax + 16, 2 * (size_t)nbMax + 16);
-
-	// zero-pressure fixing (pressure.cpp:349-388)
-	if (o.zeroPressureFixing || o.cgAccuracy < 1e-07f) {
-		R.fix = find_fix_cell(s, flags);
-		if (R.fix >= 0) MFX_HIP(hipMemsetAsync(rhs + R.fix, 0, sizeof(float), s->stream));
+	if (R.pooled) {
+		s->ensurePartials(3 * (size_t)nbMax + 16, 2 * (size_t)nbMax + 16);
+		R.dpart = s->dpart;
+		R.fpart = s->fpart;
 	}
-	// matrix
-	if (!s->code) MFX_HIP(hipMalloc((void**)&s->code, (size_t)d.n));
-	R.code = s->code;
-	{
-		const Tiles tc = tiles_for(d, kLightKZ);
-		hipLaunchKernelGGL(k_build_code, dim3((unsigned)tc.count()), dim3(kBlock), 0, s->stream, d, tc, flags->f8,
-		                   R.code, R.fix);
-	}
-	autotune_zmarch(s, R.code);
-	R.nbM = (unsigned)matvec_tiles(d, &R.kchunk, true);
-	R.npM = (int)zm_dot_parts(d, kMvTY);
 
 	// preconditioner choice (pressure.cpp:404-433, conjugategrad.cpp:302-336)
 	int pc = o.preconditioner;
@@ -1181,6 +1311,32 @@ static void cg_setup(CgRun& R, Scene* s, Grid* flags, float* rhs, float* x, cons
 	R.pc = pc;
 	R.pcNone = pc == MFX_PC_NONE;
 
+	R.st = (CgState*)s->dscal;
+	hipLaunchKernelGGL(k_cg_state_init, dim3(1), dim3(64), 0, s->stream, R.st, o.cgAccuracy, R.maxIter, o.useL2Norm,
+	                   o.relTol);
+	// zero-pressure fixing (pressure.cpp:349-388), chosen and applied on the
+	// device (no copy of the flags to the host)
+	R.fixing = o.zeroPressureFixing || o.cgAccuracy < 1e-07f;
+	if (R.fixing) {
+		const Tiles t1 = tiles_for(d);
+		hipLaunchKernelGGL(k_fix_scan, dim3((unsigned)t1.count()), dim3(kBlock), 0, s->stream, d, t1, flags->f8,
+		                   R.st);
+		hipLaunchKernelGGL(k_fix_select, dim3(1), dim3(64), 0, s->stream, d, flags->f8, R.st, rhs);
+	}
+	// matrix
+	if (R.pooled) {
+		if (!s->code) MFX_HIP(hipMalloc((void**)&s->code, (size_t)d.n));
+		R.code = s->code;
+	}
+	{
+		const Tiles tc = tiles_for(d, kLightKZ);
+		hipLaunchKernelGGL(k_build_code, dim3((unsigned)tc.count()), dim3(kBlock), 0, s->stream, d, tc, flags->f8,
+		                   R.code, (const CgState*)R.st);
+	}
+	if (R.pooled) autotune_zmarch(s, R.code);
+	R.nbM = (unsigned)matvec_tiles(d, &R.kchunk, true);
+	R.npM = (int)zm_dot_parts(d, kMvTY);
+
 	if (!R.pcNone) {
 		if (o.preconditioner == MFX_PC_MGDYNAMIC) {
 			s->mgStatic.reset();
@@ -1188,7 +1344,7 @@ static void cg_setup(CgRun& R, Scene* s, Grid* flags, float* rhs, float* x, cons
 			R.mgDyn->build(s, R.code, o.precision);
 			R.mg = R.mgDyn.get();
 		} else if (o.preconditioner == MFX_PC_MIC) {
-			R.mg = mic_hierarchy(s, flags, R.code, R.fix, o.precision);
+			R.mg = mic_hierarchy(s, flags, R.code, R.fixing, o.precision);
 		} else {
 			if (!s->mgStatic || s->mgStatic->precision() != o.precision) {
 				s->mgStatic.reset(new MgHierarchy());
@@ -1198,27 +1354,25 @@ static void cg_setup(CgRun& R, Scene* s, Grid* flags, float* rhs, float* x, cons
 		}
 	}
 
-	R.r = s->getReal();
-	R.pa = s->getReal();
-	R.pb = s->getReal();
-	R.q = s->getReal();
-	R.z = R.pcNone ? R.r : s->getReal();
-
-	R.st = (CgState*)s->dscal;
-	hipLaunchKernelGGL(k_cg_state_init, dim3(1), dim3(64), 0, s->stream, R.st, o.cgAccuracy, R.maxIter, o.useL2Norm,
-	                   o.relTol);
+	if (R.pooled) {
+		R.r = s->getReal();
+		R.pa = s->getReal();
+		R.pb = s->getReal();
+		R.q = s->getReal();
+		R.z = R.pcNone ? R.r : s->getReal();
+	}
 
 	// doInit: x = 0, r = b, bnorm; preconditioned: z = M r, sigma = dot(z, r)
 	hipLaunchKernelGGL(k_cg_init, dim3(R.nbU), dim3(kBlock), 0, s->stream, d, R.per, (const float*)rhs, x, R.r,
-	                   R.pcNone ? 1 : 0, s->dpart);
-	run_finalize_one(s->stream, FinP{R.st, s->dpart, s->fpart, (int)R.nbU, FIN_INIT, 0});
+	                   R.pcNone ? 1 : 0, R.dpart);
+	run_finalize_one(s->stream, FinP{R.st, R.dpart, R.fpart, (int)R.nbU, FIN_INIT, 0});
 	if (!R.pcNone && !batchInit) {
 		int np = R.mg->vcycle(s, R.code, R.r, R.z, R.st);
 		if (np == 0) {
-			run_dot_one(s->stream, R.nbF, DotP{d, R.st, R.z, R.r, s->dpart});
+			run_dot_one(s->stream, R.nbF, DotP{d, R.st, R.z, R.r, R.dpart});
 			np = (int)R.nbF;
 		}
-		run_finalize_one(s->stream, FinP{R.st, s->dpart, s->fpart, np, FIN_BETA0, 0});
+		run_finalize_one(s->stream, FinP{R.st, R.dpart, R.fpart, np, FIN_BETA0, 0});
 	}
 	R.launched = 0;
 }
@@ -1241,14 +1395,14 @@ static void cg_iteration(CgRun& R) {
 		// FP64 residual replacement; its partials replace the update's
 		hipLaunchKernelGGL(k_cg_replace_residual, dim3(R.nbT), dim3(kBlock), 0, s->stream, s->d, R.t,
 		                   (const CgState*)R.st, R.code, (const float*)R.rhs, (const float*)R.x, R.r,
-		                   R.pcNone ? 1 : 0, s->dpart, s->fpart);
+		                   R.pcNone ? 1 : 0, R.dpart, R.fpart);
 		nbRes = (int)R.nbT;
 	}
 	run_finalize_one(s->stream, R.fin(nbRes, FIN_RES));
 	if (!R.pcNone) {
 		int np = R.mg->vcycle(s, R.code, R.r, R.z, R.st);
 		if (np == 0) {
-			run_dot_one(s->stream, R.nbF, DotP{s->d, R.st, R.z, R.r, s->dpart});
+			run_dot_one(s->stream, R.nbF, DotP{s->d, R.st, R.z, R.r, R.dpart});
 			np = (int)R.nbF;
 		}
 		run_finalize_one(s->stream, R.fin(np, FIN_BETA));
@@ -1261,20 +1415,25 @@ static void cg_iteration(CgRun& R) {
 static void cg_finish_enqueue(CgRun& R) {
 	Scene* s = R.s;
 	const Tiles tr = tiles_for(s->d, kLightKZ);
-	if (s->d.n >= 3)
+	if (R.scaled)
+		hipLaunchKernelGGL((k_true_residual<false, true>), dim3((unsigned)tr.count()), dim3(kBlock), 0, s->stream,
+		                   s->d, tr, R.code, (const float*)R.rhs, (const float*)R.x, R.dpart, R.scale);
+	else if (s->d.n >= 3)
 		hipLaunchKernelGGL(k_true_residual<true>, dim3((unsigned)tr.count()), dim3(kBlock), 0, s->stream, s->d, tr,
-		                   R.code, (const float*)R.rhs, (const float*)R.x, s->dpart);
+		                   R.code, (const float*)R.rhs, (const float*)R.x, R.dpart);
 	else
 		hipLaunchKernelGGL(k_true_residual<false>, dim3((unsigned)tr.count()), dim3(kBlock), 0, s->stream, s->d, tr,
-		                   R.code, (const float*)R.rhs, (const float*)R.x, s->dpart);
-	hipLaunchKernelGGL(k_reduce_true_rr, dim3(1), dim3(1024), 0, s->stream, R.st, (const double*)s->dpart,
+		                   R.code, (const float*)R.rhs, (const float*)R.x, R.dpart);
+	hipLaunchKernelGGL(k_reduce_true_rr, dim3(1), dim3(1024), 0, s->stream, R.st, (const double*)R.dpart,
 	                   (int)tr.count());
 	MFX_HIP(hipMemcpyAsync(s->hscal, R.st, sizeof(CgState), hipMemcpyDeviceToHost, s->stream));
-	s->putReal(R.r);
-	s->putReal(R.pa);
-	s->putReal(R.pb);
-	s->putReal(R.q);
-	if (!R.pcNone) s->putReal(R.z);
+	if (R.pooled) {
+		s->putReal(R.r);
+		s->putReal(R.pa);
+		s->putReal(R.pb);
+		s->putReal(R.q);
+		if (!R.pcNone) s->putReal(R.z);
+	}
 }
 
 // after the scene stream has drained: statistics and the divergence error
@@ -1285,7 +1444,7 @@ static void cg_finish_collect(CgRun& R, mfx_solve_stats* stats) {
 		stats->iterations = hs->iter;
 		stats->resNorm = hs->resNorm;
 		stats->relResidual = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
-		stats->fixPidx = (int)R.fix;
+		stats->fixPidx = (int)hs->fix;
 		stats->maxIter = R.maxIter;
 		stats->preconditionerUsed = R.pc;
 		stats->converged = hs->converged;
@@ -1303,10 +1462,32 @@ static void cg_finish_collect(CgRun& R, mfx_solve_stats* stats) {
 // pressure system solve: rhs (modified by fixPressure), pressure out.
 // Iterations are queued in batches whose size is predicted from the
 // residual history; kernels stop on st->done.
+static void cg_drive(CgRun& R, double target, mfx_solve_stats* stats);
+
 static void solve_system(Scene* s, Grid* flags, float* rhs, float* x, const mfx_pressure_opts& o,
-                         mfx_solve_stats* stats) {
+                         mfx_solve_stats* stats, const CgRun* bufs = nullptr) {
 	CgRun R;
+	if (bufs) {
+		// caller-provided buffers (allocation-free solve)
+		R.pooled = false;
+		R.code = bufs->code;
+		R.r = bufs->r;
+		R.pa = bufs->pa;
+		R.pb = bufs->pb;
+		R.q = bufs->q;
+		R.z = bufs->r;
+		R.dpart = bufs->dpart;
+		R.fpart = bufs->fpart;
+	}
 	cg_setup(R, s, flags, rhs, x, o);
+	cg_drive(R, o.relTol > 0 ? o.relTol : (double)o.cgAccuracy, stats);
+}
+
+// Queue iterations in batches sized from the residual history until the
+// stopping rule holds or maxIter is reached, then finish (statistics,
+// divergence error).  target: the stopping threshold the prediction aims at.
+static void cg_drive(CgRun& R, double target, mfx_solve_stats* stats) {
+	Scene* s = R.s;
 	int batch = 4;
 	CgState* hs = (CgState*)s->hscal;
 	while (true) {
@@ -1319,7 +1500,6 @@ static void solve_system(Scene* s, Grid* flags, float* rhs, float* x, const mfx_
 		int predict = batch;
 		if (itNow >= 2 && itNow <= 64) {
 			const double a = hs->hist[itNow - 2], b = hs->hist[itNow - 1];
-			const double target = o.relTol > 0 ? o.relTol : (double)o.cgAccuracy;
 			if (a > 0 && b > 0 && b < a && target > 0) {
 				const double rate = b / a;
 				double need = std::log(target / b) / std::log(rate);
@@ -1334,6 +1514,62 @@ static void solve_system(Scene* s, Grid* flags, float* rhs, float* x, const mfx_
 	cg_finish_collect(R, stats);
 }
 
+
+// GridCg without preconditioner over a scaled Laplace matrix (the other
+// GridCg clients): kind 0 builds cgSolveDiffusion's matrix code, kind 1 the
+// pressure matrix of the flags (cgSolveWE); x is cleared by doInit and
+// solved for rhs.  maxIter = (int)(cgMaxIterFac * max size) * (4 in 2D).
+static void solve_scaled(Scene* s, Grid* flags, float* rhs, float* x, int kind, float scale, float cgAccuracy,
+                         float cgMaxIterFac, mfx_solve_stats* stats) {
+	CgRun R;
+	R.t0 = std::chrono::steady_clock::now();
+	R.s = s;
+	R.flags = flags;
+	R.rhs = rhs;
+	R.x = x;
+	mfx_pressure_opts_default(&R.o);
+	R.o.cgAccuracy = cgAccuracy;
+	R.o.preconditioner = MFX_PC_NONE;
+	const Dims& d = s->d;
+	R.t = tiles_for(d);
+	R.nbT = (unsigned)R.t.count();
+	R.nbF = flat_count(d.n);
+	R.nbU = (unsigned)update_tiles(d, &R.per);
+	const unsigned nbMax = std::max(R.nbT, R.nbF);
+	s->ensurePartials(3 * (size_t)nbMax + 16, 2 * (size_t)nbMax + 16);
+	R.dpart = s->dpart;
+	R.fpart = s->fpart;
+	int mx = d.nx > d.ny ? d.nx : d.ny;
+	if (d.nz > mx) mx = d.nz;
+	R.maxIter = (int)(cgMaxIterFac * (float)mx) * (d.is3d ? 1 : 4);
+	R.pc = MFX_PC_NONE;
+	R.pcNone = true;
+	R.scaled = 1;
+	R.scale = scale;
+	R.st = (CgState*)s->dscal;
+	hipLaunchKernelGGL(k_cg_state_init, dim3(1), dim3(64), 0, s->stream, R.st, cgAccuracy, R.maxIter, 0, 0.0);
+	if (!s->code) MFX_HIP(hipMalloc((void**)&s->code, (size_t)d.n));
+	R.code = s->code;
+	const Tiles tc = tiles_for(d, kLightKZ);
+	if (kind == 0)
+		hipLaunchKernelGGL(k_build_code_diff, dim3((unsigned)tc.count()), dim3(kBlock), 0, s->stream, d, tc,
+		                   flags->f8, R.code);
+	else
+		hipLaunchKernelGGL(k_build_code, dim3((unsigned)tc.count()), dim3(kBlock), 0, s->stream, d, tc, flags->f8,
+		                   R.code, (const CgState*)R.st);
+	R.nbM = (unsigned)matvec_tiles(d, &R.kchunk, true);
+	R.npM = (int)zm_dot_parts(d, kMvTY);
+	R.r = s->getReal();
+	R.pa = s->getReal();
+	R.pb = s->getReal();
+	R.q = s->getReal();
+	R.z = R.r;
+	hipLaunchKernelGGL(k_cg_init, dim3(R.nbU), dim3(kBlock), 0, s->stream, d, R.per, (const float*)rhs, x, R.r, 1,
+	                   R.dpart);
+	run_finalize_one(s->stream, FinP{R.st, R.dpart, R.fpart, (int)R.nbU, FIN_INIT, 0});
+	R.launched = 0;
+	cg_drive(R, (double)cgAccuracy, stats);
+}
 
 static void correct_velocity(Scene* s, Grid* flags, Grid* vel, const float* pressure) {
 	const Tiles t = tiles_for(s->d, kLightKZ);
@@ -1849,6 +2085,48 @@ int mfx_solve_pressure(mfx_grid* vel, mfx_grid* pressure, const mfx_grid* flags,
 	API_END
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free plain-CG solve: every buffer beyond the caller's grids
+// (right-hand side, CG vectors, matrix code, FP64 partial sums) lives in one
+// file-scope device array.  Each GPU holds its own copy of the array, whose
+// address hipGetSymbolAddress gives once per device; host code uses only that
+// address, never the array's name.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr long long kNoAllocMaxCells = 1ll << 20;
+constexpr size_t kNoAllocMaxPartBlocks = 65536;  // partial-sum slots (blocks) of one launch
+constexpr size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t kNaVec = al256((size_t)kNoAllocMaxCells * sizeof(float));
+constexpr size_t kNaCode = al256((size_t)kNoAllocMaxCells);
+constexpr size_t kNaDPart = al256((3 * kNoAllocMaxPartBlocks + 16) * sizeof(double));
+constexpr size_t kNaFPart = al256((2 * kNoAllocMaxPartBlocks + 16) * sizeof(float));
+constexpr size_t kNoAllocBytes = 5 * kNaVec + kNaCode + kNaDPart + kNaFPart;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_dev[64];
+
+// device address of `scratch` on device dev (looked up once per device, with
+// that device current)
+static unsigned char* scratch_on(int dev) {
+	MFX_CHECK(dev >= 0 && dev < 64, "solvePressureNoAlloc: device index out of range");
+	std::lock_guard<std::mutex> lk(g_scratch_mu);
+	if (!g_scratch_dev[dev]) {
+		int cur = 0;
+		MFX_HIP(hipGetDevice(&cur));
+		if (cur != dev) MFX_HIP(hipSetDevice(dev));
+		void* p = nullptr;
+		const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if (cur != dev) MFX_HIP(hipSetDevice(cur));
+		MFX_CHECK(e == hipSuccess && p != nullptr,
+		          std::string("solvePressureNoAlloc: hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+		g_scratch_dev[dev] = p;
+	}
+	return (unsigned char*)g_scratch_dev[dev];
+}
+
 int mfx_solve_pressure_batch(int count, mfx_grid* const* vel, mfx_grid* const* pressure,
                              const mfx_grid* const* flags, const mfx_pressure_opts* opts, mfx_solve_stats* stats) {
 	API_BEGIN
@@ -1863,6 +2141,118 @@ int mfx_solve_pressure_batch(int count, mfx_grid* const* vel, mfx_grid* const* p
 			MFX_CHECK(v[(size_t)j]->s != v[(size_t)i]->s, "solvePressureBatch: each problem needs its own solver");
 	}
 	solve_batch(v, p, f, opts_or_default(opts), stats);
+	API_END
+}
+
+int mfx_solve_pressure_noalloc(mfx_grid* vel, mfx_grid* pressure, const mfx_grid* flags,
+                               const mfx_pressure_opts* opts, mfx_solve_stats* stats) {
+	API_BEGIN
+	Grid *v = G(vel), *p = G(pressure), *f = G(flags);
+	check_pressure_args(v, p, f);
+	const mfx_pressure_opts o = opts_or_default(opts);
+	Scene* s = v->s;
+	const Dims& d = s->d;
+	MFX_CHECK(o.preconditioner == MFX_PC_NONE || (o.preconditioner == MFX_PC_MIC && !d.is3d),
+	          "solvePressureNoAlloc: only unpreconditioned CG (PcNone, or PcMIC in 2D) runs without allocation");
+	MFX_CHECK(d.n <= kNoAllocMaxCells, "solvePressureNoAlloc: grid has " + std::to_string(d.n) +
+	                                       " cells, more than the allocation-free solve's maximum of " +
+	                                       std::to_string(kNoAllocMaxCells));
+	// partial-sum slots of the launches of this shape (RHS, CG init/update,
+	// matvec, true residual) must fit the scratch partials
+	{
+		long long per = 0;
+		const long long nbT = tiles_for(d).count();
+		const long long nbF = flat_count(d.n);
+		const long long nbL = tiles_for(d, kLightKZ).count();
+		const long long nbU = update_tiles(d, &per);
+		int kc = 1;
+		matvec_tiles(d, &kc, true);
+		const long long nbMv = zm_dot_parts(d, kMvTY);
+		const long long need = std::max(std::max(nbT, nbF), std::max(std::max(nbL, nbU), nbMv));
+		MFX_CHECK(need <= (long long)kNoAllocMaxPartBlocks,
+		          "solvePressureNoAlloc: grid shape needs " + std::to_string(need) +
+		              " partial sums per launch, more than the allocation-free solve's maximum of " +
+		              std::to_string(kNoAllocMaxPartBlocks));
+	}
+	int dev = 0;
+	MFX_HIP(hipGetDevice(&dev));
+	MFX_CHECK(dev == s->device, "solvePressureNoAlloc: the solver's device is not the current device");
+	unsigned char* base = scratch_on(dev);
+	float* rhs = (float*)base;
+	CgRun B;
+	B.r = (float*)(base + kNaVec);
+	B.pa = (float*)(base + 2 * kNaVec);
+	B.pb = (float*)(base + 3 * kNaVec);
+	B.q = (float*)(base + 4 * kNaVec);
+	B.code = base + 5 * kNaVec;
+	B.dpart = (double*)(base + 5 * kNaVec + kNaCode);
+	B.fpart = (float*)(base + 5 * kNaVec + kNaCode + kNaDPart);
+	compute_rhs(s, f, v, rhs, o.enforceCompatibility, B.dpart);
+	solve_system(s, f, rhs, p->real(), o, stats, &B);
+	correct_velocity(s, f, v, p->real());
+	API_END
+}
+
+int mfx_cg_solve_diffusion(const mfx_grid* flags, mfx_grid* grid, float alpha, float cgMaxIterFac, float cgAccuracy,
+                           mfx_solve_stats* stats) {
+	API_BEGIN
+	MFX_CHECK(flags && grid, "cgSolveDiffusion: flags and grid are required");
+	Grid *f = G(flags), *g = G(grid);
+	MFX_CHECK(f->isFlags(), "cgSolveDiffusion: flags must be a FlagGrid");
+	MFX_CHECK(f->s == g->s, "grids belong to different solvers");
+	MFX_CHECK(g->isReal() || g->isVec(),
+	          "cgSolveDiffusion: Grid Type is not supported (only Real, Vec3, MAC, or Levelset)");
+	Scene* s = f->s;
+	const size_t bytes = sizeof(float) * (size_t)s->d.n;
+	float* rhs = s->getReal();
+	try {
+		// Vec3 / MAC grids: every component (two in 2D) with the same matrix;
+		// a component of the SoA grid is itself a Real grid, so getComponent
+		// and setComponent are the solve's own input and output
+		const int nc = g->isVec() ? (s->d.is3d ? 3 : 2) : 1;
+		for (int c = 0; c < nc; c++) {
+			float* u = g->comp(c);
+			MFX_HIP(hipMemcpyAsync(rhs, u, bytes, hipMemcpyDeviceToDevice, s->stream));
+			solve_scaled(s, f, rhs, u, 0, alpha, cgAccuracy, cgMaxIterFac, stats);
+		}
+	} catch (...) {
+		s->putReal(rhs);
+		throw;
+	}
+	s->putReal(rhs);
+	API_END
+}
+
+int mfx_cg_solve_we(const mfx_grid* flags, mfx_grid* ut, mfx_grid* utm1, mfx_grid* out, int crankNic, float cSqr,
+                    float cgMaxIterFac, float cgAccuracy, mfx_solve_stats* stats) {
+	API_BEGIN
+	MFX_CHECK(flags && ut && utm1 && out, "cgSolveWE: flags, ut, utm1 and out are required");
+	Grid *f = G(flags), *a = G(ut), *b = G(utm1), *o = G(out);
+	MFX_CHECK(f->isFlags() && a->isReal() && b->isReal() && o->isReal(),
+	          "cgSolveWE: needs a FlagGrid and three RealGrids");
+	MFX_CHECK(f->s == a->s && f->s == b->s && f->s == o->s, "grids belong to different solvers");
+	MFX_CHECK(a != b && a != o && b != o, "cgSolveWE: ut, utm1 and out must be different grids");
+	Scene* s = f->s;
+	// Real s = dt*dt*cSqr * 0.5 (waves.cpp:107)
+	const float dt = s->dt;
+	volatile float dd = dt * dt;
+	volatile float dc = dd * cSqr;
+	const float sc = (float)((double)dc * 0.5);
+	float* rhs = s->getReal();
+	try {
+		const Tiles t = tiles_for(s->d, kLightKZ);
+		hipLaunchKernelGGL(k_make_rhs_we, dim3((unsigned)t.count()), dim3(kBlock), 0, s->stream, s->d, t,
+		                   (const float*)a->real(), (const float*)b->real(), sc, crankNic ? 1 : 0, rhs);
+		solve_scaled(s, f, rhs, o->real(), 1, sc, cgAccuracy, cgMaxIterFac, stats);
+		// utm1.swap(ut); ut.copyFrom(out)
+		std::swap(a->data, b->data);
+		MFX_HIP(hipMemcpyAsync(a->real(), o->real(), sizeof(float) * (size_t)s->d.n, hipMemcpyDeviceToDevice,
+		                       s->stream));
+	} catch (...) {
+		s->putReal(rhs);
+		throw;
+	}
+	s->putReal(rhs);
 	API_END
 }
 

@@ -716,6 +716,9 @@ def advectSemiLagrange(*args, **kwargs):
         if _debug_level >= 0:
             print("Warning: boundaryWidth and openBounds parameters in AdvectSemiLagrange plugin are deprecated "
                   "(and have no more effect), please remove.")
+    if isinstance(a["grid"], MACGrid) and a["orderSpace"] != 1 and _debug_level >= 1:
+        # advection.cpp:415, printed after the forward step there
+        print("Warning higher order for MAC grids not yet implemented...")
     check(lib().mfx_advect_semi_lagrange(_h(a["flags"]), _h(a["vel"]), _h(a["grid"]), int(a["order"]),
                                          C.c_float(a["strength"]), int(a["orderSpace"]), int(a["clampMode"]),
                                          int(a["orderTrace"])))
@@ -787,12 +790,15 @@ def _opts(a, solver):
     return o
 
 
-def _report(solver, st):
+def _report(solver, st, what="solvePressure"):
     solver._solve_stats = {"iterations": st.iterations, "resNorm": st.resNorm, "relResidual": st.relResidual,
                            "fixPidx": st.fixPidx, "maxIter": st.maxIter, "preconditioner": st.preconditionerUsed,
                            "converged": bool(st.converged), "seconds": st.seconds}
-    if _debug_level >= 2:
+    if what == "solvePressure" and _debug_level >= 2:
         print("FluidSolver::solvePressure done. Iterations:%d, residual:%g" % (st.iterations, st.resNorm))
+    elif what == "cgSolveWaveEq" and _debug_level >= 1:
+        # waves.cpp:137 (debug level 1)
+        print("cgSolveWaveEq iterations:%d, res:%g" % (st.iterations, st.resNorm))
 
 
 @_plugin
@@ -860,6 +866,35 @@ def correctVelocity(*args, **kwargs):
     a = _parse(sig, args, kwargs)
     _opts(dict(a, obvel=None), a["vel"].parent)
     check(lib().mfx_correct_velocity(_h(a["vel"]), _h(a["pressure"]), _h(a["flags"])))
+
+
+@_plugin
+def cgSolveDiffusion(*args, **kwargs):
+    """conjugategrad.cpp:347-423: implicit diffusion of a Real, Vec3 or MAC
+    grid (every component, two in 2D) by unpreconditioned CG on the GPU"""
+    a = _parse([("flags", _REQ), ("grid", _REQ), ("alpha", 0.25), ("cgMaxIterFac", 1.0), ("cgAccuracy", 1e-4)],
+               args, kwargs)
+    st = _abi.SolveStats()
+    check(lib().mfx_cg_solve_diffusion(_h(a["flags"]), _h(a["grid"]), C.c_float(_f32(a["alpha"])),
+                                       C.c_float(_f32(a["cgMaxIterFac"])), C.c_float(_f32(a["cgAccuracy"])),
+                                       C.byref(st)))
+    solver = a["flags"].parent
+    _report(solver, st, "solveDiffusion")
+
+
+@_plugin
+def cgSolveWE(*args, **kwargs):
+    """plugin/waves.cpp:87-141: one implicit wave-equation step by
+    unpreconditioned CG on the GPU; afterwards utm1 holds the old ut and ut
+    the new heights (= out)"""
+    a = _parse([("flags", _REQ), ("ut", _REQ), ("utm1", _REQ), ("out", _REQ), ("crankNic", False),
+                ("cSqr", 0.25), ("cgMaxIterFac", 1.5), ("cgAccuracy", 1e-5)], args, kwargs)
+    st = _abi.SolveStats()
+    check(lib().mfx_cg_solve_we(_h(a["flags"]), _h(a["ut"]), _h(a["utm1"]), _h(a["out"]),
+                                int(bool(a["crankNic"])), C.c_float(_f32(a["cSqr"])),
+                                C.c_float(_f32(a["cgMaxIterFac"])), C.c_float(_f32(a["cgAccuracy"])), C.byref(st)))
+    solver = a["flags"].parent
+    _report(solver, st, "cgSolveWaveEq")
 
 
 @_plugin

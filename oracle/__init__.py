@@ -94,6 +94,10 @@ class Oracle:
             "og_mg_level_types": (I, [P, I, P]),
             "og_mg_level_matrix": (I, [P, I, P]),
             "og_apply_shape": (None, [D, I, P, P, R, P, I, P, P]),
+            "og_diffusion_matrix": (None, [D, P, R, P, P, P, P]),
+            "og_cg_solve_diffusion": (None, [D, P, P, I, R, R, R, P]),
+            "og_make_rhs_we": (None, [D, P, P, P, P, R, I]),
+            "og_cg_solve_we": (None, [D, P, P, P, P, I, R, R, R, R, P]),
         }
         for k, (res, args) in sig.items():
             f = getattr(L, k)
@@ -244,6 +248,40 @@ class Oracle:
                                                self._arr(flags, np.int32), C.byref(o), C.byref(st))
         return rc, {"iterations": st.iterations, "resNorm": st.resNorm,
                     "relResidual": st.relResidual, "fixPidx": st.fixPidx, "maxIter": st.maxIter}
+
+    # -- other GridCg clients ------------------------------------------------
+    def _stats(self, st):
+        return {"iterations": st.iterations, "resNorm": st.resNorm, "relResidual": st.relResidual,
+                "maxIter": st.maxIter, "diverged": bool(st.diverged)}
+
+    def diffusion_matrix(self, shape, flags, alpha):
+        n = int(np.prod(shape))
+        A = [np.zeros(n, self.dtype) for _ in range(4)]
+        self.lib.og_diffusion_matrix(self.dims(shape), self._arr(flags, np.int32), alpha,
+                                     *[self._arr(a) for a in A])
+        return A
+
+    def cg_solve_diffusion(self, shape, flags, grid, alpha=0.25, cgMaxIterFac=1.0, cgAccuracy=1e-4):
+        """grid: float[n] (Real) or float[n, 3] (Vec3/MAC), updated in place"""
+        st = self.Stats()
+        ncomp = 1 if grid.ndim == 1 else 3
+        self.lib.og_cg_solve_diffusion(self.dims(shape), self._arr(flags, np.int32), self._arr(grid), ncomp,
+                                       alpha, cgMaxIterFac, cgAccuracy, C.byref(st))
+        return self._stats(st)
+
+    def make_rhs_we(self, shape, flags, ut, utm1, s, crankNic=False):
+        rhs = np.zeros_like(ut)
+        self.lib.og_make_rhs_we(self.dims(shape), self._arr(flags, np.int32), self._arr(rhs), self._arr(ut),
+                                self._arr(utm1), s, int(bool(crankNic)))
+        return rhs
+
+    def cg_solve_we(self, shape, flags, ut, utm1, out, dt=1.0, crankNic=False, cSqr=0.25, cgMaxIterFac=1.5,
+                    cgAccuracy=1e-5):
+        st = self.Stats()
+        self.lib.og_cg_solve_we(self.dims(shape), self._arr(flags, np.int32), self._arr(ut), self._arr(utm1),
+                                self._arr(out), int(bool(crankNic)), cSqr, cgMaxIterFac, cgAccuracy, dt,
+                                C.byref(st))
+        return self._stats(st)
 
     # -- multigrid -----------------------------------------------------------
     def mg_create(self, shape):

@@ -55,7 +55,9 @@
  *  15  midpoint trace of centred grids: full step to the midpoint
  *  16  interpolCubicMAC: no half-cell shift of the component's lookup
  *  17  MAC midpoint trace: velocity instead of the advected field
- *  18  interpolMAC: x component without its shifted index */
+ *  18  interpolMAC: x component without its shifted index
+ *  19  cgSolveDiffusion matrix: diagonal scaled by -alpha
+ *  20  MakeRhsWE (Crank-Nicolson): x+1 neighbour read at x-1 */
 #ifndef OG_MUTANT
 #define OG_MUTANT 0
 #endif
@@ -1322,6 +1324,162 @@ int og_solve_pressure(og_dims d, og_real* vel, og_real* pressure, const int* fla
 	if (rhsOut) memcpy(rhsOut, rhs, sizeof(og_real) * (size_t)n);
 	free(rhs);
 	return rc;
+}
+
+/* ===========================================================================
+ * Other GridCg clients: cgSolveDiffusion (conjugategrad.cpp:347-423) and
+ * cgSolveWE (plugin/waves.cpp:69-141)
+ * ========================================================================= */
+
+/* GridCg<ApplyMatrix(2D)> without preconditioner: doInit clears x, r = rhs,
+ * then iterate() until ||r||_inf < acc or maxIter (conjugategrad.cpp:210-310) */
+static void plain_gridcg(og_dims d, const int* flags, og_real* x, const og_real* rhs, const og_real* A0,
+                         const og_real* Ai, const og_real* Aj, const og_real* Ak, og_real acc, int maxIter,
+                         og_solve_stats* st) {
+	long n = NCELLS(d);
+	size_t nb = sizeof(og_real) * (size_t)n;
+	og_real *res = (og_real*)calloc(1, nb), *search = (og_real*)calloc(1, nb), *tmp = (og_real*)calloc(1, nb);
+	memset(x, 0, nb);
+	memcpy(res, rhs, nb);
+	memcpy(tmp, res, nb);
+	memcpy(search, tmp, nb);
+	og_real sigma = (og_real)grid_dot(d, tmp, res);
+	og_real resNorm = 1e20;
+	int iters = 0, diverged = 0;
+	for (int iter = 0; iter < maxIter; iter++) {
+		iters++;
+		og_apply_matrix(d, flags, tmp, search, A0, Ai, Aj, Ak);
+		og_real dp = (og_real)grid_dot(d, tmp, search);
+		og_real alpha = 0.;
+		if (OG_FABS(dp) > 0.) alpha = sigma / (og_real)dp;
+		for (long i = 0; i < n; i++) x[i] += alpha * search[i];
+		for (long i = 0; i < n; i++) res[i] += (-alpha) * tmp[i];
+		memcpy(tmp, res, nb);
+		resNorm = grid_max_abs(d, res);
+		if (resNorm < acc) {
+			sigma = resNorm;
+			break;
+		}
+		og_real sigmaNew = (og_real)grid_dot(d, tmp, res);
+		og_real beta = sigmaNew / sigma;
+		for (long i = 0; i < n; i++) search[i] = tmp[i] + beta * search[i];
+		sigma = sigmaNew;
+		if (!(resNorm < 1e35)) {
+			diverged = 1;
+			break;
+		}
+	}
+	if (st) {
+		st->iterations = iters;
+		st->resNorm = resNorm;
+		st->relResidual = rel_residual(d, flags, rhs, x, A0, Ai, Aj, Ak);
+		st->fixPidx = -1;
+		st->diverged = diverged;
+		st->maxIter = maxIter;
+	}
+	free(res);
+	free(search);
+	free(tmp);
+}
+
+/* the diffusion matrix of cgSolveDiffusion (conjugategrad.cpp:355-373):
+ * MakeLaplaceMatrix on an all-fluid flag grid, then obstacle rows of the
+ * real flags become identity (A0 = 1) and every other row is scaled by alpha
+ * with 1 added to the diagonal */
+void og_diffusion_matrix(og_dims d, const int* flags, og_real alpha, og_real* A0, og_real* Ai, og_real* Aj,
+                         og_real* Ak) {
+	long n = NCELLS(d);
+	int* dummy = (int*)calloc((size_t)n, sizeof(int));
+	for (long i = 0; i < n; i++) dummy[i] = OG_FLUID;
+	og_make_laplace(d, dummy, A0, Ai, Aj, Ak);
+	free(dummy);
+	for (long ix = 0; ix < n; ix++) {
+		if (isObs(flags, ix)) {
+			Ai[ix] = Aj[ix] = Ak[ix] = 0.0;
+			A0[ix] = 1.0;
+		} else {
+			Ai[ix] *= alpha;
+			Aj[ix] *= alpha;
+			Ak[ix] *= alpha;
+			A0[ix] *= (OG_MUT(19) ? -alpha : alpha);
+			A0[ix] += 1.;
+		}
+	}
+}
+
+/* cgSolveDiffusion (conjugategrad.cpp:347-423).  ncomp 1: Real grid; ncomp
+ * 3: Vec3 or MAC grid (AoS), whose components 0..2 (0..1 in 2D) are solved
+ * one after another with the same matrix.  st (optional) gets the last
+ * component's statistics.  maxIter = (int)(cgMaxIterFac * max size) * (4 in 2D). */
+void og_cg_solve_diffusion(og_dims d, const int* flags, og_real* grid, int ncomp, og_real alpha,
+                           og_real cgMaxIterFac, og_real cgAccuracy, og_solve_stats* st) {
+	long n = NCELLS(d);
+	size_t nb = sizeof(og_real) * (size_t)n;
+	og_real *A0 = (og_real*)calloc(1, nb), *Ai = (og_real*)calloc(1, nb), *Aj = (og_real*)calloc(1, nb),
+	        *Ak = (og_real*)calloc(1, nb), *rhs = (og_real*)calloc(1, nb), *u = (og_real*)calloc(1, nb);
+	og_diffusion_matrix(d, flags, alpha, A0, Ai, Aj, Ak);
+	int mx = d.nx > d.ny ? d.nx : d.ny;
+	if (d.nz > mx) mx = d.nz;
+	const int maxIter = (int)(cgMaxIterFac * (og_real)mx) * (d.is3d ? 1 : 4);
+	if (ncomp == 1) {
+		memcpy(rhs, grid, nb);
+		plain_gridcg(d, flags, grid, rhs, A0, Ai, Aj, Ak, cgAccuracy, maxIter, st);
+	} else {
+		for (int c = 0; c < (d.is3d ? 3 : 2); c++) {
+			for (long i = 0; i < n; i++) u[i] = grid[3 * i + c]; /* getComponent */
+			memcpy(rhs, u, nb);
+			plain_gridcg(d, flags, u, rhs, A0, Ai, Aj, Ak, cgAccuracy, maxIter, st);
+			for (long i = 0; i < n; i++) grid[3 * i + c] = u[i]; /* setComponent */
+		}
+	}
+	free(A0); free(Ai); free(Aj); free(Ak); free(rhs); free(u);
+}
+
+/* MakeRhsWE (waves.cpp:69-79), bnd=1, rhs cleared first */
+void og_make_rhs_we(og_dims d, const int* flags, og_real* rhs, const og_real* ut, const og_real* utm1, og_real s,
+                    int crankNic) {
+	(void)flags;
+	const int b = 1;
+	memset(rhs, 0, sizeof(og_real) * (size_t)NCELLS(d));
+	const long X = 1, Y = d.nx;
+	for (int k = KMIN(d, b); k < KMAX(d, b); k++)
+		for (int j = b; j < d.ny - b; j++)
+			for (int i = b; i < d.nx - b; i++) {
+				long ix = IX(d, i, j, k);
+				rhs[ix] = (og_real)(2. * ut[ix] - utm1[ix]);
+				if (crankNic)
+					rhs[ix] += s * (-4. * ut[ix] + 1. * ut[ix - X] + 1. * ut[ix + (OG_MUT(20) ? -X : X)] +
+					                1. * ut[ix - Y] + 1. * ut[ix + Y]);
+			}
+}
+
+/* cgSolveWE (waves.cpp:87-141): s = dt*dt*cSqr*0.5; MakeLaplaceMatrix of the
+ * flags scaled by s with 1 added to every diagonal; rhs = MakeRhsWE; GridCg
+ * into out; then utm1 <- ut and ut <- out */
+void og_cg_solve_we(og_dims d, const int* flags, og_real* ut, og_real* utm1, og_real* out, int crankNic,
+                    og_real cSqr, og_real cgMaxIterFac, og_real cgAccuracy, og_real dt, og_solve_stats* st) {
+	long n = NCELLS(d);
+	size_t nb = sizeof(og_real) * (size_t)n;
+	og_real *A0 = (og_real*)calloc(1, nb), *Ai = (og_real*)calloc(1, nb), *Aj = (og_real*)calloc(1, nb),
+	        *Ak = (og_real*)calloc(1, nb), *rhs = (og_real*)calloc(1, nb);
+	memset(out, 0, nb);
+	og_make_laplace(d, flags, A0, Ai, Aj, Ak);
+	const og_real s = (og_real)(dt * dt * cSqr * 0.5);
+	for (long ix = 0; ix < n; ix++) {
+		Ai[ix] *= s;
+		Aj[ix] *= s;
+		Ak[ix] *= s;
+		A0[ix] *= s;
+		A0[ix] += 1.;
+	}
+	og_make_rhs_we(d, flags, rhs, ut, utm1, s, crankNic);
+	int mx = d.nx > d.ny ? d.nx : d.ny;
+	if (d.nz > mx) mx = d.nz;
+	const int maxIter = (int)(cgMaxIterFac * (og_real)mx) * (d.is3d ? 1 : 4);
+	plain_gridcg(d, flags, out, rhs, A0, Ai, Aj, Ak, cgAccuracy, maxIter, st);
+	memcpy(utm1, ut, nb); /* utm1.swap(ut); ut.copyFrom(out) */
+	memcpy(ut, out, nb);
+	free(A0); free(Ai); free(Aj); free(Ak); free(rhs);
 }
 
 /* ===========================================================================

@@ -155,6 +155,17 @@ int og_solve_pressure_system(og_dims d, og_real* rhs, og_real* pressure, const i
                              const og_pressure_opts* opts, og_solve_stats* st);
 void og_pressure_opts_default(og_pressure_opts* o);
 
+/* ---- other GridCg clients (conjugategrad.cpp:347-423, plugin/waves.cpp:69-141) ---- */
+void og_diffusion_matrix(og_dims d, const int* flags, og_real alpha, og_real* A0, og_real* Ai, og_real* Aj,
+                         og_real* Ak);
+/* ncomp 1: Real grid; 3: Vec3/MAC grid (AoS) solved per component */
+void og_cg_solve_diffusion(og_dims d, const int* flags, og_real* grid, int ncomp, og_real alpha,
+                           og_real cgMaxIterFac, og_real cgAccuracy, og_solve_stats* st);
+void og_make_rhs_we(og_dims d, const int* flags, og_real* rhs, const og_real* ut, const og_real* utm1, og_real s,
+                    int crankNic);
+void og_cg_solve_we(og_dims d, const int* flags, og_real* ut, og_real* utm1, og_real* out, int crankNic,
+                    og_real cSqr, og_real cgMaxIterFac, og_real cgAccuracy, og_real dt, og_solve_stats* st);
+
 /* ---- multigrid preconditioner after Dick et al. (multigrid.cpp) ---- */
 typedef struct og_mg og_mg;
 og_mg* og_mg_create(og_dims d);

@@ -342,7 +342,7 @@ def all_pins():
     for is3d in (False, True):
         pins.append(("outflow-%dd" % (3 if is3d else 2), lambda o, a=is3d: pin_outflow_column(o, a)))
     pins.append(("wall_bcs_3d", pin_wall_bcs_3d))
-    return pins + high_order_pins()
+    return pins + high_order_pins() + gridcg_client_pins()
 
 
 # ---------------------------------------------------------------------------
@@ -561,4 +561,153 @@ def high_order_pins():
                 continue  # the MAC midpoint trace follows the advected field, not vel (pin_midpoint_mac)
             pins.append(("hi_maccormack-t%d-s%d-tr%d" % (gt, os_, ot),
                          lambda o, a=(gt, os_, ot): pin_high_order_maccormack(o, *a)))
+    return pins
+
+
+# ---------------------------------------------------------------------------
+# other GridCg clients: cgSolveDiffusion (conjugategrad.cpp:347-423) and
+# cgSolveWE (plugin/waves.cpp:69-141), pinned by eigenvector closed forms of
+# the operators they solve and a closed form of the Crank-Nicolson RHS
+# ---------------------------------------------------------------------------
+def _interior_modes(shape, modes, kind):
+    """Product of 1-D eigenvectors on the cells 1..m of each axis (0 on the
+    outer layer) and the matching eigenvalue of the 5/7-point Laplacian:
+    kind 'dirichlet' (neighbours outside the interior held at 0): sin(pi a i/(m+1)),
+    2 - 2 cos(pi a/(m+1)); kind 'neumann' (those neighbours dropped):
+    cos(pi a (i - 1/2)/m), 2 - 2 cos(pi a/m)."""
+    nx, ny, nz = shape
+    dims = [nx, ny] + ([nz] if nz > 1 else [])
+    field = np.ones((nz, ny, nx), np.float64)
+    lam = 0.0
+    for ax, (n, a) in enumerate(zip(dims, modes)):
+        m = n - 2
+        i = np.arange(n, dtype=np.float64)
+        if kind == "dirichlet":
+            e = np.sin(np.pi * a * i / (m + 1))
+            lam += 2 - 2 * np.cos(np.pi * a / (m + 1))
+        else:
+            e = np.cos(np.pi * a * (i - 0.5) / m)
+            lam += 2 - 2 * np.cos(np.pi * a / m)
+        e[0] = e[-1] = 0.0
+        shp = [1, 1, 1]
+        shp[2 - ax] = n
+        field = field * e.reshape(shp)
+    return field, lam
+
+
+def pin_diffusion_eigen(o, is3d, vec):
+    """Walls on every side, fluid inside: cgSolveDiffusion solves
+    (I + alpha L) x = b on the interior with L the Dirichlet Laplacian (the
+    wall cells are identity rows holding 0), so a Dirichlet eigenvector b of
+    eigenvalue lam gives x = b / (1 + alpha lam).  Vec3/MAC grids: each
+    solved component (two in 2D) with its own mode; the z component of a 2D
+    grid is left alone."""
+    shape = (14, 12, 10) if is3d else (18, 14, 1)
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    flags = o.init_domain(shape)
+    o.fill_grid(shape, flags)
+    alpha = 0.3
+    modes = [(1, 2, 1), (2, 1, 3), (3, 2, 2)]
+    if not vec:
+        v, lam = _interior_modes(shape, modes[0], "dirichlet")
+        g = v.reshape(-1).astype(np.float32)
+        o.cg_solve_diffusion(shape, flags, g, alpha=alpha, cgMaxIterFac=4.0, cgAccuracy=1e-7)
+        exp = v.reshape(-1) / (1 + alpha * lam)
+        assert np.abs(g - exp).max() < 2e-5, ("diffusion real", is3d, np.abs(g - exp).max())
+        return
+    g = np.zeros((n, 3), np.float32)
+    exps = []
+    for c in range(3):
+        v, lam = _interior_modes(shape, modes[c], "dirichlet")
+        g[:, c] = (c + 1) * v.reshape(-1)
+        exps.append((c + 1) * v.reshape(-1) / (1 + alpha * lam))
+    g0 = g.copy()
+    o.cg_solve_diffusion(shape, flags, g, alpha=alpha, cgMaxIterFac=4.0, cgAccuracy=1e-7)
+    for c in range(3 if is3d else 2):
+        d = np.abs(g[:, c] - exps[c]).max()
+        assert d < 5e-5, ("diffusion vec component", is3d, c, d)
+    if not is3d:
+        assert np.array_equal(g[:, 2], g0[:, 2]), "2D diffusion touched the z component"
+
+
+def pin_diffusion_obstacle_rows(o):
+    """Obstacle rows of the diffusion matrix are identity (A0 = 1, no
+    off-diagonals) whatever alpha is; every other interior row has diagonal
+    1 + alpha * (2 dim) and off-diagonals -alpha toward every upper
+    neighbour (the matrix is built on an all-fluid grid); the outer layer's
+    non-obstacle rows are 1 on the diagonal."""
+    shape = (9, 8, 7)
+    rng = np.random.default_rng(5)
+    flags = make_flags(o, shape, bw=1, obstacles=3, rng=rng)
+    A0, Ai, Aj, Ak = o.diffusion_matrix(shape, flags, 0.4)
+    nx, ny, nz = shape
+    f = flags.reshape(nz, ny, nx)
+    inner = np.zeros((nz, ny, nx), bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    obs = (f & OBSTACLE) != 0
+    A0, Ai, Aj, Ak = [a.reshape(nz, ny, nx) for a in (A0, Ai, Aj, Ak)]
+    assert np.all(A0[obs] == 1) and np.all(Ai[obs] == 0) and np.all(Aj[obs] == 0) and np.all(Ak[obs] == 0)
+    m = inner & ~obs
+    assert np.allclose(A0[m], 1 + 0.4 * 6) and np.all(Ai[m] == np.float32(-0.4)) and np.all(Ak[m] == np.float32(-0.4))
+    outer = ~inner & ~obs
+    assert np.all(A0[outer] == 1) and np.all(Ai[outer] == 0)
+
+
+def pin_we_eigen(o, is3d):
+    """Walls on every side, fluid inside: cgSolveWE solves (I + s L) x = b
+    with s = dt^2 cSqr / 2 and L the Neumann Laplacian of the interior (the
+    pressure matrix: wall neighbours are not counted), b = 2 ut - utm1.  A
+    Neumann eigenvector gives out = b / (1 + s lam); afterwards ut = out and
+    utm1 = the old ut."""
+    shape = (12, 11, 10) if is3d else (16, 13, 1)
+    flags = o.init_domain(shape)
+    o.fill_grid(shape, flags)
+    v, lam = _interior_modes(shape, (2, 1, 1), "neumann")
+    ut = v.reshape(-1).astype(np.float32)
+    utm1 = (0.3 * v.reshape(-1)).astype(np.float32)
+    ut0 = ut.copy()
+    out = np.zeros_like(ut)
+    dt, cSqr = 1.5, 0.25
+    s = np.float32(np.float32(dt * dt) * np.float32(cSqr) * 0.5)
+    o.cg_solve_we(shape, flags, ut, utm1, out, dt=dt, cSqr=cSqr, cgMaxIterFac=4.0, cgAccuracy=1e-7)
+    exp = (2 * ut0.astype(np.float64) - 0.3 * v.reshape(-1)) / (1 + float(s) * lam)
+    assert np.abs(out - exp).max() < 2e-5, ("wave solve", is3d, np.abs(out - exp).max())
+    assert np.array_equal(ut, out) and np.array_equal(utm1, ut0)
+
+
+def pin_we_rhs_crank_nicolson(o, is3d):
+    """MakeRhsWE with crankNic: for a quadratic ut = a x^2 + b y^2 + c x y (+ e z^2)
+    the 5-point x-y stencil -4 ut + ut(i+-1) + ut(j+-1) is 2a + 2b at every
+    interior cell, so rhs = 2 ut - utm1 + s (2a + 2b); the stencil has no z
+    terms, so e drops out in 3D.  The outer layer stays 0."""
+    shape = (11, 9, 7) if is3d else (13, 10, 1)
+    nx, ny, nz = shape
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    a, b, c, e = 0.011, 0.007, 0.003, 0.02
+    ut = (a * x * x + b * y * y + c * x * y + e * z * z).astype(np.float32).reshape(-1)
+    utm1 = (0.5 * ut).astype(np.float32)
+    flags = o.init_domain(shape)
+    o.fill_grid(shape, flags)
+    s = 0.37
+    rhs = o.make_rhs_we(shape, flags, ut, utm1, np.float32(s), True).reshape(nz, ny, nx)
+    exp = (2.0 * ut - utm1).reshape(nz, ny, nx) + np.float32(s) * (2 * a + 2 * b)
+    inner = (slice(1, -1) if is3d else slice(None), slice(1, -1), slice(1, -1))
+    assert np.abs(rhs[inner] - exp[inner]).max() < 1e-5, ("crankNic rhs", np.abs(rhs[inner] - exp[inner]).max())
+    mask = np.ones_like(rhs, bool)
+    mask[inner] = False
+    assert np.all(rhs[mask] == 0)
+    plain = o.make_rhs_we(shape, flags, ut, utm1, np.float32(s), False).reshape(nz, ny, nx)
+    assert np.abs(plain[inner] - (2.0 * ut - utm1).reshape(nz, ny, nx)[inner]).max() < 1e-5
+
+
+def gridcg_client_pins():
+    pins = []
+    for is3d in (False, True):
+        for vec in (False, True):
+            pins.append(("diffusion_eigen-%dd-v%d" % (3 if is3d else 2, vec),
+                         lambda o, a=(is3d, vec): pin_diffusion_eigen(o, *a)))
+        pins.append(("we_eigen-%dd" % (3 if is3d else 2), lambda o, a=is3d: pin_we_eigen(o, a)))
+        pins.append(("we_rhs_cn-%dd" % (3 if is3d else 2), lambda o, a=is3d: pin_we_rhs_crank_nicolson(o, a)))
+    pins.append(("diffusion_rows", pin_diffusion_obstacle_rows))
     return pins

@@ -2,7 +2,8 @@
 the oracle with one planted mistake (-DOG_MUTANT=k, the list at the top of
 oracle/mfx_oracle.c: swapped MAC-accessor neighbours, a dropped MacCormack
 skip rule, swapped clamp bounds, an unshifted MAC lookup, outflow and wall
-rule slips, cubic-lookup and midpoint-trace slips) must fail at least one
+rule slips, cubic-lookup and midpoint-trace slips, diffusion-matrix and
+wave-RHS slips) must fail at least one
 check of tests/oracle_pins.py.  CPU only;
 the builds go to a temporary directory."""
 import os
@@ -15,7 +16,7 @@ from oracle import Oracle
 from tests.oracle_pins import all_pins
 
 ORACLE_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
-MUTANTS = list(range(1, 19))
+MUTANTS = list(range(1, 21))
 
 
 @pytest.fixture(scope="module")
