@@ -343,6 +343,16 @@ __global__ void k_cg_state_init(CgState* st, float acc, int maxIter, int useL2, 
 	*st = z;
 }
 
+// CG restart (cg_restart): iterate again from the current x; FIN_BETA0 then
+// sets sigma from the replaced residual
+__global__ void k_cg_restart(CgState* st) {
+	if (threadIdx.x != 0) return;
+	st->done = 0;
+	st->converged = 0;
+	st->first = 1;
+	st->beta = 0.f;
+}
+
 // Zero-pressure cell selection (pressure.cpp:349-381) on the device, in two
 // kernels: k_fix_scan notes whether any cell is empty (then nothing is
 // pinned) and the first fluid cell of the bnd=1 interior in ijk order, i.e.
@@ -1412,7 +1422,7 @@ static void cg_iteration(CgRun& R) {
 
 // FP64 true residual reduced on the device, CgState to the scene's pinned
 // mirror, buffers back to the pool (queued; nothing waits)
-static void cg_finish_enqueue(CgRun& R) {
+static void cg_true_residual_enqueue(CgRun& R) {
 	Scene* s = R.s;
 	const Tiles tr = tiles_for(s->d, kLightKZ);
 	if (R.scaled)
@@ -1427,12 +1437,61 @@ static void cg_finish_enqueue(CgRun& R) {
 	hipLaunchKernelGGL(k_reduce_true_rr, dim3(1), dim3(1024), 0, s->stream, R.st, (const double*)R.dpart,
 	                   (int)tr.count());
 	MFX_HIP(hipMemcpyAsync(s->hscal, R.st, sizeof(CgState), hipMemcpyDeviceToHost, s->stream));
+}
+
+static void cg_release(CgRun& R) {
+	Scene* s = R.s;
 	if (R.pooled) {
 		s->putReal(R.r);
 		s->putReal(R.pa);
 		s->putReal(R.pb);
 		s->putReal(R.q);
 		if (!R.pcNone) s->putReal(R.z);
+	}
+}
+
+// CG restarted from the current x: r = b - A x (FP64 row sums), z = M r,
+// p = z on the next iteration; the iteration count and ||b|| carry on
+static void cg_restart(CgRun& R) {
+	Scene* s = R.s;
+	hipLaunchKernelGGL(k_cg_restart, dim3(1), dim3(64), 0, s->stream, R.st);
+	hipLaunchKernelGGL(k_cg_replace_residual, dim3(R.nbT), dim3(kBlock), 0, s->stream, s->d, R.t,
+	                   (const CgState*)R.st, R.code, (const float*)R.rhs, (const float*)R.x, R.r,
+	                   R.pcNone ? 1 : 0, R.dpart, R.fpart);
+	int np = (int)R.nbT;
+	if (!R.pcNone) {
+		np = R.mg->vcycle(s, R.code, R.r, R.z, R.st);
+		if (np == 0) {
+			run_dot_one(s->stream, R.nbF, DotP{s->d, R.st, R.z, R.r, R.dpart});
+			np = (int)R.nbF;
+		}
+	}
+	run_finalize_one(s->stream, R.fin(np, FIN_BETA0));
+}
+
+// With relTol the stopping rule reads the recursively updated FP32 residual,
+// which can drift from b - A x.  When it has met relTol but the FP64 true
+// residual has not, CG restarts from the current x (at most kMaxRestarts
+// times, within maxIter).  The scene's host mirror holds the state after the
+// true residual.
+constexpr int kMaxRestarts = 3;
+static bool cg_needs_restart(const CgRun& R) {
+	const CgState* hs = (const CgState*)R.s->hscal;
+	if (!(R.o.relTol > 0) || R.scaled || !hs->converged || hs->diverged || hs->iter >= R.maxIter) return false;
+	const double bb = hs->bnorm2;
+	const double rel = bb > 0 ? std::sqrt(hs->trueRR / bb) : std::sqrt(hs->trueRR);
+	return rel > R.o.relTol;
+}
+static void cg_iterate(CgRun& R, double target);
+static void cg_verify(CgRun& R, double target) {
+	for (int k = 0; k < kMaxRestarts && cg_needs_restart(R); k++) {
+		// iterations queued past convergence did nothing; the budget follows
+		// the device count (p = z after the restart, so the buffer parity is free)
+		R.launched = ((const CgState*)R.s->hscal)->iter;
+		cg_restart(R);
+		cg_iterate(R, target);
+		cg_true_residual_enqueue(R);
+		sync(R.s);
 	}
 }
 
@@ -1487,6 +1546,15 @@ static void solve_system(Scene* s, Grid* flags, float* rhs, float* x, const mfx_
 // stopping rule holds or maxIter is reached, then finish (statistics,
 // divergence error).  target: the stopping threshold the prediction aims at.
 static void cg_drive(CgRun& R, double target, mfx_solve_stats* stats) {
+	cg_iterate(R, target);
+	cg_true_residual_enqueue(R);
+	sync(R.s);
+	cg_verify(R, target);
+	cg_release(R);
+	cg_finish_collect(R, stats);
+}
+
+static void cg_iterate(CgRun& R, double target) {
 	Scene* s = R.s;
 	int batch = 4;
 	CgState* hs = (CgState*)s->hscal;
@@ -1509,9 +1577,6 @@ static void cg_drive(CgRun& R, double target, mfx_solve_stats* stats) {
 		}
 		batch = std::max(1, std::min(predict, 64));
 	}
-	cg_finish_enqueue(R);
-	sync(s);
-	cg_finish_collect(R, stats);
 }
 
 
@@ -1964,8 +2029,16 @@ static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>&
 		for (int i = 0; i < P; i++) {
 			Scene* si = vel[(size_t)i]->s;
 			if (i > 0) MFX_HIP(hipStreamWaitEvent(si->stream, s0->ev, 0));
-			cg_finish_enqueue(R[(size_t)i]);
-			correct_velocity(si, flg[(size_t)i], vel[(size_t)i], pres[(size_t)i]->real());
+			cg_true_residual_enqueue(R[(size_t)i]);
+		}
+		// the relTol check on the true residual, as in a single solve; a
+		// problem that needs a restart continues alone on its own stream
+		const double target = o.relTol > 0 ? o.relTol : (double)o.cgAccuracy;
+		for (int i = 0; i < P; i++) {
+			sync(vel[(size_t)i]->s);
+			cg_verify(R[(size_t)i], target);
+			cg_release(R[(size_t)i]);
+			correct_velocity(vel[(size_t)i]->s, flg[(size_t)i], vel[(size_t)i], pres[(size_t)i]->real());
 		}
 		std::string err;
 		for (int i = 0; i < P; i++) {

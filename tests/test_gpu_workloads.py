@@ -128,5 +128,10 @@ def test_served_sessions_full_size(manta, oracle64, monkeypatch):
             keep = ~_outflow_adjacent(flags, sh)
             nr = float(np.sqrt((r[keep] ** 2).sum()))
             nb = float(np.sqrt((bvec[keep] ** 2).sum()))
-            assert nr <= 1.5 * TOL * nb, (i, sh, nr / max(nb, 1e-30))
+            # the downloaded velocity is float: its divergence carries a
+            # rounding error of a few ulps of |u| per cell, which bounds what
+            # this reconstruction can show (a projection of an already
+            # divergence-free field has ||b|| at that level)
+            floor = 8 * 2.0 ** -24 * float(np.sqrt((u ** 2).sum()))
+            assert nr <= 1.5 * TOL * nb + floor, (i, sh, nr, nb, floor)
     assert n_req == 2 * len(trace)
