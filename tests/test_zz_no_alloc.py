@@ -29,6 +29,8 @@ def _opts(**kw):
     lib().mfx_pressure_opts_default(C.byref(o))
     o.preconditioner = 0
     o.cgAccuracy = 1e-4
+    # plain CG needs more than the default 1.5 * max(n) iterations here
+    o.cgMaxIterFac = 8.0
     for k, v in kw.items():
         setattr(o, k, v)
     return o
@@ -58,7 +60,7 @@ def _same(a, b, what):
 
 @pytest.mark.parametrize("shape1,shape2,open_bound", [
     ((40, 36, 28), (21, 17, 13), "Y"),
-    ((120, 96, 1), (50, 44, 1), ""),
+    ((120, 96, 1), (50, 44, 1), "xX"),
 ])
 def test_noalloc_equals_pooled_solve(manta, oracle, rng, shape1, shape2, open_bound):
     for shape in (shape1, shape2):
