@@ -104,6 +104,67 @@ def plume_step(manta, st):
     return st["s"]._solve_stats
 
 
+def _oracle_plume_step(o, shape, flags, keep, srcval, dens, vel, tol):
+    """One plume step on the oracle (the bench's step order), in place."""
+    from oracle import PC_MGSTATIC, TYPE_MAC, TYPE_REAL
+    one = o.dtype(1.0)
+    dens[:] = dens * keep + srcval
+    o.advect(shape, one, flags, vel, dens, TYPE_REAL, 2, 1.0, 2)
+    o.advect(shape, one, flags, vel, vel, TYPE_MAC, 2, 1.0, 2)
+    o.reset_outflow(shape, flags, None, dens)
+    o.set_wall_bcs(shape, flags, vel)
+    o.add_buoyancy(shape, one, flags, dens, vel, (0, -1e-3, 0))
+    st = o.solve_pressure(shape, vel, np.zeros(dens.shape[0], o.dtype), flags, cgAccuracy=min(1e-4, tol),
+                          preconditioner=PC_MGSTATIC, relTol=tol)
+    o.set_wall_bcs(shape, flags, vel)
+    return st
+
+
+def plume_step_error(manta, st, threads=16):
+    """Error of one GPU plume step against the FP64 oracle's step from the
+    same (downloaded) state, with the projection solved to relative residual
+    1e-10, next to the CPU FP32 oracle's error for the same step at the
+    bench's tolerance (1e-4): max |x - x64| / max |x64| for density and
+    velocity."""
+    from oracle import Oracle
+    s = st["s"]
+    n = st["flags"].getSizeX()
+    shape = (n, n, n)
+    flags = st["flags"].toNumpy()
+    keep, srcval = st["keep"].toNumpy(), st["srcval"].toNumpy()
+    d0, v0 = st["density"].toNumpy().copy(), st["vel"].toNumpy().copy()
+    plume_step(manta, st)
+    s.sync()
+    gd, gv = st["density"].toNumpy(), st["vel"].toNumpy()
+    out = {"grid": [n, n, n], "threads": threads}
+    ref = None
+    for f64, tol in ((True, 1e-10), (False, 1e-4)):
+        o = Oracle(f64=f64)
+        o.set_threads(threads)
+        dens, vel = d0.astype(o.dtype), v0.astype(o.dtype)
+        t0 = time.perf_counter()
+        sto = _oracle_plume_step(o, shape, flags, keep.astype(o.dtype), srcval.astype(o.dtype), dens, vel, tol)
+        dt = time.perf_counter() - t0
+        if f64:
+            ref = (dens, vel)
+            out["fp64_oracle"] = {"rel_tol": tol, "s": round(dt, 2), "iterations": sto["iterations"]}
+        else:
+            d32 = dens
+            out["cpu_fp32_oracle"] = {"rel_tol": tol, "s": round(dt, 2), "iterations": sto["iterations"],
+                                      "density_err_vs_fp64": _rel_err(dens, ref[0]),
+                                      "velocity_err_vs_fp64": _rel_err(vel, ref[1]),
+                                      "density_err_vs_fp64_detail": _err_detail(dens, ref[0]),
+                                      "velocity_err_vs_fp64_detail": _err_detail(vel, ref[1])}
+    out["gpu"] = {"rel_tol": 1e-4, "density_err_vs_fp64": _rel_err(gd, ref[0]),
+                  "velocity_err_vs_fp64": _rel_err(gv, ref[1]),
+                  "density_err_vs_fp64_detail": _err_detail(gd, ref[0]),
+                  "velocity_err_vs_fp64_detail": _err_detail(gv, ref[1]),
+                  # advection and forces are bit-exact; the density does not
+                  # pass through the pressure solve
+                  "density_bitexact_vs_cpu_fp32_oracle": bool(np.array_equal(gd, d32))}
+    return out
+
+
 def cpu_baseline(n_sample=128, steps=3, threads=None, warm=10):
     """The CPU oracle (C restatement of the reference's plugins, OpenMP over
     the data-parallel loops) on the same plume step at a smaller grid."""
@@ -187,16 +248,18 @@ def bench_batched(manta, n2d=32, n3d=32):
             "iterations_max": int(max(its)), "rel_residual_max": rel}
 
 
-def bench_projection(manta, n=256, frac=0.15):
+def bench_projection(manta, n=256, frac=0.15, fp64=True, threads=16):
     """n^3 pressure projection: walls plus seeded boxes/spheres covering 15 %
     of the interior, face velocities N(0,1) smoothed by three binomial passes
     per axis, then setWallBcs.  Multigrid-preconditioned CG to relative
     residual 1e-4 and 1e-6, FP32 and mixed precision (BF16 coarse-level
     stencils), FP64 true residual recomputed every 50 iterations.  Reports
     time, iterations, the FP64 true relative residual, the velocity error
-    against a 1e-7 FP32 solution of the same system, and the projection's
-    roofline fraction at 115 algorithmic bytes per cell per MG-PCG iteration
-    (CG init counted as one iteration)."""
+    against the FP64 oracle's solution of the same system (MG-PCG to relative
+    residual 1e-10; max |u - u64| / max |u64|) next to the CPU FP32 oracle's
+    error at the same tolerances, and the projection's roofline fraction at
+    115 algorithmic bytes per cell per MG-PCG iteration (CG init counted as
+    one iteration)."""
     from mantaflow_amd import workloads
     p = workloads.Problem((n, n, n), frac, seed=3)
     kw = dict(cgAccuracy=1e-4, preconditioner=manta.PcMGStatic)
@@ -213,25 +276,69 @@ def bench_projection(manta, n=256, frac=0.15):
         s.sync()
         return time.perf_counter() - t0, manta.lastSolveStats(s)
 
-    s.mfxTrueResidualEvery = 10
-    solve(0, 1e-7)
-    ref = p.vel.toNumpy().copy()
-    scale = float(np.abs(ref).max())
-    s.mfxTrueResidualEvery = 50
     out = {"grid": [n, n, n], "obstacle_fraction": frac, "true_residual_every": 50, "runs": {}}
+    ref = None
+    if fp64:
+        from oracle import Oracle, PC_MGSTATIC
+        flags, vel0 = p.flags.toNumpy(), p.vel0
+        o64 = Oracle(f64=True)
+        o64.set_threads(threads)
+        v64 = vel0.astype(np.float64)
+        o64.set_wall_bcs((n, n, n), flags, v64)
+        t0 = time.perf_counter()
+        st64 = o64.solve_pressure((n, n, n), v64, np.zeros(n ** 3), flags, cgAccuracy=1e-10,
+                                  preconditioner=PC_MGSTATIC, relTol=1e-10)
+        ref = v64
+        out["fp64_oracle"] = {"s": round(time.perf_counter() - t0, 2), "iterations": st64["iterations"],
+                              "rel_residual": st64["relResidual"], "threads": threads}
+        o32 = Oracle()
+        o32.set_threads(threads)
+        out["cpu_fp32_oracle"] = {}
+        for tol in (1e-4, 1e-6):
+            v32 = vel0.copy()
+            o32.set_wall_bcs((n, n, n), flags, v32)
+            t0 = time.perf_counter()
+            # (bounded: in FP32 the reference's recursive residual may stall above 1e-6)
+            st32 = o32.solve_pressure((n, n, n), v32, np.zeros(n ** 3, np.float32), flags, cgAccuracy=1e-4,
+                                      preconditioner=PC_MGSTATIC, relTol=tol, maxIterOverride=40)
+            out["cpu_fp32_oracle"]["%g" % tol] = {
+                "s": round(time.perf_counter() - t0, 2), "iterations": st32["iterations"],
+                "rel_residual": st32["relResidual"], "velocity_err_vs_fp64": _rel_err(v32, ref),
+                "velocity_err_vs_fp64_detail": _err_detail(v32, ref)}
+    else:
+        s.mfxTrueResidualEvery = 10
+        solve(0, 1e-7)
+        ref = p.vel.toNumpy().copy()
+        s.mfxTrueResidualEvery = 50
     for prec, name in ((0, "fp32"), (1, "mixed")):
         solve(prec, 1e-4)  # builds the hierarchy for this precision
         for tol in (1e-4, 1e-6):
             t, st = solve(prec, tol)
             its = int(st["iterations"])
             gbs = (its + 1) * 115.0 * n ** 3 / t / 1e9
-            err = float(np.abs(p.vel.toNumpy() - ref).max()) / scale
+            u = p.vel.toNumpy()
             out["runs"]["%s_%g" % (name, tol)] = {
                 "ms": round(t * 1e3, 3), "iterations": its, "true_rel_residual": st["relResidual"],
-                "converged": bool(st["converged"]), "velocity_err_vs_1e-7": err,
+                "converged": bool(st["converged"]),
+                ("velocity_err_vs_fp64" if fp64 else "velocity_err_vs_1e-7"): _rel_err(u, ref),
+                ("velocity_err_vs_fp64_detail" if fp64 else "velocity_err_vs_1e-7_detail"): _err_detail(u, ref),
                 "achieved_GBs": round(gbs, 1), "roofline_frac": round(gbs / HBM_PEAK_GBS, 4)}
     s.mfxPrecision = 0
     return out
+
+
+def _err_detail(a, b):
+    """RMS error relative to the RMS of b, and the number of cells whose
+    largest component error exceeds 1e-3 of max |b| (a max-norm error can
+    sit in a handful of cells, e.g. one-cell-wide channels between
+    obstacles)."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    d = np.abs(a - b)
+    scale = max(1e-30, float(np.abs(b).max()))
+    cell = d.max(axis=1) if d.ndim == 2 else d
+    return {"rms": float(np.sqrt((d * d).mean()) / max(1e-30, np.sqrt((b * b).mean()))),
+            "cells_over_1e-3": int((cell > 1e-3 * scale).sum())}
 
 
 def _rel_err(a, b):
@@ -288,30 +395,47 @@ def config1_plume2d(manta, steps=200, threads=16):
     return out
 
 
-def config2_plume3d(manta, n=128, steps=100, threads=16):
-    """128^3 open-top plume, MacCormack, MG-static PCG at 1e-4: GPU time for
-    100 steps, CPU oracle (FP32) time per step, FP64 oracle over the same 100
-    steps and the GPU's error against it after 1 and after 100 steps."""
+def _plume_integrals(dens, vel, shape):
+    """Total density, kinetic energy (half the sum of squared face
+    velocities) and the density-weighted mean height (cells) of a plume."""
+    d = np.asarray(dens, np.float64)
+    v = np.asarray(vel, np.float64)
+    tot = float(d.sum())
+    j = (np.arange(d.size) // shape[0]) % shape[1]
+    return {"density_total": tot, "kinetic_energy": 0.5 * float((v * v).sum()),
+            "plume_height": float((d * j).sum() / tot) if tot > 0 else 0.0}
+
+
+def config2_plume3d(manta, n=128, steps=100, threads=16, record=(1, 10, 25, 50, 75, 100)):
+    """128^3 open-top plume, MacCormack, MG-static PCG at rel. res. 1e-4, 100
+    steps on the GPU, the CPU oracle (FP32) and the FP64 oracle from the same
+    initial state.  Reports the GPU time per step, the oracle times, the
+    pairwise field errors (max |a - b| / max |b|) of FP32 oracle vs FP64
+    oracle, GPU vs FP64 oracle and GPU vs FP32 oracle at the recorded steps,
+    and per-step integrals (total density, kinetic energy, plume height) of
+    the three runs with the largest relative spread between them."""
     from oracle import Oracle, PC_MGSTATIC, TYPE_MAC, TYPE_REAL
     st = plume_setup(manta, n)
     s = st["s"]
-    its = []
-    first = None
+    its, snaps, integ = [], {}, {"gpu": [], "cpu_oracle_fp32": [], "fp64_oracle": []}
     s.sync()
-    t0 = time.perf_counter()
-    for k in range(steps):
+    tg = 0.0
+    for k in range(1, steps + 1):
+        t0 = time.perf_counter()
         its.append(plume_step(manta, st)["iterations"])
-        if k == 0:
-            s.sync()
-            first = (st["density"].toNumpy().copy(), st["vel"].toNumpy().copy())
-            t0 = time.perf_counter()  # timed: steps 2..100
-    s.sync()
-    tg = time.perf_counter() - t0
-    gd, gv = st["density"].toNumpy(), st["vel"].toNumpy()
-    out = {"gpu_ms_per_step": round(tg / (steps - 1) * 1e3, 3), "gpu_mcell_steps_per_s": round(n ** 3 * (steps - 1) / tg / 1e6, 1),
+        s.sync()
+        if k > 1:
+            tg += time.perf_counter() - t0
+        gd, gv = st["density"].toNumpy(), st["vel"].toNumpy()
+        integ["gpu"].append(_plume_integrals(gd, gv, (n, n, n)))
+        if k in record:
+            snaps[("gpu", k)] = (gd.copy(), gv.copy())
+    out = {"gpu_ms_per_step": round(tg / (steps - 1) * 1e3, 3),
+           "gpu_mcell_steps_per_s": round(n ** 3 * (steps - 1) / tg / 1e6, 1),
            "cg_iterations_mean": float(np.mean(its))}
     shape, N = (n, n, n), n ** 3
     for f64 in (False, True):
+        key = "fp64_oracle" if f64 else "cpu_oracle_fp32"
         o = Oracle(f64=f64)
         o.set_threads(threads)
         flags = o.init_domain(shape, boundaryWidth=1)
@@ -321,31 +445,33 @@ def config2_plume3d(manta, n=128, steps=100, threads=16):
         o.apply_shape(shape, 3, [0.5 * n, 0.1 * n, 0.5 * n], [0, 0.02 * n, 0], 0.14 * n, mask, TYPE_REAL, 1.0)
         val = (np.random.default_rng(1).uniform(0.8, 1.0, N).astype(np.float32) * mask.astype(np.float32)).astype(o.dtype)
         keep = (1.0 - mask).astype(o.dtype)
-        vel, dens, p = np.zeros((N, 3), o.dtype), np.zeros(N, o.dtype), np.zeros(N, o.dtype)
-        nsteps = steps if f64 else 5
-        err1 = None
+        vel, dens = np.zeros((N, 3), o.dtype), np.zeros(N, o.dtype)
         t0 = time.perf_counter()
-        for k in range(nsteps):
-            dens = (dens * keep + val).astype(o.dtype)
-            o.advect(shape, 1.0, flags, vel, dens, TYPE_REAL, 2, 1.0, 2)
-            o.advect(shape, 1.0, flags, vel, vel, TYPE_MAC, 2, 1.0, 2)
-            o.reset_outflow(shape, flags, None, dens)
-            o.set_wall_bcs(shape, flags, vel)
-            o.add_buoyancy(shape, 1.0, flags, dens, vel, (0, -1e-3, 0))
-            o.solve_pressure(shape, vel, p, flags, cgAccuracy=1e-4, preconditioner=PC_MGSTATIC, relTol=1e-4)
-            o.set_wall_bcs(shape, flags, vel)
-            if k == 0 and f64:
-                err1 = (_rel_err(first[0], dens), _rel_err(first[1], vel))
+        for k in range(1, steps + 1):
+            _oracle_plume_step(o, shape, flags, keep, val, dens, vel, 1e-4)
+            integ[key].append(_plume_integrals(dens, vel, shape))
+            if k in record:
+                snaps[(key, k)] = (dens.copy(), vel.copy())
         tc = time.perf_counter() - t0
-        if f64:
-            out["fp64_oracle"] = {"steps": nsteps, "s": round(tc, 1), "threads": threads,
-                                  "gpu_err_after_1_step": {"density": err1[0], "velocity": err1[1]},
-                                  "gpu_err_after_%d_steps" % steps: {"density": _rel_err(gd, dens),
-                                                                      "velocity": _rel_err(gv, vel)}}
-        else:
-            out["cpu_oracle_fp32"] = {"ms_per_step": round(tc / nsteps * 1e3, 1), "threads": threads,
-                                      "mcell_steps_per_s": round(N * nsteps / tc / 1e6, 2),
-                                      "sample": "%d steps from the initial state" % nsteps}
+        out[key] = {"ms_per_step": round(tc / steps * 1e3, 1), "threads": threads,
+                    "mcell_steps_per_s": round(N * steps / tc / 1e6, 2)}
+    errs = {}
+    for a, b in (("cpu_oracle_fp32", "fp64_oracle"), ("gpu", "fp64_oracle"), ("gpu", "cpu_oracle_fp32")):
+        errs["%s_vs_%s" % (a, b)] = {str(k): {"density": _rel_err(snaps[(a, k)][0], snaps[(b, k)][0]),
+                                              "velocity": _rel_err(snaps[(a, k)][1], snaps[(b, k)][1])}
+                                     for k in record}
+    out["field_errors"] = errs
+    spread = {}
+    for q in ("density_total", "kinetic_energy", "plume_height"):
+        worst = 0.0
+        for k in range(steps):
+            vals = [integ[r][k][q] for r in integ]
+            ref = abs(integ["fp64_oracle"][k][q])
+            if ref > 0:
+                worst = max(worst, (max(vals) - min(vals)) / ref)
+        spread[q] = worst
+    out["integrals"] = {r: {str(k): integ[r][k - 1] for k in record} for r in integ}
+    out["integrals_max_rel_spread"] = spread
     return out
 
 
@@ -506,6 +632,10 @@ def main():
                          "64 batched problems, as one JSON object (a few minutes)")
     ap.add_argument("--no-projection", action="store_true",
                     help="skip the obstacle-rich projection leg (FP32 and mixed precision, rel. res. 1e-4/1e-6)")
+    ap.add_argument("--no-developed", action="store_true",
+                    help="skip the developed-plume window (10 steps timed after step 100)")
+    ap.add_argument("--no-errors", action="store_true",
+                    help="skip the FP64-oracle error legs (one plume step, the projection)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run 5 plain 64 MiB device copies first (PMC byte-counter calibration)")
     a = ap.parse_args()
@@ -573,11 +703,35 @@ def main():
                 "traffic": measured_traffic(best, n),
                 "algo_bytes_per_launch": algo, "avg_launch_ms": round(kstats[best]["avg_ms"], 4)}
 
-    batched = served = proj = None
+    # a developed plume: continue untimed to step 100, then time 10 steps
+    developed = None
+    if not a.no_developed:
+        done = a.warmup + a.steps
+        for _ in range(max(0, 100 - done)):
+            plume_step(manta, st)
+        s.sync()
+        dits = []
+        t0 = time.perf_counter()
+        for _ in range(10):
+            dits.append(plume_step(manta, st)["iterations"])
+        s.sync()
+        tdev = time.perf_counter() - t0
+        first = max(done, 100) + 1
+        developed = {"steps": [first, first + 9], "ms_per_step": round(tdev / 10 * 1e3, 3),
+                     "mcell_steps_per_s": round(cells * 10 / tdev / 1e6, 1),
+                     "cg_iterations_mean": float(np.mean(dits))}
+
+    batched = served = proj = steperr = None
+    if not a.no_errors:
+        print("bench: one plume step against the FP64 oracle ...", file=sys.stderr, flush=True)
+        try:
+            steperr = plume_step_error(manta, st)
+        except Exception as e:
+            steperr = {"error": str(e)}
     if not a.no_projection:
         print("bench: 256^3 projection ...", file=sys.stderr, flush=True)
         try:
-            proj = bench_projection(manta, n)
+            proj = bench_projection(manta, n, fp64=not a.no_errors)
         except Exception as e:
             proj = {"error": str(e)}
     if not a.no_batched:
@@ -613,11 +767,14 @@ def main():
         "config": {"workload": "3D smoke plume step %d^3: MacCormack advection (density + MAC velocity), "
                                "open top, wall BCs, buoyancy, MG-preconditioned CG projection to rel. "
                                "residual 1e-4" % n,
-                   "grid": [n, n, n], "cg_iterations_mean": float(np.mean(iters)),
+                   "grid": [n, n, n], "step_window": [a.warmup + 1, a.warmup + a.steps],
+                   "cg_iterations_mean": float(np.mean(iters)),
                    "rel_residual_max": float(np.max(rel)), "parallelism": "single GPU",
                    "zmarch_plane_chunks_autotuned": {"cg_matvec": mvc.value, "mg_level0": l0c.value}},
         "roofline": roof,
         "kernels": kstats,
+        "developed_plume": developed,
+        "step_error": steperr,
         "projection": proj,
         "batched_solves": batched,
         "serving": served,
