@@ -120,6 +120,27 @@ def test_advect_mac_self(manta, oracle, rng, shape, order, clamp, open_bound):
     assert_bitexact(v.toNumpy(), ref, "advect mac")
 
 
+@pytest.mark.parametrize("shape", [(70, 13, 37), (130, 9, 20), (5, 7, 3)])
+@pytest.mark.parametrize("scale", [0.4, 4.0])
+@pytest.mark.parametrize("order,clamp", [(1, 2), (2, 1), (2, 2)])
+def test_advect_mac_self_wide(manta, oracle, rng, shape, scale, order, clamp):
+    """Self-advection of 3D MAC grids wider than one 64-cell tile row, with
+    more planes than a tile, a grid of 3 planes, and velocities whose traces
+    stay within a cell or two (scale 0.4) or reach ~5 cells (scale 4);
+    bit-exact against the oracle."""
+    n = int(np.prod(shape))
+    flags = make_flags(oracle, shape, bw=1, open_bound="Y", obstacles=4, rng=rng)
+    oracle.reset_outflow(shape, flags)
+    vel = rand_vel(rng, n, scale)
+    dt = 1.2
+    sc = GpuScene(manta, shape, dt)
+    f, v = sc.flags(flags), sc.mac(vel)
+    ref = vel.copy()
+    oracle.advect(shape, np.float32(dt), flags, ref, ref, TYPE_MAC, order, 1.0, clamp)
+    manta.advectSemiLagrange(flags=f, vel=v, grid=v, order=order, strength=1.0, clampMode=clamp)
+    assert_bitexact(v.toNumpy(), ref, "advect mac %s scale %g" % (shape, scale))
+
+
 @pytest.mark.parametrize("shape", [(24, 20, 1), (17, 13, 11)])
 @pytest.mark.parametrize("clamp", [1, 2])
 def test_advect_index64_path(manta, oracle, rng, shape, clamp):
