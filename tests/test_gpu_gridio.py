@@ -55,3 +55,19 @@ def test_uni_load_refuses_other_type_or_size(tmp_path):
     t = manta.Solver(name="io2", gridSize=manta.vec3(8, 8, 8), dim=3)
     with pytest.raises(RuntimeError, match="grid dim doesn't match"):
         t.create(manta.RealGrid).load(p)
+
+
+@pytest.mark.parametrize("name,dims,cls,dim", [("real_5x4x3", (5, 4, 3), "RealGrid", 3),
+                                                ("vec3_4x3x2", (4, 3, 2), "MACGrid", 3),
+                                                ("level_6x5x1", (6, 5, 1), "LevelsetGrid", 2)])
+def test_device_grid_load_reference_file(name, dims, cls, dim):
+    """Grid.load of files written by the reference's own .uni writer
+    (tests/golden): the device grid holds the file's cells exactly."""
+    import os
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    ref = np.load(os.path.join(here, "uniio_read.npz"))
+    s = manta.Solver(name="gold", gridSize=manta.vec3(*dims), dim=dim)
+    g = s.create(getattr(manta, cls))
+    g.load(os.path.join(here, name + ".uni"))
+    want = ref[name + "_data"].reshape(-1, 3) if cls == "MACGrid" else ref[name + "_data"].reshape(-1)
+    np.testing.assert_array_equal(g.toNumpy(), want)

@@ -1,8 +1,10 @@
 """Grid file formats (.uni, .raw) of the reference's Grid<T>::save / load
 (fileio/iogrids.cpp), host side: byte layout of the MNT3 header at the
-offsets of the reference's packed struct, round trips, and the checks and
-error texts of readGridUni."""
+offsets of the reference's packed struct, round trips, the checks and
+error texts of readGridUni, and files written and read by the reference's own
+Python .uni module (golden files under tests/golden)."""
 import gzip
+import os
 import struct
 
 import numpy as np
@@ -71,3 +73,43 @@ def test_raw_round_trip_and_length_check(tmp_path):
     assert np.array_equal(gridio.read_raw(p, 60, False), a)
     with pytest.raises(gridio.GridIOError, match="stream length does not match"):
         gridio.read_raw(p, 61, False)
+
+
+# golden files written by the reference's own Python .uni writer
+# (tensorflow/tools/uniio.py, tests/golden/make_uni_golden.py), with what its
+# reader returned for them
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+GOLDEN_CASES = [("real_5x4x3", (5, 4, 3), 1, 1), ("vec3_4x3x2", (4, 3, 2), 12, 3), ("level_6x5x1", (6, 5, 1), 17, 1)]
+
+
+@pytest.mark.parametrize("name,dims,gtype,nc", GOLDEN_CASES)
+def test_uni_reads_reference_writer_files(name, dims, gtype, nc):
+    ref = np.load(os.path.join(GOLDEN, "uniio_read.npz"))
+    h, a = gridio.read_uni(os.path.join(GOLDEN, name + ".uni"), dims=dims, grid_type=gtype)
+    hv = ref[name + "_header"]
+    assert h["dims"] == tuple(int(v) for v in hv[:3]) == dims
+    assert (h["gridType"], h["elementType"], h["bytesPerElement"], h["dimT"], h["timestamp"]) == \
+        tuple(int(v) for v in hv[3:])
+    want = ref[name + "_content"].reshape(-1, nc) if nc == 3 else ref[name + "_content"].reshape(-1)
+    assert a.dtype == np.float32 and np.array_equal(a, want)
+    assert np.array_equal(a.reshape(-1), ref[name + "_data"].reshape(-1))
+
+
+@pytest.mark.parametrize("name,dims,gtype,nc", GOLDEN_CASES)
+def test_uni_writes_reference_reader_bytes(tmp_path, name, dims, gtype, nc):
+    """This package's writer and the reference's writer give the same
+    uncompressed stream (id, header fields, cell data); only the free-text
+    info field differs."""
+    import gzip
+    ref = np.load(os.path.join(GOLDEN, "uniio_read.npz"))
+    data = ref[name + "_data"].reshape(-1, nc) if nc == 3 else ref[name + "_data"].reshape(-1)
+    ts = int(ref[name + "_header"][7])
+    p = str(tmp_path / "x.uni")
+    gridio.write_uni(p, data, dims, gtype, timestamp=ts)
+    with gzip.open(p, "rb") as f:
+        mine = f.read()
+    with gzip.open(os.path.join(GOLDEN, name + ".uni"), "rb") as f:
+        theirs = f.read()
+    assert len(mine) == len(theirs) == 4 + 288 + 4 * nc * dims[0] * dims[1] * dims[2]
+    assert mine[:28] == theirs[:28]          # id + dimX..bytesPerElement
+    assert mine[280:] == theirs[280:]        # dimT, timestamp, data
