@@ -34,6 +34,13 @@ def _opts_struct(real):
     return Opts
 
 
+def _bnd_struct(real):
+    class Bnd(C.Structure):
+        _fields_ = [("phi", C.c_void_p), ("perCellCorr", C.c_void_p), ("fractions", C.c_void_p),
+                    ("obvel", C.c_void_p), ("curv", C.c_void_p), ("gfClamp", real), ("surfTens", real)]
+    return Bnd
+
+
 def _stats_struct(real):
     class Stats(C.Structure):
         _fields_ = [("iterations", C.c_int), ("resNorm", real), ("relResidual", C.c_double),
@@ -55,6 +62,7 @@ class Oracle:
         self.creal = C.c_double if f64 else C.c_float
         self.Opts = _opts_struct(self.creal)
         self.Stats = _stats_struct(self.creal)
+        self.Bnd = _bnd_struct(self.creal)
         L, R, P = self.lib, self.creal, C.c_void_p
         D, I = Dims, C.c_int
         sig = {
@@ -97,6 +105,14 @@ class Oracle:
             "og_diffusion_matrix": (None, [D, P, R, P, P, P, P]),
             "og_cg_solve_diffusion": (None, [D, P, P, I, R, R, R, P]),
             "og_make_rhs_we": (None, [D, P, P, P, P, R, I]),
+            "og_pressure_bnd_default": (None, [P]),
+            "og_ghost_fluid_factor": (R, [R, R, R]),
+            "og_make_rhs_bnd": (I, [D, P, P, P, P, C.POINTER(C.c_double)]),
+            "og_make_laplace_bnd": (None, [D, P, P, P, P, P, P]),
+            "og_correct_velocity_bnd": (None, [D, P, P, P, P]),
+            "og_solve_pressure_system_bnd": (I, [D, P, P, P, P, P, P]),
+            "og_solve_pressure_bnd": (I, [D, P, P, P, P, P, P, P]),
+            "og_set_wall_bcs_frac": (None, [D, P, P, P]),
             "og_cg_solve_we": (None, [D, P, P, P, P, I, R, R, R, R, P]),
         }
         for k, (res, args) in sig.items():
@@ -240,6 +256,63 @@ class Oracle:
             raise RuntimeError("GridCg::iterate: The CG solver diverged, residual norm > 1e30, stopping.")
         return {"iterations": st.iterations, "resNorm": st.resNorm, "relResidual": st.relResidual,
                 "fixPidx": st.fixPidx, "maxIter": st.maxIter, "rhs": rhs}
+
+    # -- second-order obstacle / free-surface boundaries ------------------
+    def bnd(self, phi=None, perCellCorr=None, fractions=None, obvel=None, curv=None, gfClamp=1e-4,
+            surfTens=0.0):
+        """The optional solvePressure inputs (pressure.cpp:480-495); the arrays
+        are kept referenced by the returned struct's ``_keep``."""
+        b = self.Bnd()
+        keep = []
+        for name, a, nc in (("phi", phi, 1), ("perCellCorr", perCellCorr, 1), ("fractions", fractions, 3),
+                            ("obvel", obvel, 3), ("curv", curv, 1)):
+            if a is not None:
+                ptr = self._arr(a, ncomp=nc)
+                keep.append(a)
+                setattr(b, name, ptr)
+        b.gfClamp = gfClamp
+        b.surfTens = surfTens
+        b._keep = keep
+        return b
+
+    def ghost_fluid_factor(self, phi_in, phi_out, gfClamp=1e-4):
+        return self.lib.og_ghost_fluid_factor(phi_in, phi_out, gfClamp)
+
+    def make_rhs_bnd(self, shape, flags, vel, b):
+        n = int(np.prod(shape))
+        rhs = np.zeros(n, self.dtype)
+        s = C.c_double(0)
+        cnt = self.lib.og_make_rhs_bnd(self.dims(shape), self._arr(flags, np.int32), self._arr(rhs),
+                                       self._arr(vel), C.byref(b), C.byref(s))
+        return rhs, cnt, s.value
+
+    def make_laplace_bnd(self, shape, flags, b):
+        n = int(np.prod(shape))
+        A = [np.zeros(n, self.dtype) for _ in range(4)]
+        self.lib.og_make_laplace_bnd(self.dims(shape), self._arr(flags, np.int32), *[self._arr(a) for a in A],
+                                     C.byref(b))
+        return A
+
+    def correct_velocity_bnd(self, shape, flags, vel, pressure, b):
+        self.lib.og_correct_velocity_bnd(self.dims(shape), self._arr(flags, np.int32), self._arr(vel),
+                                         self._arr(pressure), C.byref(b))
+
+    def solve_pressure_bnd(self, shape, vel, pressure, flags, b, **kw):
+        o = self.opts(**kw)
+        st = self.Stats()
+        n = int(np.prod(shape))
+        rhs = np.zeros(n, self.dtype)
+        rc = self.lib.og_solve_pressure_bnd(self.dims(shape), self._arr(vel), self._arr(pressure),
+                                            self._arr(flags, np.int32), C.byref(o), C.byref(b), C.byref(st),
+                                            self._arr(rhs))
+        if rc != 0:
+            raise RuntimeError("GridCg::iterate: The CG solver diverged, residual norm > 1e30, stopping.")
+        return {"iterations": st.iterations, "resNorm": st.resNorm, "relResidual": st.relResidual,
+                "fixPidx": st.fixPidx, "maxIter": st.maxIter, "rhs": rhs}
+
+    def set_wall_bcs_frac(self, shape, flags, vel, phiObs):
+        self.lib.og_set_wall_bcs_frac(self.dims(shape), self._arr(flags, np.int32), self._arr(vel),
+                                      self._arr(phiObs))
 
     def solve_pressure_system(self, shape, rhs, pressure, flags, **kw):
         o = self.opts(**kw)

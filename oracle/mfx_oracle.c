@@ -57,7 +57,11 @@
  *  17  MAC midpoint trace: velocity instead of the advected field
  *  18  interpolMAC: x component without its shifted index
  *  19  cgSolveDiffusion matrix: diagonal scaled by -alpha
- *  20  MakeRhsWE (Crank-Nicolson): x+1 neighbour read at x-1 */
+ *  20  MakeRhsWE (Crank-Nicolson): x+1 neighbour read at x-1
+ *  21  MakeRhs with fractions: the +y face's fraction read at the cell
+ *  22  ghostFluidHelper: 1 - alpha instead of 1 - 1/alpha
+ *  23  KnSetWallBcsFrac, x face: y gradient with the sides swapped
+ *  24  knCorrectVelocityGhostFluid: empty cell's x face scaled by its own pressure */
 #ifndef OG_MUTANT
 #define OG_MUTANT 0
 #endif
@@ -1205,6 +1209,13 @@ static double rel_residual(og_dims d, const int* flags, const og_real* b, const 
 /* solvePressureSystem (pressure.cpp:312-452) with GridCg (conjugategrad.cpp:198-299) */
 int og_solve_pressure_system(og_dims d, og_real* rhs, og_real* pressure, const int* flags,
                              const og_pressure_opts* opts, og_solve_stats* st) {
+	return og_solve_pressure_system_bnd(d, rhs, pressure, flags, opts, NULL, st);
+}
+
+/* solvePressureSystem (pressure.cpp:312-452): matrix (with fractions and the
+ * ghost-fluid diagonal when given), optional pinned cell, preconditioned CG */
+int og_solve_pressure_system_bnd(og_dims d, og_real* rhs, og_real* pressure, const int* flags,
+                                 const og_pressure_opts* opts, const og_pressure_bnd* bnd, og_solve_stats* st) {
 	og_pressure_opts o;
 	if (opts) o = *opts;
 	else og_pressure_opts_default(&o);
@@ -1214,7 +1225,7 @@ int og_solve_pressure_system(og_dims d, og_real* rhs, og_real* pressure, const i
 	        *A0 = (og_real*)calloc(1, nb), *Ai = (og_real*)calloc(1, nb),
 	        *Aj = (og_real*)calloc(1, nb), *Ak = (og_real*)calloc(1, nb),
 	        *tmp = (og_real*)calloc(1, nb), *P = NULL;
-	og_make_laplace(d, flags, A0, Ai, Aj, Ak);
+	og_make_laplace_bnd(d, flags, A0, Ai, Aj, Ak, bnd);
 	int fix = -1;
 	if (o.zeroPressureFixing || o.cgAccuracy < 1e-07) {
 		fix = og_find_fix_cell(d, flags);
@@ -1308,22 +1319,343 @@ int og_solve_pressure_system(og_dims d, og_real* rhs, og_real* pressure, const i
 /* solvePressure (pressure.cpp:480-521) */
 int og_solve_pressure(og_dims d, og_real* vel, og_real* pressure, const int* flags,
                       const og_pressure_opts* opts, og_solve_stats* st, og_real* rhsOut) {
+	return og_solve_pressure_bnd(d, vel, pressure, flags, opts, NULL, st, rhsOut);
+}
+
+int og_solve_pressure_bnd(og_dims d, og_real* vel, og_real* pressure, const int* flags,
+                          const og_pressure_opts* opts, const og_pressure_bnd* bnd, og_solve_stats* st,
+                          og_real* rhsOut) {
 	og_pressure_opts o;
 	if (opts) o = *opts;
 	else og_pressure_opts_default(&o);
 	long n = NCELLS(d);
 	og_real* rhs = (og_real*)calloc((size_t)n, sizeof(og_real));
 	double sum = 0;
-	int cnt = og_make_rhs(d, flags, rhs, vel, &sum);
+	int cnt = og_make_rhs_bnd(d, flags, rhs, vel, bnd, &sum);
 	if (o.enforceCompatibility) {
 		og_real corr = (og_real)(-sum / (og_real)cnt);
 		for (long i = 0; i < n; i++) rhs[i] += corr;
 	}
-	int rc = og_solve_pressure_system(d, rhs, pressure, flags, &o, st);
-	if (rc == 0) og_correct_velocity(d, flags, vel, pressure);
+	int rc = og_solve_pressure_system_bnd(d, rhs, pressure, flags, &o, bnd, st);
+	if (rc == 0) og_correct_velocity_bnd(d, flags, vel, pressure, bnd);
 	if (rhsOut) memcpy(rhsOut, rhs, sizeof(og_real) * (size_t)n);
 	free(rhs);
 	return rc;
+}
+
+/* ===========================================================================
+ * Second-order obstacle and free-surface boundaries
+ * ========================================================================= */
+void og_pressure_bnd_default(og_pressure_bnd* b) {
+	memset(b, 0, sizeof(*b));
+	b->gfClamp = (og_real)1e-04;
+	b->surfTens = (og_real)0;
+}
+
+/* thetaHelper (pressure.cpp:114-120): fraction of the way from the cell to
+ * the interface; denom is Real, the guard compares against a double */
+static og_real gf_theta(og_real inside, og_real outside) {
+	const og_real denom = inside - outside;
+	if ((double)denom > -1e-04) return (og_real)0.5;
+	og_real q = inside / denom;
+	if (q > (og_real)1) q = (og_real)1;  /* std::min(Real(1), q) */
+	if ((og_real)0 > q) q = (og_real)0;  /* std::max(Real(0), .) */
+	return q;
+}
+
+/* ghostFluidHelper (pressure.cpp:122-127): returns gfClamp itself when the
+ * fraction falls below it (the reference's "return alpha = gfClamp"), else
+ * 1 - 1/alpha evaluated in double and stored as Real */
+og_real og_ghost_fluid_factor(og_real phi_in, og_real phi_out, og_real gfClamp) {
+	og_real alpha = gf_theta(phi_in, phi_out);
+	if (alpha < gfClamp) return gfClamp;
+	if (OG_MUT(22)) return (og_real)(1. - (double)alpha);
+	return (og_real)(1. - (1. / (double)alpha));
+}
+static og_real gf_helper(const og_real* phi, long idx, long off, og_real gfClamp) {
+	return og_ghost_fluid_factor(phi[idx], phi[idx + off], gfClamp);
+}
+/* surfTensHelper (pressure.cpp:129-131) */
+static og_real st_helper(const og_real* phi, const og_real* curv, long idx, long off, og_real surfTens,
+                         og_real gfClamp) {
+	return surfTens * (curv[idx + off] - gf_helper(phi, idx, off, gfClamp) * curv[idx]);
+}
+/* ghostFluidWasClamped (pressure.cpp:194-199) */
+static int gf_clamped(const og_real* phi, long idx, long off, og_real gfClamp) {
+	return gf_theta(phi[idx], phi[idx + off]) < gfClamp;
+}
+
+/* MakeRhs (pressure.cpp:32-84), bnd=1, with fractions (+obvel), surface
+ * tension (phi and curv) and perCellCorr */
+int og_make_rhs_bnd(og_dims d, const int* flags, og_real* rhs, const og_real* vel, const og_pressure_bnd* b,
+                    double* sumOut) {
+	if (!b) return og_make_rhs(d, flags, rhs, vel, sumOut);
+	const int bw = 1;
+	long n = NCELLS(d);
+	memset(rhs, 0, sizeof(og_real) * (size_t)n);
+	const long X = 1, Y = d.nx, Z = SZ(d);
+	const og_real* fr = b->fractions;
+	const og_real* ob = b->obvel;
+	int cnt = 0;
+	double sum = 0;
+	for (int k = KMIN(d, bw); k < KMAX(d, bw); k++)
+		for (int j = bw; j < d.ny - bw; j++)
+			for (int i = bw; i < d.nx - bw; i++) {
+				long ix = IX(d, i, j, k);
+				if (!isFluid(flags, ix)) { rhs[ix] = 0; continue; }
+				og_real set = 0;
+				if (!fr) {
+					set = vel[3 * ix] - vel[3 * (ix + X)] + vel[3 * ix + 1] - vel[3 * (ix + Y) + 1];
+					if (d.is3d) set += vel[3 * ix + 2] - vel[3 * (ix + Z) + 2];
+				} else {
+					set = fr[3 * ix] * vel[3 * ix] - fr[3 * (ix + X)] * vel[3 * (ix + X)] +
+					      fr[3 * ix + 1] * vel[3 * ix + 1] -
+					      fr[3 * (OG_MUT(21) ? ix : ix + Y) + 1] * vel[3 * (ix + Y) + 1];
+					if (d.is3d) set += fr[3 * ix + 2] * vel[3 * ix + 2] - fr[3 * (ix + Z) + 2] * vel[3 * (ix + Z) + 2];
+					if (ob) {
+						set += (1 - fr[3 * ix]) * ob[3 * ix] - (1 - fr[3 * (ix + X)]) * ob[3 * (ix + X)] +
+						       (1 - fr[3 * ix + 1]) * ob[3 * ix + 1] - (1 - fr[3 * (ix + Y) + 1]) * ob[3 * (ix + Y) + 1];
+						if (d.is3d)
+							set += (1 - fr[3 * ix + 2]) * ob[3 * ix + 2] - (1 - fr[3 * (ix + Z) + 2]) * ob[3 * (ix + Z) + 2];
+					}
+				}
+				if (b->phi && b->curv) {
+					const og_real *ph = b->phi, *cv = b->curv, st = b->surfTens, gc = b->gfClamp;
+					if (isEmpty(flags, ix - X)) set += st_helper(ph, cv, ix, -X, st, gc);
+					if (isEmpty(flags, ix + X)) set += st_helper(ph, cv, ix, +X, st, gc);
+					if (isEmpty(flags, ix - Y)) set += st_helper(ph, cv, ix, -Y, st, gc);
+					if (isEmpty(flags, ix + Y)) set += st_helper(ph, cv, ix, +Y, st, gc);
+					if (d.is3d) {
+						if (isEmpty(flags, ix - Z)) set += st_helper(ph, cv, ix, -Z, st, gc);
+						if (isEmpty(flags, ix + Z)) set += st_helper(ph, cv, ix, +Z, st, gc);
+					}
+				}
+				if (b->perCellCorr) set += b->perCellCorr[ix];
+				sum += set;
+				cnt++;
+				rhs[ix] = set;
+			}
+	if (sumOut) *sumOut = sum;
+	return cnt;
+}
+
+/* MakeLaplaceMatrix (conjugategrad.h:154-187), fractions branch, then
+ * ApplyGhostFluidDiagonal (pressure.cpp:133-149) when phi is given */
+void og_make_laplace_bnd(og_dims d, const int* flags, og_real* A0, og_real* Ai, og_real* Aj, og_real* Ak,
+                         const og_pressure_bnd* b) {
+	const og_real* fr = b ? b->fractions : NULL;
+	if (!fr) {
+		og_make_laplace(d, flags, A0, Ai, Aj, Ak);
+	} else {
+		const int bw = 1;
+		long n = NCELLS(d);
+		memset(A0, 0, sizeof(og_real) * (size_t)n);
+		memset(Ai, 0, sizeof(og_real) * (size_t)n);
+		memset(Aj, 0, sizeof(og_real) * (size_t)n);
+		memset(Ak, 0, sizeof(og_real) * (size_t)n);
+		const long X = 1, Y = d.nx, Z = SZ(d);
+		for (int k = KMIN(d, bw); k < KMAX(d, bw); k++)
+			for (int j = bw; j < d.ny - bw; j++)
+				for (int i = bw; i < d.nx - bw; i++) {
+					long ix = IX(d, i, j, k);
+					if (!isFluid(flags, ix)) continue;
+					A0[ix] += fr[3 * ix];
+					A0[ix] += fr[3 * (ix + X)];
+					A0[ix] += fr[3 * ix + 1];
+					A0[ix] += fr[3 * (ix + Y) + 1];
+					if (d.is3d) A0[ix] += fr[3 * ix + 2];
+					if (d.is3d) A0[ix] += fr[3 * (ix + Z) + 2];
+					if (isFluid(flags, ix + X)) Ai[ix] = -fr[3 * (ix + X)];
+					if (isFluid(flags, ix + Y)) Aj[ix] = -fr[3 * (ix + Y) + 1];
+					if (d.is3d && isFluid(flags, ix + Z)) Ak[ix] = -fr[3 * (ix + Z) + 2];
+				}
+	}
+	if (b && b->phi) {
+		const int bw = 1;
+		const long X = 1, Y = d.nx, Z = SZ(d);
+		const og_real* ph = b->phi;
+		const og_real gc = b->gfClamp;
+		for (int k = KMIN(d, bw); k < KMAX(d, bw); k++)
+			for (int j = bw; j < d.ny - bw; j++)
+				for (int i = bw; i < d.nx - bw; i++) {
+					long ix = IX(d, i, j, k);
+					if (!isFluid(flags, ix)) continue;
+					if (isEmpty(flags, ix - X)) A0[ix] -= gf_helper(ph, ix, -X, gc);
+					if (isEmpty(flags, ix + X)) A0[ix] -= gf_helper(ph, ix, +X, gc);
+					if (isEmpty(flags, ix - Y)) A0[ix] -= gf_helper(ph, ix, -Y, gc);
+					if (isEmpty(flags, ix + Y)) A0[ix] -= gf_helper(ph, ix, +Y, gc);
+					if (d.is3d) {
+						if (isEmpty(flags, ix - Z)) A0[ix] -= gf_helper(ph, ix, -Z, gc);
+						if (isEmpty(flags, ix + Z)) A0[ix] -= gf_helper(ph, ix, +Z, gc);
+					}
+				}
+	}
+}
+
+/* correctVelocity (pressure.cpp:455-476): knCorrectVelocity, then with phi
+ * knCorrectVelocityGhostFluid (:151-190) and knReplaceClampedGhostFluidVels
+ * (:201-219), all bnd=1 */
+void og_correct_velocity_bnd(og_dims d, const int* flags, og_real* vel, const og_real* p, const og_pressure_bnd* b) {
+	og_correct_velocity(d, flags, vel, p);
+	if (!b || !b->phi) return;
+	const int bw = 1;
+	const long X = 1, Y = d.nx, Z = SZ(d);
+	const og_real* ph = b->phi;
+	const og_real gc = b->gfClamp;
+	for (int k = KMIN(d, bw); k < KMAX(d, bw); k++)
+		for (int j = bw; j < d.ny - bw; j++)
+			for (int i = bw; i < d.nx - bw; i++) {
+				long ix = IX(d, i, j, k);
+				og_real* v = vel + 3 * ix;
+				if (isFluid(flags, ix)) {
+					if (isEmpty(flags, ix - X)) v[0] += p[ix] * gf_helper(ph, ix, -X, gc);
+					if (isEmpty(flags, ix - Y)) v[1] += p[ix] * gf_helper(ph, ix, -Y, gc);
+					if (d.is3d && isEmpty(flags, ix - Z)) v[2] += p[ix] * gf_helper(ph, ix, -Z, gc);
+				} else if (isEmpty(flags, ix) && !isOutflow(flags, ix)) {
+					if (isFluid(flags, ix - X)) v[0] -= p[OG_MUT(24) ? ix : ix - X] * gf_helper(ph, ix - X, +X, gc);
+					else v[0] = 0.f;
+					if (isFluid(flags, ix - Y)) v[1] -= p[ix - Y] * gf_helper(ph, ix - Y, +Y, gc);
+					else v[1] = 0.f;
+					if (d.is3d) {
+						if (isFluid(flags, ix - Z)) v[2] -= p[ix - Z] * gf_helper(ph, ix - Z, +Z, gc);
+						else v[2] = 0.f;
+					}
+				}
+				if (b->curv) {
+					const og_real* cv = b->curv;
+					const og_real st = b->surfTens;
+					if (isFluid(flags, ix)) {
+						if (isEmpty(flags, ix - X)) v[0] += st_helper(ph, cv, ix, -X, st, gc);
+						if (isEmpty(flags, ix - Y)) v[1] += st_helper(ph, cv, ix, -Y, st, gc);
+						if (d.is3d && isEmpty(flags, ix - Z)) v[2] += st_helper(ph, cv, ix, -Z, st, gc);
+					} else if (isEmpty(flags, ix) && !isOutflow(flags, ix)) {
+						v[0] -= isFluid(flags, ix - X) ? st_helper(ph, cv, ix - X, +X, st, gc) : (og_real)0;
+						v[1] -= isFluid(flags, ix - Y) ? st_helper(ph, cv, ix - Y, +Y, st, gc) : (og_real)0;
+						if (d.is3d) v[2] -= isFluid(flags, ix - Z) ? st_helper(ph, cv, ix - Z, +Z, st, gc) : (og_real)0;
+					}
+				}
+			}
+	/* knReplaceClampedGhostFluidVels: empty cells take the fluid neighbour's
+	 * face value where that face's fraction was clamped (reads fluid cells'
+	 * faces only, which this pass leaves alone) */
+	for (int k = KMIN(d, bw); k < KMAX(d, bw); k++)
+		for (int j = bw; j < d.ny - bw; j++)
+			for (int i = bw; i < d.nx - bw; i++) {
+				long ix = IX(d, i, j, k);
+				if (!isEmpty(flags, ix)) continue;
+				og_real* v = vel + 3 * ix;
+				if (isFluid(flags, ix - X) && gf_clamped(ph, ix - X, +X, gc)) v[0] = vel[3 * (ix - X)];
+				if (isFluid(flags, ix - Y) && gf_clamped(ph, ix - Y, +Y, gc)) v[1] = vel[3 * (ix - Y) + 1];
+				if (d.is3d && isFluid(flags, ix - Z) && gf_clamped(ph, ix - Z, +Z, gc)) v[2] = vel[3 * (ix - Z) + 2];
+				if (isFluid(flags, ix + X) && gf_clamped(ph, ix + X, -X, gc)) v[0] = vel[3 * (ix + X)];
+				if (isFluid(flags, ix + Y) && gf_clamped(ph, ix + Y, -Y, gc)) v[1] = vel[3 * (ix + Y) + 1];
+				if (d.is3d && isFluid(flags, ix + Z) && gf_clamped(ph, ix + Z, -Z, gc)) v[2] = vel[3 * (ix + Z) + 2];
+			}
+}
+
+/* normalize(Vec3&) (util/vectorbase.h:416-429): unchanged when |v|^2 is
+ * within eps^2 of 1, scaled by (Real)(1/norm) when above eps^2, else 0 */
+static void vec_normalize(og_real* v) {
+	const og_real l = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+	const og_real eps2 = OG_VEC_EPS * OG_VEC_EPS;
+	if (fabs((double)l - 1.) < (double)eps2) return;
+	if (l > eps2) {
+		const og_real norm = (og_real)sqrt((double)l);
+		const og_real s = (og_real)(1. / (double)norm);
+		v[0] *= s;
+		v[1] *= s;
+		v[2] *= s;
+	} else {
+		v[0] = v[1] = v[2] = 0;
+	}
+}
+
+/* KnSetWallBcsFrac (extforces.cpp:220-303) into a copy of vel, then
+ * vel.swap(copy) (:307-315): at interior fluid or obstacle cells, a face
+ * that touches an obstacle loses the component of the face velocity
+ * (getAtMAC) along the normalised obstacle-SDF gradient */
+void og_set_wall_bcs_frac(og_dims d, const int* flags, og_real* vel, const og_real* phiObs) {
+	long n = NCELLS(d);
+	og_real* tgt = (og_real*)malloc(sizeof(og_real) * 3 * (size_t)n);
+	memcpy(tgt, vel, sizeof(og_real) * 3 * (size_t)n);
+	const long X = 1, Y = d.nx, Z = SZ(d);
+	const og_real* P = phiObs;
+	for (int k = 0; k < d.nz; k++)
+		for (int j = 0; j < d.ny; j++)
+			for (int i = 0; i < d.nx; i++) {
+				long ix = IX(d, i, j, k);
+				const int curFluid = isFluid(flags, ix) != 0, curObs = isObs(flags, ix) != 0;
+				if (!curFluid && !curObs) continue;
+				const int inb = i >= 1 && j >= 1 && i < d.nx - 1 && j < d.ny - 1 && (!d.is3d || (k >= 1 && k < d.nz - 1));
+				if (!inb) continue;
+				if (curObs | (isObs(flags, ix - X) != 0)) {
+					og_real dphi[3] = {0, 0, 0};
+					const long my = OG_MUT(23) ? -Y : Y;
+					const og_real tmp1 = (og_real)((P[ix] + P[ix - X]) * .5);
+					og_real tmp2 = (og_real)((P[ix + my] + P[ix - X + my]) * .5);
+					og_real phi1 = (og_real)((tmp1 + tmp2) * .5);
+					tmp2 = (og_real)((P[ix - my] + P[ix - X - my]) * .5);
+					og_real phi2 = (og_real)((tmp1 + tmp2) * .5);
+					dphi[0] = P[ix] - P[ix - X];
+					dphi[1] = phi1 - phi2;
+					if (d.is3d) {
+						tmp2 = (og_real)((P[ix + Z] + P[ix - X + Z]) * .5);
+						phi1 = (og_real)((tmp1 + tmp2) * .5);
+						tmp2 = (og_real)((P[ix - Z] + P[ix - X - Z]) * .5);
+						phi2 = (og_real)((tmp1 + tmp2) * .5);
+						dphi[2] = phi1 - phi2;
+					}
+					vec_normalize(dphi);
+					og_real vm[3];
+					getAtMACX(d, vel, i, j, k, vm);
+					const og_real dt = dphi[0] * vm[0] + dphi[1] * vm[1] + dphi[2] * vm[2];
+					tgt[3 * ix] = vm[0] - dt * dphi[0];
+				}
+				if (curObs | (isObs(flags, ix - Y) != 0)) {
+					og_real dphi[3] = {0, 0, 0};
+					const og_real tmp1 = (og_real)((P[ix] + P[ix - Y]) * .5);
+					og_real tmp2 = (og_real)((P[ix + X] + P[ix + X - Y]) * .5);
+					og_real phi1 = (og_real)((tmp1 + tmp2) * .5);
+					tmp2 = (og_real)((P[ix - X] + P[ix - X - Y]) * .5);
+					og_real phi2 = (og_real)((tmp1 + tmp2) * .5);
+					dphi[0] = phi1 - phi2;
+					dphi[1] = P[ix] - P[ix - Y];
+					if (d.is3d) {
+						tmp2 = (og_real)((P[ix + Z] + P[ix - Y + Z]) * .5);
+						phi1 = (og_real)((tmp1 + tmp2) * .5);
+						tmp2 = (og_real)((P[ix - Z] + P[ix - Y - Z]) * .5);
+						phi2 = (og_real)((tmp1 + tmp2) * .5);
+						dphi[2] = phi1 - phi2;
+					}
+					vec_normalize(dphi);
+					og_real vm[3];
+					getAtMACY(d, vel, i, j, k, vm);
+					const og_real dt = dphi[0] * vm[0] + dphi[1] * vm[1] + dphi[2] * vm[2];
+					tgt[3 * ix + 1] = vm[1] - dt * dphi[1];
+				}
+				if (d.is3d && (curObs | (isObs(flags, ix - Z) != 0))) {
+					og_real dphi[3] = {0, 0, 0};
+					const og_real tmp1 = (og_real)((P[ix] + P[ix - Z]) * .5);
+					og_real tmp2 = (og_real)((P[ix + X] + P[ix + X - Z]) * .5);
+					og_real phi1 = (og_real)((tmp1 + tmp2) * .5);
+					tmp2 = (og_real)((P[ix - X] + P[ix - X - Z]) * .5);
+					og_real phi2 = (og_real)((tmp1 + tmp2) * .5);
+					dphi[0] = phi1 - phi2;
+					tmp2 = (og_real)((P[ix + Y] + P[ix + Y - Z]) * .5);
+					phi1 = (og_real)((tmp1 + tmp2) * .5);
+					tmp2 = (og_real)((P[ix - Y] + P[ix - Y - Z]) * .5);
+					phi2 = (og_real)((tmp1 + tmp2) * .5);
+					dphi[1] = phi1 - phi2;
+					dphi[2] = P[ix] - P[ix - Z];
+					vec_normalize(dphi);
+					og_real vm[3];
+					getAtMACZ(d, vel, i, j, k, vm);
+					const og_real dt = dphi[0] * vm[0] + dphi[1] * vm[1] + dphi[2] * vm[2];
+					tgt[3 * ix + 2] = vm[2] - dt * dphi[2];
+				}
+			}
+	memcpy(vel, tgt, sizeof(og_real) * 3 * (size_t)n);
+	free(tgt);
 }
 
 /* ===========================================================================

@@ -155,6 +155,38 @@ int og_solve_pressure_system(og_dims d, og_real* rhs, og_real* pressure, const i
                              const og_pressure_opts* opts, og_solve_stats* st);
 void og_pressure_opts_default(og_pressure_opts* o);
 
+/* ---- second-order obstacle and free-surface boundaries (pressure.cpp:32-84,
+ *      111-214,455-476, conjugategrad.h:154-187, extforces.cpp:220-315) ---- */
+/* the optional inputs of solvePressure (pressure.cpp:480-495); a NULL
+ * pointer is an absent argument */
+typedef struct {
+	const og_real* phi;          /* Real: fluid level set (ghost fluid) */
+	const og_real* perCellCorr;  /* Real: per-cell divergence correction */
+	const og_real* fractions;    /* MAC (AoS): open face fractions */
+	const og_real* obvel;        /* MAC (AoS): obstacle velocity, used with fractions */
+	const og_real* curv;         /* Real: curvature, used with phi (surface tension) */
+	og_real gfClamp;             /* 1e-4 */
+	og_real surfTens;            /* 0 */
+} og_pressure_bnd;
+void og_pressure_bnd_default(og_pressure_bnd* b);
+/* ghostFluidHelper (pressure.cpp:114-126) for the cell value phi_in and its
+ * neighbour's phi_out */
+og_real og_ghost_fluid_factor(og_real phi_in, og_real phi_out, og_real gfClamp);
+int og_make_rhs_bnd(og_dims d, const int* flags, og_real* rhs, const og_real* vel, const og_pressure_bnd* b,
+                    double* sum);
+/* MakeLaplaceMatrix with fractions, then ApplyGhostFluidDiagonal with phi */
+void og_make_laplace_bnd(og_dims d, const int* flags, og_real* A0, og_real* Ai, og_real* Aj, og_real* Ak,
+                         const og_pressure_bnd* b);
+void og_correct_velocity_bnd(og_dims d, const int* flags, og_real* vel, const og_real* pressure,
+                             const og_pressure_bnd* b);
+int og_solve_pressure_system_bnd(og_dims d, og_real* rhs, og_real* pressure, const int* flags,
+                                 const og_pressure_opts* opts, const og_pressure_bnd* b, og_solve_stats* st);
+int og_solve_pressure_bnd(og_dims d, og_real* vel, og_real* pressure, const int* flags,
+                          const og_pressure_opts* opts, const og_pressure_bnd* b, og_solve_stats* st,
+                          og_real* rhsOut);
+/* setWallBcs with fractions and phiObs: KnSetWallBcsFrac into a copy, then swap */
+void og_set_wall_bcs_frac(og_dims d, const int* flags, og_real* vel, const og_real* phiObs);
+
 /* ---- other GridCg clients (conjugategrad.cpp:347-423, plugin/waves.cpp:69-141) ---- */
 void og_diffusion_matrix(og_dims d, const int* flags, og_real alpha, og_real* A0, og_real* Ai, og_real* Aj,
                          og_real* Ak);
