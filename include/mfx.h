@@ -137,6 +137,13 @@ int mfx_advect_semi_lagrange(const mfx_grid* flags, const mfx_grid* vel, mfx_gri
 /* ------------------------------------------------- forces / boundaries */
 /* setWallBcs without fractions/phiObs (source/plugin/extforces.cpp:307-315); obvel may be NULL */
 int mfx_set_wall_bcs(const mfx_grid* flags, mfx_grid* vel, const mfx_grid* obvel);
+/* setWallBcs with all its arguments (source/plugin/extforces.cpp:307-315):
+ * with both fractions and phiObs, KnSetWallBcsFrac (:220-303: the face
+ * velocity loses its component along the obstacle-SDF normal at faces touching
+ * an obstacle), otherwise mfx_set_wall_bcs.  obvel, fractions, phiObs may be
+ * NULL; boundaryWidth is accepted and unused, as in the reference. */
+int mfx_set_wall_bcs_frac(const mfx_grid* flags, mfx_grid* vel, const mfx_grid* obvel, const mfx_grid* fractions,
+                          const mfx_grid* phiObs, int boundaryWidth);
 /* addBuoyancy (source/plugin/extforces.cpp:83-87) */
 int mfx_add_buoyancy(const mfx_grid* flags, const mfx_grid* density, mfx_grid* vel,
                      const float gravity[3], float coefficient, int scale);
@@ -176,6 +183,41 @@ void mfx_pressure_opts_default(mfx_pressure_opts* o);
  * plain CG in 2D, where the reference also disables it (conjugategrad.cpp:315-321). */
 int mfx_solve_pressure(mfx_grid* vel, mfx_grid* pressure, const mfx_grid* flags,
                        const mfx_pressure_opts* opts, mfx_solve_stats* stats, mfx_grid* retRhs);
+/* The optional inputs of solvePressure, computePressureRhs,
+ * solvePressureSystem and correctVelocity for second-order obstacle and
+ * free-surface boundaries (source/plugin/pressure.cpp:277-495): every grid
+ * may be NULL.  fractions (MAC: open face fractions) give MakeLaplaceMatrix's
+ * fractions branch (source/conjugategrad.h:172-185) and MakeRhs's weighted
+ * divergence, with obvel (MAC) its obstacle term (pressure.cpp:47-58); phi
+ * (Real or Levelset: the liquid's level set) adds the ghost-fluid diagonal
+ * (:133-149) and velocity correction (:151-219), with curv (Real) and
+ * surfTens the surface tension terms; perCellCorr (Real) is added to the
+ * divergence.  With fractions or phi the system matrix is stored as four
+ * float arrays; the multigrid preconditioner (PcMG*, and PcMIC in 3D) is
+ * then built from the flags' plain Laplacian, an SPD approximation of it. */
+typedef struct {
+	const mfx_grid* phi;
+	const mfx_grid* perCellCorr;
+	const mfx_grid* fractions;
+	const mfx_grid* obvel;
+	const mfx_grid* curv;
+	float gfClamp;             /* 1e-4 */
+	float surfTens;            /* 0 */
+} mfx_pressure_bnd;
+void mfx_pressure_bnd_default(mfx_pressure_bnd* b);
+/* solvePressure with the optional boundary inputs (source/plugin/pressure.cpp:480-521) */
+int mfx_solve_pressure_bnd(mfx_grid* vel, mfx_grid* pressure, const mfx_grid* flags, const mfx_pressure_opts* opts,
+                           const mfx_pressure_bnd* bnd, mfx_solve_stats* stats, mfx_grid* retRhs);
+/* computePressureRhs (pressure.cpp:277-299) with the optional boundary inputs */
+int mfx_compute_pressure_rhs_bnd(mfx_grid* rhs, const mfx_grid* vel, const mfx_grid* flags,
+                                 const mfx_pressure_opts* opts, const mfx_pressure_bnd* bnd);
+/* solvePressureSystem (pressure.cpp:312-452) with the optional boundary inputs */
+int mfx_solve_pressure_system_bnd(mfx_grid* rhs, mfx_grid* pressure, const mfx_grid* flags,
+                                  const mfx_pressure_opts* opts, const mfx_pressure_bnd* bnd, mfx_solve_stats* stats);
+/* correctVelocity (pressure.cpp:455-476) with the optional boundary inputs */
+int mfx_correct_velocity_bnd(mfx_grid* vel, const mfx_grid* pressure, const mfx_grid* flags,
+                             const mfx_pressure_bnd* bnd);
+
 /* Batched solvePressure: count independent problems (vel[i], pressure[i],
  * flags[i]), each on its own scene (any mix of sizes, 2D and 3D), solved
  * together.  Every CG/multigrid phase is one launch whose work list holds the

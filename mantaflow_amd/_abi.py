@@ -28,6 +28,11 @@ class SolveStats(C.Structure):
                 ("converged", C.c_int), ("seconds", C.c_double)]
 
 
+class PressureBnd(C.Structure):
+    _fields_ = [("phi", C.c_void_p), ("perCellCorr", C.c_void_p), ("fractions", C.c_void_p),
+                ("obvel", C.c_void_p), ("curv", C.c_void_p), ("gfClamp", C.c_float), ("surfTens", C.c_float)]
+
+
 # every exported symbol of include/mfx.h with its ctypes signature
 _P = C.c_void_p
 _I = C.c_int
@@ -75,6 +80,14 @@ SIGNATURES = {
     "mfx_apply_shape": (_I, [_P, _I, _FP, _FP, _F, _FP, _P]),
     "mfx_advect_semi_lagrange": (_I, [_P, _P, _P, _I, _F, _I, _I, _I]),
     "mfx_set_wall_bcs": (_I, [_P, _P, _P]),
+    "mfx_set_wall_bcs_frac": (_I, [_P, _P, _P, _P, _P, _I]),
+    "mfx_pressure_bnd_default": (None, [C.POINTER(PressureBnd)]),
+    "mfx_solve_pressure_bnd": (_I, [_P, _P, _P, C.POINTER(PressureOpts), C.POINTER(PressureBnd),
+                                    C.POINTER(SolveStats), _P]),
+    "mfx_compute_pressure_rhs_bnd": (_I, [_P, _P, _P, C.POINTER(PressureOpts), C.POINTER(PressureBnd)]),
+    "mfx_solve_pressure_system_bnd": (_I, [_P, _P, _P, C.POINTER(PressureOpts), C.POINTER(PressureBnd),
+                                           C.POINTER(SolveStats)]),
+    "mfx_correct_velocity_bnd": (_I, [_P, _P, _P, C.POINTER(PressureBnd)]),
     "mfx_add_buoyancy": (_I, [_P, _P, _P, _FP, _F, _I]),
     "mfx_pressure_opts_default": (None, [C.POINTER(PressureOpts)]),
     "mfx_solve_pressure": (_I, [_P, _P, _P, C.POINTER(PressureOpts), C.POINTER(SolveStats), _P]),

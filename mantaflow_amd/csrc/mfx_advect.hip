@@ -1100,6 +1100,133 @@ __global__ void __launch_bounds__(kBlock) k_add_buoyancy(Dims d, Tiles t, const 
 // ---------------------------------------------------------------------------
 static inline Vel vel_of(Grid* g) { return Vel{g->comp(0), g->comp(1), g->comp(2)}; }
 
+// ---------------------------------------------------------------------------
+// setWallBcs with fractions and phiObs (extforces.cpp:220-315):
+// KnSetWallBcsFrac into a second buffer, then swapped in.  At interior fluid
+// or obstacle cells, a face touching an obstacle (the cell or its lower
+// neighbour) loses the component of its getAtMAC velocity along the
+// normalised obstacle-SDF gradient; every other value is copied.  Float and
+// double roundings follow the reference (normalize: util/vectorbase.h:416-429).
+// ---------------------------------------------------------------------------
+__device__ inline void frac_normalize(float* v) {
+	const float l = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+	const float eps2 = 1e-6f * 1e-6f;
+	if (fabs((double)l - 1.) < (double)eps2) return;
+	if (l > eps2) {
+		const float norm = (float)sqrt((double)l);
+		const float sc = (float)(1. / (double)norm);
+		v[0] *= sc;
+		v[1] *= sc;
+		v[2] *= sc;
+	} else {
+		v[0] = v[1] = v[2] = 0.f;
+	}
+}
+
+__global__ void __launch_bounds__(256) k_set_wall_bcs_frac(Dims d, const uint8_t* __restrict__ f8,
+                                                           const float* __restrict__ u, const float* __restrict__ v,
+                                                           const float* __restrict__ w, const float* __restrict__ P,
+                                                           float* __restrict__ ou, float* __restrict__ ov,
+                                                           float* __restrict__ ow) {
+	const long long X = 1, Y = d.nx, Z = d.sz;
+	for (long long c = blockIdx.x * 256ll + threadIdx.x; c < d.n; c += (long long)gridDim.x * 256) {
+		const long long row = c / d.nx;
+		const int i = (int)(c - row * d.nx);
+		const int j = d.is3d ? (int)(row % d.ny) : (int)row;
+		const int k = d.is3d ? (int)(row / d.ny) : 0;
+		float nu = u[c], nv = v[c], nw = w[c];
+		const uint8_t f = f8[c];
+		const bool curObs = (f & kObstacle) != 0;
+		const bool inb = i >= 1 && j >= 1 && i < d.nx - 1 && j < d.ny - 1 && (!d.is3d || (k >= 1 && k < d.nz - 1));
+		if (((f & kFluid) || curObs) && inb) {
+			if (curObs || (f8[c - X] & kObstacle)) {
+				float dphi[3] = {0.f, 0.f, 0.f};
+				const float tmp1 = (float)((P[c] + P[c - X]) * .5);
+				float tmp2 = (float)((P[c + Y] + P[c - X + Y]) * .5);
+				float phi1 = (float)((tmp1 + tmp2) * .5);
+				tmp2 = (float)((P[c - Y] + P[c - X - Y]) * .5);
+				float phi2 = (float)((tmp1 + tmp2) * .5);
+				dphi[0] = P[c] - P[c - X];
+				dphi[1] = phi1 - phi2;
+				if (d.is3d) {
+					tmp2 = (float)((P[c + Z] + P[c - X + Z]) * .5);
+					phi1 = (float)((tmp1 + tmp2) * .5);
+					tmp2 = (float)((P[c - Z] + P[c - X - Z]) * .5);
+					phi2 = (float)((tmp1 + tmp2) * .5);
+					dphi[2] = phi1 - phi2;
+				}
+				frac_normalize(dphi);
+				// getAtMACX (grid.h:437-448)
+				const float m0 = u[c];
+				const float m1 = 0.25f * (v[c] + v[c - 1] + v[c + Y] + v[c + Y - 1]);
+				const float m2 = d.is3d ? 0.25f * (w[c] + w[c - 1] + w[c + Z] + w[c + Z - 1]) : 0.f;
+				const float dt = dphi[0] * m0 + dphi[1] * m1 + dphi[2] * m2;
+				nu = m0 - dt * dphi[0];
+			}
+			if (curObs || (f8[c - Y] & kObstacle)) {
+				float dphi[3] = {0.f, 0.f, 0.f};
+				const float tmp1 = (float)((P[c] + P[c - Y]) * .5);
+				float tmp2 = (float)((P[c + X] + P[c + X - Y]) * .5);
+				float phi1 = (float)((tmp1 + tmp2) * .5);
+				tmp2 = (float)((P[c - X] + P[c - X - Y]) * .5);
+				float phi2 = (float)((tmp1 + tmp2) * .5);
+				dphi[0] = phi1 - phi2;
+				dphi[1] = P[c] - P[c - Y];
+				if (d.is3d) {
+					tmp2 = (float)((P[c + Z] + P[c - Y + Z]) * .5);
+					phi1 = (float)((tmp1 + tmp2) * .5);
+					tmp2 = (float)((P[c - Z] + P[c - Y - Z]) * .5);
+					phi2 = (float)((tmp1 + tmp2) * .5);
+					dphi[2] = phi1 - phi2;
+				}
+				frac_normalize(dphi);
+				// getAtMACY (grid.h:450-459)
+				const float m0 = 0.25f * (u[c] + u[c - Y] + u[c + 1] + u[c + 1 - Y]);
+				const float m1 = v[c];
+				const float m2 = d.is3d ? 0.25f * (w[c] + w[c - Y] + w[c + Z] + w[c + Z - Y]) : 0.f;
+				const float dt = dphi[0] * m0 + dphi[1] * m1 + dphi[2] * m2;
+				nv = m1 - dt * dphi[1];
+			}
+			if (d.is3d && (curObs || (f8[c - Z] & kObstacle))) {
+				float dphi[3] = {0.f, 0.f, 0.f};
+				const float tmp1 = (float)((P[c] + P[c - Z]) * .5);
+				float tmp2 = (float)((P[c + X] + P[c + X - Z]) * .5);
+				float phi1 = (float)((tmp1 + tmp2) * .5);
+				tmp2 = (float)((P[c - X] + P[c - X - Z]) * .5);
+				float phi2 = (float)((tmp1 + tmp2) * .5);
+				dphi[0] = phi1 - phi2;
+				tmp2 = (float)((P[c + Y] + P[c + Y - Z]) * .5);
+				phi1 = (float)((tmp1 + tmp2) * .5);
+				tmp2 = (float)((P[c - Y] + P[c - Y - Z]) * .5);
+				phi2 = (float)((tmp1 + tmp2) * .5);
+				dphi[1] = phi1 - phi2;
+				dphi[2] = P[c] - P[c - Z];
+				frac_normalize(dphi);
+				// getAtMACZ (grid.h:461-470)
+				const float m0 = 0.25f * (u[c] + u[c - Z] + u[c + 1] + u[c + 1 - Z]);
+				const float m1 = 0.25f * (v[c] + v[c - Z] + v[c + Y] + v[c + Y - Z]);
+				const float m2 = w[c];
+				const float dt = dphi[0] * m0 + dphi[1] * m1 + dphi[2] * m2;
+				nw = m2 - dt * dphi[2];
+			}
+		}
+		ou[c] = nu;
+		ov[c] = nv;
+		ow[c] = nw;
+	}
+}
+
+void op_set_wall_bcs_frac(Scene* s, Grid* flags, Grid* vel, Grid* phiObs) {
+	float* out = s->getVec();
+	hipLaunchKernelGGL(k_set_wall_bcs_frac, dim3(flat_blocks(s->d.n)), dim3(256), 0, s->stream, s->d,
+	                   (const uint8_t*)flags->f8, (const float*)vel->comp(0), (const float*)vel->comp(1),
+	                   (const float*)vel->comp(2), (const float*)phiObs->real(), out, out + s->cstride,
+	                   out + 2 * s->cstride);
+	// vel.swap(tmpvel)
+	s->putVec(vel->real());
+	vel->data = out;
+}
+
 void op_set_wall_bcs(Scene* s, Grid* flags, Grid* vel, Grid* obvel) {
 	Tiles t = tiles_for(s->d, kLightKZ);
 	hipLaunchKernelGGL(k_set_wall_bcs, dim3((unsigned)t.count()), dim3(kBlock), 0, s->stream, s->d, t, flags->f8,
@@ -1398,6 +1525,31 @@ int mfx_set_wall_bcs(const mfx_grid* flags, mfx_grid* vel, const mfx_grid* obvel
 		same_scene(f, G(obvel));
 	}
 	op_set_wall_bcs(f->s, f, v, obvel ? G(obvel) : nullptr);
+	API_END
+}
+
+int mfx_set_wall_bcs_frac(const mfx_grid* flags, mfx_grid* vel, const mfx_grid* obvel, const mfx_grid* fractions,
+                          const mfx_grid* phiObs, int boundaryWidth) {
+	API_BEGIN
+	MFX_CHECK(flags && vel, "setWallBcs: flags and vel are required");
+	Grid *f = G(flags), *v = G(vel);
+	MFX_CHECK(f->isFlags() && v->isMAC(), "setWallBcs: needs a FlagGrid and a MACGrid");
+	same_scene(f, v);
+	if (obvel) {
+		MFX_CHECK(G(obvel)->isVec(), "setWallBcs: obvel must be a MACGrid");
+		same_scene(f, G(obvel));
+	}
+	(void)boundaryWidth;  // unused by KnSetWallBcsFrac (extforces.cpp:220-303)
+	if (!fractions || !phiObs) {
+		// setWallBcs without both (extforces.cpp:308-309)
+		op_set_wall_bcs(f->s, f, v, obvel ? G(obvel) : nullptr);
+	} else {
+		MFX_CHECK(G(fractions)->isVec(), "setWallBcs: fractions must be a MACGrid");
+		MFX_CHECK(G(phiObs)->isReal(), "setWallBcs: phiObs must be a Real or Levelset grid");
+		same_scene(f, G(fractions));
+		same_scene(f, G(phiObs));
+		op_set_wall_bcs_frac(f->s, f, v, G(phiObs));
+	}
 	API_END
 }
 

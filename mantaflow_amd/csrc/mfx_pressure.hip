@@ -1095,6 +1095,393 @@ __global__ void k_code_hash(const uint8_t* __restrict__ code, long long n, unsig
 }
 
 // ---------------------------------------------------------------------------
+// Second-order obstacle and free-surface boundaries: MakeRhs with fractions,
+// obstacle velocity, surface tension and per-cell correction
+// (pressure.cpp:32-84), the fractions Laplacian with the ghost-fluid diagonal
+// as four float arrays (conjugategrad.h:154-187, pressure.cpp:133-149), CG
+// on that matrix, and the ghost-fluid velocity correction
+// (pressure.cpp:151-219).  One thread per cell with a grid-stride loop; the
+// arithmetic keeps the reference's float / double rounding, so RHS, matrix
+// and corrections are bit-exact.
+// ---------------------------------------------------------------------------
+struct BndDev {
+	const float* phi;   // fluid level set (ghost fluid), or null
+	const float* pcc;   // per-cell divergence correction, or null
+	const float* fx;    // fractions, SoA components, or null
+	const float* fy;
+	const float* fz;
+	const float* ox;    // obstacle velocity (used with fractions), or null
+	const float* oy;
+	const float* oz;
+	const float* curv;  // curvature (used with phi), or null
+	float gfClamp, surfTens;
+};
+
+// thetaHelper (pressure.cpp:114-120)
+__device__ inline float gf_theta(float inside, float outside) {
+	const float denom = inside - outside;
+	if ((double)denom > -1e-04) return 0.5f;
+	float q = inside / denom;
+	if (q > 1.f) q = 1.f;
+	if (0.f > q) q = 0.f;
+	return q;
+}
+// ghostFluidHelper (pressure.cpp:122-127): gfClamp itself below the clamp,
+// else 1 - 1/alpha in double, stored as float
+__device__ inline float gf_factor(float pin, float pout, float gfClamp) {
+	const float alpha = gf_theta(pin, pout);
+	if (alpha < gfClamp) return gfClamp;
+	return (float)(1. - (1. / (double)alpha));
+}
+__device__ inline float gf_at(const BndDev& B, long long c, long long off) {
+	return gf_factor(B.phi[c], B.phi[c + off], B.gfClamp);
+}
+// surfTensHelper (pressure.cpp:129-131)
+__device__ inline float st_at(const BndDev& B, long long c, long long off) {
+	return B.surfTens * (B.curv[c + off] - gf_at(B, c, off) * B.curv[c]);
+}
+
+__device__ inline void cell_of(const Dims& d, long long c, int& i, int& j, int& k) {
+	const long long row = c / d.nx;
+	i = (int)(c - row * d.nx);
+	if (d.is3d) {
+		j = (int)(row % d.ny);
+		k = (int)(row / d.ny);
+	} else {
+		j = (int)row;
+		k = 0;
+	}
+}
+
+__global__ void __launch_bounds__(256) k_make_rhs_bnd(Dims d, const uint8_t* __restrict__ f8,
+                                                      const float* __restrict__ u, const float* __restrict__ v,
+                                                      const float* __restrict__ w, BndDev B,
+                                                      float* __restrict__ rhs, double* __restrict__ part) {
+	__shared__ double sh[8];
+	double sum = 0, cnt = 0;
+	const long long X = 1, Y = d.nx, Z = d.sz;
+	for (long long c = blockIdx.x * 256ll + threadIdx.x; c < d.n; c += (long long)gridDim.x * 256) {
+		int i, j, k;
+		cell_of(d, c, i, j, k);
+		float set = 0.f;
+		if (interior_c(d, i, j, k) && (f8[c] & kFluid)) {
+			if (!B.fx) {
+				set = u[c] - u[c + X] + v[c] - v[c + Y];
+				if (d.is3d) set += w[c] - w[c + Z];
+			} else {
+				set = B.fx[c] * u[c] - B.fx[c + X] * u[c + X] + B.fy[c] * v[c] - B.fy[c + Y] * v[c + Y];
+				if (d.is3d) set += B.fz[c] * w[c] - B.fz[c + Z] * w[c + Z];
+				if (B.ox) {
+					set += (1 - B.fx[c]) * B.ox[c] - (1 - B.fx[c + X]) * B.ox[c + X] + (1 - B.fy[c]) * B.oy[c] -
+					       (1 - B.fy[c + Y]) * B.oy[c + Y];
+					if (d.is3d) set += (1 - B.fz[c]) * B.oz[c] - (1 - B.fz[c + Z]) * B.oz[c + Z];
+				}
+			}
+			if (B.phi && B.curv) {
+				if (f8[c - X] & kEmpty) set += st_at(B, c, -X);
+				if (f8[c + X] & kEmpty) set += st_at(B, c, +X);
+				if (f8[c - Y] & kEmpty) set += st_at(B, c, -Y);
+				if (f8[c + Y] & kEmpty) set += st_at(B, c, +Y);
+				if (d.is3d) {
+					if (f8[c - Z] & kEmpty) set += st_at(B, c, -Z);
+					if (f8[c + Z] & kEmpty) set += st_at(B, c, +Z);
+				}
+			}
+			if (B.pcc) set += B.pcc[c];
+			sum += (double)set;
+			cnt += 1;
+		}
+		rhs[c] = set;
+	}
+	const double s = block_sum(sum, sh);
+	const double nn = block_sum(cnt, sh);
+	if (threadIdx.x == 0) {
+		part[2 * blockIdx.x] = s;
+		part[2 * blockIdx.x + 1] = nn;
+	}
+}
+
+// MakeLaplaceMatrix (plain or with fractions) + ApplyGhostFluidDiagonal
+__global__ void __launch_bounds__(256) k_make_laplace_bnd(Dims d, const uint8_t* __restrict__ f8, BndDev B,
+                                                          float* __restrict__ A0, float* __restrict__ Ai,
+                                                          float* __restrict__ Aj, float* __restrict__ Ak) {
+	const long long X = 1, Y = d.nx, Z = d.sz;
+	for (long long c = blockIdx.x * 256ll + threadIdx.x; c < d.n; c += (long long)gridDim.x * 256) {
+		int i, j, k;
+		cell_of(d, c, i, j, k);
+		float a0 = 0.f, ai = 0.f, aj = 0.f, ak = 0.f;
+		if (interior_c(d, i, j, k) && (f8[c] & kFluid)) {
+			if (!B.fx) {
+				if (!(f8[c - X] & kObstacle)) a0 += 1.f;
+				if (!(f8[c + X] & kObstacle)) a0 += 1.f;
+				if (!(f8[c - Y] & kObstacle)) a0 += 1.f;
+				if (!(f8[c + Y] & kObstacle)) a0 += 1.f;
+				if (d.is3d && !(f8[c - Z] & kObstacle)) a0 += 1.f;
+				if (d.is3d && !(f8[c + Z] & kObstacle)) a0 += 1.f;
+				if (f8[c + X] & kFluid) ai = -1.f;
+				if (f8[c + Y] & kFluid) aj = -1.f;
+				if (d.is3d && (f8[c + Z] & kFluid)) ak = -1.f;
+			} else {
+				a0 += B.fx[c];
+				a0 += B.fx[c + X];
+				a0 += B.fy[c];
+				a0 += B.fy[c + Y];
+				if (d.is3d) a0 += B.fz[c];
+				if (d.is3d) a0 += B.fz[c + Z];
+				if (f8[c + X] & kFluid) ai = -B.fx[c + X];
+				if (f8[c + Y] & kFluid) aj = -B.fy[c + Y];
+				if (d.is3d && (f8[c + Z] & kFluid)) ak = -B.fz[c + Z];
+			}
+			if (B.phi) {
+				if (f8[c - X] & kEmpty) a0 -= gf_at(B, c, -X);
+				if (f8[c + X] & kEmpty) a0 -= gf_at(B, c, +X);
+				if (f8[c - Y] & kEmpty) a0 -= gf_at(B, c, -Y);
+				if (f8[c + Y] & kEmpty) a0 -= gf_at(B, c, +Y);
+				if (d.is3d) {
+					if (f8[c - Z] & kEmpty) a0 -= gf_at(B, c, -Z);
+					if (f8[c + Z] & kEmpty) a0 -= gf_at(B, c, +Z);
+				}
+			}
+		}
+		A0[c] = a0;
+		Ai[c] = ai;
+		Aj[c] = aj;
+		Ak[c] = ak;
+	}
+}
+
+// fixPressure (pressure.cpp:226-245) with value 0 on the float matrix, at
+// the cell k_fix_select chose (which already set rhs there)
+__global__ void k_fix_general(Dims d, const CgState* __restrict__ st, float* __restrict__ rhs,
+                              float* __restrict__ A0, float* __restrict__ Ai, float* __restrict__ Aj,
+                              float* __restrict__ Ak) {
+	if (threadIdx.x != 0) return;
+	const long long f = st->fix;
+	if (f < 0) return;
+	const long long X = 1, Y = d.nx, Z = d.sz;
+	rhs[f + X] -= Ai[f] * 0.f;
+	rhs[f + Y] -= Aj[f] * 0.f;
+	rhs[f - X] -= Ai[f - X] * 0.f;
+	rhs[f - Y] -= Aj[f - Y] * 0.f;
+	if (d.is3d) {
+		rhs[f + Z] -= Ak[f] * 0.f;
+		rhs[f - Z] -= Ak[f - Z] * 0.f;
+	}
+	rhs[f] = 0.f;
+	A0[f] = 1.f;
+	Ai[f] = Aj[f] = Ak[f] = 0.f;
+	Ai[f - X] = 0.f;
+	Aj[f - Y] = 0.f;
+	if (d.is3d) Ak[f - Z] = 0.f;
+}
+
+struct GenMat {
+	const float *A0, *Ai, *Aj, *Ak;
+};
+// ApplyMatrix (conjugategrad.h:118-151) with src values from a functor;
+// out-of-range neighbours contribute 0
+template <typename SRC>
+__device__ inline float gen_row(const Dims& d, const GenMat& M, long long c, SRC src) {
+	const long long X = 1, Y = d.nx, Z = d.sz, n = d.n;
+	auto S = [&](long long o) { return (c + o >= 0 && c + o < n) ? src(c + o) : 0.f; };
+	auto Cf = [&](const float* A, long long o) { return (c + o >= 0 && c + o < n) ? A[c + o] : 0.f; };
+	if (d.is3d)
+		return src(c) * M.A0[c] + S(-X) * Cf(M.Ai, -X) + S(X) * M.Ai[c] + S(-Y) * Cf(M.Aj, -Y) + S(Y) * M.Aj[c] +
+		       S(-Z) * Cf(M.Ak, -Z) + S(Z) * M.Ak[c];
+	return src(c) * M.A0[c] + S(-X) * Cf(M.Ai, -X) + S(X) * M.Ai[c] + S(-Y) * Cf(M.Aj, -Y) + S(Y) * M.Aj[c];
+}
+// the same row sum in double (true residual, residual replacement)
+__device__ inline double gen_row_d(const Dims& d, const GenMat& M, long long c, const float* __restrict__ x) {
+	const long long X = 1, Y = d.nx, Z = d.sz, n = d.n;
+	auto T = [&](const float* A, long long ao, long long xo) {
+		return (c + xo >= 0 && c + xo < n && c + ao >= 0 && c + ao < n) ? (double)x[c + xo] * (double)A[c + ao] : 0.0;
+	};
+	double ax = (double)x[c] * (double)M.A0[c];
+	ax += T(M.Ai, -X, -X) + T(M.Ai, 0, X) + T(M.Aj, -Y, -Y) + T(M.Aj, 0, Y);
+	if (d.is3d) ax += T(M.Ak, -Z, -Z) + T(M.Ak, 0, Z);
+	return ax;
+}
+
+// GridCg::iterate's search direction and matrix product on the float
+// matrix: p = z (first iteration) or z + beta p_old, q = A p (identity rows
+// outside the fluid), partial sums of fl(q p) per block
+__global__ void __launch_bounds__(256) k_matvec_gen(Dims d, const CgState* __restrict__ st,
+                                                    const uint8_t* __restrict__ f8, GenMat M,
+                                                    const float* __restrict__ z, const float* __restrict__ pold,
+                                                    float* __restrict__ pnew, float* __restrict__ q,
+                                                    double* __restrict__ part) {
+	__shared__ double sh[8];
+	if (st->done) return;
+	const int first = st->first;
+	const float beta = st->beta;
+	auto pn = [&](long long x) { return first ? z[x] : z[x] + beta * pold[x]; };
+	double acc = 0;
+	for (long long c = blockIdx.x * 256ll + threadIdx.x; c < d.n; c += (long long)gridDim.x * 256) {
+		const float pc = pn(c);
+		const float qc = (f8[c] & kFluid) ? gen_row(d, M, c, pn) : pc;
+		pnew[c] = pc;
+		q[c] = qc;
+		acc += (double)(qc * pc);
+	}
+	const double s = block_sum(acc, sh);
+	if (threadIdx.x == 0) part[3 * blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(256) k_true_residual_gen(Dims d, const uint8_t* __restrict__ f8, GenMat M,
+                                                           const float* __restrict__ b, const float* __restrict__ x,
+                                                           double* __restrict__ part) {
+	__shared__ double sh[8];
+	double rr = 0;
+	for (long long c = blockIdx.x * 256ll + threadIdx.x; c < d.n; c += (long long)gridDim.x * 256) {
+		const double ax = (f8[c] & kFluid) ? gen_row_d(d, M, c, x) : (double)x[c];
+		const double r = (double)b[c] - ax;
+		rr += r * r;
+	}
+	const double s = block_sum(rr, sh);
+	if (threadIdx.x == 0) part[3 * blockIdx.x] = s;
+}
+
+// residual replacement on the float matrix (same partials as
+// k_cg_replace_residual)
+__global__ void __launch_bounds__(256) k_replace_residual_gen(Dims d, const CgState* __restrict__ st,
+                                                              const uint8_t* __restrict__ f8, GenMat M,
+                                                              const float* __restrict__ b,
+                                                              const float* __restrict__ x, float* __restrict__ r,
+                                                              int withDot, double* __restrict__ part,
+                                                              float* __restrict__ fpart) {
+	__shared__ double sh[8];
+	__shared__ float shf[8];
+	if (st->done) return;
+	float mn = 3.402823466e+38f, mx = -3.402823466e+38f;
+	double zr = 0, rr = 0;
+	for (long long c = blockIdx.x * 256ll + threadIdx.x; c < d.n; c += (long long)gridDim.x * 256) {
+		const double ax = (f8[c] & kFluid) ? gen_row_d(d, M, c, x) : (double)x[c];
+		const float rn = (float)((double)b[c] - ax);
+		r[c] = rn;
+		mn = fminf(mn, rn);
+		mx = fmaxf(mx, rn);
+		zr += (double)(rn * rn);
+		rr += (double)rn * (double)rn;
+	}
+	const double s0 = block_sum(zr, sh);
+	const double s1 = block_sum(rr, sh);
+	mn = wave_min(mn);
+	mx = wave_max(mx);
+	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+	if (lane == 0) {
+		shf[wv] = mn;
+		shf[4 + wv] = mx;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		float a = shf[0], bb = shf[4];
+		for (int k = 1; k < 4; k++) {
+			a = fminf(a, shf[k]);
+			bb = fmaxf(bb, shf[4 + k]);
+		}
+		part[3 * blockIdx.x] = withDot ? s0 : 0.0;
+		part[3 * blockIdx.x + 1] = s1;
+		fpart[2 * blockIdx.x] = a;
+		fpart[2 * blockIdx.x + 1] = bb;
+	}
+}
+
+// knCorrectVelocityGhostFluid (pressure.cpp:151-190), bnd=1, after
+// knCorrectVelocity
+__global__ void __launch_bounds__(256) k_correct_gf(Dims d, const uint8_t* __restrict__ f8,
+                                                    const float* __restrict__ p, BndDev B, float* __restrict__ u,
+                                                    float* __restrict__ v, float* __restrict__ w) {
+	const long long X = 1, Y = d.nx, Z = d.sz;
+	for (long long c = blockIdx.x * 256ll + threadIdx.x; c < d.n; c += (long long)gridDim.x * 256) {
+		int i, j, k;
+		cell_of(d, c, i, j, k);
+		if (!interior_c(d, i, j, k)) continue;
+		const uint8_t f = f8[c];
+		float vu = u[c], vv = v[c], vw = w[c];
+		if (f & kFluid) {
+			if (f8[c - X] & kEmpty) vu += p[c] * gf_at(B, c, -X);
+			if (f8[c - Y] & kEmpty) vv += p[c] * gf_at(B, c, -Y);
+			if (d.is3d && (f8[c - Z] & kEmpty)) vw += p[c] * gf_at(B, c, -Z);
+		} else if ((f & kEmpty) && !(f & kOutflow)) {
+			if (f8[c - X] & kFluid) vu -= p[c - X] * gf_at(B, c - X, +X);
+			else vu = 0.f;
+			if (f8[c - Y] & kFluid) vv -= p[c - Y] * gf_at(B, c - Y, +Y);
+			else vv = 0.f;
+			if (d.is3d) {
+				if (f8[c - Z] & kFluid) vw -= p[c - Z] * gf_at(B, c - Z, +Z);
+				else vw = 0.f;
+			}
+		}
+		if (B.curv) {
+			if (f & kFluid) {
+				if (f8[c - X] & kEmpty) vu += st_at(B, c, -X);
+				if (f8[c - Y] & kEmpty) vv += st_at(B, c, -Y);
+				if (d.is3d && (f8[c - Z] & kEmpty)) vw += st_at(B, c, -Z);
+			} else if ((f & kEmpty) && !(f & kOutflow)) {
+				vu -= (f8[c - X] & kFluid) ? st_at(B, c - X, +X) : 0.f;
+				vv -= (f8[c - Y] & kFluid) ? st_at(B, c - Y, +Y) : 0.f;
+				if (d.is3d) vw -= (f8[c - Z] & kFluid) ? st_at(B, c - Z, +Z) : 0.f;
+			}
+		}
+		u[c] = vu;
+		v[c] = vv;
+		if (d.is3d) w[c] = vw;
+	}
+}
+
+// ghostFluidWasClamped (pressure.cpp:194-199)
+__device__ inline bool gf_clamped(const BndDev& B, long long c, long long off) {
+	return gf_theta(B.phi[c], B.phi[c + off]) < B.gfClamp;
+}
+// knReplaceClampedGhostFluidVels (pressure.cpp:201-219), bnd=1: empty cells
+// take a fluid neighbour's face where that face's fraction was clamped (the
+// faces read belong to fluid cells, which this pass does not write)
+__global__ void __launch_bounds__(256) k_replace_clamped(Dims d, const uint8_t* __restrict__ f8, BndDev B,
+                                                         float* __restrict__ u, float* __restrict__ v,
+                                                         float* __restrict__ w) {
+	const long long X = 1, Y = d.nx, Z = d.sz;
+	for (long long c = blockIdx.x * 256ll + threadIdx.x; c < d.n; c += (long long)gridDim.x * 256) {
+		int i, j, k;
+		cell_of(d, c, i, j, k);
+		if (!interior_c(d, i, j, k) || !(f8[c] & kEmpty)) continue;
+		float vu = u[c], vv = v[c], vw = w[c];
+		if ((f8[c - X] & kFluid) && gf_clamped(B, c - X, +X)) vu = u[c - X];
+		if ((f8[c - Y] & kFluid) && gf_clamped(B, c - Y, +Y)) vv = v[c - Y];
+		if (d.is3d && (f8[c - Z] & kFluid) && gf_clamped(B, c - Z, +Z)) vw = w[c - Z];
+		if ((f8[c + X] & kFluid) && gf_clamped(B, c + X, -X)) vu = u[c + X];
+		if ((f8[c + Y] & kFluid) && gf_clamped(B, c + Y, -Y)) vv = v[c + Y];
+		if (d.is3d && (f8[c + Z] & kFluid) && gf_clamped(B, c + Z, -Z)) vw = w[c + Z];
+		u[c] = vu;
+		v[c] = vv;
+		if (d.is3d) w[c] = vw;
+	}
+}
+
+// Multigrid on the float matrix: the V-cycle is built from the flags' plain
+// Laplacian L; it is applied as S V S with S = sqrt(diag L / diag A) on fluid
+// rows (1 elsewhere), a symmetric congruence that stays SPD and gives the
+// preconditioned rows the system's diagonal (large ghost-fluid diagonals
+// otherwise leave those rows under-weighted)
+__global__ void __launch_bounds__(256) k_diag_scale(Dims d, const uint8_t* __restrict__ code,
+                                                    const uint8_t* __restrict__ f8, const float* __restrict__ A0,
+                                                    float* __restrict__ S) {
+	for (long long c = blockIdx.x * 256ll + threadIdx.x; c < d.n; c += (long long)gridDim.x * 256) {
+		float sc = 1.f;
+		if (f8[c] & kFluid) {
+			const float dl = (float)(code[c] & 7), da = A0[c];
+			if (dl > 0.f && da > 0.f) sc = sqrtf(dl / da);
+		}
+		S[c] = sc;
+	}
+}
+__global__ void __launch_bounds__(256) k_scale_vec(long long n, const CgState* __restrict__ st,
+                                                   const float* __restrict__ S, const float* __restrict__ in,
+                                                   float* __restrict__ out) {
+	if (st->done) return;
+	for (long long c = blockIdx.x * 256ll + threadIdx.x; c < n; c += (long long)gridDim.x * 256) out[c] = S[c] * in[c];
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 static unsigned flat_count(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
@@ -1140,16 +1527,22 @@ static MgHierarchy* mic_hierarchy(Scene* s, const Grid* flags, const uint8_t* co
 // mean correction is reduced and added on the device.  part: FP64 partials
 // (2 per block of the k_make_rhs launch, plus one float slot after them).
 static void compute_rhs(Scene* s, Grid* flags, Grid* vel, float* rhs, int enforceCompatibility,
-                        double* part = nullptr) {
+                        double* part = nullptr, const BndDev* bnd = nullptr) {
 	const Tiles t = tiles_for(s->d, kLightKZ);
-	const unsigned nb = (unsigned)t.count();
+	const bool extra = bnd && (bnd->fx || bnd->pcc || (bnd->phi && bnd->curv));
+	const unsigned nb = extra ? flat_blocks(s->d.n) : (unsigned)t.count();
 	if (!part) {
 		s->ensurePartials(3 * (size_t)nb + 16, 2 * (size_t)nb + 16);
 		part = s->dpart;
 	}
-	hipLaunchKernelGGL(k_make_rhs, dim3(nb), dim3(kBlock), 0, s->stream, s->d, t, flags->f8,
-	                   (const float*)vel->comp(0), (const float*)vel->comp(1), (const float*)vel->comp(2), rhs,
-	                   part);
+	if (extra)
+		hipLaunchKernelGGL(k_make_rhs_bnd, dim3(nb), dim3(256), 0, s->stream, s->d, (const uint8_t*)flags->f8,
+		                   (const float*)vel->comp(0), (const float*)vel->comp(1), (const float*)vel->comp(2), *bnd,
+		                   rhs, part);
+	else
+		hipLaunchKernelGGL(k_make_rhs, dim3(nb), dim3(kBlock), 0, s->stream, s->d, t, flags->f8,
+		                   (const float*)vel->comp(0), (const float*)vel->comp(1), (const float*)vel->comp(2), rhs,
+		                   part);
 	if (enforceCompatibility) {
 		float* corr = (float*)(part + 2 * (size_t)nb);
 		hipLaunchKernelGGL(k_compat_corr, dim3(1), dim3(1024), 0, s->stream, (const double*)part, (int)nb, corr);
@@ -1191,6 +1584,12 @@ struct CgRun {
 	int launched = 0;
 	int scaled = 0;     // scaled Laplace matrix (cgSolveDiffusion, cgSolveWE)
 	float scale = 0.f;
+	// float matrix of the second-order boundaries (fractions / ghost fluid),
+	// pooled grids; null on the matrix-code path
+	float *gA0 = nullptr, *gAi = nullptr, *gAj = nullptr, *gAk = nullptr;
+	float *gS = nullptr, *gT = nullptr;  // V-cycle scaling and its scaled input (preconditioned only)
+	unsigned nbG = 0;  // blocks of the float-matrix kernels
+	GenMat gen() const { return GenMat{gA0, gAi, gAj, gAk}; }
 	MatvecP mvp(bool odd) const {
 		MatvecP p{s->d, kchunk, st, code, z, odd ? pb : pa, odd ? pa : pb, q, dpart, scaled, scale};
 		return p;
@@ -1285,8 +1684,10 @@ static void autotune_zmarch(Scene* s, const uint8_t* code) {
 	cleanup();
 }
 
+static int precondition(CgRun& R);
+
 static void cg_setup(CgRun& R, Scene* s, Grid* flags, float* rhs, float* x, const mfx_pressure_opts& o,
-                     bool batchInit = false) {
+                     bool batchInit = false, const BndDev* bnd = nullptr) {
 	R.t0 = std::chrono::steady_clock::now();
 	R.s = s;
 	R.flags = flags;
@@ -1333,6 +1734,20 @@ static void cg_setup(CgRun& R, Scene* s, Grid* flags, float* rhs, float* x, cons
 		                   R.st);
 		hipLaunchKernelGGL(k_fix_select, dim3(1), dim3(64), 0, s->stream, d, flags->f8, R.st, rhs);
 	}
+	// the float matrix of fractions / ghost fluid (the matrix code below
+	// then serves the multigrid preconditioner only)
+	if (bnd && (bnd->fx || bnd->phi)) {
+		R.nbG = flat_blocks(d.n);
+		R.gA0 = s->getReal();
+		R.gAi = s->getReal();
+		R.gAj = s->getReal();
+		R.gAk = s->getReal();
+		hipLaunchKernelGGL(k_make_laplace_bnd, dim3(R.nbG), dim3(256), 0, s->stream, d, (const uint8_t*)flags->f8,
+		                   *bnd, R.gA0, R.gAi, R.gAj, R.gAk);
+		if (R.fixing)
+			hipLaunchKernelGGL(k_fix_general, dim3(1), dim3(64), 0, s->stream, d, (const CgState*)R.st, rhs, R.gA0,
+			                   R.gAi, R.gAj, R.gAk);
+	}
 	// matrix
 	if (R.pooled) {
 		if (!s->code) MFX_HIP(hipMalloc((void**)&s->code, (size_t)d.n));
@@ -1376,26 +1791,72 @@ static void cg_setup(CgRun& R, Scene* s, Grid* flags, float* rhs, float* x, cons
 	hipLaunchKernelGGL(k_cg_init, dim3(R.nbU), dim3(kBlock), 0, s->stream, d, R.per, (const float*)rhs, x, R.r,
 	                   R.pcNone ? 1 : 0, R.dpart);
 	run_finalize_one(s->stream, FinP{R.st, R.dpart, R.fpart, (int)R.nbU, FIN_INIT, 0});
+	if (R.gA0 && !R.pcNone) {
+		R.gS = s->getReal();
+		R.gT = s->getReal();
+		hipLaunchKernelGGL(k_diag_scale, dim3(R.nbG), dim3(256), 0, s->stream, d, (const uint8_t*)R.code,
+		                   (const uint8_t*)flags->f8, (const float*)R.gA0, R.gS);
+	}
 	if (!R.pcNone && !batchInit) {
-		int np = R.mg->vcycle(s, R.code, R.r, R.z, R.st);
-		if (np == 0) {
-			run_dot_one(s->stream, R.nbF, DotP{d, R.st, R.z, R.r, R.dpart});
-			np = (int)R.nbF;
-		}
+		const int np = precondition(R);
 		run_finalize_one(s->stream, FinP{R.st, R.dpart, R.fpart, np, FIN_BETA0, 0});
 	}
 	R.launched = 0;
+}
+
+// r = b - A x with FP64 row sums in place of the recursive residual (plus
+// the update's partials); returns the number of partial-sum slots written
+static int enqueue_replace_residual(CgRun& R) {
+	Scene* s = R.s;
+	if (R.gA0) {
+		hipLaunchKernelGGL(k_replace_residual_gen, dim3(R.nbG), dim3(256), 0, s->stream, s->d, (const CgState*)R.st,
+		                   (const uint8_t*)R.flags->f8, R.gen(), (const float*)R.rhs, (const float*)R.x, R.r,
+		                   R.pcNone ? 1 : 0, R.dpart, R.fpart);
+		return (int)R.nbG;
+	}
+	hipLaunchKernelGGL(k_cg_replace_residual, dim3(R.nbT), dim3(kBlock), 0, s->stream, s->d, R.t, (const CgState*)R.st,
+	                   R.code, (const float*)R.rhs, (const float*)R.x, R.r, R.pcNone ? 1 : 0, R.dpart, R.fpart);
+	return (int)R.nbT;
+}
+
+// z = M r (V-cycle, scaled on the float matrix) and the partials of
+// fl(z r); returns their count
+static int precondition(CgRun& R) {
+	Scene* s = R.s;
+	int np;
+	if (R.gS) {
+		const unsigned nb = flat_blocks(s->d.n);
+		hipLaunchKernelGGL(k_scale_vec, dim3(nb), dim3(256), 0, s->stream, s->d.n, (const CgState*)R.st,
+		                   (const float*)R.gS, (const float*)R.r, R.gT);
+		np = R.mg->vcycle(s, R.code, R.gT, R.z, R.st);
+		hipLaunchKernelGGL(k_scale_vec, dim3(nb), dim3(256), 0, s->stream, s->d.n, (const CgState*)R.st,
+		                   (const float*)R.gS, (const float*)R.z, R.z);
+	} else {
+		np = R.mg->vcycle(s, R.code, R.r, R.z, R.st);
+	}
+	if (np == 0) {
+		run_dot_one(s->stream, R.nbF, DotP{s->d, R.st, R.z, R.r, R.dpart});
+		np = (int)R.nbF;
+	}
+	return np;
 }
 
 // GridCg::iterate (conjugategrad.cpp:238-299), queued on the scene stream
 static void cg_iteration(CgRun& R) {
 	Scene* s = R.s;
 	const bool odd = (R.launched & 1) != 0;
-	{
-		KtScope kt("cg_matvec", s->stream);
-		run_matvec_one(s->stream, R.nbM, R.mvp(odd));
+	if (R.gA0) {
+		const MatvecP m = R.mvp(odd);
+		hipLaunchKernelGGL(k_matvec_gen, dim3(R.nbG), dim3(256), 0, s->stream, s->d, (const CgState*)R.st,
+		                   (const uint8_t*)R.flags->f8, R.gen(), m.z, m.pold, m.pnew, m.q, R.dpart);
+		run_finalize_one(s->stream, R.fin((int)R.nbG, FIN_ALPHA));
+	} else {
+		{
+			KtScope kt("cg_matvec", s->stream);
+			run_matvec_one(s->stream, R.nbM, R.mvp(odd));
+		}
+		run_finalize_one(s->stream, R.fin(R.npM, FIN_ALPHA));
 	}
-	run_finalize_one(s->stream, R.fin(R.npM, FIN_ALPHA));
 	{
 		KtScope kt("cg_update", s->stream);
 		run_update_one(s->stream, R.nbU, R.upd(odd));
@@ -1403,18 +1864,11 @@ static void cg_iteration(CgRun& R) {
 	int nbRes = (int)R.nbU;
 	if (R.o.trueResidualEvery > 0 && (R.launched + 1) % R.o.trueResidualEvery == 0) {
 		// FP64 residual replacement; its partials replace the update's
-		hipLaunchKernelGGL(k_cg_replace_residual, dim3(R.nbT), dim3(kBlock), 0, s->stream, s->d, R.t,
-		                   (const CgState*)R.st, R.code, (const float*)R.rhs, (const float*)R.x, R.r,
-		                   R.pcNone ? 1 : 0, R.dpart, R.fpart);
-		nbRes = (int)R.nbT;
+		nbRes = enqueue_replace_residual(R);
 	}
 	run_finalize_one(s->stream, R.fin(nbRes, FIN_RES));
 	if (!R.pcNone) {
-		int np = R.mg->vcycle(s, R.code, R.r, R.z, R.st);
-		if (np == 0) {
-			run_dot_one(s->stream, R.nbF, DotP{s->d, R.st, R.z, R.r, R.dpart});
-			np = (int)R.nbF;
-		}
+		const int np = precondition(R);
 		run_finalize_one(s->stream, R.fin(np, FIN_BETA));
 	}
 	R.launched++;
@@ -1424,6 +1878,14 @@ static void cg_iteration(CgRun& R) {
 // mirror, buffers back to the pool (queued; nothing waits)
 static void cg_true_residual_enqueue(CgRun& R) {
 	Scene* s = R.s;
+	if (R.gA0) {
+		hipLaunchKernelGGL(k_true_residual_gen, dim3(R.nbG), dim3(256), 0, s->stream, s->d, (const uint8_t*)R.flags->f8,
+		                   R.gen(), (const float*)R.rhs, (const float*)R.x, R.dpart);
+		hipLaunchKernelGGL(k_reduce_true_rr, dim3(1), dim3(1024), 0, s->stream, R.st, (const double*)R.dpart,
+		                   (int)R.nbG);
+		MFX_HIP(hipMemcpyAsync(s->hscal, R.st, sizeof(CgState), hipMemcpyDeviceToHost, s->stream));
+		return;
+	}
 	const Tiles tr = tiles_for(s->d, kLightKZ);
 	if (R.scaled)
 		hipLaunchKernelGGL((k_true_residual<false, true>), dim3((unsigned)tr.count()), dim3(kBlock), 0, s->stream,
@@ -1448,6 +1910,11 @@ static void cg_release(CgRun& R) {
 		s->putReal(R.q);
 		if (!R.pcNone) s->putReal(R.z);
 	}
+	for (float** a : {&R.gA0, &R.gAi, &R.gAj, &R.gAk, &R.gS, &R.gT})
+		if (*a) {
+			s->putReal(*a);
+			*a = nullptr;
+		}
 }
 
 // CG restarted from the current x: r = b - A x (FP64 row sums), z = M r,
@@ -1455,17 +1922,8 @@ static void cg_release(CgRun& R) {
 static void cg_restart(CgRun& R) {
 	Scene* s = R.s;
 	hipLaunchKernelGGL(k_cg_restart, dim3(1), dim3(64), 0, s->stream, R.st);
-	hipLaunchKernelGGL(k_cg_replace_residual, dim3(R.nbT), dim3(kBlock), 0, s->stream, s->d, R.t,
-	                   (const CgState*)R.st, R.code, (const float*)R.rhs, (const float*)R.x, R.r,
-	                   R.pcNone ? 1 : 0, R.dpart, R.fpart);
-	int np = (int)R.nbT;
-	if (!R.pcNone) {
-		np = R.mg->vcycle(s, R.code, R.r, R.z, R.st);
-		if (np == 0) {
-			run_dot_one(s->stream, R.nbF, DotP{s->d, R.st, R.z, R.r, R.dpart});
-			np = (int)R.nbF;
-		}
-	}
+	int np = enqueue_replace_residual(R);
+	if (!R.pcNone) np = precondition(R);
 	run_finalize_one(s->stream, R.fin(np, FIN_BETA0));
 }
 
@@ -1524,7 +1982,7 @@ static void cg_finish_collect(CgRun& R, mfx_solve_stats* stats) {
 static void cg_drive(CgRun& R, double target, mfx_solve_stats* stats);
 
 static void solve_system(Scene* s, Grid* flags, float* rhs, float* x, const mfx_pressure_opts& o,
-                         mfx_solve_stats* stats, const CgRun* bufs = nullptr) {
+                         mfx_solve_stats* stats, const CgRun* bufs = nullptr, const BndDev* bnd = nullptr) {
 	CgRun R;
 	if (bufs) {
 		// caller-provided buffers (allocation-free solve)
@@ -1538,7 +1996,7 @@ static void solve_system(Scene* s, Grid* flags, float* rhs, float* x, const mfx_
 		R.dpart = bufs->dpart;
 		R.fpart = bufs->fpart;
 	}
-	cg_setup(R, s, flags, rhs, x, o);
+	cg_setup(R, s, flags, rhs, x, o, false, bnd);
 	cg_drive(R, o.relTol > 0 ? o.relTol : (double)o.cgAccuracy, stats);
 }
 
@@ -1636,10 +2094,18 @@ static void solve_scaled(Scene* s, Grid* flags, float* rhs, float* x, int kind, 
 	cg_drive(R, (double)cgAccuracy, stats);
 }
 
-static void correct_velocity(Scene* s, Grid* flags, Grid* vel, const float* pressure) {
+static void correct_velocity(Scene* s, Grid* flags, Grid* vel, const float* pressure, const BndDev* bnd = nullptr) {
 	const Tiles t = tiles_for(s->d, kLightKZ);
 	hipLaunchKernelGGL(k_correct_velocity, dim3((unsigned)t.count()), dim3(kBlock), 0, s->stream, s->d, t, flags->f8,
 	                   pressure, vel->comp(0), vel->comp(1), vel->comp(2));
+	if (bnd && bnd->phi) {
+		// correctVelocity with phi (pressure.cpp:470-474)
+		const unsigned nb = flat_blocks(s->d.n);
+		hipLaunchKernelGGL(k_correct_gf, dim3(nb), dim3(256), 0, s->stream, s->d, (const uint8_t*)flags->f8, pressure,
+		                   *bnd, vel->comp(0), vel->comp(1), vel->comp(2));
+		hipLaunchKernelGGL(k_replace_clamped, dim3(nb), dim3(256), 0, s->stream, s->d, (const uint8_t*)flags->f8, *bnd,
+		                   vel->comp(0), vel->comp(1), vel->comp(2));
+	}
 }
 
 // ---------------------------------------------------------------------------
@@ -2132,6 +2598,114 @@ void mfx_pressure_opts_default(mfx_pressure_opts* o) {
 	o->maxIterOverride = 0;
 	o->precision = 0;
 	o->trueResidualEvery = 0;
+}
+
+void mfx_pressure_bnd_default(mfx_pressure_bnd* b) {
+	if (!b) return;
+	memset(b, 0, sizeof(*b));
+	b->gfClamp = 1e-04f;
+	b->surfTens = 0.f;
+}
+
+}  // extern "C"
+
+// the optional grids of the second-order boundaries, checked and resolved to
+// device pointers (MAC / Vec3 grids as their three SoA components)
+static BndDev bnd_of(const mfx_pressure_bnd* b, const Grid* flags) {
+	BndDev B{};
+	B.gfClamp = b ? b->gfClamp : 1e-04f;
+	B.surfTens = b ? b->surfTens : 0.f;
+	if (!b) return B;
+	auto real = [&](const mfx_grid* h, const char* name) -> const float* {
+		if (!h) return nullptr;
+		const Grid* g = G(h);
+		MFX_CHECK(g->isReal(), std::string("solvePressure: ") + name + " must be a Real or Levelset grid");
+		MFX_CHECK(g->s == flags->s, "grids belong to different solvers");
+		return g->real();
+	};
+	auto vec = [&](const mfx_grid* h, const char* name, const float** c) {
+		c[0] = c[1] = c[2] = nullptr;
+		if (!h) return;
+		const Grid* g = G(h);
+		MFX_CHECK(g->isVec(), std::string("solvePressure: ") + name + " must be a MACGrid");
+		MFX_CHECK(g->s == flags->s, "grids belong to different solvers");
+		for (int q = 0; q < 3; q++) c[q] = g->comp(q);
+	};
+	B.phi = real(b->phi, "phi");
+	B.pcc = real(b->perCellCorr, "perCellCorr");
+	B.curv = real(b->curv, "curv");
+	const float* fr[3];
+	vec(b->fractions, "fractions", fr);
+	B.fx = fr[0];
+	B.fy = fr[1];
+	B.fz = fr[2];
+	const float* ob[3];
+	vec(b->obvel, "obvel", ob);
+	B.ox = ob[0];
+	B.oy = ob[1];
+	B.oz = ob[2];
+	return B;
+}
+
+extern "C" {
+
+int mfx_solve_pressure_bnd(mfx_grid* vel, mfx_grid* pressure, const mfx_grid* flags, const mfx_pressure_opts* opts,
+                           const mfx_pressure_bnd* bnd, mfx_solve_stats* stats, mfx_grid* retRhs) {
+	API_BEGIN
+	Grid *v = G(vel), *p = G(pressure), *f = G(flags);
+	check_pressure_args(v, p, f);
+	MFX_CHECK(!retRhs || G(retRhs)->isReal(), "solvePressure: retRhs must be a RealGrid");
+	const mfx_pressure_opts o = opts_or_default(opts);
+	const BndDev B = bnd_of(bnd, f);
+	Scene* s = v->s;
+	float* rhs = s->getReal();
+	try {
+		compute_rhs(s, f, v, rhs, o.enforceCompatibility, nullptr, &B);
+		solve_system(s, f, rhs, p->real(), o, stats, nullptr, &B);
+		correct_velocity(s, f, v, p->real(), &B);
+		if (retRhs)
+			MFX_HIP(hipMemcpyAsync(G(retRhs)->real(), rhs, sizeof(float) * (size_t)s->d.n, hipMemcpyDeviceToDevice,
+			                       s->stream));
+	} catch (...) {
+		s->putReal(rhs);
+		throw;
+	}
+	s->putReal(rhs);
+	API_END
+}
+
+int mfx_compute_pressure_rhs_bnd(mfx_grid* rhs, const mfx_grid* vel, const mfx_grid* flags,
+                                 const mfx_pressure_opts* opts, const mfx_pressure_bnd* bnd) {
+	API_BEGIN
+	Grid *r = G(rhs), *v = G(vel), *f = G(flags);
+	check_pressure_args(v, r, f);
+	const mfx_pressure_opts o = opts_or_default(opts);
+	const BndDev B = bnd_of(bnd, f);
+	compute_rhs(v->s, f, v, r->real(), o.enforceCompatibility, nullptr, &B);
+	API_END
+}
+
+int mfx_solve_pressure_system_bnd(mfx_grid* rhs, mfx_grid* pressure, const mfx_grid* flags,
+                                  const mfx_pressure_opts* opts, const mfx_pressure_bnd* bnd, mfx_solve_stats* stats) {
+	API_BEGIN
+	Grid *r = G(rhs), *p = G(pressure), *f = G(flags);
+	MFX_CHECK(r && p && f && r->isReal() && p->isReal() && f->isFlags(),
+	          "solvePressureSystem: needs rhs, pressure (RealGrid) and flags (FlagGrid)");
+	MFX_CHECK(r->s == p->s && r->s == f->s, "grids belong to different solvers");
+	const mfx_pressure_opts o = opts_or_default(opts);
+	const BndDev B = bnd_of(bnd, f);
+	solve_system(r->s, f, r->real(), p->real(), o, stats, nullptr, &B);
+	API_END
+}
+
+int mfx_correct_velocity_bnd(mfx_grid* vel, const mfx_grid* pressure, const mfx_grid* flags,
+                             const mfx_pressure_bnd* bnd) {
+	API_BEGIN
+	Grid *v = G(vel), *p = G(pressure), *f = G(flags);
+	check_pressure_args(v, p, f);
+	const BndDev B = bnd_of(bnd, f);
+	correct_velocity(v->s, f, v, p->real(), &B);
+	API_END
 }
 
 int mfx_solve_pressure(mfx_grid* vel, mfx_grid* pressure, const mfx_grid* flags, const mfx_pressure_opts* opts,

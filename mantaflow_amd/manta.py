@@ -729,9 +729,11 @@ def setWallBcs(*args, **kwargs):
     """extforces.cpp:307-315"""
     a = _parse([("flags", _REQ), ("vel", _REQ), ("obvel", None), ("fractions", None), ("phiObs", None),
                 ("boundaryWidth", 0)], args, kwargs)
-    if a["fractions"] is not None and a["phiObs"] is not None:
-        _err("setWallBcs: second-order obstacle boundaries (fractions/phiObs) are not supported by this build")
-    check(lib().mfx_set_wall_bcs(_h(a["flags"]), _h(a["vel"]), _h(a["obvel"])))
+    if _h(a["fractions"]) is not None and _h(a["phiObs"]) is not None:
+        check(lib().mfx_set_wall_bcs_frac(_h(a["flags"]), _h(a["vel"]), _h(a["obvel"]), _h(a["fractions"]),
+                                          _h(a["phiObs"]), int(a["boundaryWidth"])))
+    else:
+        check(lib().mfx_set_wall_bcs(_h(a["flags"]), _h(a["vel"]), _h(a["obvel"])))
 
 
 @_plugin
@@ -768,11 +770,24 @@ _PRESSURE_SIG = [("cgAccuracy", 1e-3), ("phi", None), ("perCellCorr", None), ("f
                  ("zeroPressureFixing", False), ("curv", None), ("surfTens", 0.)]
 
 
+_BND_GRIDS = ("phi", "perCellCorr", "fractions", "obvel", "curv")
+
+
+def _bnd(a):
+    """The optional boundary inputs (pressure.cpp:480-495) as the C struct,
+    or None when none of the grids is given (None or 0 both mean absent)."""
+    if all(_h(a.get(k)) is None for k in _BND_GRIDS):
+        return None
+    b = _abi.PressureBnd()
+    lib().mfx_pressure_bnd_default(C.byref(b))
+    for k in _BND_GRIDS:
+        setattr(b, k, _h(a.get(k)))
+    b.gfClamp = _f32(a["gfClamp"])
+    b.surfTens = _f32(a["surfTens"])
+    return b
+
+
 def _opts(a, solver):
-    for k in ("phi", "perCellCorr", "fractions", "obvel", "curv"):
-        if a.get(k) is not None and a.get(k) != 0:
-            _err("solvePressure: argument '%s' (ghost-fluid / fractions / surface tension path) is not "
-                 "supported by this build" % k)
     o = _abi.PressureOpts()
     lib().mfx_pressure_opts_default(C.byref(o))
     o.cgAccuracy = a["cgAccuracy"]
@@ -809,8 +824,13 @@ def solvePressure(*args, **kwargs):
     solver = a["vel"].parent
     o = _opts(a, solver)
     st = _abi.SolveStats()
-    check(lib().mfx_solve_pressure(_h(a["vel"]), _h(a["pressure"]), _h(a["flags"]), C.byref(o), C.byref(st),
-                                   _h(a["retRhs"])))
+    b = _bnd(a)
+    if b is None:
+        check(lib().mfx_solve_pressure(_h(a["vel"]), _h(a["pressure"]), _h(a["flags"]), C.byref(o), C.byref(st),
+                                       _h(a["retRhs"])))
+    else:
+        check(lib().mfx_solve_pressure_bnd(_h(a["vel"]), _h(a["pressure"]), _h(a["flags"]), C.byref(o),
+                                           C.byref(b), C.byref(st), _h(a["retRhs"])))
     _report(solver, st)
 
 
@@ -824,6 +844,9 @@ def solvePressureBatch(problems, **kwargs):
     if not problems:
         return
     a = _parse(_PRESSURE_SIG, (), kwargs)
+    if _bnd(a) is not None:
+        _err("solvePressureBatch: the second-order boundary inputs (%s) are taken by solvePressure only"
+             % ", ".join(_BND_GRIDS))
     solver = problems[0][0].parent
     o = _opts(a, solver)
     n = len(problems)
@@ -843,7 +866,12 @@ def computePressureRhs(*args, **kwargs):
         [x for x in _PRESSURE_SIG]
     a = _parse(sig, args, kwargs)
     o = _opts(a, a["vel"].parent)
-    check(lib().mfx_compute_pressure_rhs(_h(a["rhs"]), _h(a["vel"]), _h(a["flags"]), C.byref(o)))
+    b = _bnd(a)
+    if b is None:
+        check(lib().mfx_compute_pressure_rhs(_h(a["rhs"]), _h(a["vel"]), _h(a["flags"]), C.byref(o)))
+    else:
+        check(lib().mfx_compute_pressure_rhs_bnd(_h(a["rhs"]), _h(a["vel"]), _h(a["flags"]), C.byref(o),
+                                                 C.byref(b)))
 
 
 @_plugin
@@ -855,7 +883,13 @@ def solvePressureSystem(*args, **kwargs):
     solver = a["vel"].parent
     o = _opts(dict(a, obvel=None), solver)
     st = _abi.SolveStats()
-    check(lib().mfx_solve_pressure_system(_h(a["rhs"]), _h(a["pressure"]), _h(a["flags"]), C.byref(o), C.byref(st)))
+    b = _bnd(dict(a, obvel=None))
+    if b is None:
+        check(lib().mfx_solve_pressure_system(_h(a["rhs"]), _h(a["pressure"]), _h(a["flags"]), C.byref(o),
+                                              C.byref(st)))
+    else:
+        check(lib().mfx_solve_pressure_system_bnd(_h(a["rhs"]), _h(a["pressure"]), _h(a["flags"]), C.byref(o),
+                                                  C.byref(b), C.byref(st)))
     _report(solver, st)
 
 
@@ -865,7 +899,11 @@ def correctVelocity(*args, **kwargs):
     sig = [("vel", _REQ), ("pressure", _REQ), ("flags", _REQ)] + [x for x in _PRESSURE_SIG if x[0] != "obvel"]
     a = _parse(sig, args, kwargs)
     _opts(dict(a, obvel=None), a["vel"].parent)
-    check(lib().mfx_correct_velocity(_h(a["vel"]), _h(a["pressure"]), _h(a["flags"])))
+    b = _bnd(dict(a, obvel=None))
+    if b is None:
+        check(lib().mfx_correct_velocity(_h(a["vel"]), _h(a["pressure"]), _h(a["flags"])))
+    else:
+        check(lib().mfx_correct_velocity_bnd(_h(a["vel"]), _h(a["pressure"]), _h(a["flags"]), C.byref(b)))
 
 
 @_plugin
