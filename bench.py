@@ -566,9 +566,10 @@ def bench_configs(manta):
 
 def bench_serving(n_sessions=24, n_req=240, rate=200.0):
     """Poisson stream of plume-step / pressure-solve requests on resident
-    sessions (64^2 .. 192^3 cells, log-uniform): served (continuous batching,
-    up to four ticks in flight on worker threads) vs the plain path one request at
-    a time; in real time at `rate` req/s
+    sessions (64^2 .. 192^3 cells, log-uniform), both through the library's
+    serving engine: served (continuous batching, up to four ticks in flight on
+    worker threads) vs the plain path one request at a time; in real time at
+    `rate` req/s
     (latency) and with every request queued at t=0 (throughput).  In real
     time each request's slowdown is its latency over the isolated latency of
     the same request (same session and kind, plain path, nothing else
@@ -578,15 +579,15 @@ def bench_serving(n_sessions=24, n_req=240, rate=200.0):
     for sess in sessions:  # warm: multigrid hierarchies, pools
         sess.run_single(serve.REQ_STEP)
     trace = serve.make_trace(n_req, n_sessions, rate=rate)
-    # isolated latency of every (session, kind): median of three plain runs
+    # isolated latency of every (session, kind): median of three runs of the
+    # request alone through the engine's plain path (nothing else running)
     iso = {}
     for i, sess in enumerate(sessions):
         for kind in (serve.REQ_STEP, serve.REQ_SOLVE):
             ts = []
             for _ in range(3):
-                t0 = time.perf_counter()
-                sess.run_single(kind)
-                ts.append(time.perf_counter() - t0)
+                lat, _ = serve.serve_serial(sessions, [serve.Request(0, 0.0, i, kind)], realtime=False)
+                ts.append(lat[0])
             iso[(i, kind)] = float(np.median(ts))
     single = [iso[(i, serve.REQ_SOLVE)] for i in range(min(8, n_sessions))]
     out = {"sessions": n_sessions, "requests": n_req, "rate_req_per_s": rate,

@@ -267,6 +267,77 @@ int mfx_correct_velocity(mfx_grid* vel, const mfx_grid* pressure, const mfx_grid
 /* releaseMG (source/plugin/pressure.cpp:252-266); s == NULL releases all */
 int mfx_release_mg(mfx_scene* s);
 
+/* --------------------------------------------------------------- serving */
+/* A request loop over scenes that stay resident in HBM.  No counterpart in
+ * the reference: its scene scripts run one scene per process and call the
+ * plugins of one step in sequence (scenes/simpleplume.py, plume_2d.py); the
+ * step a request runs is that loop's body.  A session is one plume scene
+ * (its grids created with mfx_grid_create on one scene).  Request kinds:
+ *   MFX_REQ_STEP  density <- density*keep + srcval; advectSemiLagrange of
+ *                 density and vel (order 2); resetOutflow(real=density);
+ *                 setWallBcs; addBuoyancy(gravity); solvePressure;
+ *                 setWallBcs
+ *   MFX_REQ_SOLVE solvePressure of the session's current vel
+ * with the server's solve options.  Continuous batching: the dispatcher
+ * admits ready requests of idle sessions (one request per session at a time,
+ * arrival order) while slots and device memory allow (6 floats per cell of
+ * the session are counted against the free memory less hbmReserve, or
+ * against memoryLimit less what admitted requests hold when that is set);
+ * while no more are ready than there are free workers each becomes a tick of
+ * its own (the plain path: mfx_solve_pressure), otherwise they form one
+ * batched tick (their pre-solve phases, one mfx_solve_pressure_batch, their
+ * post phases).  Up to `workers` ticks run at once on their sessions'
+ * streams.  A session's results do not depend on what it was batched with. */
+enum { MFX_REQ_STEP = 0, MFX_REQ_SOLVE = 1 };
+typedef struct mfx_server mfx_server;
+typedef struct {
+	int maxBatch;              /* requests per batched tick (64) */
+	int workers;               /* ticks in flight at once, 1..16 (1) */
+	size_t hbmReserve;         /* free device memory kept unadmitted (2 GiB) */
+	size_t memoryLimit;        /* > 0: admit against this budget instead of the device's free memory */
+	mfx_pressure_opts solve;   /* options of every projection (cgAccuracy 1e-4, PcMGStatic) */
+} mfx_serve_opts;
+typedef struct {
+	mfx_grid* flags;           /* FlagGrid */
+	mfx_grid* vel;             /* MACGrid */
+	mfx_grid* pressure;        /* RealGrid */
+	mfx_grid* density;         /* RealGrid; density, keep and srcval may be NULL for solve-only sessions */
+	const mfx_grid* keep;      /* RealGrid: 1 outside the source, 0 inside */
+	const mfx_grid* srcval;    /* RealGrid: source values inside the source, 0 outside */
+	float gravity[3];
+} mfx_serve_session;
+typedef struct {
+	long long tag;             /* the caller's id of the request */
+	int session, kind;
+	int status;                /* 0 done; -1 failed (error holds the message) */
+	int batch;                 /* requests in the tick that served it */
+	long long tick;            /* serial number of that tick */
+	double submitted, done;    /* seconds on the server's clock (mfx_serve_time) */
+	int iterations;            /* CG iterations of its solve */
+	int converged;             /* 1 if its solve met the stopping rule */
+	double relResidual;        /* ||b - A x||_2 / ||b||_2 of its solve */
+	char error[200];
+} mfx_serve_result;
+void mfx_serve_opts_default(mfx_serve_opts* o);
+int mfx_serve_create(const mfx_serve_opts* opts, mfx_server** out);
+/* registers a session (its grids must outlive the server); *id = its index */
+int mfx_serve_add_session(mfx_server* sv, const mfx_serve_session* session, int* id);
+/* queues a request; returns at once */
+int mfx_serve_submit(mfx_server* sv, int session, int kind, long long tag);
+/* queues count requests that arrive at the same instant (all checked first;
+ * none is queued if one is invalid) */
+int mfx_serve_submit_many(mfx_server* sv, int count, const int* sessions, const int* kinds, const long long* tags);
+/* moves up to maxOut finished requests (completion order) into out; waits
+ * up to timeoutSec for the first one while requests are outstanding */
+int mfx_serve_poll(mfx_server* sv, mfx_serve_result* out, int maxOut, int* count, double timeoutSec);
+/* waits until every submitted request has finished (error after timeoutSec) */
+int mfx_serve_drain(mfx_server* sv, double timeoutSec);
+/* seconds since the server was created (the clock of submitted/done) */
+int mfx_serve_time(const mfx_server* sv, double* seconds);
+/* waits for the ticks in flight, drops requests not yet admitted, joins the
+ * server's threads; the sessions' grids are not freed */
+int mfx_serve_destroy(mfx_server* sv);
+
 /* ------------------------------------------------------------- profiling */
 /* optional HIP-event timing of the main kernels on their launching stream
  * (names: "cg_matvec", "cg_update", "mg_l0_smooth", "mg_l0_residual",

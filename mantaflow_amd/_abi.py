@@ -33,6 +33,24 @@ class PressureBnd(C.Structure):
                 ("obvel", C.c_void_p), ("curv", C.c_void_p), ("gfClamp", C.c_float), ("surfTens", C.c_float)]
 
 
+class ServeOpts(C.Structure):
+    _fields_ = [("maxBatch", C.c_int), ("workers", C.c_int), ("hbmReserve", C.c_size_t),
+                ("memoryLimit", C.c_size_t), ("solve", PressureOpts)]
+
+
+class ServeSession(C.Structure):
+    _fields_ = [("flags", C.c_void_p), ("vel", C.c_void_p), ("pressure", C.c_void_p), ("density", C.c_void_p),
+                ("keep", C.c_void_p), ("srcval", C.c_void_p), ("gravity", C.c_float * 3)]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [("tag", C.c_longlong), ("session", C.c_int), ("kind", C.c_int), ("status", C.c_int),
+                ("batch", C.c_int), ("tick", C.c_longlong), ("submitted", C.c_double), ("done", C.c_double),
+                ("iterations", C.c_int), ("converged", C.c_int), ("relResidual", C.c_double), ("error", C.c_char * 200)]
+
+
+REQ_STEP, REQ_SOLVE = 0, 1
+
 # every exported symbol of include/mfx.h with its ctypes signature
 _P = C.c_void_p
 _I = C.c_int
@@ -100,6 +118,15 @@ SIGNATURES = {
                                       C.POINTER(PressureOpts), C.POINTER(SolveStats)]),
     "mfx_correct_velocity": (_I, [_P, _P, _P]),
     "mfx_release_mg": (_I, [_P]),
+    "mfx_serve_opts_default": (None, [C.POINTER(ServeOpts)]),
+    "mfx_serve_create": (_I, [C.POINTER(ServeOpts), C.POINTER(_P)]),
+    "mfx_serve_add_session": (_I, [_P, C.POINTER(ServeSession), C.POINTER(_I)]),
+    "mfx_serve_submit": (_I, [_P, _I, _I, C.c_longlong]),
+    "mfx_serve_submit_many": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(C.c_longlong)]),
+    "mfx_serve_poll": (_I, [_P, C.POINTER(ServeResult), _I, C.POINTER(_I), C.c_double]),
+    "mfx_serve_drain": (_I, [_P, C.c_double]),
+    "mfx_serve_time": (_I, [_P, C.POINTER(C.c_double)]),
+    "mfx_serve_destroy": (_I, [_P]),
     "mfx_kernel_timing": (_I, [_I]),
     "mfx_kernel_timing_reset": (_I, []),
     "mfx_kernel_timing_get": (_I, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),

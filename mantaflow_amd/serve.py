@@ -1,5 +1,6 @@
 """Serving loop for smoke-step and pressure-solve requests on scenes that stay
-resident in HBM.
+resident in HBM: the Python face of the library's serving engine
+(mfx_serve_* in include/mfx.h).
 
 A request names a session (one plume scene: flags, MAC velocity, density,
 pressure, source) and a kind: "step" = one plume step (density source,
@@ -8,33 +9,32 @@ addBuoyancy, solvePressure, setWallBcs, as scenes/plume_2d.py and
 scenes/simpleplume.py do), "solve" = the pressure projection of the
 session's current velocity.
 
-Server (continuous batching): every engine tick admits the requests that
-have arrived, whose session is idle, while batch slots and free HBM allow;
-their pre-solve phases run on their own HIP streams (so they overlap on the
-device), then all their pressure systems are solved by one batched solve
-(mfx_solve_pressure_batch: one launch per CG/multigrid phase for every
-admitted problem), then the post phases.  Requests arriving during a tick
-are admitted by the next one.  A session's results do not depend on what it
-was batched with (the batched solve is bit-identical to a single solve).
+Server (continuous batching, native threads): the engine's dispatcher admits
+the requests that have arrived, whose session is idle, while batch slots and
+free HBM allow, and hands them to worker threads (one tick in flight per
+worker): one request per worker while no more are ready than there are free
+workers (the plain path, so a small request runs beside a large one instead
+of queueing behind it), else all of them as one batched tick - their
+pre-solve phases on their own HIP streams, one mfx_solve_pressure_batch
+(one launch per CG/multigrid phase for every admitted problem), then the
+post phases.  Requests arriving during a tick are admitted as workers free
+up.  A session's results do not depend on what it was batched with (the
+batched solve is bit-identical to a single solve).
 
-With workers > 1 the ticks run on worker threads (one per hardware queue of
-the GPU; the library call releases the interpreter lock), so a small request
-no longer waits for a large one to finish: the dispatcher hands the ready
-requests of idle sessions to free workers, one request per worker while
-there are no more than free workers, else all of them as one batched tick.
-
-serve_serial replays the same trace on the plain path: one request at a
-time, each with its own solvePressure.
+serve_serial replays the same trace on the plain path: the same engine with
+one worker and one request per tick, each with its own solvePressure.
+PlumeScene.run_single is the same plain path driven from Python, one plugin
+call at a time, as a scene script would.
 """
 import collections
 import ctypes as C
+import itertools
 import math
-import queue
-import threading
 import time
 
 import numpy as np
 
+from . import _abi
 from . import manta as m
 from ._abi import check, lib
 
@@ -100,6 +100,14 @@ class PlumeScene:
 
     def problem(self):
         return (self.vel, self.pressure, self.flags)
+
+    def session_struct(self):
+        """The session as the serving engine takes it (mfx_serve_session)."""
+        g = _abi.ServeSession()
+        g.flags, g.vel, g.pressure = self.flags._h.value, self.vel._h.value, self.pressure._h.value
+        g.density, g.keep, g.srcval = self.density._h.value, self.keep._h.value, self.srcval._h.value
+        g.gravity[:] = [self.gravity.x, self.gravity.y, self.gravity.z]
+        return g
 
     def run_single(self, kind):
         """The plain path for one request."""
@@ -186,173 +194,94 @@ def _percentiles(lat):
 
 
 class Server:
-    """Continuous-batching engine over resident sessions (workers: ticks in
-    flight at once, each on its own thread)."""
+    """Continuous-batching engine over resident sessions (the library's
+    mfx_serve_*): `workers` ticks in flight at once, batched ticks of at most
+    `max_batch` requests, admission by free device memory less
+    `hbm_reserve` (or by `memory_limit` bytes when that is set)."""
 
-    def __init__(self, sessions, max_batch=64, hbm_reserve=2 << 30, workers=1):
+    def __init__(self, sessions, max_batch=64, hbm_reserve=2 << 30, workers=1, memory_limit=0):
         self.sessions = sessions
         self.max_batch = max_batch
         self.hbm_reserve = hbm_reserve
         self.workers = max(1, int(workers))
+        self.memory_limit = int(memory_limit)
         self.ticks = 0
         self.batch_sizes = []
-        self._lock = threading.Lock()
+        self.results = []
 
-    def _admit(self, ready, free_bytes):
-        chosen, used, budget = [], set(), free_bytes - self.hbm_reserve
-        for r in ready:
-            if len(chosen) >= self.max_batch:
-                break
-            if r.session in used:
-                continue  # one request per session per tick, in arrival order
-            need = 6 * 4 * self.sessions[r.session].cells  # rhs and CG vectors of the solve
-            if need > budget:
-                continue
-            budget -= need
-            used.add(r.session)
-            chosen.append(r)
-        return chosen
-
-    def _count(self, n):
-        with self._lock:
-            self.ticks += 1
-            self.batch_sizes.append(n)
-
-    def tick(self, batch):
-        if len(batch) == 1:
-            # nothing to share: the plain path has the lowest latency
-            self.sessions[batch[0].session].run_single(batch[0].kind)
-            self._count(1)
-            return
-        for r in batch:
-            if r.kind == REQ_STEP:
-                self.sessions[r.session].pre()
-        m.solvePressureBatch([self.sessions[r.session].problem() for r in batch], cgAccuracy=1e-4,
-                             preconditioner=self.sessions[batch[0].session].pc)
-        for r in batch:
-            self.sessions[r.session].post(r.kind)
-        for r in batch:
-            self.sessions[r.session].sync()
-        self._count(len(batch))
+    def _options(self):
+        o = _abi.ServeOpts()
+        lib().mfx_serve_opts_default(C.byref(o))
+        o.maxBatch = int(self.max_batch)
+        o.workers = self.workers
+        o.hbmReserve = int(self.hbm_reserve)
+        o.memoryLimit = self.memory_limit
+        first = self.sessions[0]
+        o.solve.cgAccuracy = 1e-4
+        o.solve.preconditioner = int(first.pc)
+        o.solve.relTol = float(getattr(first.s, "mfxRelTol", 0.0) or 0.0)
+        return o
 
     def run(self, trace, realtime=True):
-        """Replay `trace`; returns per-request latency (s) and the makespan."""
-        if self.workers > 1:
-            return self._run_workers(trace, realtime)
-        pending = collections.deque(sorted(trace, key=lambda r: r.arrival))
-        ready = []
-        lat = {}
-        t0 = time.perf_counter()
-        while pending or ready:
-            now = time.perf_counter() - t0
-            while pending and (not realtime or pending[0].arrival <= now):
-                ready.append(pending.popleft())
-            if not ready:
-                time.sleep(max(0.0, pending[0].arrival - now))
-                continue
-            free, _ = device_memory()
-            batch = self._admit(ready, free)
-            if not batch:
-                raise RuntimeError("serve: no request fits in free device memory")
-            self.tick(batch)
-            done = time.perf_counter() - t0
-            ids = {r.rid for r in batch}
-            for r in batch:
-                lat[r.rid] = done - (r.arrival if realtime else 0.0)
-            ready = [r for r in ready if r.rid not in ids]
-        return lat, time.perf_counter() - t0
-
-    def _split(self, batch, free_workers):
-        """Few ready requests (no more than the free workers): one tick each,
-        in flight at once (latency); more: one batched tick (a batch shares
-        every launch, which is what raises throughput)."""
-        if len(batch) <= free_workers:
-            return [[r] for r in batch]
-        return [batch]
-
-    def _run_workers(self, trace, realtime):
-        pending = collections.deque(sorted(trace, key=lambda r: r.arrival))
-        ready = []
-        busy = set()
-        lat = {}
-        work_q, done_q = queue.Queue(), queue.Queue()
-        t0 = time.perf_counter()
-
-        def worker():
-            while True:
-                batch = work_q.get()
-                if batch is None:
-                    return
-                err = None
-                try:
-                    self.tick(batch)
-                except BaseException as e:  # reported by the dispatcher
-                    err = e
-                done_q.put((batch, time.perf_counter() - t0, err))
-
-        threads = [threading.Thread(target=worker, daemon=True) for _ in range(self.workers)]
-        for th in threads:
-            th.start()
-        in_flight = 0
+        """Replay `trace` (arrival times in seconds from the start; all at
+        once when realtime is False); returns per-request latency (s, from
+        the request's arrival to its completion) and the makespan."""
+        L = lib()
+        h = C.c_void_p()
+        check(L.mfx_serve_create(C.byref(self._options()), C.byref(h)))
+        res = []
         try:
-            while pending or ready or in_flight:
-                now = time.perf_counter() - t0
-                while pending and (not realtime or pending[0].arrival <= now):
-                    ready.append(pending.popleft())
-                free_workers = self.workers - in_flight
-                if free_workers > 0:
-                    cand = [r for r in ready if r.session not in busy]
-                    batch = []
-                    if cand:
-                        free, _ = device_memory()
-                        batch = self._admit(cand, free)
-                        if not batch and in_flight == 0:
-                            raise RuntimeError("serve: no request fits in free device memory")
-                    if batch:
-                        for group in self._split(batch, free_workers):
-                            busy.update(r.session for r in group)
-                            ids = {r.rid for r in group}
-                            ready = [r for r in ready if r.rid not in ids]
-                            work_q.put(group)
-                            in_flight += 1
-                        continue
-                # wait for a completion, or for the next arrival
-                wait = None
-                if realtime and pending:
-                    wait = max(0.0, pending[0].arrival - (time.perf_counter() - t0))
-                if in_flight == 0:
-                    time.sleep(wait or 0.0)
-                    continue
-                try:
-                    group, done, err = done_q.get(timeout=wait)
-                except queue.Empty:
-                    continue
-                in_flight -= 1
-                if err is not None:
-                    raise err
-                for r in group:
-                    lat[r.rid] = done - (r.arrival if realtime else 0.0)
-                    busy.discard(r.session)
+            for sess in self.sessions:
+                check(L.mfx_serve_add_session(h, C.byref(sess.session_struct()), None))
+            t = C.c_double()
+            check(L.mfx_serve_time(h, C.byref(t)))
+            start = t.value
+            w0 = time.perf_counter()
+            order = sorted(trace, key=lambda r: r.arrival)
+            # requests of one arrival instant (all of them when not in real
+            # time) are queued together
+            groups = [order] if not realtime else [list(g) for _, g in itertools.groupby(order, lambda r: r.arrival)]
+            for g in groups:
+                if realtime:
+                    wait = g[0].arrival - (time.perf_counter() - w0)
+                    if wait > 0:
+                        time.sleep(wait)
+                n = len(g)
+                sess = (C.c_int * n)(*[r.session for r in g])
+                kinds = (C.c_int * n)(*[_abi.REQ_STEP if r.kind == REQ_STEP else _abi.REQ_SOLVE for r in g])
+                tags = (C.c_longlong * n)(*[r.rid for r in g])
+                check(L.mfx_serve_submit_many(h, n, sess, kinds, tags))
+            check(L.mfx_serve_drain(h, 3600.0))
+            buf = (_abi.ServeResult * 256)()
+            n = C.c_int()
+            while True:
+                check(L.mfx_serve_poll(h, buf, len(buf), C.byref(n), 0.0))
+                if n.value == 0:
+                    break
+                res.extend(_abi.ServeResult.from_buffer_copy(buf[i]) for i in range(n.value))
         finally:
-            for _ in threads:
-                work_q.put(None)
-            for th in threads:
-                th.join()
-        return lat, time.perf_counter() - t0
+            check(L.mfx_serve_destroy(h))
+        self.results = res
+        bad = [x for x in res if x.status != 0]
+        if bad:
+            raise RuntimeError(bad[0].error.decode(errors="replace"))
+        ticks = {}
+        for x in res:
+            ticks[x.tick] = x.batch
+            if x.kind == _abi.REQ_STEP:
+                self.sessions[x.session].steps += 1
+        self.ticks += len(ticks)
+        self.batch_sizes.extend(ticks[k] for k in sorted(ticks))
+        arrival = {r.rid: r.arrival for r in trace}
+        lat = {x.tag: x.done - start - (arrival[x.tag] if realtime else 0.0) for x in res}
+        makespan = max((x.done for x in res), default=start) - start
+        return lat, makespan
 
 
 def serve_serial(sessions, trace, realtime=True):
-    """Plain path: requests one at a time in arrival order."""
-    lat = {}
-    t0 = time.perf_counter()
-    for r in sorted(trace, key=lambda r: r.arrival):
-        if realtime:
-            now = time.perf_counter() - t0
-            if r.arrival > now:
-                time.sleep(r.arrival - now)
-        sessions[r.session].run_single(r.kind)
-        lat[r.rid] = time.perf_counter() - t0 - (r.arrival if realtime else 0.0)
-    return lat, time.perf_counter() - t0
+    """Plain path: requests one at a time in arrival order, each with its
+    own solvePressure (the engine with one worker and one request per tick)."""
+    return Server(sessions, max_batch=1, workers=1).run(trace, realtime)
 
 
 def summarize(lat, makespan):

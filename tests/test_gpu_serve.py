@@ -33,13 +33,48 @@ def test_served_equals_serial():
 
 
 def test_admission_respects_memory():
-    sessions = [serve.PlumeScene((16, 16, 16), seed=1), serve.PlumeScene((16, 16, 16), seed=2)]
-    srv = serve.Server(sessions, max_batch=8, hbm_reserve=0)
-    ready = [serve.Request(0, 0.0, 0, serve.REQ_SOLVE), serve.Request(1, 0.0, 1, serve.REQ_SOLVE),
+    """One request per session per tick, in arrival order; nothing admitted
+    beyond the memory budget; a request larger than the whole budget fails
+    with the engine's message."""
+    need = 6 * 4 * 16 ** 3
+    trace = [serve.Request(0, 0.0, 0, serve.REQ_SOLVE), serve.Request(1, 0.0, 1, serve.REQ_SOLVE),
              serve.Request(2, 0.0, 0, serve.REQ_STEP)]
-    # one request per session per tick, and nothing beyond the memory budget
-    assert [r.rid for r in srv._admit(ready, 1 << 40)] == [0, 1]
-    assert [r.rid for r in srv._admit(ready, 6 * 4 * 16 ** 3)] == [0]
+    sessions = [serve.PlumeScene((16, 16, 16), seed=1), serve.PlumeScene((16, 16, 16), seed=2)]
+    srv = serve.Server(sessions, max_batch=8, workers=1, memory_limit=1 << 40)
+    srv.run(trace, realtime=False)
+    ticks = {}
+    for x in srv.results:
+        ticks.setdefault(x.tick, []).append(x.tag)
+    assert sorted(sorted(v) for v in ticks.values()) == [[0, 1], [2]]
+    assert sessions[0].steps == 1 and sessions[1].steps == 0
+    # budget for one session's solve at a time: every tick holds one request
+    srv = serve.Server(sessions, max_batch=8, workers=4, memory_limit=need)
+    srv.run(trace, realtime=False)
+    assert srv.batch_sizes == [1, 1, 1]
+    starts = sorted((x.done, x.tag) for x in srv.results)
+    assert [t for _, t in starts][-1] == 2  # the session's step after its solve
+    with pytest.raises(RuntimeError, match="no request fits"):
+        serve.Server(sessions, max_batch=8, workers=2, memory_limit=need - 1).run(trace[:1], realtime=False)
+
+
+def test_engine_plain_path_equals_plugin_calls():
+    """The engine's plain path (one request per tick) leaves a session in
+    the same state, bit for bit, as the same requests run from Python one
+    plugin call at a time (PlumeScene.run_single, the scene script's loop)."""
+    shapes = [(20, 20, 20), (40, 40, 1)]
+    a = [serve.PlumeScene(sh, seed=50 + i) for i, sh in enumerate(shapes)]
+    b = [serve.PlumeScene(sh, seed=50 + i) for i, sh in enumerate(shapes)]
+    trace = serve.make_trace(12, len(shapes), rate=1e9, seed=11)
+    for r in sorted(trace, key=lambda r: r.arrival):
+        a[r.session].run_single(r.kind)
+    srv = serve.Server(b, max_batch=1, workers=1)
+    lat, _ = srv.run(trace, realtime=False)
+    assert sorted(lat) == sorted(r.rid for r in trace) and srv.batch_sizes == [1] * len(trace)
+    assert all(x.iterations > 0 and x.relResidual < 1e-4 for x in srv.results)
+    for i, (x, y) in enumerate(zip(a, b)):
+        assert x.steps == y.steps
+        for u, v, name in zip(state(x), state(y), ("density", "vel", "pressure")):
+            np.testing.assert_array_equal(u, v, err_msg="session %d %s" % (i, name))
 
 
 def test_threaded_served_equals_serial():

@@ -86,23 +86,15 @@ def test_batched_problems_full_size(manta, oracle, oracle64):
     print("64 batched problems: worst FP64 relative residual (CPU recomputation) %.3g" % worst)
 
 
-def test_served_sessions_full_size(manta, oracle64, monkeypatch):
+def test_served_sessions_full_size(manta, oracle64):
     from mantaflow_amd import serve
     shapes = serve.make_sessions(24)
     assert max(a * b * c for a, b, c in shapes) > 100 ** 3
     a = [serve.PlumeScene(sh, seed=20 + i) for i, sh in enumerate(shapes)]
     b = [serve.PlumeScene(sh, seed=20 + i) for i, sh in enumerate(shapes)]
     trace = serve.make_trace(72, len(shapes), rate=1e9, seed=8)
-    stats = {}
-    orig_post = serve.PlumeScene.post
-
-    def post(self, kind):
-        st = manta.lastSolveStats(self.s)
-        stats.setdefault(id(self), []).append((st["converged"], st["relResidual"]))
-        orig_post(self, kind)
-
-    monkeypatch.setattr(serve.PlumeScene, "post", post)
-    serve.serve_serial(a, trace, realtime=False)
+    ser = serve.Server(a, max_batch=1, workers=1)
+    ser.run(trace, realtime=False)
     srv = serve.Server(b, workers=4)
     lat, _ = srv.run(trace, realtime=False)
     assert sorted(lat) == sorted(r.rid for r in trace) and max(srv.batch_sizes) > 1
@@ -112,11 +104,12 @@ def test_served_sessions_full_size(manta, oracle64, monkeypatch):
         for g in ("density", "vel", "pressure"):
             np.testing.assert_array_equal(getattr(x, g).toNumpy(), getattr(y, g).toNumpy(),
                                           err_msg="session %d %s" % (i, g))
-        for s in (x, y):
-            for conv, rel in stats.get(id(s), []):
-                assert conv and rel <= TOL, (i, rel)
-                n_req += 1
-        if id(y) in stats:
+        for res in (ser.results, srv.results):
+            for r in res:
+                if r.session == i:
+                    assert r.status == 0 and r.converged and r.relResidual <= TOL, (i, r.relResidual)
+                    n_req += 1
+        if any(r.session == i for r in srv.results):
             # the last projection, checked on the CPU: undo the correction to
             # recover its right-hand side, b = rhs(u) + A p (correctVelocity's
             # identity, away from cells whose upper face borders an outflow cell)

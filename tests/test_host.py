@@ -80,3 +80,41 @@ def test_errors_surface_as_messages_without_a_device():
     assert rc == -1 and L.mfx_last_error()
     assert L.mfx_scene_create(16, 16, 2, 2, C.byref(h)) == -1
     assert b"2D solver" in L.mfx_last_error()
+
+
+def test_serve_engine_host_logic():
+    """The serving engine's bookkeeping without a device: option checks,
+    requests for unknown sessions refused, poll/drain with nothing
+    outstanding, the clock, and shutdown."""
+    from mantaflow_amd import _abi
+    L = _abi.lib()
+    o = _abi.ServeOpts()
+    L.mfx_serve_opts_default(C.byref(o))
+    assert (o.maxBatch, o.workers, o.hbmReserve, o.memoryLimit) == (64, 1, 2 << 30, 0)
+    assert abs(o.solve.cgAccuracy - 1e-4) < 1e-9 and o.solve.preconditioner == 3
+    h = C.c_void_p()
+    bad = _abi.ServeOpts.from_buffer_copy(o)
+    bad.workers = 0
+    assert L.mfx_serve_create(C.byref(bad), C.byref(h)) == -1
+    assert b"workers" in L.mfx_last_error()
+    o.workers = 3
+    assert L.mfx_serve_create(C.byref(o), C.byref(h)) == 0
+    try:
+        assert L.mfx_serve_submit(h, 0, _abi.REQ_SOLVE, 7) == -1
+        assert b"unknown session" in L.mfx_last_error()
+        assert L.mfx_serve_submit(h, 0, 5, 7) == -1
+        assert b"kind" in L.mfx_last_error()
+        ss, kk, tt = (C.c_int * 2)(0, 0), (C.c_int * 2)(1, 1), (C.c_longlong * 2)(1, 2)
+        assert L.mfx_serve_submit_many(h, 2, ss, kk, tt) == -1  # nothing queued
+        buf = (_abi.ServeResult * 4)()
+        n = C.c_int(-1)
+        assert L.mfx_serve_poll(h, buf, 4, C.byref(n), 0.01) == 0 and n.value == 0
+        assert L.mfx_serve_drain(h, 0.01) == 0
+        t1, t2 = C.c_double(), C.c_double()
+        assert L.mfx_serve_time(h, C.byref(t1)) == 0
+        assert L.mfx_serve_time(h, C.byref(t2)) == 0 and 0.0 <= t1.value <= t2.value
+        s = _abi.ServeSession()
+        assert L.mfx_serve_add_session(h, C.byref(s), None) == -1
+        assert b"required" in L.mfx_last_error()
+    finally:
+        assert L.mfx_serve_destroy(h) == 0
