@@ -165,9 +165,11 @@ def plume_step_error(manta, st, threads=16):
     return out
 
 
-def cpu_baseline(n_sample=128, steps=3, threads=None, warm=10):
+def cpu_baseline(n_sample=256, steps=3, threads=None, warm=2):
     """The CPU oracle (C restatement of the reference's plugins, OpenMP over
-    the data-parallel loops) on the same plume step at a smaller grid."""
+    the data-parallel loops) on the same plume step: by default the 256^3
+    workload itself, timed over steps 3-5 of the plume (a step takes seconds
+    there; --configs times the 128^3 plume over all 100 steps)."""
     from oracle import Oracle, PC_MGSTATIC, TYPE_MAC, TYPE_REAL
     o = Oracle()
     th = threads or min(16, os.cpu_count() or 1)
@@ -623,8 +625,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--res", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-res", type=int, default=128)
-    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-res", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-problem batched-solve leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the served request-stream leg")
     ap.add_argument("--configs", action="store_true",
