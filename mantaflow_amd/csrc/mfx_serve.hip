@@ -36,6 +36,10 @@ struct Request {
 	long long tag;
 	int session, kind;
 	double submitted;
+	// the session's grids, copied at dispatch (under the lock) so that a
+	// running tick never reads the session table, which add_session may grow
+	mfx_serve_session g;
+	Scene* scene;
 };
 
 struct Session {
@@ -118,12 +122,12 @@ void run_tick(mfx_server* sv, const std::vector<Request>& batch, std::vector<mfx
 			copy_msg(res[i].error, sizeof(res[i].error), m);
 		}
 	};
-	auto sess = [&](size_t i) -> const mfx_serve_session& { return sv->sessions[batch[i].session].g; };
+	auto sess = [&](size_t i) -> const mfx_serve_session& { return batch[i].g; };
 	if (n == 1) {
 		const mfx_serve_session& s = sess(0);
 		if ((batch[0].kind == MFX_REQ_STEP && step_pre(s)) ||
 		    mfx_solve_pressure(s.vel, s.pressure, s.flags, &sv->opts.solve, &st[0], nullptr) ||
-		    step_post(s, batch[0].kind) || mfx_scene_sync(static_cast<mfx_scene*>(sv->sessions[batch[0].session].scene))) {
+		    step_post(s, batch[0].kind) || mfx_scene_sync(static_cast<mfx_scene*>(batch[0].scene))) {
 			fail(0, 1);
 			return;
 		}
@@ -150,7 +154,7 @@ void run_tick(mfx_server* sv, const std::vector<Request>& batch, std::vector<mfx
 				return;
 			}
 		for (size_t i = 0; i < n; ++i)
-			if (mfx_scene_sync(static_cast<mfx_scene*>(sv->sessions[batch[i].session].scene))) {
+			if (mfx_scene_sync(static_cast<mfx_scene*>(batch[i].scene))) {
 				fail(0, n);
 				return;
 			}
@@ -246,10 +250,12 @@ void dispatcher_loop(mfx_server* sv) {
 				std::vector<Request> batch;
 				for (size_t i : pick) batch.push_back(sv->ready[i]);
 				for (size_t k = pick.size(); k-- > 0;) sv->ready.erase(sv->ready.begin() + (long)pick[k]);
-				for (const Request& r : batch) {
+				for (Request& r : batch) {
 					Session& s = sv->sessions[r.session];
 					s.busy = true;
 					sv->heldBytes += s.need;
+					r.g = s.g;
+					r.scene = s.scene;
 				}
 				if ((int)batch.size() <= freeWorkers) {
 					for (const Request& r : batch) {
@@ -375,7 +381,7 @@ int mfx_serve_submit_many(mfx_server* sv, int count, const int* sessions, const 
 		          "mfx_serve_submit: step request on a solve-only session");
 	}
 	const double t = sv->now();
-	for (int i = 0; i < count; ++i) sv->ready.push_back(Request{tags[i], sessions[i], kinds[i], t});
+	for (int i = 0; i < count; ++i) sv->ready.push_back(Request{tags[i], sessions[i], kinds[i], t, {}, nullptr});
 	sv->outstanding += count;
 	sv->cvDispatch.notify_one();
 	SV_END
