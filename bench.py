@@ -214,8 +214,9 @@ def cpu_baseline(n_sample=256, steps=3, threads=None, warm=2):
 def bench_batched(manta, n2d=32, n3d=32):
     """64 uneven pressure problems: one batched solve vs the problems solved
     one after another on the plain path (same inputs, rel. residual 1e-4).
-    The batch is solved twice untimed (queued launches, then the capture of
-    its two-iteration graph) and the median of three graph replays is
+    The batch is solved ten times untimed (the plane-chunk autotuner's eight
+    trials, then the chosen policy's queued launches and the capture of its
+    two-iteration graph) and the median of three graph replays is
     reported."""
     from mantaflow_amd import workloads
     probs = workloads.batched_problems(n2d, n3d)
@@ -223,7 +224,7 @@ def bench_batched(manta, n2d=32, n3d=32):
     for p in probs:  # build the static multigrid hierarchies
         manta.solvePressure(flags=p.flags, vel=p.vel, pressure=p.pressure, **kw)
     times = []
-    for rep in range(5):
+    for rep in range(13):
         for p in probs:
             p.reset()
             p.s.sync()
@@ -231,7 +232,7 @@ def bench_batched(manta, n2d=32, n3d=32):
         manta.solvePressureBatch([p.problem() for p in probs], **kw)
         times.append(time.perf_counter() - t0)
     tfirst = times[0]
-    tb = float(np.median(times[2:]))
+    tb = float(np.median(times[-3:]))
     its = [manta.lastSolveStats(p.s)["iterations"] for p in probs]
     rel = max(manta.lastSolveStats(p.s)["relResidual"] for p in probs)
     for p in probs:
@@ -245,6 +246,7 @@ def bench_batched(manta, n2d=32, n3d=32):
     cells = sum(p.cells for p in probs)
     return {"problems": len(probs), "cells": cells, "batched_ms": round(tb * 1e3, 3),
             "batched_first_ms": round(tfirst * 1e3, 3),
+            "batched_trials_ms": [round(t * 1e3, 3) for t in times[:8]],
             "serial_ms": round(ts * 1e3, 3), "batched_solves_per_s": round(len(probs) / tb, 1),
             "serial_solves_per_s": round(len(probs) / ts, 1), "speedup": round(ts / tb, 2),
             "iterations_max": int(max(its)), "rel_residual_max": rel}

@@ -64,7 +64,7 @@ public:
 	// level-0 plane chunk of single solves, else the batched heuristic
 	struct Steps;
 	Steps steps(const Dims& d, const uint8_t* code, const float* r, float* z, const CgState* st, double* part,
-	            bool tuned) const;
+	            bool tuned, int forceChunk = 0) const;
 	bool dim3d() const { return is3d; }
 	int levels() const { return (int)L.size(); }
 	int precision() const { return prec; }

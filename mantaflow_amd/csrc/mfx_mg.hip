@@ -1681,11 +1681,11 @@ void run_fill_many(hipStream_t s, unsigned nb, WorkList items, const FillP* p) {
 // Parameters of every V-cycle step of this hierarchy (shared by the single
 // launch sequence below and by batched work lists).
 MgHierarchy::Steps MgHierarchy::steps(const Dims& d, const uint8_t* code, const float* r, float* z, const CgState* st,
-                                      double* part, bool tuned) const {
+                                      double* part, bool tuned, int forceChunk) const {
 	Steps S_;
 	const int nl = (int)L.size();
 	S_.nl = nl;
-	S_.l0Tiles = l0_tiles(d, &S_.kchunk, tuned);
+	S_.l0Tiles = l0_tiles(d, &S_.kchunk, tuned, forceChunk);
 	S_.nPart = is3d ? zm_dot_parts(d, kZmTY) : S_.l0Tiles;
 	const void* noA = nullptr;
 	if (nl == 1) {

@@ -1614,8 +1614,8 @@ struct CgRun {
 // planes on the problem's own matrix code and pooled scratch grids under a
 // scratch CgState; the fastest chunk of each is kept for the shape (the
 // up-sweep shares the down-sweep's).  Single solves are latency-bound, so
-// the winners are short chunks for small grids; batched solves keep the
-// heuristic chunk (fewer halo planes per cell, throughput-bound).  The dot
+// the winners are short chunks for small grids; batched solves pick their
+// own chunk policy per batch layout (batch_policy below).  The dot
 // partition of the z-marching kernels makes the result independent of the
 // chunk, so single and batched solves still agree bit for bit.
 static void autotune_zmarch(Scene* s, const uint8_t* code) {
@@ -2220,6 +2220,73 @@ struct BatchSolve {
 
 }  // namespace
 
+// Online autotuner of the batches' plane chunks.  A batch layout (precision
+// and the problems' shapes, in order) is solved with one chunk policy per
+// solve - the throughput heuristic, the single-solve autotuner's per-shape
+// winners, and the heuristic's chunks doubled and quadrupled (capped at the
+// grid depth) - twice each, interleaved, timing the iteration phase on the
+// host; from then on the layout uses the fastest.  The dot partition of the z-marching kernels
+// makes every policy give the same bits.  A layout seen once (a serving tick)
+// gets the heuristic, the first policy.
+constexpr int kBatchPolicies = 4, kBatchTrials = 2, kBatchTuneMax = 256;
+struct BatchTune {
+	double best[kBatchPolicies] = {1e30, 1e30, 1e30, 1e30};
+	int runs[kBatchPolicies] = {0, 0, 0, 0};
+	int chosen = -1;
+	unsigned long long used = 0;
+};
+static std::mutex g_btune_mu;
+static std::map<std::vector<int>, BatchTune> g_btune;
+static unsigned long long g_btune_clock = 0;
+
+static int batch_policy(const std::vector<int>& key) {
+	std::lock_guard<std::mutex> g(g_btune_mu);
+	auto it = g_btune.find(key);
+	if (it == g_btune.end()) {
+		if ((int)g_btune.size() >= kBatchTuneMax) {
+			// forget the least recently used layout
+			auto old = g_btune.begin();
+			for (auto j = g_btune.begin(); j != g_btune.end(); ++j)
+				if (j->second.used < old->second.used) old = j;
+			g_btune.erase(old);
+		}
+		it = g_btune.emplace(key, BatchTune{}).first;
+	}
+	BatchTune& t = it->second;
+	t.used = ++g_btune_clock;
+	if (t.chosen >= 0) return t.chosen;
+	int total = 0;
+	for (int p = 0; p < kBatchPolicies; p++) total += t.runs[p];
+	return total % kBatchPolicies;
+}
+static void batch_policy_done(const std::vector<int>& key, int policy, double seconds) {
+	std::lock_guard<std::mutex> g(g_btune_mu);
+	auto it = g_btune.find(key);
+	if (it == g_btune.end() || it->second.chosen >= 0) return;
+	BatchTune& t = it->second;
+	t.runs[policy] += 1;
+	t.best[policy] = std::min(t.best[policy], seconds);
+	for (int p = 0; p < kBatchPolicies; p++)
+		if (t.runs[p] < kBatchTrials) return;
+	int b = 0;
+	for (int p = 1; p < kBatchPolicies; p++)
+		if (t.best[p] < t.best[b]) b = p;
+	t.chosen = b;
+}
+// plane chunks of a 3D problem under a batch policy: tuned (per-shape
+// winners) and the forced chunk for the matvec and the level-0 kernels
+static void batch_chunks(const Dims& d, int policy, bool* tuned, int* forceMv, int* forceL0) {
+	*tuned = policy == 1;
+	*forceMv = *forceL0 = 0;
+	if (policy < 2 || !d.is3d) return;
+	int kmv = 0, kl0 = 0;
+	matvec_tiles(d, &kmv, false);
+	l0_tiles(d, &kl0, false);
+	const int cap = zm_chunk(d.nz), f = policy == 2 ? 2 : 4;
+	*forceMv = std::min(f * kmv, cap);
+	*forceL0 = std::min(f * kl0, cap);
+}
+
 static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>& pres,
                         const std::vector<Grid*>& flg, const mfx_pressure_opts& o, mfx_solve_stats* stats) {
 	const int P = (int)vel.size();
@@ -2255,13 +2322,24 @@ static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>&
 			B.down[d3] = Step<L0DownP>((size_t)P);
 			B.l0up[d3] = Step<L0UpP>((size_t)P);
 		}
+		std::vector<int> tkey = {o.precision, P};
+		for (int i = 0; i < P; i++) {
+			const Dims& d = R[(size_t)i].s->d;
+			tkey.push_back(d.nx);
+			tkey.push_back(d.ny);
+			tkey.push_back(d.nz);
+		}
+		const int policy = batch_policy(tkey);
 		int maxIt = 0;
 		for (int i = 0; i < P; i++) {
 			CgRun& r = R[(size_t)i];
 			const Dims& d = r.s->d;
 			maxIt = std::max(maxIt, r.maxIter);
+			bool tuned = false;
+			int fMv = 0, fL0 = 0;
+			batch_chunks(d, policy, &tuned, &fMv, &fL0);
 			int kb = 1;
-			const long long nbMv = matvec_tiles(d, &kb, false);
+			const long long nbMv = matvec_tiles(d, &kb, tuned, fMv);
 			for (int par = 0; par < 2; par++) {
 				B.mv[par].params[(size_t)i] = r.mvp(par == 1);
 				B.mv[par].params[(size_t)i].kchunk = kb;
@@ -2275,7 +2353,7 @@ static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>&
 			B.fr.add_tiles(i, 1);
 			if (r.pcNone) continue;
 			B.anyMg = true;
-			const MgHierarchy::Steps S = r.mg->steps(d, r.code, r.r, r.z, r.st, r.s->dpart, false);
+			const MgHierarchy::Steps S = r.mg->steps(d, r.code, r.r, r.z, r.st, r.s->dpart, tuned, fL0);
 			const int d3 = d.is3d ? 1 : 0;
 			int np;
 			if (S.nl == 1) {
@@ -2446,6 +2524,7 @@ static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>&
 			return e;
 		};
 		bool converged = false;
+		const auto tIter = std::chrono::steady_clock::now();
 		try {
 			while (!converged) {
 				while (its < maxIt && its - checked < kAhead && (pinned ? (int)pending.size() < nslot : pending.empty())) {
@@ -2488,6 +2567,8 @@ static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>&
 		}
 		for (auto& c : pending) evPool.push_back(c.ev);
 		for (hipEvent_t e : evPool) (void)hipEventDestroy(e);
+		batch_policy_done(tkey, policy,
+		                  std::chrono::duration<double>(std::chrono::steady_clock::now() - tIter).count());
 		for (int i = 0; i < P; i++) R[(size_t)i].launched = its;
 		// finish and correct velocities, each problem on its own stream
 		if (!s0->ev) MFX_HIP(hipEventCreateWithFlags(&s0->ev, hipEventDisableTiming));

@@ -43,10 +43,11 @@ def test_batch_equals_single(manta, oracle, pc):
 
 
 def test_batch_repeated_direct_capture_replay(manta, oracle):
-    """The same batch solved three times in a row: queued launches, then
-    capture of the two-iteration graph, then its replay.  Every pass starts
-    from the same velocity and must give results bit-identical to single
-    solves."""
+    """The same batch solved twelve times in a row: the chunk autotuner's
+    eight trial solves (four plane-chunk policies, queued launches), then the
+    chosen policy's queued launches, the capture of its two-iteration graph
+    and the replays.  Every pass starts from the same velocity and must give
+    results bit-identical to single solves."""
     kw = dict(preconditioner=manta.PcMGStatic, cgAccuracy=1e-4)
     single, batch, vel0 = [], [], []
     for k, (shape, frac) in enumerate(PROBLEMS):
@@ -56,7 +57,7 @@ def test_batch_repeated_direct_capture_replay(manta, oracle):
         vel0.append(v.toNumpy().copy())
     for g, f, v, p in single:
         manta.solvePressure(flags=f, vel=v, pressure=p, **kw)
-    for rep in range(3):
+    for rep in range(12):
         for (g, f, v, p), v0 in zip(batch, vel0):
             v.fromNumpy(v0)
         manta.solvePressureBatch([(v, p, f) for g, f, v, p in batch], **kw)
