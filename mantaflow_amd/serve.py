@@ -243,9 +243,14 @@ class Server:
             groups = [order] if not realtime else [list(g) for _, g in itertools.groupby(order, lambda r: r.arrival)]
             for g in groups:
                 if realtime:
+                    # sleep until shortly before the arrival, then spin: a
+                    # request is submitted within microseconds of its
+                    # arrival time (sleep alone overshoots by ~0.1 ms)
                     wait = g[0].arrival - (time.perf_counter() - w0)
-                    if wait > 0:
-                        time.sleep(wait)
+                    if wait > 1e-3:
+                        time.sleep(wait - 1e-3)
+                    while time.perf_counter() - w0 < g[0].arrival:
+                        pass
                 n = len(g)
                 sess = (C.c_int * n)(*[r.session for r in g])
                 kinds = (C.c_int * n)(*[_abi.REQ_STEP if r.kind == REQ_STEP else _abi.REQ_SOLVE for r in g])
