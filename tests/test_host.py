@@ -118,3 +118,20 @@ def test_serve_engine_host_logic():
         assert b"required" in L.mfx_last_error()
     finally:
         assert L.mfx_serve_destroy(h) == 0
+
+
+def test_header_compiles_as_c_and_host_program_as_cpp(tmp_path):
+    """include/mfx.h is plain C (C99) and the C++ host program of the C-ABI
+    tests compiles against it (syntax only, no device needed)."""
+    import subprocess
+    src = tmp_path / "abi.c"
+    names = header_functions()
+    src.write_text('#include "mfx.h"\ntypedef void (*fn)(void);\nfn table[] = {%s};\n'
+                   'int main(void) { return table[0] == 0; }\n' % ", ".join("(fn)&%s" % n for n in names))
+    inc = os.path.join(ROOT, "include")
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-pedantic", "-fsyntax-only", "-I", inc, str(src)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I", inc,
+                        os.path.join(ROOT, "tests", "cabi", "plume_host.cpp")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
