@@ -34,6 +34,9 @@
 #include <set>
 #include <mutex>
 #include <chrono>
+#include <atomic>
+#include <exception>
+#include <thread>
 #include <deque>
 #include <cmath>
 #include <cstring>
@@ -2287,6 +2290,39 @@ static void batch_chunks(const Dims& d, int policy, bool* tuned, int* forceMv, i
 	*forceL0 = std::min(f * kl0, cap);
 }
 
+// f(i) for i in [0, n) on up to eight host threads (the calling one
+// included), each problem's calls on its own scene and stream; the first
+// exception (by index) is rethrown after all have finished.  The per-problem
+// set-up and finishing of a batch are host-bound (launches, an event wait
+// per problem), so spreading them over threads shortens the batch.
+template <typename F>
+static void parallel_for(int n, int device, F&& f) {
+	const int nt = std::min(n, 8);
+	std::vector<std::exception_ptr> err((size_t)n);
+	std::atomic<int> next{0};
+	auto work = [&]() {
+		for (;;) {
+			const int i = next.fetch_add(1);
+			if (i >= n) return;
+			try {
+				f(i);
+			} catch (...) {
+				err[(size_t)i] = std::current_exception();
+			}
+		}
+	};
+	std::vector<std::thread> th;
+	for (int t = 1; t < nt; t++)
+		th.emplace_back([&, device]() {
+			(void)hipSetDevice(device);
+			work();
+		});
+	work();
+	for (auto& t : th) t.join();
+	for (auto& e : err)
+		if (e) std::rethrow_exception(e);
+}
+
 static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>& pres,
                         const std::vector<Grid*>& flg, const mfx_pressure_opts& o, mfx_solve_stats* stats) {
 	const int P = (int)vel.size();
@@ -2296,13 +2332,14 @@ static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>&
 		for (int i = 0; i < P; i++)
 			if (rhs[(size_t)i]) vel[(size_t)i]->s->putReal(rhs[(size_t)i]);
 	};
+	const int device = vel[0]->s->device;
 	try {
-		for (int i = 0; i < P; i++) {
+		parallel_for(P, device, [&](int i) {
 			Scene* s = vel[(size_t)i]->s;
 			rhs[(size_t)i] = s->getReal();
 			compute_rhs(s, flg[(size_t)i], vel[(size_t)i], rhs[(size_t)i], o.enforceCompatibility);
 			cg_setup(R[(size_t)i], s, flg[(size_t)i], rhs[(size_t)i], pres[(size_t)i]->real(), o, true);
-		}
+		});
 
 		BatchSolve B;
 		B.P = P;
@@ -2581,21 +2618,22 @@ static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>&
 		// the relTol check on the true residual, as in a single solve; a
 		// problem that needs a restart continues alone on its own stream
 		const double target = o.relTol > 0 ? o.relTol : (double)o.cgAccuracy;
-		for (int i = 0; i < P; i++) {
+		std::vector<std::string> errs((size_t)P);
+		parallel_for(P, device, [&](int i) {
 			sync(vel[(size_t)i]->s);
 			cg_verify(R[(size_t)i], target);
 			cg_release(R[(size_t)i]);
 			correct_velocity(vel[(size_t)i]->s, flg[(size_t)i], vel[(size_t)i], pres[(size_t)i]->real());
-		}
-		std::string err;
-		for (int i = 0; i < P; i++) {
 			sync(vel[(size_t)i]->s);
 			try {
 				cg_finish_collect(R[(size_t)i], stats ? &stats[i] : nullptr);
 			} catch (const Error& e) {
-				if (err.empty()) err = e.what();
+				errs[(size_t)i] = e.what();
 			}
-		}
+		});
+		std::string err;
+		for (const auto& e : errs)
+			if (err.empty() && !e.empty()) err = e;
 		release();
 		if (!err.empty()) throw Error(err);
 	} catch (...) {
