@@ -2339,6 +2339,9 @@ static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>&
 			rhs[(size_t)i] = s->getReal();
 			compute_rhs(s, flg[(size_t)i], vel[(size_t)i], rhs[(size_t)i], o.enforceCompatibility);
 			cg_setup(R[(size_t)i], s, flg[(size_t)i], rhs[(size_t)i], pres[(size_t)i]->real(), o, true);
+			// marks the end of this problem's set-up for the batch stream
+			if (!s->ev) MFX_HIP(hipEventCreateWithFlags(&s->ev, hipEventDisableTiming));
+			MFX_HIP(hipEventRecord(s->ev, s->stream));
 		});
 
 		BatchSolve B;
@@ -2488,12 +2491,7 @@ static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>&
 		B.dStates = (CgState**)(B.dmem + stOff);
 		B.dDone = (int*)(B.dmem + doneOff);
 		// the batch stream waits for every problem's set-up
-		for (int i = 1; i < P; i++) {
-			Scene* si = vel[(size_t)i]->s;
-			if (!si->ev) MFX_HIP(hipEventCreateWithFlags(&si->ev, hipEventDisableTiming));
-			MFX_HIP(hipEventRecord(si->ev, si->stream));
-			MFX_HIP(hipStreamWaitEvent(q, si->ev, 0));
-		}
+		for (int i = 1; i < P; i++) MFX_HIP(hipStreamWaitEvent(q, vel[(size_t)i]->s->ev, 0));
 		// initial V-cycle of every preconditioned problem, sigma = dot(z, r)
 		if (B.anyMg) {
 			B.enqueue_vcycle(q);
@@ -2610,17 +2608,18 @@ static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>&
 		// finish and correct velocities, each problem on its own stream
 		if (!s0->ev) MFX_HIP(hipEventCreateWithFlags(&s0->ev, hipEventDisableTiming));
 		MFX_HIP(hipEventRecord(s0->ev, q));
-		for (int i = 0; i < P; i++) {
-			Scene* si = vel[(size_t)i]->s;
-			if (i > 0) MFX_HIP(hipStreamWaitEvent(si->stream, s0->ev, 0));
-			cg_true_residual_enqueue(R[(size_t)i]);
-		}
 		// the relTol check on the true residual, as in a single solve; a
 		// problem that needs a restart continues alone on its own stream
 		const double target = o.relTol > 0 ? o.relTol : (double)o.cgAccuracy;
 		std::vector<std::string> errs((size_t)P);
 		parallel_for(P, device, [&](int i) {
-			sync(vel[(size_t)i]->s);
+			Scene* si = vel[(size_t)i]->s;
+			if (i > 0) MFX_HIP(hipStreamWaitEvent(si->stream, s0->ev, 0));
+			cg_true_residual_enqueue(R[(size_t)i]);
+		});
+		parallel_for(P, device, [&](int i) {
+			Scene* si = vel[(size_t)i]->s;
+			sync(si);
 			cg_verify(R[(size_t)i], target);
 			cg_release(R[(size_t)i]);
 			correct_velocity(vel[(size_t)i]->s, flg[(size_t)i], vel[(size_t)i], pres[(size_t)i]->real());
