@@ -118,6 +118,28 @@ struct DotP {
 	const float* r;
 	double* part;
 };
+// end of a solve: ||b - A x||^2 partials, their sum, the velocity correction
+struct TrueResP {
+	Dims d;
+	Tiles t;
+	const uint8_t* code;
+	const float* b;
+	const float* x;
+	double* part;
+	float scale;
+};
+struct TrueRedP {
+	CgState* st;
+	const double* part;
+	int nb;
+};
+struct CorrVelP {
+	Dims d;
+	Tiles t;
+	const uint8_t* f8;
+	const float* pr;
+	float *u, *v, *w;
+};
 
 // FIN_BETA0: FIN_BETA of the initial V-cycle, which also marks the next
 // iteration as the first (p = z)

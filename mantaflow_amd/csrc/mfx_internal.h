@@ -192,8 +192,8 @@ __device__ inline unsigned xcd_tile(unsigned wg, unsigned nwg) {
 struct CellPos {
 	int i, j, k;
 };
-__device__ inline CellPos tile_cell(const Tiles& t) {
-	const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+// cells of logical tile `tile` (the thread's column of kTX x kTY rows)
+__device__ inline CellPos tile_cell_of(const Tiles& t, unsigned tile) {
 	const unsigned rest = fastdiv(tile, t.dx);
 	const unsigned bx = tile - rest * (unsigned)t.tx;
 	const unsigned k = fastdiv(rest, t.dy);
@@ -204,6 +204,7 @@ __device__ inline CellPos tile_cell(const Tiles& t) {
 	p.k = (int)k * t.kz;
 	return p;
 }
+__device__ inline CellPos tile_cell(const Tiles& t) { return tile_cell_of(t, xcd_tile(blockIdx.x, gridDim.x)); }
 
 // Planes per thread of the light per-cell kernels: a thread walks kLightKZ
 // planes of its 64x4 tile and issues the loads of all of them before the

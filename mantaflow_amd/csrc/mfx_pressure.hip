@@ -219,10 +219,14 @@ __global__ void k_add_const(float* a, long long n, float v) {
 // knCorrectVelocity (pressure.cpp:87-109), bnd=1.  kLightKZ planes per
 // thread with all loads issued first; each face is updated in a register in
 // the reference's order and written once.
-__global__ void __launch_bounds__(kBlock) k_correct_velocity(Dims d, Tiles t, const uint8_t* __restrict__ f8,
-                                                             const float* __restrict__ pr, float* __restrict__ u,
-                                                             float* __restrict__ v, float* __restrict__ w) {
-	const CellPos p = tile_cell(t);
+__device__ __forceinline__ void body_correct_velocity(const CorrVelP& P, long long blk, long long) {
+	const Dims& d = P.d;
+	const uint8_t* __restrict__ f8 = P.f8;
+	const float* __restrict__ pr = P.pr;
+	float* __restrict__ u = P.u;
+	float* __restrict__ v = P.v;
+	float* __restrict__ w = P.w;
+	const CellPos p = tile_cell_of(P.t, (unsigned)blk);
 	if (p.i >= d.nx || p.j >= d.ny) return;
 	const int i = p.i, j = p.j;
 	constexpr int KZ = kLightKZ;
@@ -483,8 +487,8 @@ __global__ void __launch_bounds__(kBlock) k_make_rhs_we(Dims d, Tiles t, const f
 	}
 }
 
-// st->trueRR = sum of nb true-residual partials (fixed order)
-__global__ void __launch_bounds__(1024) k_reduce_true_rr(CgState* st, const double* __restrict__ part, int nb) {
+// st->trueRR = sum of nb true-residual partials (fixed order; 1024 threads)
+__device__ __forceinline__ void reduce_true_rr(CgState* st, const double* __restrict__ part, int nb) {
 	__shared__ double sh[16];
 	double a = 0;
 	for (int b = threadIdx.x; b < nb; b += blockDim.x) a += part[3 * b];
@@ -496,6 +500,17 @@ __global__ void __launch_bounds__(1024) k_reduce_true_rr(CgState* st, const doub
 		for (int q = 0; q < (int)(blockDim.x >> 6); q++) s += sh[q];
 		st->trueRR = s;
 	}
+}
+__global__ void __launch_bounds__(1024) k_reduce_true_rr(CgState* st, const double* __restrict__ part, int nb) {
+	reduce_true_rr(st, part, nb);
+}
+__device__ __forceinline__ void body_reduce_true_rr(const TrueRedP& P, long long, long long) {
+	reduce_true_rr(P.st, P.part, P.nb);
+}
+// every problem's CgState into one array (one copy to the host for a batch)
+__global__ void k_gather_states(int np, CgState* const* __restrict__ st, CgState* __restrict__ out) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < np) out[i] = *st[i];
 }
 
 // sum fl(z*r) (GridDotProduct(mTmp, mResidual)) of 256 cells into part[3*slot]
@@ -853,12 +868,18 @@ struct __attribute__((packed, aligned(4))) FTriple3 {
 // grids of at least 3 cells); the
 // z-neighbours of inner planes are the thread's own adjacent planes, so only
 // the first plane's lower and the last plane's upper neighbour are loaded.
+// (a __device__ body: one launch per problem, or one for a batch; the
+// partial goes to the logical tile's slot in both)
 template <bool TRI, bool SC = false>
-__global__ void __launch_bounds__(kBlock) k_true_residual(Dims d, Tiles t, const uint8_t* __restrict__ code,
-                                                          const float* __restrict__ b, const float* __restrict__ x,
-                                                          double* __restrict__ part, float scale = 0.f) {
+__device__ __forceinline__ void body_true_residual(const TrueResP& P, long long blk, long long slot) {
+	const Dims& d = P.d;
+	const uint8_t* __restrict__ code = P.code;
+	const float* __restrict__ b = P.b;
+	const float* __restrict__ x = P.x;
+	double* __restrict__ part = P.part;
+	const float scale = P.scale;
 	__shared__ double sh[8];
-	const CellPos p = tile_cell(t);
+	const CellPos p = tile_cell_of(P.t, (unsigned)blk);
 	double rr = 0;
 	if (p.i < d.nx && p.j < d.ny) {
 		constexpr int KZ = kLightKZ;
@@ -938,7 +959,7 @@ __global__ void __launch_bounds__(kBlock) k_true_residual(Dims d, Tiles t, const
 		}
 	}
 	const double s = block_sum(rr, sh);
-	if (threadIdx.x == 0) part[3 * blockIdx.x] = s;
+	if (threadIdx.x == 0) part[3 * slot] = s;
 }
 
 // Residual replacement: r = b - A x with FP64 accumulation of the row sum,
@@ -1076,6 +1097,15 @@ void run_dot_one(hipStream_t s, unsigned nb, const DotP& p) {
 }
 void run_dot_many(hipStream_t s, unsigned nb, WorkList items, const DotP* p) {
 	hipLaunchKernelGGL((k_many<DotP, body_dot, kBlock>), dim3(nb), dim3(kBlock), 0, s, items, p);
+}
+static void run_true_residual_many(hipStream_t s, unsigned nb, WorkList items, const TrueResP* p) {
+	launch_many<TrueResP, body_true_residual<true>, kBlock>(s, nb, items, p);
+}
+static void run_reduce_true_rr_many(hipStream_t s, unsigned nb, WorkList items, const TrueRedP* p) {
+	launch_many<TrueRedP, body_reduce_true_rr, 1024>(s, nb, items, p);
+}
+static void run_correct_velocity_many(hipStream_t s, unsigned nb, WorkList items, const CorrVelP* p) {
+	launch_many<CorrVelP, body_correct_velocity, kBlock>(s, nb, items, p);
 }
 
 // 64-bit hash of a matrix code array: the wrapping sum over cells of a
@@ -1605,6 +1635,8 @@ struct CgRun {
 		FinP p{st, dpart, fpart, nb, phase, pcNone ? 1 : 0};
 		return p;
 	}
+	TrueResP trueResP() const { return TrueResP{s->d, tiles_for(s->d, kLightKZ), code, rhs, x, dpart, scale}; }
+	TrueRedP trueRedP() const { return TrueRedP{st, dpart, (int)tiles_for(s->d, kLightKZ).count()}; }
 };
 
 // matrix, preconditioner, buffers, CgState and GridCg::doInit
@@ -1890,15 +1922,10 @@ static void cg_true_residual_enqueue(CgRun& R) {
 		return;
 	}
 	const Tiles tr = tiles_for(s->d, kLightKZ);
-	if (R.scaled)
-		hipLaunchKernelGGL((k_true_residual<false, true>), dim3((unsigned)tr.count()), dim3(kBlock), 0, s->stream,
-		                   s->d, tr, R.code, (const float*)R.rhs, (const float*)R.x, R.dpart, R.scale);
-	else if (s->d.n >= 3)
-		hipLaunchKernelGGL(k_true_residual<true>, dim3((unsigned)tr.count()), dim3(kBlock), 0, s->stream, s->d, tr,
-		                   R.code, (const float*)R.rhs, (const float*)R.x, R.dpart);
-	else
-		hipLaunchKernelGGL(k_true_residual<false>, dim3((unsigned)tr.count()), dim3(kBlock), 0, s->stream, s->d, tr,
-		                   R.code, (const float*)R.rhs, (const float*)R.x, R.dpart);
+	const TrueResP tp = R.trueResP();
+	if (R.scaled) launch_one<TrueResP, body_true_residual<false, true>, kBlock>(s->stream, tr.count(), tp);
+	else if (s->d.n >= 3) launch_one<TrueResP, body_true_residual<true>, kBlock>(s->stream, tr.count(), tp);
+	else launch_one<TrueResP, body_true_residual<false>, kBlock>(s->stream, tr.count(), tp);
 	hipLaunchKernelGGL(k_reduce_true_rr, dim3(1), dim3(1024), 0, s->stream, R.st, (const double*)R.dpart,
 	                   (int)tr.count());
 	MFX_HIP(hipMemcpyAsync(s->hscal, R.st, sizeof(CgState), hipMemcpyDeviceToHost, s->stream));
@@ -2099,8 +2126,8 @@ static void solve_scaled(Scene* s, Grid* flags, float* rhs, float* x, int kind, 
 
 static void correct_velocity(Scene* s, Grid* flags, Grid* vel, const float* pressure, const BndDev* bnd = nullptr) {
 	const Tiles t = tiles_for(s->d, kLightKZ);
-	hipLaunchKernelGGL(k_correct_velocity, dim3((unsigned)t.count()), dim3(kBlock), 0, s->stream, s->d, t, flags->f8,
-	                   pressure, vel->comp(0), vel->comp(1), vel->comp(2));
+	launch_one<CorrVelP, body_correct_velocity, kBlock>(
+	    s->stream, t.count(), CorrVelP{s->d, t, flags->f8, pressure, vel->comp(0), vel->comp(1), vel->comp(2)});
 	if (bnd && bnd->phi) {
 		// correctVelocity with phi (pressure.cpp:470-474)
 		const unsigned nb = flat_blocks(s->d.n);
@@ -2184,6 +2211,10 @@ struct BatchSolve {
 	Step<CoarsestP> crs;
 	Step<L0UpP> l0up[2];
 	Step<DotP> dot;
+	// finish: true residual partials, their sums, the velocity correction
+	Step<TrueResP> tres;
+	Step<TrueRedP> tred;
+	Step<CorrVelP> corr;
 	bool anyMg = false;
 	uint8_t* dmem = nullptr;
 	CgState** dStates = nullptr;
@@ -2358,6 +2389,9 @@ static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>&
 		B.fill = Step<FillP>((size_t)P);
 		B.crs = Step<CoarsestP>((size_t)P);
 		B.dot = Step<DotP>((size_t)P);
+		B.tres = Step<TrueResP>((size_t)P);
+		B.tred = Step<TrueRedP>((size_t)P);
+		B.corr = Step<CorrVelP>((size_t)P);
 		for (int d3 = 0; d3 < 2; d3++) {
 			B.down[d3] = Step<L0DownP>((size_t)P);
 			B.l0up[d3] = Step<L0UpP>((size_t)P);
@@ -2387,6 +2421,15 @@ static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>&
 				B.up[par].params[(size_t)i] = r.upd(par == 1);
 				B.up[par].add_tiles(i, r.nbU);
 			}
+			MFX_CHECK(!r.scaled && !r.gA0 && d.n >= 3, "batched solve: unexpected system kind");
+			const long long nLight = tiles_for(d, kLightKZ).count();
+			B.tres.params[(size_t)i] = r.trueResP();
+			B.tres.add_tiles(i, nLight);
+			B.tred.params[(size_t)i] = r.trueRedP();
+			B.tred.add_tiles(i, 1);
+			B.corr.params[(size_t)i] = CorrVelP{d, tiles_for(d, kLightKZ), flg[(size_t)i]->f8, pres[(size_t)i]->real(),
+			                                    vel[(size_t)i]->comp(0), vel[(size_t)i]->comp(1), vel[(size_t)i]->comp(2)};
+			B.corr.add_tiles(i, nLight);
 			B.fa.params[(size_t)i] = r.fin(r.npM, FIN_ALPHA);
 			B.fa.add_tiles(i, 1);
 			B.fr.params[(size_t)i] = r.fin((int)r.nbU, FIN_RES);
@@ -2450,6 +2493,9 @@ static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>&
 			f(B.fill);
 			f(B.crs);
 			f(B.dot);
+			f(B.tres);
+			f(B.tred);
+			f(B.corr);
 			for (int d3 = 0; d3 < 2; d3++) {
 				f(B.down[d3]);
 				f(B.l0up[d3]);
@@ -2465,6 +2511,8 @@ static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>&
 		const size_t stOff = A.add(states.data(), sizeof(CgState*) * (size_t)P);
 		std::vector<int> zeros((size_t)P, 0);
 		const size_t doneOff = A.add(zeros.data(), sizeof(int) * (size_t)P);
+		std::vector<CgState> gatherInit((size_t)P);
+		const size_t gatherOff = A.add(gatherInit.data(), sizeof(CgState) * (size_t)P);
 		Scene* s0 = vel[0]->s;
 		hipStream_t q = s0->stream;
 		if (s0->arenaCap < A.host.size()) {
@@ -2605,31 +2653,36 @@ static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>&
 		batch_policy_done(tkey, policy,
 		                  std::chrono::duration<double>(std::chrono::steady_clock::now() - tIter).count());
 		for (int i = 0; i < P; i++) R[(size_t)i].launched = its;
-		// finish and correct velocities, each problem on its own stream
-		if (!s0->ev) MFX_HIP(hipEventCreateWithFlags(&s0->ev, hipEventDisableTiming));
-		MFX_HIP(hipEventRecord(s0->ev, q));
+		// finish, batched: every problem's true residual and its sum, one
+		// copy of all CgStates to the host, each problem's state into its
+		// scene's host mirror (as a single solve leaves it)
+		run_true_residual_many(q, B.tres.n(), B.tres.dItems, B.tres.dParams);
+		run_reduce_true_rr_many(q, B.tred.n(), B.tred.dItems, B.tred.dParams);
+		CgState* dGather = (CgState*)(B.dmem + gatherOff);
+		hipLaunchKernelGGL(k_gather_states, dim3((P + 255) / 256), dim3(256), 0, q, P, (CgState* const*)B.dStates,
+		                   dGather);
+		MFX_HIP(hipMemcpyAsync(s0->hstage, dGather, sizeof(CgState) * (size_t)P, hipMemcpyDeviceToHost, q));
+		sync(s0);
+		for (int i = 0; i < P; i++)
+			std::memcpy(vel[(size_t)i]->s->hscal, (const CgState*)s0->hstage + i, sizeof(CgState));
 		// the relTol check on the true residual, as in a single solve; a
 		// problem that needs a restart continues alone on its own stream
+		// (and has synchronized it when cg_verify returns)
 		const double target = o.relTol > 0 ? o.relTol : (double)o.cgAccuracy;
-		std::vector<std::string> errs((size_t)P);
-		parallel_for(P, device, [&](int i) {
-			Scene* si = vel[(size_t)i]->s;
-			if (i > 0) MFX_HIP(hipStreamWaitEvent(si->stream, s0->ev, 0));
-			cg_true_residual_enqueue(R[(size_t)i]);
-		});
-		parallel_for(P, device, [&](int i) {
-			Scene* si = vel[(size_t)i]->s;
-			sync(si);
+		for (int i = 0; i < P; i++) {
 			cg_verify(R[(size_t)i], target);
 			cg_release(R[(size_t)i]);
-			correct_velocity(vel[(size_t)i]->s, flg[(size_t)i], vel[(size_t)i], pres[(size_t)i]->real());
-			sync(vel[(size_t)i]->s);
+		}
+		run_correct_velocity_many(q, B.corr.n(), B.corr.dItems, B.corr.dParams);
+		sync(s0);
+		std::vector<std::string> errs((size_t)P);
+		for (int i = 0; i < P; i++) {
 			try {
 				cg_finish_collect(R[(size_t)i], stats ? &stats[i] : nullptr);
 			} catch (const Error& e) {
 				errs[(size_t)i] = e.what();
 			}
-		});
+		}
 		std::string err;
 		for (const auto& e : errs)
 			if (err.empty() && !e.empty()) err = e;
