@@ -147,6 +147,13 @@ int mfx_set_wall_bcs_frac(const mfx_grid* flags, mfx_grid* vel, const mfx_grid* 
 /* addBuoyancy (source/plugin/extforces.cpp:83-87) */
 int mfx_add_buoyancy(const mfx_grid* flags, const mfx_grid* density, mfx_grid* vel,
                      const float gravity[3], float coefficient, int scale);
+/* setWallBcs(flags, vel) without obvel, fractions or phiObs, then
+ * addBuoyancy(flags, density, vel, gravity, coefficient, scale), as one pass
+ * over the grid with results bit-identical to the two calls.  No counterpart
+ * in the reference (it fuses extforces.cpp:307-315 with :83-87, the order in
+ * which the plume scenes call them). */
+int mfx_set_wall_bcs_buoyancy(const mfx_grid* flags, mfx_grid* vel, const mfx_grid* density,
+                              const float gravity[3], float coefficient, int scale);
 
 /* --------------------------------------------------------------- pressure */
 typedef struct {
@@ -162,6 +169,9 @@ typedef struct {
 	int maxIterOverride;       /* > 0: replaces the reference's iteration cap */
 	int precision;             /* 0 FP32 path; 1 mixed (BF16 MG smoothing, FP64 scalars) */
 	int trueResidualEvery;     /* > 0: recompute r = b - A x with FP64 accumulation every k its */
+	int deferCorrection;       /* 1 (mfx_solve_pressure only): vel is left uncorrected; the caller
+	                              applies mfx_correct_velocity or mfx_correct_velocity_wall_bcs with
+	                              the same grids before anything reads vel (0) */
 } mfx_pressure_opts;
 
 typedef struct {
@@ -264,6 +274,12 @@ int mfx_solve_pressure_system(mfx_grid* rhs, mfx_grid* pressure, const mfx_grid*
                               const mfx_pressure_opts* opts, mfx_solve_stats* stats);
 /* correctVelocity (source/plugin/pressure.cpp:455-476) */
 int mfx_correct_velocity(mfx_grid* vel, const mfx_grid* pressure, const mfx_grid* flags);
+/* correctVelocity then setWallBcs(flags, vel) (no obvel, fractions or
+ * phiObs) as one pass, bit-identical to the two calls.  No counterpart in
+ * the reference (it fuses pressure.cpp:87-109 with extforces.cpp:307-315,
+ * the velocity correction that ends solvePressure and the setWallBcs that
+ * follows it in the plume scenes); with mfx_pressure_opts.deferCorrection. */
+int mfx_correct_velocity_wall_bcs(mfx_grid* vel, const mfx_grid* pressure, const mfx_grid* flags);
 /* releaseMG (source/plugin/pressure.cpp:252-266); s == NULL releases all */
 int mfx_release_mg(mfx_scene* s);
 

@@ -19,7 +19,7 @@ class PressureOpts(C.Structure):
                 ("enforceCompatibility", C.c_int), ("useL2Norm", C.c_int),
                 ("zeroPressureFixing", C.c_int), ("relTol", C.c_double),
                 ("maxIterOverride", C.c_int), ("precision", C.c_int),
-                ("trueResidualEvery", C.c_int)]
+                ("trueResidualEvery", C.c_int), ("deferCorrection", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -107,6 +107,7 @@ SIGNATURES = {
                                            C.POINTER(SolveStats)]),
     "mfx_correct_velocity_bnd": (_I, [_P, _P, _P, C.POINTER(PressureBnd)]),
     "mfx_add_buoyancy": (_I, [_P, _P, _P, _FP, _F, _I]),
+    "mfx_set_wall_bcs_buoyancy": (_I, [_P, _P, _P, _FP, _F, _I]),
     "mfx_pressure_opts_default": (None, [C.POINTER(PressureOpts)]),
     "mfx_solve_pressure": (_I, [_P, _P, _P, C.POINTER(PressureOpts), C.POINTER(SolveStats), _P]),
     "mfx_solve_pressure_noalloc": (_I, [_P, _P, _P, C.POINTER(PressureOpts), C.POINTER(SolveStats)]),
@@ -117,6 +118,7 @@ SIGNATURES = {
     "mfx_solve_pressure_batch": (_I, [C.c_int, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P),
                                       C.POINTER(PressureOpts), C.POINTER(SolveStats)]),
     "mfx_correct_velocity": (_I, [_P, _P, _P]),
+    "mfx_correct_velocity_wall_bcs": (_I, [_P, _P, _P]),
     "mfx_release_mg": (_I, [_P]),
     "mfx_serve_opts_default": (None, [C.POINTER(ServeOpts)]),
     "mfx_serve_create": (_I, [C.POINTER(ServeOpts), C.POINTER(_P)]),

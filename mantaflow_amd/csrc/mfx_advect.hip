@@ -1008,36 +1008,81 @@ __global__ void __launch_bounds__(kBlock) k_set_wall_bcs(Dims d, Tiles t, const 
 			if (d.is3d) ob[q][2] = ow[ix];
 		}
 	}
-	// The per-cell rules below, written as boolean algebra and selects (no
-	// branch per condition): a face next to an obstacle (or an obstacle face
-	// next to fluid) takes the obstacle velocity; a stick wall across x, y or
-	// z zeroes the two other components of a fluid cell, overriding the
-	// obstacle value (the stick tests come last in the reference).
+	// The per-cell rules (wall_rule) as boolean algebra and selects, no
+	// branch per condition.
 #pragma unroll
 	for (int q = 0; q < KZ; q++) {
 		const int k = p.k + q;
 		if (k >= d.nz) break;
 		const long long ix = d.idx(i, j, k);
-		// predicates as 0/1 integers (vector ALU) rather than lane masks
-		const int fl = f[q], xm = fxm[q], xp = fxp[q], ym = fym[q], yp = fyp[q], zm = fzm[q], zp = fzp[q];
-		const int curF = fl & 1, curO = (fl >> 1) & 1;  // kFluid = 1, kObstacle = 2, kStick = 64
-		const int act = curF | curO;
-		const int gi = min(i, 1), li = min(d.nx - 1 - i, 1), gj = min(j, 1), lj = min(d.ny - 1 - j, 1);
-		const int gk = min(k, 1), lk = min(d.nz - 1 - k, 1), is3 = d.is3d ? 1 : 0;
-		const int ou_ = act & gi & (((xm >> 1) & 1) | (curO & xm & 1));
-		const int ov_ = act & gj & (((ym >> 1) & 1) | (curO & ym & 1));
-		const int ow_ = act & ((is3 ^ 1) | (gk & (((zm >> 1) & 1) | (curO & zm & 1))));
-		const int sx = curF & ((gi & (xm >> 6)) | (li & (xp >> 6))) & 1;
-		const int sy = curF & ((gj & (ym >> 6)) | (lj & (yp >> 6))) & 1;
-		const int sz = curF & is3 & ((gk & (zm >> 6)) | (lk & (zp >> 6))) & 1;
-		const int wu = ou_ | sy | sz, wv = ov_ | sx | sz, ww = ow_ | sx | sy;
+		const WallRule r = wall_rule(d, i, j, k, f[q], fxm[q], fxp[q], fym[q], fyp[q], fzm[q], fzp[q]);
+		const int is3 = d.is3d ? 1 : 0;
 		// value masks: all ones keeps the obstacle velocity, zero gives 0.0
-		const float nu = __int_as_float(__float_as_int(ob[q][0]) & ((sy | sz) - 1));
-		const float nv = __int_as_float(__float_as_int(ob[q][1]) & ((sx | sz) - 1));
-		const float nw = __int_as_float(__float_as_int(ob[q][2]) & ((sx | sy | (is3 ^ 1)) - 1));
-		if (wu) u[ix] = nu;
-		if (wv) v[ix] = nv;
-		if (ww) w[ix] = nw;
+		const float nu = __int_as_float(__float_as_int(ob[q][0]) & ((r.sy | r.sz) - 1));
+		const float nv = __int_as_float(__float_as_int(ob[q][1]) & ((r.sx | r.sz) - 1));
+		const float nw = __int_as_float(__float_as_int(ob[q][2]) & ((r.sx | r.sy | (is3 ^ 1)) - 1));
+		if (r.wu) u[ix] = nu;
+		if (r.wv) v[ix] = nv;
+		if (r.ww) w[ix] = nw;
+	}
+}
+
+// setWallBcs (no obstacle velocity) followed by addBuoyancy in one pass:
+// each face is first zeroed where KnSetWallBcs writes it, then gets
+// KnAddBuoyancy's increment (extforces.cpp:72-80, bnd=1, evaluated in
+// double), and is written once if that changed its bits.  Same values as
+// k_set_wall_bcs then k_add_buoyancy; kLightKZ planes per thread, every
+// load issued first.
+__global__ void __launch_bounds__(kBlock) k_wall_buoyancy(Dims d, Tiles t, const uint8_t* __restrict__ f8,
+                                                          const float* __restrict__ den, float* __restrict__ u,
+                                                          float* __restrict__ v, float* __restrict__ w, float fx,
+                                                          float fy, float fz) {
+	const CellPos p = tile_cell(t);
+	if (p.i >= d.nx || p.j >= d.ny) return;
+	const int i = p.i, j = p.j;
+	const long long oxm = i > 0 ? 1 : 0, oxp = i < d.nx - 1 ? 1 : 0;
+	const long long oym = j > 0 ? d.nx : 0, oyp = j < d.ny - 1 ? d.nx : 0;
+	constexpr int KZ = kLightKZ;
+	uint8_t f[KZ], fxm[KZ], fxp[KZ], fym[KZ], fyp[KZ], fzm[KZ], fzp[KZ];
+	float dc[KZ], dxm[KZ], dym[KZ], dzm[KZ], uc[KZ], vc[KZ], wc[KZ];
+#pragma unroll
+	for (int q = 0; q < KZ; q++) {
+		const int k = min(p.k + q, d.nz - 1);  // planes past the end are loaded, never used
+		const long long ix = d.idx(i, j, k);
+		const long long ozm = (d.is3d && k > 0) ? d.sz : 0, ozp = (d.is3d && k < d.nz - 1) ? d.sz : 0;
+		f[q] = f8[ix];
+		fxm[q] = f8[ix - oxm];
+		fxp[q] = f8[ix + oxp];
+		fym[q] = f8[ix - oym];
+		fyp[q] = f8[ix + oyp];
+		fzm[q] = f8[ix - ozm];
+		fzp[q] = f8[ix + ozp];
+		dc[q] = den[ix];
+		dxm[q] = den[ix - oxm];
+		dym[q] = den[ix - oym];
+		dzm[q] = den[ix - ozm];
+		uc[q] = u[ix];
+		vc[q] = v[ix];
+		wc[q] = w[ix];
+	}
+#pragma unroll
+	for (int q = 0; q < KZ; q++) {
+		const int k = p.k + q;
+		if (k >= d.nz) break;
+		const long long ix = d.idx(i, j, k);
+		const WallRule r = wall_rule(d, i, j, k, f[q], fxm[q], fxp[q], fym[q], fyp[q], fzm[q], fzp[q]);
+		float nu = r.wu ? 0.f : uc[q], nv = r.wv ? 0.f : vc[q], nw = r.ww ? 0.f : wc[q];
+		const bool bi = interior(d, i, j, k) && (f[q] & kFluid);
+		const bool bu = bi && (fxm[q] & kFluid), bv = bi && (fym[q] & kFluid), bw = bi && d.is3d && (fzm[q] & kFluid);
+		if (bu) nu = (float)((double)nu + (0.5 * (double)fx) * (double)(dc[q] + dxm[q]));
+		if (bv) nv = (float)((double)nv + (0.5 * (double)fy) * (double)(dc[q] + dym[q]));
+		if (bw) nw = (float)((double)nw + (0.5 * (double)fz) * (double)(dc[q] + dzm[q]));
+		// a face is stored only where its bits change (an increment of 0 from
+		// a zero gravity component changes only a -0); the others already
+		// hold the result
+		if (__float_as_uint(nu) != __float_as_uint(uc[q])) u[ix] = nu;
+		if (__float_as_uint(nv) != __float_as_uint(vc[q])) v[ix] = nv;
+		if (__float_as_uint(nw) != __float_as_uint(wc[q])) w[ix] = nw;
 	}
 }
 
@@ -1232,6 +1277,28 @@ void op_set_wall_bcs(Scene* s, Grid* flags, Grid* vel, Grid* obvel) {
 	hipLaunchKernelGGL(k_set_wall_bcs, dim3((unsigned)t.count()), dim3(kBlock), 0, s->stream, s->d, t, flags->f8,
 	                   vel->comp(0), vel->comp(1), vel->comp(2), obvel ? obvel->comp(0) : nullptr,
 	                   obvel ? obvel->comp(1) : nullptr, obvel ? obvel->comp(2) : nullptr);
+}
+
+void op_wall_buoyancy(Scene* s, Grid* flags, Grid* density, Grid* vel, const float f[3]) {
+	Tiles t = tiles_for(s->d, kLightKZ);
+	hipLaunchKernelGGL(k_wall_buoyancy, dim3((unsigned)t.count()), dim3(kBlock), 0, s->stream, s->d, t, flags->f8,
+	                   (const float*)density->real(), vel->comp(0), vel->comp(1), vel->comp(2), f[0], f[1], f[2]);
+}
+
+// addBuoyancy's force (extforces.cpp:83-87): float gridScale = scale ?
+// getDx() : 1; f = -gravity * dt / gridScale * coefficient (Vec3 float ops,
+// one rounding each)
+void buoyancy_strength(const Scene* s, const float gravity[3], float coefficient, int scale, float fv[3]) {
+	int mx = s->d.nx > s->d.ny ? s->d.nx : s->d.ny;
+	if (s->d.nz > mx) mx = s->d.nz;
+	const float dx = (float)(1.0 / mx);
+	volatile float gridScale = scale ? dx : 1.f;
+	for (int c = 0; c < 3; c++) {
+		volatile float a = -gravity[c];
+		volatile float b = a * s->dt;
+		volatile float q = b / gridScale;
+		fv[c] = q * coefficient;
+	}
 }
 
 void op_add_buoyancy(Scene* s, Grid* flags, Grid* density, Grid* vel, const float f[3]) {
@@ -1562,20 +1629,23 @@ int mfx_add_buoyancy(const mfx_grid* flags, const mfx_grid* density, mfx_grid* v
 	same_scene(f, v);
 	same_scene(f, den);
 	Scene* s = f->s;
-	// addBuoyancy (extforces.cpp:83-87): float gridScale = scale ? getDx() : 1;
-	// f = -gravity * dt / gridScale * coefficient (Vec3 float ops)
-	int mx = s->d.nx > s->d.ny ? s->d.nx : s->d.ny;
-	if (s->d.nz > mx) mx = s->d.nz;
-	const float dx = (float)(1.0 / mx);
-	volatile float gridScale = scale ? dx : 1.f;
 	float fv[3];
-	for (int c = 0; c < 3; c++) {
-		volatile float a = -gravity[c];
-		volatile float b = a * s->dt;
-		volatile float q = b / gridScale;
-		fv[c] = q * coefficient;
-	}
+	buoyancy_strength(s, gravity, coefficient, scale, fv);
 	op_add_buoyancy(s, f, den, v, fv);
+	API_END
+}
+
+int mfx_set_wall_bcs_buoyancy(const mfx_grid* flags, mfx_grid* vel, const mfx_grid* density, const float gravity[3],
+                              float coefficient, int scale) {
+	API_BEGIN
+	MFX_CHECK(flags && density && vel && gravity, "setWallBcs+addBuoyancy: flags, vel, density and gravity are required");
+	Grid *f = G(flags), *den = G(density), *v = G(vel);
+	MFX_CHECK(f->isFlags() && den->isReal() && v->isMAC(), "addBuoyancy: needs FlagGrid, RealGrid, MACGrid");
+	same_scene(f, v);
+	same_scene(f, den);
+	float fv[3];
+	buoyancy_strength(f->s, gravity, coefficient, scale, fv);
+	op_wall_buoyancy(f->s, f, den, v, fv);
 	API_END
 }
 

@@ -187,6 +187,37 @@ __device__ inline unsigned xcd_tile(unsigned wg, unsigned nwg) {
 	return x < r ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
 }
 
+// KnSetWallBcs (extforces.cpp:186-217) at cell (i,j,k) as 0/1 integer
+// predicates, from the cell's flags and its six neighbours' (a neighbour's
+// flags are used only where it exists: the reference's i>0 / i<size-1
+// tests).  w* = the face is written: a face next to an obstacle, or an
+// obstacle face next to fluid, takes the obstacle velocity; s* = a stick
+// wall across that axis, which zeroes the two other components of a fluid
+// cell and overrides the obstacle value (the stick tests come last in the
+// reference).  In 2D the z face of every fluid or obstacle cell is written
+// with 0.
+struct WallRule {
+	int wu, wv, ww, sx, sy, sz;
+};
+__device__ inline WallRule wall_rule(const Dims& d, int i, int j, int k, int fl, int xm, int xp, int ym, int yp,
+                                     int zm, int zp) {
+	const int curF = fl & 1, curO = (fl >> 1) & 1;  // kFluid = 1, kObstacle = 2, kStick = 64
+	const int act = curF | curO;
+	const int gi = min(i, 1), li = min(d.nx - 1 - i, 1), gj = min(j, 1), lj = min(d.ny - 1 - j, 1);
+	const int gk = min(k, 1), lk = min(d.nz - 1 - k, 1), is3 = d.is3d ? 1 : 0;
+	const int ou = act & gi & (((xm >> 1) & 1) | (curO & xm & 1));
+	const int ov = act & gj & (((ym >> 1) & 1) | (curO & ym & 1));
+	const int ow = act & ((is3 ^ 1) | (gk & (((zm >> 1) & 1) | (curO & zm & 1))));
+	WallRule r;
+	r.sx = curF & ((gi & (xm >> 6)) | (li & (xp >> 6))) & 1;
+	r.sy = curF & ((gj & (ym >> 6)) | (lj & (yp >> 6))) & 1;
+	r.sz = curF & is3 & ((gk & (zm >> 6)) | (lk & (zp >> 6))) & 1;
+	r.wu = ou | r.sy | r.sz;
+	r.wv = ov | r.sx | r.sz;
+	r.ww = ow | r.sx | r.sy;
+	return r;
+}
+
 // Decode the calling thread's cell (i,j,k) from a 1-D tile launch; k is the
 // first of the tile's kz planes.
 struct CellPos {
@@ -286,6 +317,9 @@ double reduce_sum_double(Scene* s, const float* a, int mode, long long n);
 // kernels / ops implemented in the various .hip files
 void op_set_wall_bcs(Scene* s, Grid* flags, Grid* vel, Grid* obvel);
 void op_add_buoyancy(Scene* s, Grid* flags, Grid* density, Grid* vel, const float f[3]);
+// setWallBcs (no obstacle velocity) then addBuoyancy, one pass
+void op_wall_buoyancy(Scene* s, Grid* flags, Grid* density, Grid* vel, const float f[3]);
+void buoyancy_strength(const Scene* s, const float gravity[3], float coefficient, int scale, float fv[3]);
 void op_advect(Scene* s, Grid* flags, Grid* vel, Grid* grid, int order, float strength, int clampMode, int orderSpace,
                int orderTrace);
 

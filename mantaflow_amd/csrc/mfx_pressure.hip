@@ -66,6 +66,37 @@ __device__ inline int off_coef(const Dims& d, const uint8_t* f8, long long c, in
 	return 1;
 }
 
+// The matrix code byte of cell c = (i,j,k) from its flags and its six
+// neighbours' (used only where they exist) and the pinned cell fix (-1 for
+// none): the rules as 0/1 integer predicates (vector ALU, no branch per
+// condition), the same values as MakeLaplaceMatrix + fixPressure give.
+__device__ inline uint8_t code_byte(const Dims& d, int i, int j, int k, long long c, long long fix, int f, int fxm,
+                                   int fxp, int fym, int fyp, int fzm, int fzp) {
+	const int is3 = d.is3d ? 1 : 0;
+	const int gi = min(i, 1), li = min(d.nx - 1 - i, 1), g2i = min(max(i - 1, 0), 1);  // i>=1, i<nx-1, i>=2
+	const int gj = min(j, 1), lj = min(d.ny - 1 - j, 1), g2j = min(max(j - 1, 0), 1);
+	const int gk = min(k, 1), lk = min(d.nz - 1 - k, 1), g2k = min(max(k - 1, 0), 1);
+	const int kin = is3 ? (gk & lk) : 1;  // k inside the interior (always in 2D)
+	const int fl = f & 1;                  // kFluid = 1, kObstacle = 2
+	const int inner = gi & li & gj & lj & kin;
+	const int isfix = c == fix ? 1 : 0, nfix = isfix ^ 1;
+	const int fi = fl & inner;
+	const int nobs = (((fxm >> 1) & 1) ^ 1) + (((fxp >> 1) & 1) ^ 1) + (((fym >> 1) & 1) ^ 1) +
+	                 (((fyp >> 1) & 1) ^ 1) + is3 * ((((fzm >> 1) & 1) ^ 1) + (((fzp >> 1) & 1) ^ 1));
+	const int A0 = fi * (isfix + nfix * nobs);
+	// off_coef(c -> upper neighbour): c interior fluid, neighbour fluid, neither pinned
+	const int bx = fi & nfix & fxp & (c + 1 != fix ? 1 : 0);
+	const int by = fi & nfix & fyp & (c + d.nx != fix ? 1 : 0);
+	const int bz = fi & nfix & is3 & fzp & (c + d.sz != fix ? 1 : 0);
+	// any lower off-diagonal into c: off_coef(lower neighbour -> c)
+	const int lox = gi & g2i & gj & lj & kin & fxm & (c - 1 != fix ? 1 : 0);
+	const int loy = gj & g2j & gi & li & kin & fym & (c - d.nx != fix ? 1 : 0);
+	const int loz = is3 & gk & g2k & gi & li & gj & lj & fzm & (c - d.sz != fix ? 1 : 0);
+	const int lo = fl & nfix & (lox | loy | loz) & 1;
+	const int trivial = (A0 == 1 ? 1 : 0) & ((bx | by | bz | lo) & 1 ^ 1);
+	return (uint8_t)(A0 | ((bx & 1) << 3) | ((by & 1) << 4) | ((bz & 1) << 5) | (fl << 6) | (trivial << 7));
+}
+
 // kLightKZ planes per thread; the seven flags of every plane are loaded
 // first (offsets 0 where the neighbour is outside the grid: those flags are
 // then never used).
@@ -92,36 +123,12 @@ __global__ void __launch_bounds__(kBlock) k_build_code(Dims d, Tiles t, const ui
 		fzm[q] = f8[c - ozm];
 		fzp[q] = f8[c + ozp];
 	}
-	// The rules below as 0/1 integer predicates (vector ALU, no branch per
-	// condition); the same values as MakeLaplaceMatrix + fixPressure give.
-	const int is3 = d.is3d ? 1 : 0;
-	const int gi = min(i, 1), li = min(d.nx - 1 - i, 1), g2i = min(max(i - 1, 0), 1);  // i>=1, i<nx-1, i>=2
-	const int gj = min(j, 1), lj = min(d.ny - 1 - j, 1), g2j = min(max(j - 1, 0), 1);
 #pragma unroll
 	for (int q = 0; q < KZ; q++) {
 		const int k = p.k + q;
 		if (k >= d.nz) break;
 		const long long c = d.idx(i, j, k);
-		const int gk = min(k, 1), lk = min(d.nz - 1 - k, 1), g2k = min(max(k - 1, 0), 1);
-		const int kin = is3 ? (gk & lk) : 1;  // k inside the interior (always in 2D)
-		const int fl = f[q] & 1;               // kFluid = 1, kObstacle = 2
-		const int inner = gi & li & gj & lj & kin;
-		const int isfix = c == fix ? 1 : 0, nfix = isfix ^ 1;
-		const int fi = fl & inner;
-		const int nobs = (((fxm[q] >> 1) & 1) ^ 1) + (((fxp[q] >> 1) & 1) ^ 1) + (((fym[q] >> 1) & 1) ^ 1) +
-		                 (((fyp[q] >> 1) & 1) ^ 1) + is3 * ((((fzm[q] >> 1) & 1) ^ 1) + (((fzp[q] >> 1) & 1) ^ 1));
-		const int A0 = fi * (isfix + nfix * nobs);
-		// off_coef(c -> upper neighbour): c interior fluid, neighbour fluid, neither pinned
-		const int bx = fi & nfix & fxp[q] & (c + 1 != fix ? 1 : 0);
-		const int by = fi & nfix & fyp[q] & (c + d.nx != fix ? 1 : 0);
-		const int bz = fi & nfix & is3 & fzp[q] & (c + d.sz != fix ? 1 : 0);
-		// any lower off-diagonal into c: off_coef(lower neighbour -> c)
-		const int lox = gi & g2i & gj & lj & kin & fxm[q] & (c - 1 != fix ? 1 : 0);
-		const int loy = gj & g2j & gi & li & kin & fym[q] & (c - d.nx != fix ? 1 : 0);
-		const int loz = is3 & gk & g2k & gi & li & gj & lj & fzm[q] & (c - d.sz != fix ? 1 : 0);
-		const int lo = fl & nfix & (lox | loy | loz) & 1;
-		const int trivial = (A0 == 1 ? 1 : 0) & ((bx | by | bz | lo) & 1 ^ 1);
-		code[c] = (uint8_t)(A0 | ((bx & 1) << 3) | ((by & 1) << 4) | ((bz & 1) << 5) | (fl << 6) | (trivial << 7));
+		code[c] = code_byte(d, i, j, k, c, fix, f[q], fxm[q], fxp[q], fym[q], fyp[q], fzm[q], fzp[q]);
 	}
 }
 
@@ -162,11 +169,14 @@ __device__ inline float apply_row(const Dims& d, const uint8_t* __restrict__ cod
 // RHS: MakeRhs (pressure.cpp:32-84) without fractions/obvel/surface tension
 // ---------------------------------------------------------------------------
 // kLightKZ planes per thread, all loads issued first (upper-neighbour offsets
-// are 0 on the last layer, where the value is not used).
+// are 0 on the last layer, where the value is not used).  CODE: the same
+// pass also writes the matrix code (k_build_code's values with no pinned
+// cell), from the six neighbours' flags.
+template <bool CODE>
 __global__ void __launch_bounds__(kBlock) k_make_rhs(Dims d, Tiles t, const uint8_t* __restrict__ f8,
                                                      const float* __restrict__ u, const float* __restrict__ v,
                                                      const float* __restrict__ w, float* __restrict__ rhs,
-                                                     double* __restrict__ part) {
+                                                     double* __restrict__ part, uint8_t* __restrict__ code) {
 	__shared__ double sh[8];
 	const CellPos p = tile_cell(t);
 	double sum = 0, cnt = 0;
@@ -174,25 +184,53 @@ __global__ void __launch_bounds__(kBlock) k_make_rhs(Dims d, Tiles t, const uint
 		constexpr int KZ = kLightKZ;
 		const int i = p.i, j = p.j;
 		const long long ox = i < d.nx - 1 ? 1 : 0, oy = j < d.ny - 1 ? d.nx : 0;
-		uint8_t f[KZ];
+		const long long oxm = i > 0 ? 1 : 0, oym = j > 0 ? d.nx : 0;
+		uint8_t f[KZ], fxm[KZ], fxp[KZ], fym[KZ], fyp[KZ], fzm[KZ], fzp[KZ];
 		float u0[KZ], u1[KZ], v0[KZ], v1[KZ], w0[KZ], w1[KZ];
 #pragma unroll
 		for (int q = 0; q < KZ; q++) {
 			const int k = min(p.k + q, d.nz - 1);
 			const long long c = d.idx(i, j, k);
-			const long long oz = (d.is3d && k < d.nz - 1) ? d.sz : 0;
 			f[q] = f8[c];
+			if (CODE) {
+				fxm[q] = f8[c - oxm];
+				fxp[q] = f8[c + ox];
+				fym[q] = f8[c - oym];
+				fyp[q] = f8[c + oy];
+			}
 			u0[q] = u[c];
 			u1[q] = u[c + ox];
 			v0[q] = v[c];
 			v1[q] = v[c + oy];
 			w0[q] = w[c];
-			w1[q] = w[c + oz];
+		}
+		// z-neighbours from the thread's adjacent planes (clamped planes
+		// repeat the last one, as the zero offsets there do); the planes
+		// below the first and above the last are loaded
+		{
+			const int kl = min(p.k + KZ - 1, d.nz - 1), k0 = min(p.k, d.nz - 1);
+			const long long cl = d.idx(i, j, kl) + ((d.is3d && kl < d.nz - 1) ? d.sz : 0);
+			w1[KZ - 1] = w[cl];
+			if (CODE) {
+				fzp[KZ - 1] = f8[cl];
+				fzm[0] = f8[d.idx(i, j, k0) - ((d.is3d && k0 > 0) ? d.sz : 0)];
+			}
+		}
+#pragma unroll
+		for (int q = 0; q < KZ - 1; q++) {
+			w1[q] = w0[q + 1];
+			if (CODE) {
+				fzp[q] = f[q + 1];
+				fzm[q + 1] = f[q];
+			}
 		}
 #pragma unroll
 		for (int q = 0; q < KZ; q++) {
 			const int k = p.k + q;
 			if (k >= d.nz) break;
+			if (CODE)
+				code[d.idx(i, j, k)] =
+				    code_byte(d, i, j, k, d.idx(i, j, k), -1, f[q], fxm[q], fxp[q], fym[q], fyp[q], fzm[q], fzp[q]);
 			float set = 0.f;
 			if (interior_c(d, i, j, k) && (f[q] & kFluid)) {
 				set = u0[q] - u1[q] + v0[q] - v1[q];
@@ -219,7 +257,8 @@ __global__ void k_add_const(float* a, long long n, float v) {
 // knCorrectVelocity (pressure.cpp:87-109), bnd=1.  kLightKZ planes per
 // thread with all loads issued first; each face is updated in a register in
 // the reference's order and written once.
-__device__ __forceinline__ void body_correct_velocity(const CorrVelP& P, long long blk, long long) {
+template <bool WALL>
+__device__ __forceinline__ void correct_velocity_cells(const CorrVelP& P, long long blk) {
 	const Dims& d = P.d;
 	const uint8_t* __restrict__ f8 = P.f8;
 	const float* __restrict__ pr = P.pr;
@@ -232,7 +271,8 @@ __device__ __forceinline__ void body_correct_velocity(const CorrVelP& P, long lo
 	constexpr int KZ = kLightKZ;
 	// neighbour offsets, valid for every cell (interior cells use the true ones)
 	const long long ox = i > 0 ? 1 : 0, oy = j > 0 ? d.nx : 0;
-	uint8_t f[KZ], gx[KZ], gy[KZ], gz[KZ];
+	const long long oxp = i < d.nx - 1 ? 1 : 0, oyp = j < d.ny - 1 ? d.nx : 0;
+	uint8_t f[KZ], gx[KZ], gy[KZ], gz[KZ], hx[KZ], hy[KZ], hz[KZ];
 	float pc[KZ], px[KZ], py[KZ], pz[KZ], uc[KZ], vc[KZ], wc[KZ];
 #pragma unroll
 	for (int q = 0; q < KZ; q++) {
@@ -241,6 +281,10 @@ __device__ __forceinline__ void body_correct_velocity(const CorrVelP& P, long lo
 		f[q] = f8[c];
 		gx[q] = f8[c - ox];
 		gy[q] = f8[c - oy];
+		if (WALL) {
+			hx[q] = f8[c + oxp];
+			hy[q] = f8[c + oyp];
+		}
 		pc[q] = pr[c];
 		px[q] = pr[c - ox];
 		py[q] = pr[c - oy];
@@ -249,7 +293,8 @@ __device__ __forceinline__ void body_correct_velocity(const CorrVelP& P, long lo
 		wc[q] = w[c];
 	}
 	// z-neighbours: the plane below the first is loaded; for the others it is
-	// the thread's previous plane (planes past nz are clamped and never used)
+	// the thread's previous plane (planes past nz are clamped and never used);
+	// the plane above the last one likewise
 	{
 		const int k0 = min(p.k, d.nz - 1);
 		const long long c0 = d.idx(i, j, k0) - (k0 > 0 ? d.sz : 0);
@@ -261,38 +306,67 @@ __device__ __forceinline__ void body_correct_velocity(const CorrVelP& P, long lo
 		gz[q] = f[q - 1];
 		pz[q] = pc[q - 1];
 	}
+	if (WALL) {
+#pragma unroll
+		for (int q = 0; q < KZ - 1; q++) hz[q] = f[q + 1];
+		const int k3 = min(p.k + KZ - 1, d.nz - 1);
+		hz[KZ - 1] = f8[d.idx(i, j, k3) + ((d.is3d && k3 < d.nz - 1) ? d.sz : 0)];
+	}
 #pragma unroll
 	for (int q = 0; q < KZ; q++) {
 		const int k = p.k + q;
 		if (k >= d.nz) break;
-		if (!interior_c(d, i, j, k)) continue;
 		const long long c = d.idx(i, j, k);
 		const uint8_t fc = f[q];
 		float nu = uc[q], nv = vc[q], nw = wc[q];
-		if (fc & kFluid) {
-			const float pcc = pc[q];
-			if (gx[q] & kFluid) nu -= (pcc - px[q]);
-			if (gy[q] & kFluid) nv -= (pcc - py[q]);
-			if (d.is3d && (gz[q] & kFluid)) nw -= (pcc - pz[q]);
-			if (gx[q] & kEmpty) nu -= pcc;
-			if (gy[q] & kEmpty) nv -= pcc;
-			if (d.is3d && (gz[q] & kEmpty)) nw -= pcc;
-		} else if ((fc & kEmpty) && !(fc & kOutflow)) {
-			if (gx[q] & kFluid) nu += px[q];
-			else nu = 0.f;
-			if (gy[q] & kFluid) nv += py[q];
-			else nv = 0.f;
-			if (d.is3d) {
-				if (gz[q] & kFluid) nw += pz[q];
-				else nw = 0.f;
+		bool cor = interior_c(d, i, j, k);
+		if (cor) {
+			if (fc & kFluid) {
+				const float pcc = pc[q];
+				if (gx[q] & kFluid) nu -= (pcc - px[q]);
+				if (gy[q] & kFluid) nv -= (pcc - py[q]);
+				if (d.is3d && (gz[q] & kFluid)) nw -= (pcc - pz[q]);
+				if (gx[q] & kEmpty) nu -= pcc;
+				if (gy[q] & kEmpty) nv -= pcc;
+				if (d.is3d && (gz[q] & kEmpty)) nw -= pcc;
+			} else if ((fc & kEmpty) && !(fc & kOutflow)) {
+				if (gx[q] & kFluid) nu += px[q];
+				else nu = 0.f;
+				if (gy[q] & kFluid) nv += py[q];
+				else nv = 0.f;
+				if (d.is3d) {
+					if (gz[q] & kFluid) nw += pz[q];
+					else nw = 0.f;
+				}
+			} else {
+				cor = false;
 			}
-		} else {
-			continue;
 		}
-		u[c] = nu;
-		v[c] = nv;
-		if (d.is3d) w[c] = nw;
+		if (WALL) {
+			// then setWallBcs (no obstacle velocity): written faces take 0
+			const WallRule r = wall_rule(d, i, j, k, fc, gx[q], hx[q], gy[q], hy[q], gz[q], hz[q]);
+			if (r.wu) nu = 0.f;
+			if (r.wv) nv = 0.f;
+			if (r.ww) nw = 0.f;
+			if (cor || r.wu) u[c] = nu;
+			if (cor || r.wv) v[c] = nv;
+			if ((cor && d.is3d) || r.ww) w[c] = nw;
+		} else if (cor) {
+			u[c] = nu;
+			v[c] = nv;
+			if (d.is3d) w[c] = nw;
+		}
 	}
+}
+
+__device__ __forceinline__ void body_correct_velocity(const CorrVelP& P, long long blk, long long) {
+	correct_velocity_cells<false>(P, blk);
+}
+// correctVelocity then setWallBcs(flags, vel) in one pass: each face's
+// corrected value is zeroed where KnSetWallBcs writes it (no obstacle
+// velocity), and written once
+__device__ __forceinline__ void body_correct_velocity_wall(const CorrVelP& P, long long blk, long long) {
+	correct_velocity_cells<true>(P, blk);
 }
 
 // ---------------------------------------------------------------------------
@@ -1559,8 +1633,10 @@ static MgHierarchy* mic_hierarchy(Scene* s, const Grid* flags, const uint8_t* co
 // RHS (MakeRhs, pressure.cpp:32-84) into rhs; with enforceCompatibility the
 // mean correction is reduced and added on the device.  part: FP64 partials
 // (2 per block of the k_make_rhs launch, plus one float slot after them).
+// code (not with the boundary inputs): the matrix code with no pinned cell
+// is written by the same pass.
 static void compute_rhs(Scene* s, Grid* flags, Grid* vel, float* rhs, int enforceCompatibility,
-                        double* part = nullptr, const BndDev* bnd = nullptr) {
+                        double* part = nullptr, const BndDev* bnd = nullptr, uint8_t* code = nullptr) {
 	const Tiles t = tiles_for(s->d, kLightKZ);
 	const bool extra = bnd && (bnd->fx || bnd->pcc || (bnd->phi && bnd->curv));
 	const unsigned nb = extra ? flat_blocks(s->d.n) : (unsigned)t.count();
@@ -1568,14 +1644,19 @@ static void compute_rhs(Scene* s, Grid* flags, Grid* vel, float* rhs, int enforc
 		s->ensurePartials(3 * (size_t)nb + 16, 2 * (size_t)nb + 16);
 		part = s->dpart;
 	}
+	MFX_CHECK(!(extra && code), "the matrix code is built with the RHS only without boundary inputs");
 	if (extra)
 		hipLaunchKernelGGL(k_make_rhs_bnd, dim3(nb), dim3(256), 0, s->stream, s->d, (const uint8_t*)flags->f8,
 		                   (const float*)vel->comp(0), (const float*)vel->comp(1), (const float*)vel->comp(2), *bnd,
 		                   rhs, part);
-	else
-		hipLaunchKernelGGL(k_make_rhs, dim3(nb), dim3(kBlock), 0, s->stream, s->d, t, flags->f8,
+	else if (code)
+		hipLaunchKernelGGL(k_make_rhs<true>, dim3(nb), dim3(kBlock), 0, s->stream, s->d, t, flags->f8,
 		                   (const float*)vel->comp(0), (const float*)vel->comp(1), (const float*)vel->comp(2), rhs,
-		                   part);
+		                   part, code);
+	else
+		hipLaunchKernelGGL(k_make_rhs<false>, dim3(nb), dim3(kBlock), 0, s->stream, s->d, t, flags->f8,
+		                   (const float*)vel->comp(0), (const float*)vel->comp(1), (const float*)vel->comp(2), rhs,
+		                   part, nullptr);
 	if (enforceCompatibility) {
 		float* corr = (float*)(part + 2 * (size_t)nb);
 		hipLaunchKernelGGL(k_compat_corr, dim3(1), dim3(1024), 0, s->stream, (const double*)part, (int)nb, corr);
@@ -1590,6 +1671,7 @@ static void compute_rhs(Scene* s, Grid* flags, Grid* vel, float* rhs, int enforc
 // problems through work lists.
 struct CgRun {
 	Scene* s = nullptr;
+	bool codeReady = false;  // the matrix code was written with the RHS (no pinned cell)
 	Grid* flags = nullptr;
 	float* rhs = nullptr;
 	float* x = nullptr;
@@ -1788,7 +1870,9 @@ static void cg_setup(CgRun& R, Scene* s, Grid* flags, float* rhs, float* x, cons
 		if (!s->code) MFX_HIP(hipMalloc((void**)&s->code, (size_t)d.n));
 		R.code = s->code;
 	}
-	{
+	if (R.codeReady) {
+		MFX_CHECK(R.pooled && !R.fixing && R.code == s->code, "matrix code built with the RHS for another solve");
+	} else {
 		const Tiles tc = tiles_for(d, kLightKZ);
 		hipLaunchKernelGGL(k_build_code, dim3((unsigned)tc.count()), dim3(kBlock), 0, s->stream, d, tc, flags->f8,
 		                   R.code, (const CgState*)R.st);
@@ -2012,8 +2096,10 @@ static void cg_finish_collect(CgRun& R, mfx_solve_stats* stats) {
 static void cg_drive(CgRun& R, double target, mfx_solve_stats* stats);
 
 static void solve_system(Scene* s, Grid* flags, float* rhs, float* x, const mfx_pressure_opts& o,
-                         mfx_solve_stats* stats, const CgRun* bufs = nullptr, const BndDev* bnd = nullptr) {
+                         mfx_solve_stats* stats, const CgRun* bufs = nullptr, const BndDev* bnd = nullptr,
+                         bool codeReady = false) {
 	CgRun R;
+	R.codeReady = codeReady;
 	if (bufs) {
 		// caller-provided buffers (allocation-free solve)
 		R.pooled = false;
@@ -2769,6 +2855,7 @@ void mfx_pressure_opts_default(mfx_pressure_opts* o) {
 	o->maxIterOverride = 0;
 	o->precision = 0;
 	o->trueResidualEvery = 0;
+	o->deferCorrection = 0;
 }
 
 void mfx_pressure_bnd_default(mfx_pressure_bnd* b) {
@@ -2827,6 +2914,7 @@ int mfx_solve_pressure_bnd(mfx_grid* vel, mfx_grid* pressure, const mfx_grid* fl
 	check_pressure_args(v, p, f);
 	MFX_CHECK(!retRhs || G(retRhs)->isReal(), "solvePressure: retRhs must be a RealGrid");
 	const mfx_pressure_opts o = opts_or_default(opts);
+	MFX_CHECK(!o.deferCorrection, "deferCorrection is honoured by mfx_solve_pressure only");
 	const BndDev B = bnd_of(bnd, f);
 	Scene* s = v->s;
 	float* rhs = s->getReal();
@@ -2887,11 +2975,14 @@ int mfx_solve_pressure(mfx_grid* vel, mfx_grid* pressure, const mfx_grid* flags,
 	MFX_CHECK(!retRhs || G(retRhs)->isReal(), "solvePressure: retRhs must be a RealGrid");
 	const mfx_pressure_opts o = opts_or_default(opts);
 	Scene* s = v->s;
+	// without a pinned cell the matrix code is written by the RHS pass
+	const bool fuseCode = !(o.zeroPressureFixing || o.cgAccuracy < 1e-07f);
+	if (fuseCode && !s->code) MFX_HIP(hipMalloc((void**)&s->code, (size_t)s->d.n));
 	float* rhs = s->getReal();
 	try {
-		compute_rhs(s, f, v, rhs, o.enforceCompatibility);
-		solve_system(s, f, rhs, p->real(), o, stats);
-		correct_velocity(s, f, v, p->real());
+		compute_rhs(s, f, v, rhs, o.enforceCompatibility, nullptr, nullptr, fuseCode ? s->code : nullptr);
+		solve_system(s, f, rhs, p->real(), o, stats, nullptr, nullptr, fuseCode);
+		if (!o.deferCorrection) correct_velocity(s, f, v, p->real());
 		if (retRhs)
 			MFX_HIP(hipMemcpyAsync(G(retRhs)->real(), rhs, sizeof(float) * (size_t)s->d.n, hipMemcpyDeviceToDevice,
 			                       s->stream));
@@ -2958,6 +3049,7 @@ int mfx_solve_pressure_batch(int count, mfx_grid* const* vel, mfx_grid* const* p
 		for (int j = 0; j < i; j++)
 			MFX_CHECK(v[(size_t)j]->s != v[(size_t)i]->s, "solvePressureBatch: each problem needs its own solver");
 	}
+	MFX_CHECK(!opts || !opts->deferCorrection, "deferCorrection is honoured by mfx_solve_pressure only");
 	solve_batch(v, p, f, opts_or_default(opts), stats);
 	API_END
 }
@@ -2968,6 +3060,7 @@ int mfx_solve_pressure_noalloc(mfx_grid* vel, mfx_grid* pressure, const mfx_grid
 	Grid *v = G(vel), *p = G(pressure), *f = G(flags);
 	check_pressure_args(v, p, f);
 	const mfx_pressure_opts o = opts_or_default(opts);
+	MFX_CHECK(!o.deferCorrection, "deferCorrection is honoured by mfx_solve_pressure only");
 	Scene* s = v->s;
 	const Dims& d = s->d;
 	MFX_CHECK(o.preconditioner == MFX_PC_NONE || (o.preconditioner == MFX_PC_MIC && !d.is3d),
@@ -3100,6 +3193,17 @@ int mfx_correct_velocity(mfx_grid* vel, const mfx_grid* pressure, const mfx_grid
 	Grid *v = G(vel), *p = G(pressure), *f = G(flags);
 	check_pressure_args(v, p, f);
 	correct_velocity(v->s, f, v, p->real());
+	API_END
+}
+
+int mfx_correct_velocity_wall_bcs(mfx_grid* vel, const mfx_grid* pressure, const mfx_grid* flags) {
+	API_BEGIN
+	Grid *v = G(vel), *p = G(pressure), *f = G(flags);
+	check_pressure_args(v, p, f);
+	Scene* s = v->s;
+	const Tiles t = tiles_for(s->d, kLightKZ);
+	launch_one<CorrVelP, body_correct_velocity_wall, kBlock>(
+	    s->stream, t.count(), CorrVelP{s->d, t, f->f8, p->real(), v->comp(0), v->comp(1), v->comp(2)});
 	API_END
 }
 

@@ -103,12 +103,21 @@ int step_pre(const mfx_serve_session& s) {
 	if (mfx_advect_semi_lagrange(s.flags, s.vel, s.density, 2, 1.0f, 1, 2, 1)) return -1;
 	if (mfx_advect_semi_lagrange(s.flags, s.vel, s.vel, 2, 1.0f, 1, 2, 1)) return -1;
 	if (mfx_reset_outflow(s.flags, nullptr, s.density)) return -1;
-	if (mfx_set_wall_bcs(s.flags, s.vel, nullptr)) return -1;
-	return mfx_add_buoyancy(s.flags, s.density, s.vel, s.gravity, 1.0f, 1);
+	// setWallBcs then addBuoyancy, one pass
+	return mfx_set_wall_bcs_buoyancy(s.flags, s.vel, s.density, s.gravity, 1.0f, 1);
 }
 
 int step_post(const mfx_serve_session& s, int kind) {
 	return kind == MFX_REQ_STEP ? mfx_set_wall_bcs(s.flags, s.vel, nullptr) : 0;
+}
+
+// solvePressure; for a step its velocity correction and the setWallBcs that
+// follows run as one pass
+int solve_single(const mfx_serve_session& s, int kind, const mfx_pressure_opts& opts, mfx_solve_stats* st) {
+	mfx_pressure_opts o = opts;
+	o.deferCorrection = kind == MFX_REQ_STEP ? 1 : 0;
+	if (mfx_solve_pressure(s.vel, s.pressure, s.flags, &o, st, nullptr)) return -1;
+	return kind == MFX_REQ_STEP ? mfx_correct_velocity_wall_bcs(s.vel, s.pressure, s.flags) : 0;
 }
 
 // Runs one tick; fills res[i].status / error / stats.
@@ -125,9 +134,8 @@ void run_tick(mfx_server* sv, const std::vector<Request>& batch, std::vector<mfx
 	auto sess = [&](size_t i) -> const mfx_serve_session& { return batch[i].g; };
 	if (n == 1) {
 		const mfx_serve_session& s = sess(0);
-		if ((batch[0].kind == MFX_REQ_STEP && step_pre(s)) ||
-		    mfx_solve_pressure(s.vel, s.pressure, s.flags, &sv->opts.solve, &st[0], nullptr) ||
-		    step_post(s, batch[0].kind) || mfx_scene_sync(static_cast<mfx_scene*>(batch[0].scene))) {
+		if ((batch[0].kind == MFX_REQ_STEP && step_pre(s)) || solve_single(s, batch[0].kind, sv->opts.solve, &st[0]) ||
+		    mfx_scene_sync(static_cast<mfx_scene*>(batch[0].scene))) {
 			fail(0, 1);
 			return;
 		}
@@ -144,7 +152,9 @@ void run_tick(mfx_server* sv, const std::vector<Request>& batch, std::vector<mfx
 			P[i] = sess(i).pressure;
 			F[i] = sess(i).flags;
 		}
-		if (mfx_solve_pressure_batch((int)n, V.data(), P.data(), F.data(), &sv->opts.solve, st.data())) {
+		mfx_pressure_opts o = sv->opts.solve;
+		o.deferCorrection = 0;
+		if (mfx_solve_pressure_batch((int)n, V.data(), P.data(), F.data(), &o, st.data())) {
 			fail(0, n);
 			return;
 		}

@@ -24,9 +24,39 @@ import time
 import numpy as np
 
 from . import _abi, gridio
-from ._abi import check, lib
+from ._abi import check
+from ._abi import lib as _lib_now
 
-lib()  # the GPU path is the only path: fail at import if libmfx.so is missing
+_lib_now()  # the GPU path is the only path: fail at import if libmfx.so is missing
+
+# Held-back plugin calls.  setWallBcs (without obvel, fractions or phiObs)
+# and the velocity correction that ends solvePressure are kept for one call,
+# so that the call the plume scenes make next runs them fused with its own
+# pass over the grid (setWallBcs + addBuoyancy; correctVelocity +
+# setWallBcs: mfx_set_wall_bcs_buoyancy, mfx_correct_velocity_wall_bcs, each
+# bit-identical to the separate calls).  Every other call into the library,
+# grid methods included, goes through lib(), which first runs the held-back
+# call on its own, so no grid is ever read or written out of order.
+_pending = None
+
+
+def _flush():
+    """Run the held-back call, if any."""
+    global _pending
+    p, _pending = _pending, None
+    if p is None:
+        return
+    if p[0] == "wall":
+        check(_lib_now().mfx_set_wall_bcs(p[1]._h, p[2]._h, None))
+    else:
+        check(_lib_now().mfx_correct_velocity(p[2]._h, p[3]._h, p[1]._h))
+
+
+def lib():
+    """The library, after any held-back plugin call has run."""
+    if _pending is not None:
+        _flush()
+    return _lib_now()
 
 # ---------------------------------------------------------------------------
 # constants (source/python/defines.py:15-50, pwrapper/registry.cpp:398-420)
@@ -729,11 +759,23 @@ def setWallBcs(*args, **kwargs):
     """extforces.cpp:307-315"""
     a = _parse([("flags", _REQ), ("vel", _REQ), ("obvel", None), ("fractions", None), ("phiObs", None),
                 ("boundaryWidth", 0)], args, kwargs)
+    global _pending
+    fl, vel = a["flags"], a["vel"]
     if _h(a["fractions"]) is not None and _h(a["phiObs"]) is not None:
-        check(lib().mfx_set_wall_bcs_frac(_h(a["flags"]), _h(a["vel"]), _h(a["obvel"]), _h(a["fractions"]),
+        check(lib().mfx_set_wall_bcs_frac(_h(fl), _h(vel), _h(a["obvel"]), _h(a["fractions"]),
                                           _h(a["phiObs"]), int(a["boundaryWidth"])))
+    elif _h(a["obvel"]) is None and isinstance(fl, FlagGrid) and isinstance(vel, MACGrid) and \
+            fl.parent is vel.parent:
+        p = _pending
+        if p is not None and p[0] == "corr" and p[1] is fl and p[2] is vel:
+            # the held-back correction of solvePressure, fused with this call
+            _pending = None
+            check(_lib_now().mfx_correct_velocity_wall_bcs(vel._h, p[3]._h, fl._h))
+        else:
+            _flush()
+            _pending = ("wall", fl, vel)
     else:
-        check(lib().mfx_set_wall_bcs(_h(a["flags"]), _h(a["vel"]), _h(a["obvel"])))
+        check(lib().mfx_set_wall_bcs(_h(fl), _h(vel), _h(a["obvel"])))
 
 
 @_plugin
@@ -741,7 +783,21 @@ def addBuoyancy(*args, **kwargs):
     """extforces.cpp:83-87"""
     a = _parse([("flags", _REQ), ("density", _REQ), ("vel", _REQ), ("gravity", _REQ), ("coefficient", 1.0),
                 ("scale", True)], args, kwargs)
+    global _pending
     g = _v3(a["gravity"], "gravity")
+    p = _pending
+    if p is not None and p[0] == "wall" and p[1] is a["flags"] and p[2] is a["vel"]:
+        # the held-back setWallBcs on the same grids, fused with this call
+        _pending = None
+        try:
+            check(_lib_now().mfx_set_wall_bcs_buoyancy(_h(a["flags"]), _h(a["vel"]), _h(a["density"]),
+                                                       _abi.floats(*g), C.c_float(a["coefficient"]),
+                                                       int(bool(a["scale"]))))
+        except RuntimeError:
+            # the arguments were refused before any work: setWallBcs alone
+            check(_lib_now().mfx_set_wall_bcs(p[1]._h, p[2]._h, None))
+            raise
+        return
     check(lib().mfx_add_buoyancy(_h(a["flags"]), _h(a["density"]), _h(a["vel"]), _abi.floats(*g),
                                  C.c_float(a["coefficient"]), int(bool(a["scale"]))))
 
@@ -819,13 +875,21 @@ def _report(solver, st, what="solvePressure"):
 @_plugin
 def solvePressure(*args, **kwargs):
     """pressure.cpp:480-521"""
+    global _pending
     sig = [("vel", _REQ), ("pressure", _REQ), ("flags", _REQ)] + _PRESSURE_SIG + [("retRhs", None)]
     a = _parse(sig, args, kwargs)
     solver = a["vel"].parent
     o = _opts(a, solver)
     st = _abi.SolveStats()
     b = _bnd(a)
-    if b is None:
+    if b is None and _h(a["retRhs"]) is None:
+        # the velocity correction is held back (run by the next call, fused
+        # with it when that is setWallBcs on the same grids)
+        o.deferCorrection = 1
+        check(lib().mfx_solve_pressure(_h(a["vel"]), _h(a["pressure"]), _h(a["flags"]), C.byref(o), C.byref(st),
+                                       None))
+        _pending = ("corr", a["flags"], a["vel"], a["pressure"])
+    elif b is None:
         check(lib().mfx_solve_pressure(_h(a["vel"]), _h(a["pressure"]), _h(a["flags"]), C.byref(o), C.byref(st),
                                        _h(a["retRhs"])))
     else:

@@ -36,7 +36,8 @@ import numpy as np
 
 from . import _abi
 from . import manta as m
-from ._abi import check, lib
+from ._abi import check
+from .manta import lib  # runs any held-back plugin call first
 
 REQ_STEP, REQ_SOLVE = "step", "solve"
 Request = collections.namedtuple("Request", "rid arrival session kind")

@@ -1,0 +1,226 @@
+"""The fused per-cell passes against the separate calls they replace, bit for
+bit (float bit patterns, signed zeros included), and against the oracle:
+
+- mfx_set_wall_bcs_buoyancy = setWallBcs + addBuoyancy;
+- mfx_correct_velocity_wall_bcs = correctVelocity + setWallBcs;
+- the RHS pass that also writes the matrix code (every solve without a
+  pinned cell) = the separate matrix-code pass (the boundary-input entry
+  point with no inputs keeps the two passes);
+- the Python face, which holds back setWallBcs and solvePressure's velocity
+  correction for one call to fuse them, reads exactly as if every call had
+  run at once."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from mantaflow_amd import _abi
+from oracle import FLUID, OBSTACLE, OUTFLOW, PC_MGSTATIC, PC_NONE
+from tests.helpers import GpuScene, assert_bitexact, make_flags, rand_vel, smooth_vel
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(24, 20, 1), (17, 13, 11), (40, 33, 29)]
+GRAV = (0.0, -4e-3, 1e-3)
+
+
+def same_bits(a, b, what):
+    a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
+    assert a.shape == b.shape, what
+    bad = np.argwhere(a.view(np.uint32) != b.view(np.uint32))
+    assert bad.size == 0, "%s: %d differing words, first %s" % (what, len(bad), bad[:4].tolist())
+
+
+def _flags(oracle, rng, shape, stick, open_bound):
+    f = make_flags(oracle, shape, bw=1, open_bound=open_bound, obstacles=5, rng=rng, stick=stick)
+    if open_bound:
+        oracle.reset_outflow(shape, f)
+    return f
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(_abi.lib().mfx_last_error().decode())
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("stick,open_bound", [(False, ""), (True, ""), (False, "Y"), (True, "xY")])
+def test_wall_buoyancy_fused(manta, oracle, rng, shape, stick, open_bound):
+    n = int(np.prod(shape))
+    flags = _flags(oracle, rng, shape, stick, open_bound)
+    vel = rand_vel(rng, n)
+    vel[rng.random(n) < 0.05] *= -0.0  # some signed zeros
+    dens = rng.uniform(0, 1, n).astype(np.float32)
+    sc = GpuScene(manta, shape, 0.8)
+    f, d = sc.flags(flags), sc.real(dens)
+    # separate C calls
+    a = sc.mac(vel)
+    L = _abi.lib()
+    _check(L.mfx_set_wall_bcs(f._h, a._h, None))
+    _check(L.mfx_add_buoyancy(f._h, d._h, a._h, _abi.floats(*GRAV), C.c_float(1.3), 1))
+    # fused C call
+    b = sc.mac(vel)
+    _check(L.mfx_set_wall_bcs_buoyancy(f._h, b._h, d._h, _abi.floats(*GRAV), C.c_float(1.3), 1))
+    same_bits(b.toNumpy(), a.toNumpy(), "fused wall+buoyancy")
+    # the Python face (setWallBcs held back, fused into addBuoyancy)
+    c = sc.mac(vel)
+    manta.setWallBcs(flags=f, vel=c)
+    manta.addBuoyancy(flags=f, density=d, vel=c, gravity=manta.vec3(*GRAV), coefficient=1.3)
+    same_bits(c.toNumpy(), a.toNumpy(), "python face wall+buoyancy")
+    # and the oracle
+    ref = vel.copy()
+    oracle.set_wall_bcs(shape, flags, ref)
+    oracle.add_buoyancy(shape, np.float32(0.8), flags, dens, ref, GRAV, 1.3, True)
+    assert_bitexact(b.toNumpy(), ref, "oracle wall+buoyancy")
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("stick,open_bound", [(False, ""), (True, ""), (False, "Y"), (True, "xXyY")])
+def test_correct_wall_fused(manta, oracle, rng, shape, stick, open_bound):
+    n = int(np.prod(shape))
+    flags = _flags(oracle, rng, shape, stick, open_bound)
+    # a few empty cells inside, as a free surface would leave them
+    empt = (rng.random(n) < 0.03) & ((flags & FLUID) != 0)
+    flags[empt] = (flags[empt] & ~FLUID) | 4
+    vel = rand_vel(rng, n)
+    p = rng.standard_normal(n).astype(np.float32)
+    sc = GpuScene(manta, shape)
+    f, pr = sc.flags(flags), sc.real(p)
+    L = _abi.lib()
+    a = sc.mac(vel)
+    _check(L.mfx_correct_velocity(a._h, pr._h, f._h))
+    _check(L.mfx_set_wall_bcs(f._h, a._h, None))
+    b = sc.mac(vel)
+    _check(L.mfx_correct_velocity_wall_bcs(b._h, pr._h, f._h))
+    same_bits(b.toNumpy(), a.toNumpy(), "fused correction+wall")
+    ref = vel.copy()
+    oracle.correct_velocity(shape, flags, ref, p)
+    oracle.set_wall_bcs(shape, flags, ref)
+    assert_bitexact(b.toNumpy(), ref, "oracle correction+wall")
+
+
+def _solve_c(L, v, p, f, pc, fused, defer=False):
+    o = _abi.PressureOpts()
+    L.mfx_pressure_opts_default(C.byref(o))
+    o.cgAccuracy = 1e-4
+    o.preconditioner = pc
+    o.deferCorrection = int(defer)
+    st = _abi.SolveStats()
+    if fused:
+        _check(L.mfx_solve_pressure(v._h, p._h, f._h, C.byref(o), C.byref(st), None))
+    else:
+        b = _abi.PressureBnd()
+        L.mfx_pressure_bnd_default(C.byref(b))  # no boundary inputs: separate matrix-code pass
+        _check(L.mfx_solve_pressure_bnd(v._h, p._h, f._h, C.byref(o), C.byref(b), C.byref(st), None))
+    return st
+
+
+@pytest.mark.parametrize("shape,pc", [((40, 36, 1), PC_NONE), ((33, 28, 30), PC_MGSTATIC), ((48, 48, 48), PC_MGSTATIC)])
+def test_solve_code_with_rhs(manta, oracle, rng, shape, pc):
+    flags = _flags(oracle, rng, shape, False, "Y")
+    vel = smooth_vel(rng, shape, 2, 1.0)
+    oracle.set_wall_bcs(shape, flags, vel)
+    L = _abi.lib()
+    res = []
+    for fused in (True, False):
+        sc = GpuScene(manta, shape)
+        f, v, p = sc.flags(flags), sc.mac(vel), sc.real()
+        st = _solve_c(L, v, p, f, pc, fused)
+        res.append((st.iterations, st.resNorm, p.toNumpy(), v.toNumpy()))
+    assert res[0][0] == res[1][0] and res[0][1] == res[1][1]
+    same_bits(res[0][2], res[1][2], "pressure")
+    same_bits(res[0][3], res[1][3], "velocity")
+    # deferred correction, then the fused correction+wall pass, equals the
+    # solve followed by setWallBcs
+    sc = GpuScene(manta, shape)
+    f, v, p = sc.flags(flags), sc.mac(vel), sc.real()
+    st = _solve_c(L, v, p, f, pc, True, defer=True)
+    assert st.iterations == res[0][0]
+    _check(L.mfx_correct_velocity_wall_bcs(v._h, p._h, f._h))
+    sc2 = GpuScene(manta, shape)
+    f2, v2, p2 = sc2.flags(flags), sc2.mac(vel), sc2.real()
+    _solve_c(L, v2, p2, f2, pc, True)
+    _check(L.mfx_set_wall_bcs(f2._h, v2._h, None))
+    same_bits(v.toNumpy(), v2.toNumpy(), "deferred correction + wall")
+    same_bits(p.toNumpy(), p2.toNumpy(), "deferred correction pressure")
+
+
+def test_deferred_entry_points_refuse_other_paths(manta, oracle, rng):
+    shape = (20, 18, 1)
+    flags = _flags(oracle, rng, shape, False, "")
+    sc = GpuScene(manta, shape)
+    f, v, p = sc.flags(flags), sc.mac(rand_vel(rng, int(np.prod(shape)))), sc.real()
+    L = _abi.lib()
+    o = _abi.PressureOpts()
+    L.mfx_pressure_opts_default(C.byref(o))
+    o.deferCorrection = 1
+    b = _abi.PressureBnd()
+    L.mfx_pressure_bnd_default(C.byref(b))
+    st = _abi.SolveStats()
+    assert L.mfx_solve_pressure_bnd(v._h, p._h, f._h, C.byref(o), C.byref(b), C.byref(st), None) != 0
+    assert b"deferCorrection" in L.mfx_last_error()
+    V = (C.c_void_p * 1)(v._h)
+    P = (C.c_void_p * 1)(p._h)
+    F = (C.c_void_p * 1)(f._h)
+    assert L.mfx_solve_pressure_batch(1, V, P, F, C.byref(o), C.byref(st)) != 0
+    assert b"deferCorrection" in L.mfx_last_error()
+
+
+@pytest.mark.parametrize("shape", [(30, 26, 1), (21, 19, 17)])
+def test_python_face_holds_back_in_order(manta, oracle, rng, shape):
+    """solvePressure's correction and a plain setWallBcs are held back one
+    call: whatever comes next (a grid read, another plugin, a setWallBcs on
+    other grids) sees the grids as if they had run at once."""
+    n = int(np.prod(shape))
+    flags = _flags(oracle, rng, shape, True, "Y")
+    vel = smooth_vel(rng, shape, 2, 1.0)
+    dens = rng.uniform(0, 1, n).astype(np.float32)
+    L = _abi.lib()
+
+    def reference():
+        sc = GpuScene(manta, shape)
+        f, v, p, d = sc.flags(flags), sc.mac(vel), sc.real(), sc.real(dens)
+        _check(L.mfx_set_wall_bcs(f._h, v._h, None))
+        _check(L.mfx_add_buoyancy(f._h, d._h, v._h, _abi.floats(*GRAV), C.c_float(1.0), 1))
+        _solve_c(L, v, p, f, PC_MGSTATIC, True)
+        after_solve = v.toNumpy()
+        _check(L.mfx_set_wall_bcs(f._h, v._h, None))
+        return after_solve, v.toNumpy(), p.toNumpy()
+
+    ref_solve, ref_step, ref_p = reference()
+    kw = dict(cgAccuracy=1e-4, preconditioner=manta.PcMGStatic)
+    # a read right after solvePressure runs the held-back correction alone
+    sc = GpuScene(manta, shape)
+    f, v, p, d = sc.flags(flags), sc.mac(vel), sc.real(), sc.real(dens)
+    manta.setWallBcs(flags=f, vel=v)
+    manta.addBuoyancy(flags=f, density=d, vel=v, gravity=manta.vec3(*GRAV))
+    manta.solvePressure(flags=f, vel=v, pressure=p, **kw)
+    same_bits(v.toNumpy(), ref_solve, "read after solvePressure")
+    manta.setWallBcs(flags=f, vel=v)
+    same_bits(v.toNumpy(), ref_step, "step with a read in between")
+    # the full step with nothing in between (both fusions)
+    sc = GpuScene(manta, shape)
+    f, v, p, d = sc.flags(flags), sc.mac(vel), sc.real(), sc.real(dens)
+    manta.setWallBcs(flags=f, vel=v)
+    manta.addBuoyancy(flags=f, density=d, vel=v, gravity=manta.vec3(*GRAV))
+    manta.solvePressure(flags=f, vel=v, pressure=p, **kw)
+    manta.setWallBcs(flags=f, vel=v)
+    same_bits(v.toNumpy(), ref_step, "fused step")
+    same_bits(p.toNumpy(), ref_p, "fused step pressure")
+    # a held-back setWallBcs followed by another plugin on the same grid
+    sc = GpuScene(manta, shape)
+    f, v = sc.flags(flags), sc.mac(vel)
+    manta.setWallBcs(flags=f, vel=v)
+    v.multConst(manta.vec3(2, 2, 2))
+    ref = vel.copy()
+    oracle.set_wall_bcs(shape, flags, ref)
+    assert_bitexact(v.toNumpy(), ref * np.float32(2), "setWallBcs then multConst")
+    # addBuoyancy refused after a held-back setWallBcs: the wall still applies
+    sc = GpuScene(manta, shape)
+    f, v = sc.flags(flags), sc.mac(vel)
+    manta.setWallBcs(flags=f, vel=v)
+    with pytest.raises(RuntimeError):
+        manta.addBuoyancy(flags=f, density=sc.mac(), vel=v, gravity=manta.vec3(*GRAV))
+    ref = vel.copy()
+    oracle.set_wall_bcs(shape, flags, ref)
+    assert_bitexact(v.toNumpy(), ref, "wall applied after a refused addBuoyancy")
