@@ -92,6 +92,11 @@ int mfx_grid_mult_const(mfx_grid* g, const float* v);
 int mfx_grid_add(mfx_grid* g, const mfx_grid* a);
 int mfx_grid_sub(mfx_grid* g, const mfx_grid* a);
 int mfx_grid_mult(mfx_grid* g, const mfx_grid* a);
+/* g.mult(a) then g.add(b) on Real grids as one pass, bit-identical to the
+ * two calls (grid.cpp:205-280).  No counterpart in the reference; the
+ * manta module uses it for a mult directly followed by an add on the same
+ * grid (a density source).  b must not be g. */
+int mfx_grid_mult_add(mfx_grid* g, const mfx_grid* a, const mfx_grid* b);
 int mfx_grid_add_scaled(mfx_grid* g, const mfx_grid* a, const float* factor);
 int mfx_grid_clamp(mfx_grid* g, float vmin, float vmax);
 /* getMax/getMin/getMaxAbs/getL1/getL2 (source/grid.cpp:315-389); bnd used by L1/L2 */

@@ -80,6 +80,7 @@ SIGNATURES = {
     "mfx_grid_add": (_I, [_P, _P]),
     "mfx_grid_sub": (_I, [_P, _P]),
     "mfx_grid_mult": (_I, [_P, _P]),
+    "mfx_grid_mult_add": (_I, [_P, _P, _P]),
     "mfx_grid_add_scaled": (_I, [_P, _P, _FP]),
     "mfx_grid_clamp": (_I, [_P, _F, _F]),
     "mfx_grid_reduce": (_I, [_P, _I, _I, C.POINTER(C.c_double)]),

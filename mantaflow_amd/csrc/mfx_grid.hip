@@ -239,6 +239,21 @@ static void launch_elementwise_f(Scene* s, float* a, const float* b, long long n
 		hipLaunchKernelGGL(k_elementwise<float>, dim3(flat_blocks(n)), dim3(256), 0, s->stream, a, b, n, op, v, v2);
 }
 
+// g = g * a, then g = g + b (two roundings, as the two calls), one pass
+__global__ void k_mult_add4(float4* __restrict__ g, const float4* __restrict__ a, const float4* __restrict__ b,
+                            long long n4) {
+	for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n4;
+	     i += (long long)gridDim.x * blockDim.x) {
+		const float4 x = g[i], y = a[i], z = b[i];
+		g[i] = make_float4(x.x * y.x + z.x, x.y * y.y + z.y, x.z * y.z + z.z, x.w * y.w + z.w);
+	}
+}
+__global__ void k_mult_add(float* __restrict__ g, const float* __restrict__ a, const float* __restrict__ b,
+                           long long n) {
+	for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+		g[i] = g[i] * a[i] + b[i];
+}
+
 static void elementwise(Grid* g, const Grid* other, int op, const float* v, const float* v2) {
 	Scene* s = g->s;
 	long long n = s->d.n;
@@ -983,6 +998,28 @@ int mfx_grid_mult(mfx_grid* g, const mfx_grid* a) {
 	NEED(g, "grid");
 	NEED(a, "grid");
 	elementwise(G(g), G(a), EW_MUL, nullptr, nullptr);
+	API_END
+}
+int mfx_grid_mult_add(mfx_grid* g, const mfx_grid* a, const mfx_grid* b) {
+	API_BEGIN
+	NEED(g, "grid");
+	NEED(a, "grid");
+	NEED(b, "grid");
+	Grid *x = G(g), *y = G(a), *z = G(b);
+	MFX_CHECK(x->isReal() && y->isReal() && z->isReal(), "mult+add: needs Real grids");
+	MFX_CHECK(z != x, "mult+add: the added grid must not be the result grid");
+	for (const Grid* o : {y, z})
+		MFX_CHECK(o->s->d.nx == x->s->d.nx && o->s->d.ny == x->s->d.ny && o->s->d.nz == x->s->d.nz,
+		          "different grid resolutions");
+	Scene* s = x->s;
+	const long long n = s->d.n;
+	float* gp = x->real();
+	const float *ap = y->real(), *bp = z->real();
+	if (n % 4 == 0 && ((uintptr_t)gp | (uintptr_t)ap | (uintptr_t)bp) % 16 == 0)
+		hipLaunchKernelGGL(k_mult_add4, dim3(flat_blocks(n / 4)), dim3(256), 0, s->stream, (float4*)gp,
+		                   (const float4*)ap, (const float4*)bp, n / 4);
+	else
+		hipLaunchKernelGGL(k_mult_add, dim3(flat_blocks(n)), dim3(256), 0, s->stream, gp, ap, bp, n);
 	API_END
 }
 int mfx_grid_add_scaled(mfx_grid* g, const mfx_grid* a, const float* factor) {

@@ -29,12 +29,13 @@ from ._abi import lib as _lib_now
 
 _lib_now()  # the GPU path is the only path: fail at import if libmfx.so is missing
 
-# Held-back plugin calls.  setWallBcs (without obvel, fractions or phiObs)
-# and the velocity correction that ends solvePressure are kept for one call,
-# so that the call the plume scenes make next runs them fused with its own
-# pass over the grid (setWallBcs + addBuoyancy; correctVelocity +
-# setWallBcs: mfx_set_wall_bcs_buoyancy, mfx_correct_velocity_wall_bcs, each
-# bit-identical to the separate calls).  Every other call into the library,
+# Held-back calls.  setWallBcs (without obvel, fractions or phiObs), the
+# velocity correction that ends solvePressure and a Real grid's mult are
+# kept for one call, so that the call the plume scenes make next runs them
+# fused with its own pass over the grid (setWallBcs + addBuoyancy;
+# correctVelocity + setWallBcs; mult + add of the same grid:
+# mfx_set_wall_bcs_buoyancy, mfx_correct_velocity_wall_bcs,
+# mfx_grid_mult_add, each bit-identical to the separate calls).  Every other call into the library,
 # grid methods included, goes through lib(), which first runs the held-back
 # call on its own, so no grid is ever read or written out of order.
 _pending = None
@@ -48,6 +49,8 @@ def _flush():
         return
     if p[0] == "wall":
         check(_lib_now().mfx_set_wall_bcs(p[1]._h, p[2]._h, None))
+    elif p[0] == "mult":
+        check(_lib_now().mfx_grid_mult(p[1]._h, p[2]._h))
     else:
         check(_lib_now().mfx_correct_velocity(p[2]._h, p[3]._h, p[1]._h))
 
@@ -493,7 +496,14 @@ class _ScalarOps:
         check(lib().mfx_grid_mult_const(self._h, self._val(s)))
 
     def add(self, a):
+        global _pending
         self._check_same(a)
+        p = _pending
+        if p is not None and p[0] == "mult" and p[1] is self and isinstance(a, RealGrid) and a is not self:
+            # the held-back mult of this grid, fused with this add
+            _pending = None
+            check(_lib_now().mfx_grid_mult_add(self._h, p[2]._h, a._h))
+            return
         check(lib().mfx_grid_add(self._h, a._h))
 
     def sub(self, a):
@@ -501,7 +511,12 @@ class _ScalarOps:
         check(lib().mfx_grid_sub(self._h, a._h))
 
     def mult(self, a):
+        global _pending
         self._check_same(a)
+        if isinstance(self, RealGrid) and isinstance(a, RealGrid):
+            _flush()
+            _pending = ("mult", self, a)  # held back: fused with a following add
+            return
         check(lib().mfx_grid_mult(self._h, a._h))
 
     def addScaled(self, a, factor):

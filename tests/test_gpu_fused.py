@@ -6,9 +6,10 @@ bit (float bit patterns, signed zeros included), and against the oracle:
 - the RHS pass that also writes the matrix code (every solve without a
   pinned cell) = the separate matrix-code pass (the boundary-input entry
   point with no inputs keeps the two passes);
-- the Python face, which holds back setWallBcs and solvePressure's velocity
-  correction for one call to fuse them, reads exactly as if every call had
-  run at once."""
+- mfx_grid_mult_add = Grid.mult + Grid.add of the same grid;
+- the Python face, which holds back setWallBcs, solvePressure's velocity
+  correction and a Real grid's mult for one call to fuse them, reads
+  exactly as if every call had run at once."""
 import ctypes as C
 
 import numpy as np
@@ -224,3 +225,40 @@ def test_python_face_holds_back_in_order(manta, oracle, rng, shape):
     ref = vel.copy()
     oracle.set_wall_bcs(shape, flags, ref)
     assert_bitexact(v.toNumpy(), ref, "wall applied after a refused addBuoyancy")
+
+
+@pytest.mark.parametrize("shape", [(24, 20, 1), (17, 13, 11), (64, 64, 64)])
+def test_mult_add_fused(manta, rng, shape):
+    """Grid.mult held back and fused with a following Grid.add of the same
+    grid (mfx_grid_mult_add), against the two separate C calls; an add of
+    the grid to itself, or any other call in between, runs the mult first."""
+    n = int(np.prod(shape))
+    g0 = rng.standard_normal(n).astype(np.float32)
+    a0 = rng.uniform(-2, 2, n).astype(np.float32)
+    b0 = rng.standard_normal(n).astype(np.float32)
+    g0[rng.random(n) < 0.05] = -0.0
+    sc = GpuScene(manta, shape)
+    L = _abi.lib()
+    ref_g, a, b = sc.real(g0), sc.real(a0), sc.real(b0)
+    _check(L.mfx_grid_mult(ref_g._h, a._h))
+    _check(L.mfx_grid_add(ref_g._h, b._h))
+    ref = ref_g.toNumpy()
+    g = sc.real(g0)
+    _check(L.mfx_grid_mult_add(g._h, a._h, b._h))
+    same_bits(g.toNumpy(), ref, "C mult+add")
+    g = sc.real(g0)
+    g.mult(a)
+    g.add(b)
+    same_bits(g.toNumpy(), ref, "python mult then add")
+    # mult then add of the grid to itself: 2 * (g * a)
+    g = sc.real(g0)
+    g.mult(a)
+    g.add(g)
+    same_bits(g.toNumpy(), (g0 * a0) + (g0 * a0), "python mult then self-add")
+    # a read between the two
+    g = sc.real(g0)
+    g.mult(a)
+    same_bits(g.toNumpy(), g0 * a0, "read after mult")
+    g.add(b)
+    same_bits(g.toNumpy(), ref, "add after a read")
+    assert L.mfx_grid_mult_add(g._h, a._h, g._h) != 0
