@@ -2454,7 +2454,12 @@ static void solve_batch(const std::vector<Grid*>& vel, const std::vector<Grid*>&
 		parallel_for(P, device, [&](int i) {
 			Scene* s = vel[(size_t)i]->s;
 			rhs[(size_t)i] = s->getReal();
-			compute_rhs(s, flg[(size_t)i], vel[(size_t)i], rhs[(size_t)i], o.enforceCompatibility);
+			// without a pinned cell the matrix code is written by the RHS pass
+			const bool fuseCode = !(o.zeroPressureFixing || o.cgAccuracy < 1e-07f);
+			if (fuseCode && !s->code) MFX_HIP(hipMalloc((void**)&s->code, (size_t)s->d.n));
+			compute_rhs(s, flg[(size_t)i], vel[(size_t)i], rhs[(size_t)i], o.enforceCompatibility, nullptr, nullptr,
+			            fuseCode ? s->code : nullptr);
+			R[(size_t)i].codeReady = fuseCode;
 			cg_setup(R[(size_t)i], s, flg[(size_t)i], rhs[(size_t)i], pres[(size_t)i]->real(), o, true);
 			// marks the end of this problem's set-up for the batch stream
 			if (!s->ev) MFX_HIP(hipEventCreateWithFlags(&s->ev, hipEventDisableTiming));
