@@ -882,17 +882,12 @@ __device__ inline bool inb(const Dims& d, int i, int j, int k) {
 	return i >= 0 && j >= 0 && k >= 0 && i < d.nx && j < d.ny && k < d.nz;
 }
 
-__global__ void __launch_bounds__(kBlock) k_outflow_extrap(Dims d, Tiles t, const uint8_t* __restrict__ f8,
-                                                           const float* __restrict__ vu, const float* __restrict__ vv,
-                                                           const float* __restrict__ vw, const float* __restrict__ pu,
-                                                           const float* __restrict__ pv, const float* __restrict__ pw,
-                                                           float timeStep, float* __restrict__ du,
-                                                           float* __restrict__ dv, float* __restrict__ dw) {
-	CellPos p = tile_cell(t);
-	if (p.i >= d.nx || p.j >= d.ny) return;
-	const int i = p.i, j = p.j, k = p.k;
-	const long long ix = d.idx(i, j, k);
-	if (!(f8[ix] & kOutflow)) return;
+__device__ inline void outflow_extrap_cell(const Dims& d, const uint8_t* __restrict__ f8, const float* __restrict__ vu,
+                                           const float* __restrict__ vv, const float* __restrict__ vw,
+                                           const float* __restrict__ pu, const float* __restrict__ pv,
+                                           const float* __restrict__ pw, float timeStep, float* __restrict__ du,
+                                           float* __restrict__ dv, float* __restrict__ dw, int i, int j, int k,
+                                           long long ix) {
 	const float* vel[3] = {vu, vv, vw};
 	const float* prev[3] = {pu, pv, pw};
 	// getBulkVel
@@ -950,25 +945,41 @@ __global__ void __launch_bounds__(kBlock) k_outflow_extrap(Dims d, Tiles t, cons
 	dw[ix] = vd[2];
 }
 
-// kLightKZ planes per thread (tiles_for(d, kLightKZ)): the flags of all
-// planes load first and only outflow cells copy.
-__global__ void __launch_bounds__(kBlock) k_outflow_copy(Dims d, Tiles t, const uint8_t* __restrict__ f8,
-                                                         const float* __restrict__ du, const float* __restrict__ dv,
-                                                         const float* __restrict__ dw, float* __restrict__ vu,
-                                                         float* __restrict__ vv, float* __restrict__ vw) {
-	const CellPos p = tile_cell(t);
-	if (p.i >= d.nx || p.j >= d.ny) return;
-	uint8_t fl[kLightKZ];
-#pragma unroll
-	for (int kk = 0; kk < kLightKZ; kk++) fl[kk] = p.k + kk < d.nz ? f8[d.idx(p.i, p.j, p.k + kk)] : (uint8_t)0;
-	for (int kk = 0; kk < kLightKZ; kk++) {
-		if (!(fl[kk] & kOutflow)) continue;
-		const long long ix = d.idx(p.i, p.j, p.k + kk);
+// applyOutflowBC's extrapolation at the outflow cells listed by outflow_scan
+// (one cell per thread, grid-stride over the device count)
+__global__ void __launch_bounds__(256) k_outflow_extrap(Dims d, const unsigned* __restrict__ list,
+                                                        const unsigned* __restrict__ count,
+                                                        const uint8_t* __restrict__ f8, const float* __restrict__ vu,
+                                                        const float* __restrict__ vv, const float* __restrict__ vw,
+                                                        const float* __restrict__ pu, const float* __restrict__ pv,
+                                                        const float* __restrict__ pw, float timeStep,
+                                                        float* __restrict__ du, float* __restrict__ dv,
+                                                        float* __restrict__ dw) {
+	const unsigned m = *count;
+	const unsigned nxy = (unsigned)(d.nx * d.ny);
+	for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < m; q += gridDim.x * blockDim.x) {
+		const unsigned ix = list[q];
+		const unsigned k = ix / nxy, rest = ix - k * nxy;
+		const unsigned j = rest / (unsigned)d.nx, i = rest - j * (unsigned)d.nx;
+		outflow_extrap_cell(d, f8, vu, vv, vw, pu, pv, pw, timeStep, du, dv, dw, (int)i, (int)j, (int)k,
+		                    (long long)ix);
+	}
+}
+// and the copy back of the extrapolated values
+__global__ void __launch_bounds__(256) k_outflow_copy(const unsigned* __restrict__ list,
+                                                      const unsigned* __restrict__ count, const float* __restrict__ du,
+                                                      const float* __restrict__ dv, const float* __restrict__ dw,
+                                                      float* __restrict__ vu, float* __restrict__ vv,
+                                                      float* __restrict__ vw) {
+	const unsigned m = *count;
+	for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < m; q += gridDim.x * blockDim.x) {
+		const unsigned ix = list[q];
 		vu[ix] = du[ix];
 		vv[ix] = dv[ix];
 		vw[ix] = dw[ix];
 	}
 }
+
 
 // ---------------------------------------------------------------------------
 // KnSetWallBcs (extforces.cpp:186-217), all ijk.  kLightKZ planes per thread:
@@ -1330,17 +1341,17 @@ static void with_geo(const Dims& d, F&& f) {
 // are copied back into result
 static void outflow_bc(Scene* s, Grid* flags, Grid* grid, float* result) {
 	const Dims d = s->d;
-	const Tiles t = tiles_for(d);
 	const double tsd = (double)s->dt * 4;
 	const float ts = (float)((1.0 < tsd) ? tsd : 1.0);
 	float* vdst = s->getVec();
 	float* r3[3] = {result, result + s->cstride, result + 2 * s->cstride};
 	float* d3[3] = {vdst, vdst + s->cstride, vdst + 2 * s->cstride};
-	const Tiles tl = tiles_for(d, kLightKZ);
-	hipLaunchKernelGGL(k_outflow_extrap, dim3((unsigned)t.count()), dim3(kBlock), 0, s->stream, d, t, flags->f8,
-	                   (const float*)r3[0], (const float*)r3[1], (const float*)r3[2], (const float*)grid->comp(0),
-	                   (const float*)grid->comp(1), (const float*)grid->comp(2), ts, d3[0], d3[1], d3[2]);
-	hipLaunchKernelGGL(k_outflow_copy, dim3((unsigned)tl.count()), dim3(kBlock), 0, s->stream, d, tl, flags->f8,
+	const unsigned* count = nullptr;
+	const unsigned* list = outflow_list(flags, &count);
+	hipLaunchKernelGGL(k_outflow_extrap, dim3(kOutflowListBlocks), dim3(256), 0, s->stream, d, list, count, flags->f8, (const float*)r3[0], (const float*)r3[1], (const float*)r3[2],
+	                   (const float*)grid->comp(0), (const float*)grid->comp(1), (const float*)grid->comp(2), ts, d3[0],
+	                   d3[1], d3[2]);
+	hipLaunchKernelGGL(k_outflow_copy, dim3(kOutflowListBlocks), dim3(256), 0, s->stream, list, count,
 	                   (const float*)d3[0], (const float*)d3[1], (const float*)d3[2], r3[0], r3[1], r3[2]);
 	s->putVec(vdst);
 }

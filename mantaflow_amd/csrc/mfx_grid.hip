@@ -123,6 +123,7 @@ void flags_touched(Grid* g) { g->gen = ++g_flag_gen; }
 void flags_repack(Grid* g) {
 	Scene* s = g->s;
 	flags_touched(g);
+	g->ogen = g->gen;
 	hipLaunchKernelGGL(k_pack_flags, dim3(flat_blocks(s->d.n)), dim3(256), 0, s->stream,
 	                   (const int*)g->data, g->f8, s->d.n);
 }
@@ -155,6 +156,8 @@ void free_grid(Grid* g) {
 	hipStreamSynchronize(s->stream);
 	if (g->data) hipFree(g->data);
 	if (g->f8) hipFree(g->f8);
+	if (g->olist) hipFree(g->olist);
+	if (g->ocount) hipFree(g->ocount);
 	for (size_t i = 0; i < s->grids.size(); i++)
 		if (s->grids[i] == g) {
 			s->grids.erase(s->grids.begin() + i);
@@ -613,18 +616,70 @@ __global__ void k_set_open_bound(int* f, Dims d, int bw, int lox, int loy, int l
 	}
 }
 
-// The uint8 mirror f8 holds the outflow bit, so the scan reads 1 B per cell
-// and touches the int32 flags only at outflow cells.
-__global__ void k_reset_outflow(int* f, uint8_t* f8, float* phi, float* real, long long n) {
-	for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-	     i += (long long)gridDim.x * blockDim.x) {
-		if (!(f8[i] & kOutflow)) continue;
+// resetOutflow over the outflow cells listed by outflow_scan (the uint8
+// mirror f8 holds the outflow bit, so the scan reads 1 B per cell and the
+// int32 flags are touched only at outflow cells)
+__global__ void k_reset_outflow(int* f, uint8_t* f8, float* phi, float* real, const unsigned* __restrict__ list,
+                                const unsigned* __restrict__ count) {
+	const unsigned m = *count;
+	for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < m; q += gridDim.x * blockDim.x) {
+		const unsigned i = list[q];
 		const int v = (f[i] | kEmpty) & ~kFluid;
 		f[i] = v;
 		f8[i] = (uint8_t)(v & 0x7f);
 		if (phi) phi[i] = 0.5f;
 		if (real) real[i] = 0.f;
 	}
+}
+
+__global__ void k_zero_u32(unsigned* p) { *p = 0u; }
+
+// 16 flag bytes per thread (one 16-byte load where the run is whole); a
+// thread with outflow cells reserves its slots with one atomic
+__global__ void k_outflow_scan(const uint8_t* __restrict__ f8, long long n, unsigned* __restrict__ list,
+                               unsigned* __restrict__ count) {
+	for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t * 16 < n;
+	     t += (long long)gridDim.x * blockDim.x) {
+		const long long b0 = t * 16;
+		uint8_t fl[16];
+		if (b0 + 16 <= n) {
+			const uint4 w = *(const uint4*)(f8 + b0);
+			const unsigned ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+			for (int e = 0; e < 16; e++) fl[e] = (uint8_t)(ws[e >> 2] >> (8 * (e & 3)));
+		} else {
+#pragma unroll
+			for (int e = 0; e < 16; e++) fl[e] = b0 + e < n ? f8[b0 + e] : (uint8_t)0;
+		}
+		unsigned m = 0;
+#pragma unroll
+		for (int e = 0; e < 16; e++) m += (fl[e] & kOutflow) ? 1u : 0u;
+		if (m == 0) continue;
+		unsigned at = atomicAdd(count, m);
+#pragma unroll
+		for (int e = 0; e < 16; e++)
+			if (fl[e] & kOutflow) list[at++] = (unsigned)(b0 + e);
+	}
+}
+
+const unsigned* outflow_list(Grid* g, const unsigned** count) {
+	Scene* s = g->s;
+	MFX_CHECK(g->isFlags() && g->f8, "outflow cells: needs a FlagGrid");
+	MFX_CHECK(s->d.n < (1LL << 32), "grid too large for the outflow cell list");
+	if (!g->olist) {
+		MFX_HIP(hipMalloc((void**)&g->olist, sizeof(unsigned) * (size_t)s->d.n));
+		MFX_HIP(hipMalloc((void**)&g->ocount, 256));
+		g->olistGen = 0;
+	}
+	if (g->olistGen != g->ogen) {
+		hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s->stream, g->ocount);
+		const long long runs = (s->d.n + 15) / 16;
+		hipLaunchKernelGGL(k_outflow_scan, dim3(flat_blocks(runs)), dim3(256), 0, s->stream, (const uint8_t*)g->f8,
+		                   s->d.n, g->olist, g->ocount);
+		g->olistGen = g->ogen;
+	}
+	*count = g->ocount;
+	return g->olist;
 }
 
 __global__ void k_count_cells(const int* f, Dims d, int flag, int bnd, float* mask,
@@ -957,6 +1012,11 @@ int mfx_grid_swap(mfx_grid* x, mfx_grid* y) {
 	if (a->isFlags()) {
 		flags_touched(a);
 		flags_touched(b);
+		// the outflow cell lists follow the data they were built from
+		std::swap(a->olist, b->olist);
+		std::swap(a->ocount, b->ocount);
+		std::swap(a->olistGen, b->olistGen);
+		std::swap(a->ogen, b->ogen);
 	}
 	API_END
 }
@@ -1232,8 +1292,11 @@ int mfx_reset_outflow(mfx_grid* gg, mfx_grid* phi, mfx_grid* real) {
 	MFX_CHECK(!real || G(real)->isReal(), "resetOutflow: real must be a Real grid");
 	Scene* s = g->s;
 	flags_touched(g);
-	hipLaunchKernelGGL(k_reset_outflow, dim3(flat_blocks(s->d.n)), dim3(256), 0, s->stream, g->ints(), g->f8,
-	                   phi ? G(phi)->real() : nullptr, real ? G(real)->real() : nullptr, s->d.n);
+	// (the outflow flags themselves are not changed here: the list stays valid)
+	const unsigned* count = nullptr;
+	const unsigned* list = outflow_list(g, &count);
+	hipLaunchKernelGGL(k_reset_outflow, dim3(kOutflowListBlocks), dim3(256), 0, s->stream, g->ints(), g->f8,
+	                   phi ? G(phi)->real() : nullptr, real ? G(real)->real() : nullptr, list, count);
 	API_END
 }
 

@@ -120,6 +120,12 @@ struct Grid {
 	void* data = nullptr;           // float[n] / int32[n] / float[3*cstride] (SoA)
 	uint8_t* f8 = nullptr;          // flag grids: packed bits 0..6
 	uint64_t gen = 0;               // flag grids: bumped by every write (flags_touched)
+	// flag grids: indices of the outflow cells (outflow_list), valid while
+	// olistGen == ogen; ogen is bumped by every write that can change which
+	// cells carry the outflow flag (all but resetOutflow's)
+	unsigned* olist = nullptr;
+	unsigned* ocount = nullptr;
+	uint64_t ogen = 1, olistGen = 0;
 	bool external = false;
 	float* comp(int c) const { return (float*)data + (size_t)c * s->cstride; }
 	float* real() const { return (float*)data; }
@@ -316,6 +322,13 @@ double reduce_sum_double(Scene* s, const float* a, int mode, long long n);
 
 // kernels / ops implemented in the various .hip files
 void op_set_wall_bcs(Scene* s, Grid* flags, Grid* vel, Grid* obvel);
+// Indices of the flag grid's outflow cells, in no particular order, and
+// their number (*count, on the device), rebuilt on the scene's stream only
+// when a write may have changed which cells are outflow cells.  The kernels
+// that only touch outflow cells run over the list (grid-stride over *count)
+// instead of over every cell.
+const unsigned* outflow_list(Grid* flags, const unsigned** count);
+constexpr unsigned kOutflowListBlocks = 1024;
 void op_add_buoyancy(Scene* s, Grid* flags, Grid* density, Grid* vel, const float f[3]);
 // setWallBcs (no obstacle velocity) then addBuoyancy, one pass
 void op_wall_buoyancy(Scene* s, Grid* flags, Grid* density, Grid* vel, const float f[3]);

@@ -262,3 +262,53 @@ def test_mult_add_fused(manta, rng, shape):
     g.add(b)
     same_bits(g.toNumpy(), ref, "add after a read")
     assert L.mfx_grid_mult_add(g._h, a._h, g._h) != 0
+
+
+@pytest.mark.parametrize("shape", [(26, 22, 1), (19, 15, 13)])
+def test_outflow_list_follows_flag_changes(manta, oracle, rng, shape):
+    """The outflow-cell list that resetOutflow and the MAC advection's outflow
+    extrapolation run over is kept per flag grid and rebuilt after any write
+    that can change which cells are outflow cells (setOpenBound, a new
+    upload, a swap of flag grids); resetOutflow itself keeps it.  Each stage
+    bit-exact against the oracle on the same flags."""
+    from oracle import TYPE_MAC
+    n = int(np.prod(shape))
+    dt = 1.1
+    sc = GpuScene(manta, shape, dt)
+    flags = make_flags(oracle, shape, bw=1, open_bound="Y", obstacles=3, rng=rng)
+    flags[(flags & OUTFLOW) != 0] |= FLUID
+    f = sc.flags(flags)
+    vel = rand_vel(rng, n)
+    v = sc.mac(vel)
+    ref = vel.copy()
+
+    def step(tag):
+        oracle.reset_outflow(shape, flags)
+        manta.resetOutflow(flags=f)
+        oracle.advect(shape, np.float32(dt), flags, ref, ref, TYPE_MAC, 2, 1.0, 2)
+        manta.advectSemiLagrange(flags=f, vel=v, grid=v, order=2)
+        assert_bitexact(f.toNumpy(), flags, "flags " + tag)
+        assert_bitexact(v.toNumpy(), ref, "advect " + tag)
+
+    step("open Y")
+    step("open Y again")
+    # more open sides on the same grid
+    oracle.set_open_bound(shape, flags, 1, "xX")
+    manta.setOpenBound(f, 1, "xX", manta.FlagOutflow | manta.FlagEmpty)
+    step("open xXY")
+    # new flags uploaded into the same grid
+    flags = make_flags(oracle, shape, bw=1, open_bound="y", obstacles=2, rng=rng)
+    f.fromNumpy(flags)
+    step("open y")
+    # swap with a grid whose open side is another one
+    other = make_flags(oracle, shape, bw=1, open_bound="X", obstacles=2, rng=rng)
+    g = sc.flags(other)
+    manta.resetOutflow(flags=g)  # builds g's list
+    oracle.reset_outflow(shape, other)
+    _check(manta.lib().mfx_grid_swap(f._h, g._h))  # Grid::swap (grid.h:226)
+    flags, other = other, flags
+    step("swapped in open X")
+    assert_bitexact(g.toNumpy(), other, "swapped out")
+    oracle.reset_outflow(shape, other)
+    manta.resetOutflow(flags=g)
+    assert_bitexact(g.toNumpy(), other, "resetOutflow after the swap")
