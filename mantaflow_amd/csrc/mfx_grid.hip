@@ -44,6 +44,7 @@ Scene::~Scene() {
 	if (code) hipFree(code);
 	if (hscal) hipHostFree(hscal);
 	if (ev) hipEventDestroy(ev);
+	if (evState) hipEventDestroy(evState);
 	if (arena) hipFree(arena);
 	if (hstage) hipHostFree(hstage);
 	if (bexec) hipGraphExecDestroy(bexec);

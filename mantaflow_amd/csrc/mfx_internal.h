@@ -86,6 +86,7 @@ struct Scene {
 	void* dscal = nullptr;      // device scalar block
 	void* hscal = nullptr;      // pinned host mirror
 	hipEvent_t ev = nullptr;    // cross-stream ordering (batched solves)
+	hipEvent_t evState = nullptr;  // a CG state copy the host waits for
 	void* arena = nullptr;      // work lists of batched solves led by this scene
 	size_t arenaCap = 0;
 	void* hstage = nullptr;     // pinned staging of the arena upload

@@ -2055,15 +2055,26 @@ static bool cg_needs_restart(const CgRun& R) {
 	return rel > R.o.relTol;
 }
 static void cg_iterate(CgRun& R, double target);
+// Iterations, then the FP64 true residual and the state copy, until the
+// copied state says done or maxIter iterations are queued.  The residual is
+// queued as soon as the iterations in flight are predicted to converge; if
+// they did not, it is superseded by the one after further iterations.
+static void cg_iterate_finish(CgRun& R, double target) {
+	while (true) {
+		cg_iterate(R, target);
+		cg_true_residual_enqueue(R);
+		sync(R.s);
+		if (((const CgState*)R.s->hscal)->done || R.launched >= R.maxIter) return;
+	}
+}
+
 static void cg_verify(CgRun& R, double target) {
 	for (int k = 0; k < kMaxRestarts && cg_needs_restart(R); k++) {
 		// iterations queued past convergence did nothing; the budget follows
 		// the device count (p = z after the restart, so the buffer parity is free)
 		R.launched = ((const CgState*)R.s->hscal)->iter;
 		cg_restart(R);
-		cg_iterate(R, target);
-		cg_true_residual_enqueue(R);
-		sync(R.s);
+		cg_iterate_finish(R, target);
 	}
 }
 
@@ -2120,36 +2131,49 @@ static void solve_system(Scene* s, Grid* flags, float* rhs, float* x, const mfx_
 // stopping rule holds or maxIter is reached, then finish (statistics,
 // divergence error).  target: the stopping threshold the prediction aims at.
 static void cg_drive(CgRun& R, double target, mfx_solve_stats* stats) {
-	cg_iterate(R, target);
-	cg_true_residual_enqueue(R);
-	sync(R.s);
+	cg_iterate_finish(R, target);
 	cg_verify(R, target);
 	cg_release(R);
 	cg_finish_collect(R, stats);
 }
 
+// Queues iterations in batches sized from the residual history.  The host
+// reads the state of the next-to-last iteration of a batch while the GPU
+// runs the last one, so the queue does not drain at these checks; it
+// returns when the iterations queued are predicted to reach the target (or
+// to be done, or maxIter is reached), and the caller queues the finish.
 static void cg_iterate(CgRun& R, double target) {
 	Scene* s = R.s;
 	int batch = 4;
 	CgState* hs = (CgState*)s->hscal;
+	if (!s->evState) MFX_HIP(hipEventCreateWithFlags(&s->evState, hipEventDisableTiming));
 	while (true) {
 		const int nIt = std::min(batch, R.maxIter - R.launched);
-		for (int it = 0; it < nIt; it++) cg_iteration(R);
-		MFX_HIP(hipMemcpyAsync(hs, R.st, sizeof(CgState), hipMemcpyDeviceToHost, s->stream));
-		sync(s);
-		if (hs->done || R.launched >= R.maxIter) break;
-		int itNow = hs->iter;
-		int predict = batch;
+		for (int it = 0; it < nIt; it++) {
+			cg_iteration(R);
+			if (it == nIt - 2) {
+				MFX_HIP(hipMemcpyAsync(hs, R.st, sizeof(CgState), hipMemcpyDeviceToHost, s->stream));
+				MFX_HIP(hipEventRecord(s->evState, s->stream));
+			}
+		}
+		if (nIt < 2 || R.launched >= R.maxIter) return;
+		MFX_HIP(hipEventSynchronize(s->evState));
+		if (hs->done) return;  // the iteration in flight returns at once
+		// iterations still needed after the one in flight (without a usable
+		// rate: another batch of the same size)
+		int more = batch;
+		const int itNow = hs->iter;
 		if (itNow >= 2 && itNow <= 64) {
 			const double a = hs->hist[itNow - 2], b = hs->hist[itNow - 1];
 			if (a > 0 && b > 0 && b < a && target > 0) {
 				const double rate = b / a;
 				double need = std::log(target / b) / std::log(rate);
 				if (need < 1) need = 1;
-				predict = (int)std::ceil(need);
+				more = (int)std::ceil(need) - 1;
 			}
 		}
-		batch = std::max(1, std::min(predict, 64));
+		if (more <= 0) return;
+		batch = std::min(more, 64);
 	}
 }
 
