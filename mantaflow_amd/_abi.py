@@ -7,7 +7,8 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libmfx.so")
+# MFX_LIB names another build of the library (the sanitizer test's)
+LIB_PATH = os.environ.get("MFX_LIB") or os.path.join(_HERE, "libmfx.so")
 
 GRID_REAL, GRID_INT, GRID_VEC3, GRID_MAC, GRID_LEVELSET, GRID_FLAGS = 1, 2, 4, 12, 17, 34
 RED_MAX, RED_MIN, RED_MAXABS, RED_L1, RED_L2, RED_SUM = 0, 1, 2, 3, 4, 5
